@@ -1,0 +1,509 @@
+// The C ABI of libripley_hip (declared in include/ripley_hip.h): argument
+// checks mirroring the reference's exceptions, ownership of device memory,
+// and the host-pointer variants that stage escript Data buffers.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <iomanip>
+#include <string>
+#include <vector>
+
+#include "rg_internal.h"
+
+namespace rg {
+int solve(rg_matrix* m, double* x, const double* b, const rg_options& opt, rg_diagnostics& dg);
+void ensure_work(rg_matrix* m);
+void free_work(rg_solver_work* w);
+}  // namespace rg
+
+namespace {
+
+thread_local std::string g_error;
+
+template <class F>
+int guarded(F&& f)
+{
+    try {
+        g_error.clear();
+        return f();
+    } catch (const rg::Error& e) {
+        g_error = e.what();
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        g_error = "out of host memory";
+        return RG_MEMORY_ERROR;
+    } catch (const std::exception& e) {
+        g_error = e.what();
+        return RG_VALUE_ERROR;
+    }
+}
+
+void use_device(const rg_context* ctx) { RG_HIP(hipSetDevice(ctx->device)); }
+
+// Device staging of a host array for the *_host entry points.
+struct DevBuf {
+    double* p = nullptr;
+    ~DevBuf()
+    {
+        if (p) hipFree(p);
+    }
+};
+
+void check_matrix(const rg_matrix* m)
+{
+    RG_CHECK(m != nullptr, RG_VALUE_ERROR, "matrix handle is NULL");
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* rg_last_error(void) { return g_error.c_str(); }
+
+const char* rg_version(void) { return "ripley_hip 1.0 (gfx950)"; }
+
+int rg_context_create(int device, void* stream, rg_context** out)
+{
+    return guarded([&] {
+        RG_CHECK(out != nullptr, RG_VALUE_ERROR, "rg_context_create: out is NULL");
+        int ndev = 0;
+        RG_HIP(hipGetDeviceCount(&ndev));
+        RG_CHECK(device >= 0 && device < ndev, RG_VALUE_ERROR, "rg_context_create: no such device");
+        RG_HIP(hipSetDevice(device));
+        rg_context* c = new rg_context();
+        c->device = device;
+        c->stream = reinterpret_cast<hipStream_t>(stream);
+        hipDeviceProp_t prop;
+        RG_HIP(hipGetDeviceProperties(&prop, device));
+        c->num_cu = prop.multiProcessorCount;
+        try {
+            RG_HIP(hipMalloc(&c->partials, sizeof(double) * 16 * rg::kMaxGrid));
+            RG_HIP(hipMalloc(&c->counters, sizeof(unsigned) * rg::kNumCounters));
+            RG_HIP(hipMemset(c->counters, 0, sizeof(unsigned) * rg::kNumCounters));
+            RG_HIP(hipHostMalloc(&c->pinned, sizeof(double) * 256, hipHostMallocDefault));
+        } catch (...) {
+            if (c->partials) hipFree(c->partials);
+            if (c->counters) hipFree(c->counters);
+            delete c;
+            throw;
+        }
+        *out = c;
+        return RG_OK;
+    });
+}
+
+int rg_context_destroy(rg_context* ctx)
+{
+    return guarded([&] {
+        if (!ctx) return RG_OK;
+        hipSetDevice(ctx->device);
+        if (ctx->stream) hipStreamSynchronize(ctx->stream);
+        else hipDeviceSynchronize();
+        if (ctx->partials) hipFree(ctx->partials);
+        if (ctx->counters) hipFree(ctx->counters);
+        if (ctx->pinned) hipHostFree(ctx->pinned);
+        delete ctx;
+        return RG_OK;
+    });
+}
+
+int rg_context_set_stream(rg_context* ctx, void* stream)
+{
+    return guarded([&] {
+        RG_CHECK(ctx != nullptr, RG_VALUE_ERROR, "context is NULL");
+        ctx->stream = reinterpret_cast<hipStream_t>(stream);
+        return RG_OK;
+    });
+}
+
+int rg_context_synchronize(rg_context* ctx)
+{
+    return guarded([&] {
+        RG_CHECK(ctx != nullptr, RG_VALUE_ERROR, "context is NULL");
+        use_device(ctx);
+        RG_HIP(hipStreamSynchronize(ctx->stream));
+        return RG_OK;
+    });
+}
+
+int rg_matrix_create(rg_context* ctx, int dim, const int32_t* ne, const double* length, int block_size,
+                     int flags, rg_matrix** out)
+{
+    return guarded([&] {
+        RG_CHECK(ctx && ne && length && out, RG_VALUE_ERROR, "rg_matrix_create: NULL argument");
+        RG_CHECK(dim == 2 || dim == 3, RG_VALUE_ERROR, "rg_matrix_create: dim must be 2 or 3");
+        RG_CHECK(block_size >= 1 && block_size <= 4, RG_VALUE_ERROR,
+                 "rg_matrix_create: block size must be 1..4");
+        use_device(ctx);
+        rg::Geom g;
+        g.dim = dim;
+        g.bs = block_size;
+        for (int d = 0; d < dim; ++d) {
+            RG_CHECK(ne[d] >= 1, RG_VALUE_ERROR, "rg_matrix_create: need at least one element per axis");
+            RG_CHECK(length[d] > 0, RG_VALUE_ERROR, "rg_matrix_create: side lengths must be positive");
+            g.ne[d] = ne[d];
+            g.nn[d] = (int64_t)ne[d] + 1;
+            g.len[d] = length[d];
+            g.h[d] = length[d] / ne[d];
+        }
+        g.N = g.nn[0] * g.nn[1] * g.nn[2];
+        RG_CHECK(g.N < (int64_t)1 << 31, RG_VALUE_ERROR, "rg_matrix_create: more than 2^31 nodes");
+        g.s1 = g.nn[0];
+        g.s2 = g.nn[0] * g.nn[1];
+        g.nslot = (dim == 3) ? 27 : 9;
+        const bool sym = (flags & RG_MATRIX_SYMMETRIC) != 0;
+        g.nstore = sym ? g.nslot / 2 + 1 : g.nslot;
+        g.first = sym ? g.nslot / 2 : 0;
+        g.ld = ((g.N + 63) / 64) * 64;
+        g.nnz = rg::csr_nnz(g);
+        RG_CHECK(g.nnz < (int64_t)1 << 31, RG_VALUE_ERROR, "rg_matrix_create: PASO pattern exceeds int32 indices");
+        rg_matrix* m = new rg_matrix();
+        m->ctx = ctx;
+        m->g = g;
+        m->flags = flags;
+        m->symmetric = sym;
+        try {
+            const size_t nval = (size_t)g.nstore * g.bs * g.bs * g.ld;
+            RG_HIP(hipMalloc(&m->val, nval * sizeof(double)));
+            RG_HIP(hipMemsetAsync(m->val, 0, nval * sizeof(double), ctx->stream));
+            RG_HIP(hipMalloc(&m->dinv, (size_t)g.bs * g.bs * g.ld * sizeof(double)));
+            RG_HIP(hipMalloc(&m->dinv32, (size_t)g.bs * g.bs * g.ld * sizeof(float)));
+            RG_HIP(hipMalloc(&m->balance, (size_t)g.N * g.bs * sizeof(double)));
+            RG_HIP(hipStreamSynchronize(ctx->stream));
+        } catch (...) {
+            if (m->val) hipFree(m->val);
+            if (m->dinv) hipFree(m->dinv);
+            if (m->dinv32) hipFree(m->dinv32);
+            if (m->balance) hipFree(m->balance);
+            delete m;
+            throw;
+        }
+        *out = m;
+        return RG_OK;
+    });
+}
+
+int rg_matrix_destroy(rg_matrix* m)
+{
+    return guarded([&] {
+        if (!m) return RG_OK;
+        hipSetDevice(m->ctx->device);
+        hipStreamSynchronize(m->ctx->stream);
+        rg::free_work(m->work);
+        void* bufs[] = {m->val, m->val32, m->dinv, m->dinv32, m->balance};
+        for (void* b : bufs)
+            if (b) hipFree(b);
+        delete m;
+        return RG_OK;
+    });
+}
+
+int rg_matrix_get_info(const rg_matrix* m, int64_t* num_rows, int64_t* nnz, int* num_slots, int* block_size)
+{
+    return guarded([&] {
+        check_matrix(m);
+        if (num_rows) *num_rows = m->g.N;
+        if (nnz) *nnz = m->g.nnz;
+        if (num_slots) *num_slots = m->g.nstore;
+        if (block_size) *block_size = m->g.bs;
+        return RG_OK;
+    });
+}
+
+int rg_matrix_set_values(rg_matrix* m, double value)
+{
+    return guarded([&] {
+        check_matrix(m);
+        use_device(m->ctx);
+        rg::launch_set_values(m, value);
+        m->is_balanced = false;
+        m->val32_valid = false;
+        return RG_OK;
+    });
+}
+
+int rg_matrix_reset_values(rg_matrix* m, int preserve_solver_data)
+{
+    return guarded([&] {
+        check_matrix(m);
+        use_device(m->ctx);
+        rg::launch_set_values(m, 0.0);
+        m->is_balanced = false;
+        m->val32_valid = false;
+        if (!preserve_solver_data) m->have_prec = false;
+        return RG_OK;
+    });
+}
+
+int rg_matrix_add_element_blocks(rg_matrix* m, int64_t num_elements, int nodes_per_element,
+                                 const int32_t* nodes, int num_eq, const double* em)
+{
+    return guarded([&] {
+        check_matrix(m);
+        RG_CHECK(num_eq == m->g.bs, RG_VALUE_ERROR,
+                 "addToSystemMatrix: number of equations does not match the block size");
+        RG_CHECK(num_elements >= 0 && (num_elements == 0 || (nodes && em)), RG_VALUE_ERROR,
+                 "addToSystemMatrix: NULL element data");
+        if (num_elements == 0) return RG_OK;
+        use_device(m->ctx);
+        for (int64_t k = 0; k < num_elements * nodes_per_element; ++k)
+            RG_CHECK(nodes[k] >= 0 && nodes[k] < m->g.N, RG_VALUE_ERROR,
+                     "addToSystemMatrix: node index out of range");
+        const size_t nsz = sizeof(int32_t) * num_elements * nodes_per_element;
+        const size_t esz = sizeof(double) * num_elements * nodes_per_element * nodes_per_element * num_eq * num_eq;
+        void* dn = nullptr;
+        void* de = nullptr;
+        try {
+            RG_HIP(hipMalloc(&dn, nsz));
+            RG_HIP(hipMalloc(&de, esz));
+            RG_HIP(hipMemcpyAsync(dn, nodes, nsz, hipMemcpyHostToDevice, m->ctx->stream));
+            RG_HIP(hipMemcpyAsync(de, em, esz, hipMemcpyHostToDevice, m->ctx->stream));
+            rg::launch_add_element_blocks(m, num_elements, nodes_per_element, (const int32_t*)dn, (const double*)de);
+            RG_HIP(hipStreamSynchronize(m->ctx->stream));
+        } catch (...) {
+            if (dn) hipFree(dn);
+            if (de) hipFree(de);
+            throw;
+        }
+        hipFree(dn);
+        hipFree(de);
+        m->val32_valid = false;
+        return RG_OK;
+    });
+}
+
+int rg_matrix_assemble(rg_matrix* m, const rg_pde_coefs* coefs, int reduced, double* rhs)
+{
+    return guarded([&] {
+        check_matrix(m);
+        RG_CHECK(coefs != nullptr, RG_VALUE_ERROR, "assemble: coefficients are NULL");
+        use_device(m->ctx);
+        rg::launch_assemble(m, *coefs, reduced, rhs);
+        return RG_OK;
+    });
+}
+
+int rg_matrix_vector(rg_matrix* m, double alpha, const double* x, double beta, double* y)
+{
+    return guarded([&] {
+        check_matrix(m);
+        RG_CHECK(!m->is_balanced, RG_STATE_ERROR, "MatrixVector: balanced matrix is not supported.");
+        RG_CHECK(x && y, RG_VALUE_ERROR, "MatrixVector: NULL vector");
+        use_device(m->ctx);
+        rg::launch_spmv(m, alpha, x, beta, y);
+        return RG_OK;
+    });
+}
+
+int rg_matrix_ypAx(rg_matrix* m, double* y, const double* x) { return rg_matrix_vector(m, 1.0, x, 1.0, y); }
+
+int rg_matrix_ypAx_host(rg_matrix* m, double* y, const double* x)
+{
+    return guarded([&] {
+        check_matrix(m);
+        RG_CHECK(!m->is_balanced, RG_STATE_ERROR, "MatrixVector: balanced matrix is not supported.");
+        RG_CHECK(x && y, RG_VALUE_ERROR, "MatrixVector: NULL vector");
+        use_device(m->ctx);
+        const size_t n = (size_t)m->g.N * m->g.bs;
+        DevBuf dx, dy;
+        RG_HIP(hipMalloc(&dx.p, n * sizeof(double)));
+        RG_HIP(hipMalloc(&dy.p, n * sizeof(double)));
+        hipStream_t st = m->ctx->stream;
+        RG_HIP(hipMemcpyAsync(dx.p, x, n * sizeof(double), hipMemcpyHostToDevice, st));
+        RG_HIP(hipMemcpyAsync(dy.p, y, n * sizeof(double), hipMemcpyHostToDevice, st));
+        rg::launch_spmv(m, 1.0, dx.p, 1.0, dy.p);
+        RG_HIP(hipMemcpyAsync(y, dy.p, n * sizeof(double), hipMemcpyDeviceToHost, st));
+        RG_HIP(hipStreamSynchronize(st));
+        return RG_OK;
+    });
+}
+
+int rg_matrix_nullify_rows_and_cols(rg_matrix* m, const double* row_mask, const double* col_mask, double mdv)
+{
+    return guarded([&] {
+        check_matrix(m);
+        RG_CHECK(row_mask && col_mask, RG_VALUE_ERROR, "nullifyRowsAndCols: NULL mask");
+        use_device(m->ctx);
+        rg::ensure_work(m);
+        RG_HIP(hipMemsetAsync(m->work->flag, 0, sizeof(int), m->ctx->stream));
+        rg::launch_nullify(m, row_mask, col_mask, mdv, m->work->flag);
+        return RG_OK;
+    });
+}
+
+int rg_matrix_nullify_rows_and_cols_host(rg_matrix* m, const double* row_mask, const double* col_mask, double mdv)
+{
+    return guarded([&] {
+        check_matrix(m);
+        RG_CHECK(row_mask && col_mask, RG_VALUE_ERROR, "nullifyRowsAndCols: NULL mask");
+        use_device(m->ctx);
+        const size_t n = (size_t)m->g.N * m->g.bs;
+        DevBuf dr, dc;
+        hipStream_t st = m->ctx->stream;
+        RG_HIP(hipMalloc(&dr.p, n * sizeof(double)));
+        RG_HIP(hipMemcpyAsync(dr.p, row_mask, n * sizeof(double), hipMemcpyHostToDevice, st));
+        const double* cm = dr.p;
+        if (col_mask != row_mask) {
+            RG_HIP(hipMalloc(&dc.p, n * sizeof(double)));
+            RG_HIP(hipMemcpyAsync(dc.p, col_mask, n * sizeof(double), hipMemcpyHostToDevice, st));
+            cm = dc.p;
+        }
+        rg::ensure_work(m);
+        RG_HIP(hipMemsetAsync(m->work->flag, 0, sizeof(int), st));
+        rg::launch_nullify(m, dr.p, cm, mdv, m->work->flag);
+        RG_HIP(hipStreamSynchronize(st));
+        return RG_OK;
+    });
+}
+
+int rg_matrix_export_csr(rg_matrix* m, int32_t* ptr, int32_t* index, double* val)
+{
+    return guarded([&] {
+        check_matrix(m);
+        use_device(m->ctx);
+        const rg::Geom& g = m->g;
+        std::vector<int32_t> hptr(g.N + 1);
+        rg::csr_row_ptr(g, hptr.data());
+        if (ptr) std::memcpy(ptr, hptr.data(), sizeof(int32_t) * (g.N + 1));
+        if (!index && !val) return RG_OK;
+        hipStream_t st = m->ctx->stream;
+        void *dptr = nullptr, *didx = nullptr, *dval = nullptr;
+        const size_t bs2 = (size_t)g.bs * g.bs;
+        try {
+            RG_HIP(hipMalloc(&dptr, sizeof(int32_t) * (g.N + 1)));
+            if (index) RG_HIP(hipMalloc(&didx, sizeof(int32_t) * g.nnz));
+            if (val) RG_HIP(hipMalloc(&dval, sizeof(double) * g.nnz * bs2));
+            RG_HIP(hipMemcpyAsync(dptr, hptr.data(), sizeof(int32_t) * (g.N + 1), hipMemcpyHostToDevice, st));
+            rg::launch_export_csr_values(m, (const int32_t*)dptr, (int32_t*)didx, (double*)dval);
+            if (index) RG_HIP(hipMemcpyAsync(index, didx, sizeof(int32_t) * g.nnz, hipMemcpyDeviceToHost, st));
+            if (val) RG_HIP(hipMemcpyAsync(val, dval, sizeof(double) * g.nnz * bs2, hipMemcpyDeviceToHost, st));
+            RG_HIP(hipStreamSynchronize(st));
+        } catch (...) {
+            if (dptr) hipFree(dptr);
+            if (didx) hipFree(didx);
+            if (dval) hipFree(dval);
+            throw;
+        }
+        hipFree(dptr);
+        if (didx) hipFree(didx);
+        if (dval) hipFree(dval);
+        return RG_OK;
+    });
+}
+
+int rg_matrix_save_mm(rg_matrix* m, const char* filename)
+{
+    return guarded([&] {
+        check_matrix(m);
+        RG_CHECK(filename != nullptr, RG_VALUE_ERROR, "SparseMatrix::saveMM: no file name");
+        const rg::Geom& g = m->g;
+        const size_t bs = g.bs, bs2 = bs * bs;
+        std::vector<int32_t> ptr(g.N + 1), idx(g.nnz);
+        std::vector<double> val(g.nnz * bs2);
+        int rc = rg_matrix_export_csr(m, ptr.data(), idx.data(), val.data());
+        if (rc) throw rg::Error(rc, g_error);
+        std::ofstream f(filename);
+        RG_CHECK(!f.fail(), RG_VALUE_ERROR, "SparseMatrix::saveMM: File could not be opened for writing");
+        // layout of SparseMatrix<double>::saveMM (paso/src/SparseMatrix.cpp:182-240)
+        f << "%%MatrixMarket matrix coordinate real general\n";
+        f << g.N * bs << " " << g.N * bs << " " << (int64_t)g.nnz * bs2 << "\n";
+        f.precision(15);
+        for (int64_t i = 0; i < g.N; ++i)
+            for (int32_t k = ptr[i]; k < ptr[i + 1]; ++k)
+                for (size_t r = 0; r < bs; ++r)
+                    for (size_t c = 0; c < bs; ++c)
+                        f << i * bs + r + 1 << " " << (int64_t)idx[k] * bs + c + 1 << " "
+                          << val[(size_t)k * bs2 + r + bs * c] << "\n";
+        return RG_OK;
+    });
+}
+
+void rg_options_default(rg_options* opt)
+{
+    if (!opt) return;
+    opt->tolerance = 1e-8;
+    opt->inner_tolerance = 1e-4;
+    opt->iter_max = 100000;
+    opt->sweeps = 1;
+    opt->method = RG_METHOD_PCG;
+    opt->verbose = 0;
+    opt->accept_failed_convergence = 0;
+    opt->check_interval = 0;
+}
+
+// Error mapping of SystemMatrix<double>::solve (paso/src/solve.cpp:119-143).
+static int map_solver_result(int res, const rg_options& opt)
+{
+    if (res == RG_OK) return RG_OK;
+    if ((res == RG_DIVERGENCE || res == RG_MAX_ITER_REACHED) && opt.accept_failed_convergence) {
+        if (opt.verbose) printf("paso: failed convergence error has been canceled as requested.\n");
+        return RG_OK;
+    }
+    switch (res) {
+        case RG_DIVERGENCE:
+            g_error = "Solver: No improvement during iteration. Iterative solver gives up.";
+            break;
+        case RG_MAX_ITER_REACHED:
+            g_error = "Solver: maximum number of iteration steps reached.\n"
+                      "Returned solution does not fulfil stopping criterion.";
+            break;
+        case RG_INPUT_ERROR: g_error = "Solver: illegal dimension in iterative solver."; break;
+        case RG_NEGATIVE_NORM: g_error = "Solver: negative energy norm (try other solver or preconditioner)."; break;
+        case RG_BREAKDOWN: g_error = "Solver: fatal break down in iterative solver."; break;
+        default: g_error = "Solver: Generic error in solver."; break;
+    }
+    return res;
+}
+
+int rg_solve(rg_matrix* m, double* x, const double* b, const rg_options* opt, rg_diagnostics* diag)
+{
+    rg_options o;
+    rg_options_default(&o);
+    if (opt) o = *opt;
+    rg_diagnostics dg;
+    std::memset(&dg, 0, sizeof(dg));
+    int res = guarded([&] {
+        check_matrix(m);
+        RG_CHECK(x && b, RG_VALUE_ERROR, "solve: NULL vector");
+        use_device(m->ctx);
+        return rg::solve(m, x, b, o, dg);
+    });
+    if (diag) *diag = dg;
+    if (res >= RG_VALUE_ERROR) return res;
+    return map_solver_result(res, o);
+}
+
+int rg_solve_host(rg_matrix* m, double* x, const double* b, const rg_options* opt, rg_diagnostics* diag)
+{
+    DevBuf dx, db;
+    int rc = guarded([&] {
+        check_matrix(m);
+        RG_CHECK(x && b, RG_VALUE_ERROR, "solve: NULL vector");
+        use_device(m->ctx);
+        const size_t n = (size_t)m->g.N * m->g.bs;
+        RG_HIP(hipMalloc(&dx.p, n * sizeof(double)));
+        RG_HIP(hipMalloc(&db.p, n * sizeof(double)));
+        RG_HIP(hipMemcpyAsync(db.p, b, n * sizeof(double), hipMemcpyHostToDevice, m->ctx->stream));
+        return RG_OK;
+    });
+    if (rc) return rc;
+    const int res = rg_solve(m, dx.p, db.p, opt, diag);
+    const std::string err = g_error;
+    int rc2 = guarded([&] {
+        const size_t n = (size_t)m->g.N * m->g.bs;
+        RG_HIP(hipMemcpyAsync(x, dx.p, n * sizeof(double), hipMemcpyDeviceToHost, m->ctx->stream));
+        RG_HIP(hipStreamSynchronize(m->ctx->stream));
+        return RG_OK;
+    });
+    if (res) {
+        g_error = err;
+        return res;
+    }
+    return rc2;
+}
+
+}  // extern "C"

@@ -1,0 +1,119 @@
+// Device-side helpers shared by the kernels of libripley_hip (gfx950).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "rg_internal.h"
+
+namespace rg {
+
+// Workgroups are dispatched round-robin over the 8 XCDs, each with its own
+// L2.  Streaming kernels remap the hardware block id so that the blocks of
+// one XCD cover one contiguous range of rows: neighbouring stencil planes of
+// x are then re-read from that XCD's L2 instead of from other XCDs.
+__device__ __forceinline__ int64_t xcd_block()
+{
+    const int64_t b = blockIdx.x, G = gridDim.x;
+    if (G % kNumXcd) return b;
+    return (b % kNumXcd) * (G / kNumXcd) + b / kNumXcd;
+}
+
+// Row range [r0, r1) of this block when n rows are split into gridDim.x
+// contiguous chunks (chunk is a multiple of the block size).
+struct RowRange {
+    int64_t r0, r1;
+};
+__device__ __forceinline__ RowRange block_rows(int64_t n, int64_t chunk)
+{
+    const int64_t lb = xcd_block();
+    const int64_t r0 = lb * chunk;
+    int64_t r1 = r0 + chunk;
+    if (r1 > n) r1 = n;
+    return {r0, r1};
+}
+
+// Sum over the 64 lanes of a wavefront (butterfly; every lane gets the sum).
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+__device__ __forceinline__ double wave_max(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// Sum of NV values over a 256-thread block, in a fixed order (deterministic);
+// the result is valid in all threads.
+template <int NV>
+__device__ __forceinline__ void block_sum(double (&v)[NV])
+{
+    __shared__ double red[NV][kBlock / 64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+        const double s = wave_sum(v[k]);
+        if (lane == 0) red[k][w] = s;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+        double s = 0.0;
+#pragma unroll
+        for (int j = 0; j < kBlock / 64; ++j) s += red[k][j];
+        v[k] = s;
+    }
+    __syncthreads();
+}
+
+// Last-workgroup reduction.  Every block publishes its NV block sums to
+// partials[k*G + b]; the block that takes the last ticket of *counter sums
+// all partials in a fixed order (so the result does not depend on timing),
+// leaves the totals in tot[] (valid in all its threads), resets the counter
+// to zero and returns true.  Other blocks return false.
+template <int NV>
+__device__ __forceinline__ bool last_block_sum(double (&v)[NV], double* partials,
+                                               unsigned* counter, double (&tot)[NV])
+{
+    __shared__ bool s_last;
+    block_sum<NV>(v);
+    const unsigned G = gridDim.x;
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < NV; ++k)
+            __hip_atomic_store(&partials[k * G + blockIdx.x], v[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __threadfence();
+        const unsigned t = atomicAdd(counter, 1u);
+        s_last = (t == G - 1);
+    }
+    __syncthreads();
+    if (!s_last) return false;
+    __threadfence();
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+        double s = 0.0;
+        for (unsigned b = threadIdx.x; b < G; b += kBlock)
+            s += __hip_atomic_load(&partials[k * G + b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        tot[k] = s;
+    }
+    block_sum<NV>(tot);
+    if (threadIdx.x == 0) __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return true;
+}
+
+// Full-stencil slot s -> (dx, dy, dz) in {-1,0,1}, getDiagonalIndices order.
+template <int DIM>
+__host__ __device__ __forceinline__ void slot_delta(int s, int& dx, int& dy, int& dz)
+{
+    dx = s % 3 - 1;
+    dy = (s / 3) % 3 - 1;
+    dz = (DIM == 3) ? s / 9 - 1 : 0;
+}
+
+}  // namespace rg

@@ -1,0 +1,138 @@
+// Internal declarations of libripley_hip (not part of the C ABI).
+//
+// Data layout in HBM (one rank's share of a Rectangle/Brick):
+//   * block rows are nodes, numbered lexicographically (x fastest) as
+//     ripley numbers DOFs at one rank (ripley/src/Brick.cpp:3000-3061);
+//   * the matrix is a diagonal-offset stencil: S = 9 (2-D) or 27 (3-D) slots
+//     per node in getDiagonalIndices order (Brick.cpp:2484-2507); with
+//     symmetric storage only the main and upper slots (5 / 14) are kept;
+//   * values are structure-of-arrays: entry (slot s, ir, ic) of node i is
+//     val[(s*bs*bs + ir + bs*ic) * ld + i] -- one coalesced stream per slot
+//     and block entry, the block entry order being PASO's column-major block
+//     (paso/src/SparseMatrix_MatrixVector.cpp:288-320);
+//   * vectors are DOF-major then component, x[i*bs + c], as escript Data;
+//   * the Jacobi inverse is structure-of-arrays dinv[(ir + bs*ic) * ld + i].
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "ripley_hip.h"
+
+namespace rg {
+
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define RG_HIP(call)                                                                          \
+    do {                                                                                      \
+        hipError_t e_ = (call);                                                               \
+        if (e_ != hipSuccess)                                                                 \
+            throw ::rg::Error(RG_DEVICE_ERROR, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+#define RG_CHECK(cond, code, msg)                                \
+    do {                                                         \
+        if (!(cond)) throw ::rg::Error((code), std::string(msg)); \
+    } while (0)
+
+constexpr int kBlock = 256;          // threads per workgroup of every streaming kernel
+constexpr int kNumXcd = 8;           // MI355X: 8 XCDs, blocks dispatched round-robin
+constexpr int kMaxGrid = 2048;       // grid cap of grid-stride streaming kernels
+constexpr int kNumCounters = 64;     // last-block ticket counters per context
+
+// Geometry of one matrix (one rank's share).
+struct Geom {
+    int dim = 3;
+    int bs = 1;
+    int ne[3] = {1, 1, 1};
+    int64_t nn[3] = {1, 1, 1};
+    double len[3] = {1, 1, 1};
+    double h[3] = {1, 1, 1};
+    int64_t N = 0;       // nodes = block rows
+    int64_t s1 = 0;      // nn0
+    int64_t s2 = 0;      // nn0*nn1
+    int nslot = 27;      // slots of the full stencil
+    int nstore = 27;     // slots stored (nslot or the upper half + main)
+    int first = 0;       // full-stencil index of stored slot 0
+    int64_t ld = 0;      // leading dimension of value / dinv streams
+    int64_t nnz = 0;     // PASO block nnz
+};
+
+// Scalars of one in-flight Krylov solve, resident in device memory and
+// updated by the last workgroup of each reducing kernel.
+struct KrylovState {
+    double tau, tau_new, delta, alpha, beta, gamma1, norm, tol;
+    double rho, rnorm0, pad0, pad1;
+    int32_t num_iter, maxit, done, status;
+    int32_t pad[4];
+};
+enum { KS_RUNNING = 0, KS_CONVERGED = 1, KS_MAXITER = 2, KS_BREAKDOWN = 3 };
+
+}  // namespace rg
+
+struct rg_context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int num_cu = 256;
+    double* partials = nullptr;     // 16 * kMaxGrid doubles of per-block partial sums
+    unsigned* counters = nullptr;   // kNumCounters last-block tickets (kept at zero between uses)
+    double* pinned = nullptr;       // host-pinned staging for scalars (256 doubles)
+    hipEvent_t ev = nullptr;
+};
+
+// Work vectors and cached launch graph of the Krylov solvers of one matrix.
+struct rg_solver_work {
+    int64_t n = 0;   // N*bs
+    double *r = nullptr, *xw = nullptr, *rs = nullptr, *p = nullptr, *v = nullptr, *x2 = nullptr;
+    double *z = nullptr, *z2 = nullptr;
+    double* norms = nullptr;        // [0] sum of squares, [1] max abs
+    rg::KrylovState* st = nullptr;
+    int* flag = nullptr;
+    hipStream_t capture = nullptr;
+    hipGraphExec_t graph = nullptr;
+    int graph_method = -1, graph_sweeps = -1, graph_iters = -1;
+};
+
+struct rg_matrix {
+    rg_context* ctx = nullptr;
+    rg::Geom g;
+    int flags = 0;
+    bool symmetric = false;
+    double* val = nullptr;         // nstore*bs*bs*ld
+    float* val32 = nullptr;        // FP32 copy for the mixed-precision solver
+    bool val32_valid = false;
+    double* dinv = nullptr;        // bs*bs*ld Jacobi inverse (of the balanced matrix)
+    float* dinv32 = nullptr;
+    double* balance = nullptr;     // N*bs balance vector
+    bool is_balanced = false;
+    bool have_prec = false;
+    rg_solver_work* work = nullptr;
+};
+
+namespace rg {
+
+// ------------------------------------------------------------- kernels API
+// All launches go to m->ctx->stream.
+void launch_set_values(rg_matrix* m, double value);
+void launch_add_element_blocks(rg_matrix* m, int64_t ne, int npe, const int32_t* d_nodes,
+                               const double* d_em);
+void launch_assemble(rg_matrix* m, const rg_pde_coefs& c, int reduced, double* rhs);
+void launch_spmv(rg_matrix* m, double alpha, const double* x, double beta, double* y);
+void launch_spmv32(rg_matrix* m, const double* x, double* y);   // FP32 values, y = A32 x
+void launch_nullify(rg_matrix* m, const double* row_mask, const double* col_mask, double mdv,
+                    int* d_flag);
+void launch_export_csr_values(rg_matrix* m, const int32_t* d_ptr, int32_t* d_index, double* d_val);
+void launch_to_fp32(rg_matrix* m);
+
+// Host-side pattern helpers.
+void csr_row_ptr(const Geom& g, int32_t* ptr);   // ptr[N+1] on host
+int64_t csr_nnz(const Geom& g);
+
+}  // namespace rg

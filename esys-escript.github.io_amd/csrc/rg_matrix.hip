@@ -1,0 +1,501 @@
+// Matrix kernels of libripley_hip: value initialisation, element-block
+// insertion, colour-free gather assembly, stencil SpMV, Dirichlet
+// nullification and the PASO CSR export.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+
+#include "rg_device.h"
+#include "rg_dispatch.h"
+#include "rg_stencil.h"
+
+namespace rg {
+
+// ------------------------------------------------------------- set values
+// paso::SystemMatrix::setValues (paso/src/SystemMatrix.h:273-279): every
+// entry of the PASO pattern := value.  Stencil slots whose neighbour lies
+// outside the grid are not in the pattern and are set to zero.
+template <int DIM, int BS, bool SYM>
+__global__ __launch_bounds__(kBlock) void k_set_values(double* __restrict__ val, int64_t ld, int64_t N,
+                                                       int64_t n0, int64_t n1, int64_t n2,
+                                                       double value)
+{
+    constexpr int NST = stored_slots<DIM, SYM>();
+    constexpr int FIRST = SYM ? StencilShape<DIM>::CENTER : 0;
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= N) return;
+    const int64_t x = i % n0, y = (i / n0) % n1, z = i / (n0 * n1);
+#pragma unroll
+    for (int t = 0; t < NST; ++t) {
+        const double v = slot_in_grid<DIM>(FIRST + t, x, y, z, n0, n1, n2) ? value : 0.0;
+#pragma unroll
+        for (int e = 0; e < BS * BS; ++e) val[(int64_t)(t * BS * BS + e) * ld + i] = v;
+    }
+}
+
+void launch_set_values(rg_matrix* m, double value)
+{
+    const Geom& g = m->g;
+    const int grid = (int)((g.N + kBlock - 1) / kBlock);
+    dispatch(g, m->symmetric, [&](auto D, auto B, auto S) {
+        hipLaunchKernelGGL((k_set_values<D.value, B.value, S.value>), dim3(grid), dim3(kBlock), 0,
+                           m->ctx->stream, m->val, g.ld, g.N, g.nn[0], g.nn[1], g.nn[2], value);
+    });
+    RG_HIP(hipGetLastError());
+}
+
+// -------------------------------------------------- element block insertion
+// ripley::SystemMatrix::add (ripley/src/RipleySystemMatrix.cu:111-155), the
+// stencil-matrix counterpart of addToPasoMatrix
+// (ripley/src/RipleyDomain.cpp:1488-1517): the pair (a, b) of local element
+// nodes lands in the slot given by their local positions; with symmetric
+// storage only pairs with b >= a are added.  One workgroup walks the elements
+// in order (deterministic); its threads cover the distinct (a, b, k, m)
+// targets of one element.
+template <int DIM, int BS, bool SYM>
+__global__ __launch_bounds__(kBlock) void k_add_elements(double* __restrict__ val, int64_t ld,
+                                                         int64_t num_el, const int32_t* __restrict__ nodes,
+                                                         const double* __restrict__ em)
+{
+    constexpr int NN = 1 << DIM;
+    constexpr int C = StencilShape<DIM>::CENTER;
+    constexpr int NT = NN * NN * BS * BS;
+    for (int64_t e = 0; e < num_el; ++e) {
+        for (int t = threadIdx.x; t < NT; t += kBlock) {
+            const int k = t % BS, mm = (t / BS) % BS, a = (t / (BS * BS)) % NN, b = t / (BS * BS * NN);
+            if (SYM && b < a) continue;
+            const int s = ((b & 1) - (a & 1) + 1) + 3 * (((b >> 1) & 1) - ((a >> 1) & 1) + 1) +
+                          (DIM == 3 ? 9 * (((b >> 2) & 1) - ((a >> 2) & 1) + 1) : 0);
+            const int st = SYM ? s - C : s;
+            const int64_t row = nodes[e * NN + a];
+            val[(int64_t)(st * BS * BS + k + BS * mm) * ld + row] += em[e * NT + t];
+        }
+        __syncthreads();
+    }
+}
+
+void launch_add_element_blocks(rg_matrix* m, int64_t ne, int npe, const int32_t* d_nodes,
+                               const double* d_em)
+{
+    const Geom& g = m->g;
+    RG_CHECK(npe == (1 << g.dim), RG_VALUE_ERROR,
+             "add_element_blocks: nodes_per_element must be 4 (2-D) or 8 (3-D)");
+    dispatch(g, m->symmetric, [&](auto D, auto B, auto S) {
+        hipLaunchKernelGGL((k_add_elements<D.value, B.value, S.value>), dim3(1), dim3(kBlock), 0,
+                           m->ctx->stream, m->val, g.ld, ne, d_nodes, d_em);
+    });
+    RG_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------- assembly
+struct CoefDev {
+    const double* p;
+    int mode;
+    int stride;
+    int iso;
+};
+struct AsmArgs {
+    CoefDev A, B, C, D, X, Y;
+    int64_t n0, n1, n2;     // nodes per axis
+    int ne0, ne1, ne2;      // elements per axis
+    double h0, h1, h2;
+    int reduced;
+};
+
+__device__ __forceinline__ const double* coef_pt(const CoefDev& c, int64_t e, int q, int dps)
+{
+    const double* p = c.p + (int64_t)c.stride * e;
+    return (c.mode == RG_COEF_EXPANDED) ? p + q * dps : p;
+}
+
+// Colour-free gather assembly.  One thread per (node i, equation k,
+// component m): it visits the 2^DIM elements around node i, computes row a
+// (the node's local index) of each element matrix by Gauss quadrature and
+// adds entry (a, b) to the stencil slot of local node b:
+//   EM_S[k,m,a,b] = sum_q w ( A[k,i,m,j] dN_a/dx_i dN_b/dx_j + B[k,i,m] N_b dN_a/dx_i
+//                           + C[k,m,j] N_a dN_b/dx_j + D[k,m] N_a N_b )
+//   EM_F[k,a]     = sum_q w ( X[k,i] dN_a/dx_i + Y[k] N_a )
+// -- the weak form that DefaultAssembler3D::assemblePDESingle/System
+// integrate with the same 2-point Gauss rule (points (1-+1/sqrt3)/2, x
+// fastest; ripley/src/DefaultAssembler3D.cpp:153-2229, :3631-5745) and, for
+// reduced, the one-point rule of assemblePDE*Reduced (:2891-3270).  Each
+// matrix entry and rhs entry is written by exactly one thread: no atomics.
+template <int DIM, int NB, bool SYM>
+__global__ __launch_bounds__(kBlock) void k_assemble(AsmArgs a, double* __restrict__ val, int64_t ld,
+                                                     int64_t N, double* __restrict__ rhs)
+{
+    constexpr int NS = StencilShape<DIM>::NS;
+    constexpr int CEN = StencilShape<DIM>::CENTER;
+    constexpr int NST = stored_slots<DIM, SYM>();
+    constexpr int FIRST = SYM ? CEN : 0;
+    constexpr int NNODE = 1 << DIM;
+    const int km = blockIdx.y;
+    const int k = km % NB, m = km / NB;
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= N) return;
+    const int64_t x = i % a.n0, y = (i / a.n0) % a.n1, z = (DIM == 3) ? i / (a.n0 * a.n1) : 0;
+    const double h[3] = {a.h0, a.h1, a.h2};
+    double vol = a.h0 * a.h1;
+    if (DIM == 3) vol *= a.h2;
+    const int nq = a.reduced ? 1 : NNODE;
+    const double w = vol / nq;
+    const double g0 = 0.5 - 0.5 / 1.73205080756887719318, g1 = 0.5 + 0.5 / 1.73205080756887719318;
+    const bool doS = (a.A.mode | a.B.mode | a.C.mode | a.D.mode) != 0;
+    const bool doF = rhs != nullptr && m == 0 && (a.X.mode | a.Y.mode) != 0;
+    const int dpsA = NB * DIM * NB * DIM, dpsB = NB * DIM * NB, dpsC = NB * NB * DIM, dpsD = NB * NB;
+    const int dpsX = NB * DIM, dpsY = NB;
+
+    double acc[NS];
+#pragma unroll
+    for (int s = 0; s < NS; ++s) acc[s] = 0.0;
+    double f = 0.0;
+
+#pragma unroll
+    for (int ez = 0; ez < (DIM == 3 ? 2 : 1); ++ez)
+#pragma unroll
+        for (int ey = 0; ey < 2; ++ey)
+#pragma unroll
+            for (int ex = 0; ex < 2; ++ex) {
+                const int64_t k0 = x - 1 + ex, k1 = y - 1 + ey, k2 = (DIM == 3) ? z - 1 + ez : 0;
+                if (k0 < 0 || k0 >= a.ne0 || k1 < 0 || k1 >= a.ne1) continue;
+                if (DIM == 3 && (k2 < 0 || k2 >= a.ne2)) continue;
+                const int64_t e = k0 + (int64_t)a.ne0 * (k1 + (int64_t)a.ne1 * k2);
+                // local index bits of node i in this element
+                const int abit[3] = {1 - ex, 1 - ey, 1 - ez};
+                for (int q = 0; q < nq; ++q) {
+                    double xi[3];
+#pragma unroll
+                    for (int d = 0; d < DIM; ++d) xi[d] = a.reduced ? 0.5 : (((q >> d) & 1) ? g1 : g0);
+                    double fa[3], Ga[3];
+#pragma unroll
+                    for (int d = 0; d < DIM; ++d) fa[d] = abit[d] ? xi[d] : 1.0 - xi[d];
+                    double Na = fa[0] * fa[1];
+                    if (DIM == 3) Na *= fa[2];
+#pragma unroll
+                    for (int d = 0; d < DIM; ++d) {
+                        double gq = (abit[d] ? 1.0 : -1.0) / h[d];
+#pragma unroll
+                        for (int o = 0; o < DIM; ++o)
+                            if (o != d) gq *= fa[o];
+                        Ga[d] = gq;
+                    }
+                    if (doS) {
+                        double u[3] = {0.0, 0.0, 0.0};
+                        double c0 = 0.0;
+                        if (a.A.mode) {
+                            const double* Ap = coef_pt(a.A, e, q, a.A.iso ? 1 : dpsA);
+                            if (a.A.iso) {
+#pragma unroll
+                                for (int j = 0; j < DIM; ++j) u[j] = Ap[0] * Ga[j];
+                            } else {
+#pragma unroll
+                                for (int j = 0; j < DIM; ++j)
+#pragma unroll
+                                    for (int ii = 0; ii < DIM; ++ii)
+                                        u[j] += Ap[k + NB * (ii + DIM * (m + NB * j))] * Ga[ii];
+                            }
+                        }
+                        if (a.C.mode) {
+                            const double* Cp = coef_pt(a.C, e, q, dpsC);
+#pragma unroll
+                            for (int j = 0; j < DIM; ++j) u[j] += Na * Cp[k + NB * (m + NB * j)];
+                        }
+                        if (a.B.mode) {
+                            const double* Bp = coef_pt(a.B, e, q, dpsB);
+#pragma unroll
+                            for (int ii = 0; ii < DIM; ++ii) c0 += Bp[k + NB * (ii + DIM * m)] * Ga[ii];
+                        }
+                        if (a.D.mode) {
+                            const double* Dp = coef_pt(a.D, e, q, dpsD);
+                            c0 += Dp[k + NB * m] * Na;
+                        }
+#pragma unroll
+                        for (int b = 0; b < NNODE; ++b) {
+                            const int bb[3] = {b & 1, (b >> 1) & 1, (b >> 2) & 1};
+                            double fb[3];
+#pragma unroll
+                            for (int d = 0; d < DIM; ++d) fb[d] = bb[d] ? xi[d] : 1.0 - xi[d];
+                            double Nb = fb[0] * fb[1];
+                            if (DIM == 3) Nb *= fb[2];
+                            double sdot = c0 * Nb;
+#pragma unroll
+                            for (int d = 0; d < DIM; ++d) {
+                                double gq = (bb[d] ? 1.0 : -1.0) / h[d];
+#pragma unroll
+                                for (int o = 0; o < DIM; ++o)
+                                    if (o != d) gq *= fb[o];
+                                sdot += u[d] * gq;
+                            }
+                            const int s = (ex + bb[0]) + 3 * (ey + bb[1]) + (DIM == 3 ? 9 * (ez + bb[2]) : 0);
+                            acc[s] += w * sdot;
+                        }
+                    }
+                    if (doF) {
+                        double sf = 0.0;
+                        if (a.X.mode) {
+                            const double* Xp = coef_pt(a.X, e, q, dpsX);
+#pragma unroll
+                            for (int ii = 0; ii < DIM; ++ii) sf += Xp[k + NB * ii] * Ga[ii];
+                        }
+                        if (a.Y.mode) {
+                            const double* Yp = coef_pt(a.Y, e, q, dpsY);
+                            sf += Yp[k] * Na;
+                        }
+                        f += w * sf;
+                    }
+                }
+            }
+    if (doS) {
+#pragma unroll
+        for (int t = 0; t < NST; ++t)
+            val[(int64_t)(t * NB * NB + k + NB * m) * ld + i] += acc[FIRST + t];
+    }
+    if (doF) rhs[i * NB + k] += f;
+}
+
+void launch_assemble(rg_matrix* m, const rg_pde_coefs& c, int reduced, double* rhs)
+{
+    const Geom& g = m->g;
+    AsmArgs a;
+    auto cv = [](const rg_coef& x) { return CoefDev{x.data, x.data ? x.mode : 0, x.elem_stride, x.isotropic}; };
+    a.A = cv(c.A); a.B = cv(c.B); a.C = cv(c.C); a.D = cv(c.D); a.X = cv(c.X); a.Y = cv(c.Y);
+    const rg_coef* all[6] = {&c.A, &c.B, &c.C, &c.D, &c.X, &c.Y};
+    for (int t = 0; t < 6; ++t) {
+        const rg_coef& x = *all[t];
+        RG_CHECK(x.mode >= RG_COEF_EMPTY && x.mode <= RG_COEF_EXPANDED, RG_VALUE_ERROR,
+                 "assemble: unknown coefficient mode");
+        RG_CHECK(x.mode == RG_COEF_EMPTY || x.data != nullptr, RG_VALUE_ERROR,
+                 "assemble: coefficient data is NULL");
+        RG_CHECK(!(reduced && x.mode == RG_COEF_EXPANDED), RG_VALUE_ERROR,
+                 "assemble: reduced-order coefficients cannot be expanded");
+    }
+    RG_CHECK(!c.A.isotropic || g.bs == 1, RG_VALUE_ERROR,
+             "assemble: isotropic A is defined for a single PDE only");
+    a.n0 = g.nn[0]; a.n1 = g.nn[1]; a.n2 = g.nn[2];
+    a.ne0 = g.ne[0]; a.ne1 = g.ne[1]; a.ne2 = g.ne[2];
+    a.h0 = g.h[0]; a.h1 = g.h[1]; a.h2 = g.h[2];
+    a.reduced = reduced;
+    const dim3 grid((unsigned)((g.N + kBlock - 1) / kBlock), (unsigned)(g.bs * g.bs));
+    dispatch(g, m->symmetric, [&](auto D, auto B, auto S) {
+        hipLaunchKernelGGL((k_assemble<D.value, B.value, S.value>), grid, dim3(kBlock), 0,
+                           m->ctx->stream, a, m->val, g.ld, g.N, rhs);
+    });
+    RG_HIP(hipGetLastError());
+    m->val32_valid = false;
+}
+
+// -------------------------------------------------------------------- SpMV
+// paso SystemMatrix::MatrixVector -> SparseMatrix_MatrixVector_CSR_OFFSET0
+// (paso/src/SparseMatrix_MatrixVector.cpp:191-336): out = alpha*A*in +
+// beta*out, out first scaled by beta (zeroed for beta == 0).
+template <int DIM, int BS, bool SYM, typename VT>
+__global__ __launch_bounds__(kBlock) void k_spmv(const VT* __restrict__ val, int64_t ld, int64_t N,
+                                                 int64_t s1, int64_t s2, int64_t chunk,
+                                                 const double* __restrict__ x, double* __restrict__ y,
+                                                 double alpha, double beta)
+{
+    const RowRange rr = block_rows(N, chunk);
+    for (int64_t i = rr.r0 + threadIdx.x; i < rr.r1; i += kBlock) {
+        double acc[BS];
+#pragma unroll
+        for (int r = 0; r < BS; ++r) acc[r] = 0.0;
+        stencil_row<DIM, BS, SYM>(val, ld, N, s1, s2, i, x, acc);
+#pragma unroll
+        for (int r = 0; r < BS; ++r) {
+            const double yo = (beta == 0.0) ? 0.0 : ((beta == 1.0) ? y[i * BS + r] : beta * y[i * BS + r]);
+            y[i * BS + r] = fma(alpha, acc[r], yo);
+        }
+    }
+}
+
+void launch_spmv(rg_matrix* m, double alpha, const double* x, double beta, double* y)
+{
+    const Geom& g = m->g;
+    const Launch L = chunked_launch(g.N);
+    dispatch(g, m->symmetric, [&](auto D, auto B, auto S) {
+        hipLaunchKernelGGL((k_spmv<D.value, B.value, S.value, double>), dim3(L.grid), dim3(kBlock), 0,
+                           m->ctx->stream, m->val, g.ld, g.N, g.s1, g.s2, L.chunk, x, y, alpha, beta);
+    });
+    RG_HIP(hipGetLastError());
+}
+
+void launch_spmv32(rg_matrix* m, const double* x, double* y)
+{
+    const Geom& g = m->g;
+    const Launch L = chunked_launch(g.N);
+    dispatch(g, m->symmetric, [&](auto D, auto B, auto S) {
+        hipLaunchKernelGGL((k_spmv<D.value, B.value, S.value, float>), dim3(L.grid), dim3(kBlock), 0,
+                           m->ctx->stream, m->val32, g.ld, g.N, g.s1, g.s2, L.chunk, x, y, 1.0, 0.0);
+    });
+    RG_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------- nullify
+// SparseMatrix::nullifyRowsAndCols_CSR(_BLK1) (paso/src/SparseMatrix.h:
+// 494-535): a(i r, j c) := (i r == j c ? mdv : 0) where row_mask[i r] > 0 or
+// col_mask[j c] > 0, over the PASO pattern.  With symmetric storage the two
+// masks must be equal (checked by k_mask_differs).
+template <int DIM, int BS, bool SYM>
+__global__ __launch_bounds__(kBlock) void k_nullify(double* __restrict__ val, int64_t ld, int64_t N,
+                                                    int64_t n0, int64_t n1, int64_t n2,
+                                                    const double* __restrict__ mrow,
+                                                    const double* __restrict__ mcol, double mdv)
+{
+    constexpr int NST = stored_slots<DIM, SYM>();
+    constexpr int FIRST = SYM ? StencilShape<DIM>::CENTER : 0;
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= N) return;
+    const int64_t x = i % n0, y = (i / n0) % n1, z = i / (n0 * n1);
+    bool anyrow = false;
+    double mr[BS];
+#pragma unroll
+    for (int r = 0; r < BS; ++r) {
+        mr[r] = mrow[i * BS + r];
+        anyrow |= mr[r] > 0.0;
+    }
+#pragma unroll
+    for (int t = 0; t < NST; ++t) {
+        const int s = FIRST + t;
+        if (!slot_in_grid<DIM>(s, x, y, z, n0, n1, n2)) continue;
+        int dx, dy, dz;
+        slot_delta<DIM>(s, dx, dy, dz);
+        const int64_t j = i + dx + dy * n0 + dz * n0 * n1;
+        bool anycol = false;
+        double mc[BS];
+#pragma unroll
+        for (int c = 0; c < BS; ++c) {
+            mc[c] = mcol[j * BS + c];
+            anycol |= mc[c] > 0.0;
+        }
+        if (!anyrow && !anycol) continue;
+#pragma unroll
+        for (int c = 0; c < BS; ++c)
+#pragma unroll
+            for (int r = 0; r < BS; ++r)
+                if (mr[r] > 0.0 || mc[c] > 0.0)
+                    val[(int64_t)(t * BS * BS + r + BS * c) * ld + i] = (j == i && r == c) ? mdv : 0.0;
+    }
+}
+
+__global__ void k_mask_differs(const double* a, const double* b, int64_t n, int* flag)
+{
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
+        if ((a[i] > 0.0) != (b[i] > 0.0)) *flag = 1;
+}
+
+void launch_nullify(rg_matrix* m, const double* row_mask, const double* col_mask, double mdv, int* d_flag)
+{
+    const Geom& g = m->g;
+    const int grid = (int)((g.N + kBlock - 1) / kBlock);
+    if (m->symmetric && row_mask != col_mask) {
+        hipLaunchKernelGGL(k_mask_differs, dim3(std::min(grid, 1024)), dim3(kBlock), 0, m->ctx->stream,
+                           row_mask, col_mask, g.N * g.bs, d_flag);
+        RG_HIP(hipGetLastError());
+        int h = 0;
+        RG_HIP(hipMemcpyAsync(&h, d_flag, sizeof(int), hipMemcpyDeviceToHost, m->ctx->stream));
+        RG_HIP(hipStreamSynchronize(m->ctx->stream));
+        RG_CHECK(h == 0, RG_VALUE_ERROR,
+                 "nullifyRowsAndCols: a matrix with symmetric storage needs equal row and column masks");
+    }
+    dispatch(g, m->symmetric, [&](auto D, auto B, auto S) {
+        hipLaunchKernelGGL((k_nullify<D.value, B.value, S.value>), dim3(grid), dim3(kBlock), 0,
+                           m->ctx->stream, m->val, g.ld, g.N, g.nn[0], g.nn[1], g.nn[2], row_mask,
+                           col_mask, mdv);
+    });
+    RG_HIP(hipGetLastError());
+    m->val32_valid = false;
+}
+
+// ------------------------------------------------------------- CSR export
+// The PASO main block in CSR form: row i lists its in-grid neighbours in
+// ascending order (ripley/src/Brick.cpp:3000-3061) with the block values
+// val[k*bs*bs + ir + bs*ic] (paso/src/SystemMatrix.h:404-492).
+template <int DIM, int BS, bool SYM>
+__global__ __launch_bounds__(kBlock) void k_export_csr(const double* __restrict__ val, int64_t ld,
+                                                       int64_t N, int64_t n0, int64_t n1, int64_t n2,
+                                                       const int32_t* __restrict__ ptr,
+                                                       int32_t* __restrict__ index,
+                                                       double* __restrict__ out)
+{
+    constexpr int NS = StencilShape<DIM>::NS;
+    constexpr int C = StencilShape<DIM>::CENTER;
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= N) return;
+    const int64_t x = i % n0, y = (i / n0) % n1, z = i / (n0 * n1);
+    int64_t k = ptr[i];
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+        if (!slot_in_grid<DIM>(s, x, y, z, n0, n1, n2)) continue;
+        int dx, dy, dz;
+        slot_delta<DIM>(s, dx, dy, dz);
+        const int64_t j = i + dx + dy * n0 + dz * n0 * n1;
+        if (index) index[k] = (int32_t)j;
+        if (out) {
+#pragma unroll
+            for (int c = 0; c < BS; ++c)
+#pragma unroll
+                for (int r = 0; r < BS; ++r) {
+                    double v;
+                    if (!SYM || s >= C)
+                        v = val[(int64_t)((SYM ? s - C : s) * BS * BS + r + BS * c) * ld + i];
+                    else
+                        v = val[(int64_t)((C - s) * BS * BS + c + BS * r) * ld + j];
+                    out[k * BS * BS + r + BS * c] = v;
+                }
+        }
+        ++k;
+    }
+}
+
+void launch_export_csr_values(rg_matrix* m, const int32_t* d_ptr, int32_t* d_index, double* d_val)
+{
+    const Geom& g = m->g;
+    const int grid = (int)((g.N + kBlock - 1) / kBlock);
+    dispatch(g, m->symmetric, [&](auto D, auto B, auto S) {
+        hipLaunchKernelGGL((k_export_csr<D.value, B.value, S.value>), dim3(grid), dim3(kBlock), 0,
+                           m->ctx->stream, m->val, g.ld, g.N, g.nn[0], g.nn[1], g.nn[2], d_ptr,
+                           d_index, d_val);
+    });
+    RG_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------- FP32 values
+__global__ void k_to_fp32(const double* __restrict__ a, float* __restrict__ b, int64_t n)
+{
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
+        b[i] = (float)a[i];
+}
+
+void launch_to_fp32(rg_matrix* m)
+{
+    const Geom& g = m->g;
+    const int64_t n = (int64_t)g.nstore * g.bs * g.bs * g.ld;
+    hipLaunchKernelGGL(k_to_fp32, dim3(4096), dim3(kBlock), 0, m->ctx->stream, m->val, m->val32, n);
+    RG_HIP(hipGetLastError());
+}
+
+// ----------------------------------------------------------- host pattern
+void csr_row_ptr(const Geom& g, int32_t* ptr)
+{
+    const int64_t n0 = g.nn[0], n1 = g.nn[1], n2 = g.nn[2];
+    ptr[0] = 0;
+    int64_t acc = 0;
+    for (int64_t i = 0; i < g.N; ++i) {
+        const int64_t x = i % n0, y = (i / n0) % n1, z = i / (n0 * n1);
+        const int cx = 1 + (x > 0) + (x < n0 - 1);
+        const int cy = 1 + (y > 0) + (y < n1 - 1);
+        const int cz = (g.dim == 3) ? 1 + (z > 0) + (z < n2 - 1) : 1;
+        acc += cx * cy * cz;
+        ptr[i + 1] = (int32_t)acc;
+    }
+}
+
+int64_t csr_nnz(const Geom& g)
+{
+    int64_t n = 1;
+    for (int d = 0; d < g.dim; ++d) n *= 3 * g.nn[d] - 2;
+    return n;
+}
+
+}  // namespace rg

@@ -1,0 +1,993 @@
+// Solver kernels and host driver of libripley_hip: PASO's balance, Jacobi
+// preconditioner, residual-smoothed PCG (and the textbook / mixed-precision
+// variants) and the outer refinement loop of paso::Solver.
+//
+// Every Krylov scalar lives in device memory (KrylovState); each reducing
+// kernel publishes per-workgroup partial sums and its last workgroup reduces
+// them in a fixed order and updates the scalars, so a chunk of iterations is
+// a fixed sequence of launches with no host round trip.  The chunk is
+// captured once into a hipGraph and replayed; kernels return at entry once
+// the state says the iteration has stopped, and the host polls the state
+// between chunks.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+
+#include "rg_device.h"
+#include "rg_dispatch.h"
+#include "rg_stencil.h"
+
+namespace rg {
+
+namespace {
+
+double now_s()
+{
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+// ------------------------------------------------------------ Jacobi apply
+// BlockOps_solveAll (paso/src/BlockOps.h:219-245): z_i = inv(A_ii) r_i with
+// the inverse stored structure-of-arrays dinv[(ir + bs*ic)*ld + i].
+template <int BS, typename DT>
+__device__ __forceinline__ void jacobi_node(const DT* __restrict__ dinv, int64_t ld, int64_t i,
+                                            const double* __restrict__ r, double (&z)[BS])
+{
+    if (BS == 1) {
+        z[0] = (double)dinv[i] * r[i];
+    } else {
+        double rv[BS];
+#pragma unroll
+        for (int c = 0; c < BS; ++c) rv[c] = r[i * BS + c];
+#pragma unroll
+        for (int a = 0; a < BS; ++a) {
+            double s = 0.0;
+#pragma unroll
+            for (int c = 0; c < BS; ++c) s = fma((double)dinv[(int64_t)(a + BS * c) * ld + i], rv[c], s);
+            z[a] = s;
+        }
+    }
+}
+
+// z = M^-1 r for node i: the stored sweep result if zbuf is given, else one
+// Jacobi sweep computed on the fly.
+template <int BS, typename DT>
+__device__ __forceinline__ void prec_node(const DT* dinv, int64_t ld, int64_t i, const double* r,
+                                          const double* zbuf, double (&z)[BS])
+{
+    if (zbuf) {
+#pragma unroll
+        for (int c = 0; c < BS; ++c) z[c] = zbuf[i * BS + c];
+    } else {
+        jacobi_node<BS>(dinv, ld, i, r, z);
+    }
+}
+
+// ----------------------------------------------------------------- balance
+// SystemMatrix::balance (paso/src/SystemMatrix.cpp:262-312) with
+// SparseMatrix::maxAbsRow_CSR_OFFSET0 (SparseMatrix.cpp:266-283):
+// d[i r] = sqrt(1/max_(j,c) |a(i r, j c)|), or 1 for a zero row.
+template <int DIM, int BS, bool SYM>
+__global__ __launch_bounds__(kBlock) void k_balance_vector(const double* __restrict__ val, int64_t ld,
+                                                           int64_t N, int64_t s1, int64_t s2,
+                                                           double* __restrict__ d)
+{
+    constexpr int NS = StencilShape<DIM>::NS;
+    constexpr int C = StencilShape<DIM>::CENTER;
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= N) return;
+    double fac[BS];
+#pragma unroll
+    for (int r = 0; r < BS; ++r) fac[r] = 0.0;
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+        int dx, dy, dz;
+        slot_delta<DIM>(s, dx, dy, dz);
+        const int64_t j = i + dx + dy * s1 + dz * s2;
+        if ((uint64_t)j >= (uint64_t)N) continue;
+#pragma unroll
+        for (int c = 0; c < BS; ++c)
+#pragma unroll
+            for (int r = 0; r < BS; ++r) {
+                double a;
+                if (!SYM || s >= C) a = val[(int64_t)((SYM ? s - C : s) * BS * BS + r + BS * c) * ld + i];
+                else a = val[(int64_t)((C - s) * BS * BS + c + BS * r) * ld + j];
+                fac[r] = fmax(fac[r], fabs(a));
+            }
+    }
+#pragma unroll
+    for (int r = 0; r < BS; ++r) d[i * BS + r] = (fac[r] > 0.0) ? sqrt(1.0 / fac[r]) : 1.0;
+}
+
+// SparseMatrix::applyDiagonal_CSR_OFFSET0 (SparseMatrix.cpp:361-384):
+// a(i r, j c) *= d[i r] * d[j c].
+template <int DIM, int BS, bool SYM>
+__global__ __launch_bounds__(kBlock) void k_apply_balance(double* __restrict__ val, int64_t ld, int64_t N,
+                                                          int64_t s1, int64_t s2,
+                                                          const double* __restrict__ d)
+{
+    constexpr int NST = stored_slots<DIM, SYM>();
+    constexpr int FIRST = SYM ? StencilShape<DIM>::CENTER : 0;
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= N) return;
+    double di[BS];
+#pragma unroll
+    for (int r = 0; r < BS; ++r) di[r] = d[i * BS + r];
+#pragma unroll
+    for (int t = 0; t < NST; ++t) {
+        int dx, dy, dz;
+        slot_delta<DIM>(FIRST + t, dx, dy, dz);
+        const int64_t j = i + dx + dy * s1 + dz * s2;
+        if ((uint64_t)j >= (uint64_t)N) continue;
+#pragma unroll
+        for (int c = 0; c < BS; ++c) {
+            const double dj = d[j * BS + c];
+#pragma unroll
+            for (int r = 0; r < BS; ++r) {
+                const int64_t idx = (int64_t)(t * BS * BS + r + BS * c) * ld + i;
+                val[idx] *= di[r] * dj;
+            }
+        }
+    }
+}
+
+// SparseMatrix::invMain (SparseMatrix.cpp:387-435) with the closed-form
+// inverses BlockOps_invM_2/3 (BlockOps.h:100-145); a singular block raises
+// *failed.
+template <int BS, bool SYM, int DIM>
+__global__ __launch_bounds__(kBlock) void k_inv_main(const double* __restrict__ val, int64_t ld, int64_t N,
+                                                     double* __restrict__ dinv, float* __restrict__ dinv32,
+                                                     int* failed)
+{
+    constexpr int T = SYM ? 0 : StencilShape<DIM>::CENTER;   // stored slot of the main diagonal
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= N) return;
+    double A[BS * BS], I[BS * BS];
+#pragma unroll
+    for (int e = 0; e < BS * BS; ++e) A[e] = val[(int64_t)(T * BS * BS + e) * ld + i];
+    bool ok = true;
+    if (BS == 1) {
+        ok = fabs(A[0]) > 0.0;
+        I[0] = 1.0 / A[0];
+    } else if (BS == 2) {
+        const double A11 = A[0], A12 = A[2], A21 = A[1], A22 = A[3];
+        double D = A11 * A22 - A12 * A21;
+        ok = fabs(D) > 0.0;
+        D = 1.0 / D;
+        I[0] = A22 * D; I[1] = -A21 * D; I[2] = -A12 * D; I[3] = A11 * D;
+    } else if (BS == 3) {
+        const double A11 = A[0], A21 = A[1], A31 = A[2], A12 = A[3], A22 = A[4], A32 = A[5], A13 = A[6],
+                     A23 = A[7], A33 = A[8];
+        double D = A11 * (A22 * A33 - A23 * A32) + A12 * (A31 * A23 - A21 * A33) + A13 * (A21 * A32 - A31 * A22);
+        ok = fabs(D) > 0.0;
+        D = 1.0 / D;
+        I[0] = (A22 * A33 - A23 * A32) * D;
+        I[1] = (A31 * A23 - A21 * A33) * D;
+        I[2] = (A21 * A32 - A31 * A22) * D;
+        I[3] = (A13 * A32 - A12 * A33) * D;
+        I[4] = (A11 * A33 - A31 * A13) * D;
+        I[5] = (A12 * A31 - A11 * A32) * D;
+        I[6] = (A12 * A23 - A13 * A22) * D;
+        I[7] = (A13 * A21 - A11 * A23) * D;
+        I[8] = (A11 * A22 - A12 * A21) * D;
+    }
+    if (!ok) {
+        *failed = 1;
+        return;
+    }
+#pragma unroll
+    for (int e = 0; e < BS * BS; ++e) {
+        dinv[(int64_t)e * ld + i] = I[e];
+        dinv32[(int64_t)e * ld + i] = (float)I[e];
+    }
+}
+
+// ------------------------------------------------------- vector utilities
+__global__ void k_mul(double* __restrict__ out, const double* __restrict__ a, const double* __restrict__ b,
+                      int64_t n)
+{
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
+        out[i] = a[i] * b[i];
+}
+
+__global__ void k_axpy(double* __restrict__ y, double a, const double* __restrict__ x, int64_t n)
+{
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
+        y[i] += a * x[i];
+}
+
+__global__ void k_fill(double* __restrict__ y, double v, int64_t n)
+{
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
+        y[i] = v;
+}
+
+// ||v||_2^2 and ||v||_inf in one pass (Solver.cpp:106-130, :201-225).
+__global__ __launch_bounds__(kBlock) void k_norms(const double* __restrict__ v, int64_t n, double* partials,
+                                                  unsigned* counter, double* out)
+{
+    double s[2] = {0.0, 0.0};
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+        const double a = v[i];
+        s[0] += a * a;
+        s[1] = fmax(s[1], fabs(a));
+    }
+    double mx = wave_max(s[1]);
+    __shared__ double smax[kBlock / 64];
+    if ((threadIdx.x & 63) == 0) smax[threadIdx.x >> 6] = mx;
+    __syncthreads();
+    mx = 0.0;
+    for (int w = 0; w < kBlock / 64; ++w) mx = fmax(mx, smax[w]);
+    // the block maxima go to partial row 2, the sums through last_block_sum
+    double v1[1] = {s[0]}, t1[1];
+    const unsigned G = gridDim.x;
+    if (threadIdx.x == 0)
+        __hip_atomic_store(&partials[2 * G + blockIdx.x], mx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (last_block_sum<1>(v1, partials, counter, t1)) {
+        double m2 = 0.0;
+        for (unsigned b = threadIdx.x; b < G; b += kBlock)
+            m2 = fmax(m2, __hip_atomic_load(&partials[2 * G + b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+        m2 = wave_max(m2);
+        __shared__ double sm2[kBlock / 64];
+        if ((threadIdx.x & 63) == 0) sm2[threadIdx.x >> 6] = m2;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double mm = 0.0;
+            for (int w = 0; w < kBlock / 64; ++w) mm = fmax(mm, sm2[w]);
+            out[0] = t1[0];
+            out[1] = mm;
+        }
+    }
+}
+
+// ------------------------------------------------ PASO residual-smoothed PCG
+// Solver_PCG (paso/src/PCG.cpp:62-368).  Iteration k:
+//   (1)-(3) v = M^-1 r, tau = v.r, p = v + (tau/tau_old) p      k_paso_p
+//   (4)-(5) v = A p, delta = p.v, alpha = tau/delta             k_paso_spmv
+//   (6)     r -= alpha v, d = r - rs, s3 = d.d, s4 = d.rs,
+//           x2 += alpha p                                       k_paso_r
+//   (7)     gamma = -s4/s3, rs = (1-gamma) rs + gamma r,
+//           x = (1-gamma) x + gamma x2, s5 = rs.rs              k_paso_x
+//   (8)     stop if sqrt(s5) <= tol, iter > maxit or tau == 0
+// tau of iteration k+1 is reduced in k_paso_x of iteration k (r is final
+// there), so each iteration is four launches.
+struct PcgArgs {
+    double *r, *x, *rs, *p, *v, *x2;
+    const double* z;        // stored M^-1 r (Jacobi sweeps > 1) or nullptr
+    KrylovState* st;
+    double* partials;
+    unsigned* counter;
+    int64_t N, ld, s1, s2, chunk;
+};
+
+template <int BS, typename DT>
+__global__ __launch_bounds__(kBlock) void k_paso_init(PcgArgs a, const DT* __restrict__ dinv)
+{
+    const RowRange rr = block_rows(a.N, a.chunk);
+    double t[1] = {0.0}, tot[1];
+    for (int64_t i = rr.r0 + threadIdx.x; i < rr.r1; i += kBlock) {
+        double z[BS];
+        prec_node<BS>(dinv, a.ld, i, a.r, a.z, z);
+#pragma unroll
+        for (int c = 0; c < BS; ++c) {
+            const int64_t k = i * BS + c;
+            const double rk = a.r[k];
+            a.rs[k] = rk;
+            a.x2[k] = a.x[k];
+            a.p[k] = 0.0;
+            t[0] += z[c] * rk;
+        }
+    }
+    if (last_block_sum<1>(t, a.partials, a.counter, tot) && threadIdx.x == 0) {
+        KrylovState* s = a.st;
+        s->tau = tot[0];
+        s->beta = 0.0;
+        s->num_iter = 1;
+        s->done = 0;
+        s->status = KS_RUNNING;
+    }
+}
+
+template <int BS, typename DT>
+__global__ __launch_bounds__(kBlock) void k_paso_p(PcgArgs a, const DT* __restrict__ dinv)
+{
+    if (a.st->done) return;
+    const double beta = a.st->beta;
+    const RowRange rr = block_rows(a.N, a.chunk);
+    for (int64_t i = rr.r0 + threadIdx.x; i < rr.r1; i += kBlock) {
+        double z[BS];
+        prec_node<BS>(dinv, a.ld, i, a.r, a.z, z);
+#pragma unroll
+        for (int c = 0; c < BS; ++c) {
+            const int64_t k = i * BS + c;
+            a.p[k] = fma(beta, a.p[k], z[c]);
+        }
+    }
+}
+
+template <int DIM, int BS, bool SYM, typename VT>
+__global__ __launch_bounds__(kBlock) void k_paso_spmv(PcgArgs a, const VT* __restrict__ val)
+{
+    if (a.st->done) return;
+    const RowRange rr = block_rows(a.N, a.chunk);
+    double dl[1] = {0.0}, tot[1];
+    for (int64_t i = rr.r0 + threadIdx.x; i < rr.r1; i += kBlock) {
+        double acc[BS];
+#pragma unroll
+        for (int r = 0; r < BS; ++r) acc[r] = 0.0;
+        stencil_row<DIM, BS, SYM>(val, a.ld, a.N, a.s1, a.s2, i, a.p, acc);
+#pragma unroll
+        for (int r = 0; r < BS; ++r) {
+            a.v[i * BS + r] = acc[r];
+            dl[0] += acc[r] * a.p[i * BS + r];
+        }
+    }
+    if (last_block_sum<1>(dl, a.partials, a.counter, tot) && threadIdx.x == 0) {
+        KrylovState* s = a.st;
+        s->delta = tot[0];
+        s->alpha = s->tau / tot[0];
+        if (fabs(tot[0]) <= 0.0) {   // PCG.cpp:262 breakFlag
+            s->done = 1;
+            s->status = KS_BREAKDOWN;
+        }
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_paso_r(PcgArgs a)
+{
+    if (a.st->done) return;
+    const double alpha = a.st->alpha;
+    // a.N counts scalar entries here (nodes * block size); the update is
+    // independent of the block structure.
+    const RowRange rr = block_rows(a.N, a.chunk);
+    double s[2] = {0.0, 0.0}, tot[2];
+    for (int64_t i = rr.r0 + threadIdx.x; i < rr.r1; i += kBlock) {
+        const double ri = fma(-alpha, a.v[i], a.r[i]);
+        a.r[i] = ri;
+        const double d = ri - a.rs[i];
+        s[0] += d * d;
+        s[1] += d * a.rs[i];
+        a.x2[i] = fma(alpha, a.p[i], a.x2[i]);
+    }
+    if (last_block_sum<2>(s, a.partials, a.counter, tot) && threadIdx.x == 0) {
+        KrylovState* st = a.st;
+        st->gamma1 = (fabs(tot[0]) <= 0.0) ? 0.0 : -tot[1] / tot[0];
+    }
+}
+
+template <int BS, typename DT>
+__global__ __launch_bounds__(kBlock) void k_paso_x(PcgArgs a, const DT* __restrict__ dinv)
+{
+    if (a.st->done) return;
+    const double g1 = a.st->gamma1, g2 = 1.0 - g1;
+    const RowRange rr = block_rows(a.N, a.chunk);
+    double s[2] = {0.0, 0.0}, tot[2];
+    for (int64_t i = rr.r0 + threadIdx.x; i < rr.r1; i += kBlock) {
+        double z[BS];
+        prec_node<BS>(dinv, a.ld, i, a.r, a.z, z);
+#pragma unroll
+        for (int c = 0; c < BS; ++c) {
+            const int64_t k = i * BS + c;
+            const double rk = a.r[k];
+            const double rsk = g2 * a.rs[k] + g1 * rk;
+            a.rs[k] = rsk;
+            a.x[k] = g2 * a.x[k] + g1 * a.x2[k];
+            s[0] += rsk * rsk;
+            s[1] += z[c] * rk;
+        }
+    }
+    if (last_block_sum<2>(s, a.partials, a.counter, tot) && threadIdx.x == 0) {
+        KrylovState* st = a.st;
+        const double norm = sqrt(tot[0]);
+        st->norm = norm;
+        const bool conv = norm <= st->tol;
+        const bool maxi = st->num_iter > st->maxit;
+        const bool brk = fabs(st->tau) <= 0.0;
+        if (conv || maxi || brk) {
+            st->done = 1;
+            st->status = maxi ? KS_MAXITER : (brk ? KS_BREAKDOWN : KS_CONVERGED);
+        } else {
+            st->beta = tot[1] / st->tau;
+            st->tau = tot[1];
+            st->num_iter += 1;
+        }
+    }
+}
+
+// ------------------------------------------------------ textbook Jacobi-PCG
+//   q = A p, delta = p.q, alpha = rho/delta                  k_cg_spmv
+//   x += alpha p, r -= alpha q, rho' = r.M^-1 r, ||r||       k_cg_update
+//   p = M^-1 r + (rho'/rho) p                                k_cg_p
+// Stops when ||r||_2 <= tol (or iter >= maxit, or a zero scalar).
+template <int BS, typename DT>
+__global__ __launch_bounds__(kBlock) void k_cg_init(PcgArgs a, const DT* __restrict__ dinv)
+{
+    const RowRange rr = block_rows(a.N, a.chunk);
+    double s[2] = {0.0, 0.0}, tot[2];
+    for (int64_t i = rr.r0 + threadIdx.x; i < rr.r1; i += kBlock) {
+        double z[BS];
+        prec_node<BS>(dinv, a.ld, i, a.r, a.z, z);
+#pragma unroll
+        for (int c = 0; c < BS; ++c) {
+            const int64_t k = i * BS + c;
+            const double rk = a.r[k];
+            a.p[k] = z[c];
+            s[0] += rk * z[c];
+            s[1] += rk * rk;
+        }
+    }
+    if (last_block_sum<2>(s, a.partials, a.counter, tot) && threadIdx.x == 0) {
+        KrylovState* st = a.st;
+        st->rho = tot[0];
+        st->norm = sqrt(tot[1]);
+        st->num_iter = 0;
+        st->done = 0;
+        st->status = KS_RUNNING;
+        if (st->norm <= st->tol) { st->done = 1; st->status = KS_CONVERGED; }
+        else if (st->maxit <= 0) { st->done = 1; st->status = KS_MAXITER; }
+        else if (fabs(tot[0]) <= 0.0) { st->done = 1; st->status = KS_BREAKDOWN; }
+    }
+}
+
+template <int DIM, int BS, bool SYM, typename VT>
+__global__ __launch_bounds__(kBlock) void k_cg_spmv(PcgArgs a, const VT* __restrict__ val)
+{
+    if (a.st->done) return;
+    const RowRange rr = block_rows(a.N, a.chunk);
+    double dl[1] = {0.0}, tot[1];
+    for (int64_t i = rr.r0 + threadIdx.x; i < rr.r1; i += kBlock) {
+        double acc[BS];
+#pragma unroll
+        for (int r = 0; r < BS; ++r) acc[r] = 0.0;
+        stencil_row<DIM, BS, SYM>(val, a.ld, a.N, a.s1, a.s2, i, a.p, acc);
+#pragma unroll
+        for (int r = 0; r < BS; ++r) {
+            a.v[i * BS + r] = acc[r];
+            dl[0] += acc[r] * a.p[i * BS + r];
+        }
+    }
+    if (last_block_sum<1>(dl, a.partials, a.counter, tot) && threadIdx.x == 0) {
+        KrylovState* st = a.st;
+        st->delta = tot[0];
+        if (fabs(tot[0]) <= 0.0) { st->done = 1; st->status = KS_BREAKDOWN; }
+        else st->alpha = st->rho / tot[0];
+    }
+}
+
+template <int BS, typename DT>
+__global__ __launch_bounds__(kBlock) void k_cg_update(PcgArgs a, const DT* __restrict__ dinv)
+{
+    if (a.st->done) return;
+    const double alpha = a.st->alpha;
+    const RowRange rr = block_rows(a.N, a.chunk);
+    double s[2] = {0.0, 0.0}, tot[2];
+    for (int64_t i = rr.r0 + threadIdx.x; i < rr.r1; i += kBlock) {
+#pragma unroll
+        for (int c = 0; c < BS; ++c) {
+            const int64_t k = i * BS + c;
+            a.x[k] = fma(alpha, a.p[k], a.x[k]);
+            a.r[k] = fma(-alpha, a.v[k], a.r[k]);
+        }
+        if (a.z == nullptr) {
+            double z[BS];
+            jacobi_node<BS>(dinv, a.ld, i, a.r, z);
+#pragma unroll
+            for (int c = 0; c < BS; ++c) {
+                const double rk = a.r[i * BS + c];
+                s[0] += rk * z[c];
+                s[1] += rk * rk;
+            }
+        } else {
+#pragma unroll
+            for (int c = 0; c < BS; ++c) {
+                const double rk = a.r[i * BS + c];
+                s[1] += rk * rk;
+            }
+        }
+    }
+    if (last_block_sum<2>(s, a.partials, a.counter, tot) && threadIdx.x == 0) {
+        KrylovState* st = a.st;
+        st->num_iter += 1;
+        st->norm = sqrt(tot[1]);
+        st->tau_new = tot[0];
+        if (st->norm <= st->tol) { st->done = 1; st->status = KS_CONVERGED; }
+        else if (st->num_iter >= st->maxit) { st->done = 1; st->status = KS_MAXITER; }
+        else if (a.z == nullptr) {
+            if (fabs(tot[0]) <= 0.0) { st->done = 1; st->status = KS_BREAKDOWN; }
+            else { st->beta = tot[0] / st->rho; st->rho = tot[0]; }
+        }
+    }
+}
+
+// r.z for the sweeps > 1 path of the textbook PCG (z stored).
+__global__ __launch_bounds__(kBlock) void k_cg_rz(PcgArgs a, int64_t n)
+{
+    if (a.st->done) return;
+    double s[1] = {0.0}, tot[1];
+    for (int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x; k < n; k += (int64_t)gridDim.x * kBlock)
+        s[0] += a.r[k] * a.z[k];
+    if (last_block_sum<1>(s, a.partials, a.counter, tot) && threadIdx.x == 0) {
+        KrylovState* st = a.st;
+        if (fabs(tot[0]) <= 0.0) { st->done = 1; st->status = KS_BREAKDOWN; }
+        else { st->beta = tot[0] / st->rho; st->rho = tot[0]; }
+    }
+}
+
+template <int BS, typename DT>
+__global__ __launch_bounds__(kBlock) void k_cg_p(PcgArgs a, const DT* __restrict__ dinv)
+{
+    if (a.st->done) return;
+    const double beta = a.st->beta;
+    const RowRange rr = block_rows(a.N, a.chunk);
+    for (int64_t i = rr.r0 + threadIdx.x; i < rr.r1; i += kBlock) {
+        double z[BS];
+        prec_node<BS>(dinv, a.ld, i, a.r, a.z, z);
+#pragma unroll
+        for (int c = 0; c < BS; ++c) {
+            const int64_t k = i * BS + c;
+            a.p[k] = fma(beta, a.p[k], z[c]);
+        }
+    }
+}
+
+// Jacobi sweeps 2..n (Smoother.cpp:100-127): z += inv(D) (r - A z), into
+// zout from zin (ping-pong buffers).
+template <int DIM, int BS, bool SYM>
+__global__ __launch_bounds__(kBlock) void k_jacobi_sweep(const double* __restrict__ val, const double* __restrict__ dinv,
+                                                         int64_t ld, int64_t N, int64_t s1, int64_t s2,
+                                                         int64_t chunk, const double* __restrict__ r,
+                                                         const double* __restrict__ zin, double* __restrict__ zout,
+                                                         const KrylovState* st)
+{
+    if (st && st->done) return;
+    const RowRange rr = block_rows(N, chunk);
+    for (int64_t i = rr.r0 + threadIdx.x; i < rr.r1; i += kBlock) {
+        double acc[BS];
+#pragma unroll
+        for (int q = 0; q < BS; ++q) acc[q] = 0.0;
+        stencil_row<DIM, BS, SYM>(val, ld, N, s1, s2, i, zin, acc);
+        double res[BS];
+#pragma unroll
+        for (int q = 0; q < BS; ++q) res[q] = r[i * BS + q] - acc[q];
+        double z[BS];
+        if (BS == 1) {
+            z[0] = dinv[i] * res[0];
+        } else {
+#pragma unroll
+            for (int q = 0; q < BS; ++q) {
+                double s = 0.0;
+#pragma unroll
+                for (int c = 0; c < BS; ++c) s += dinv[(int64_t)(q + BS * c) * ld + i] * res[c];
+                z[q] = s;
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < BS; ++q) zout[i * BS + q] = zin[i * BS + q] + z[q];
+    }
+}
+
+template <int BS>
+__global__ __launch_bounds__(kBlock) void k_jacobi_first(const double* __restrict__ dinv, int64_t ld, int64_t N,
+                                                         int64_t chunk, const double* __restrict__ r,
+                                                         double* __restrict__ z, const KrylovState* st)
+{
+    if (st && st->done) return;
+    const RowRange rr = block_rows(N, chunk);
+    for (int64_t i = rr.r0 + threadIdx.x; i < rr.r1; i += kBlock) {
+        double zz[BS];
+        jacobi_node<BS>(dinv, ld, i, r, zz);
+#pragma unroll
+        for (int q = 0; q < BS; ++q) z[i * BS + q] = zz[q];
+    }
+}
+
+}  // namespace
+
+// ====================================================================== host
+
+struct SolverImpl {
+    rg_matrix* m;
+    rg_solver_work* w;
+    hipStream_t st;
+    Launch L;
+
+    explicit SolverImpl(rg_matrix* mm) : m(mm), w(mm->work), st(mm->ctx->stream), L(chunked_launch(mm->g.N)) {}
+
+    PcgArgs args(const double* zbuf)
+    {
+        PcgArgs a;
+        a.r = w->r; a.x = w->xw; a.rs = w->rs; a.p = w->p; a.v = w->v; a.x2 = w->x2;
+        a.z = zbuf;
+        a.st = w->st;
+        a.partials = m->ctx->partials;
+        a.counter = m->ctx->counters;
+        a.N = m->g.N; a.ld = m->g.ld; a.s1 = m->g.s1; a.s2 = m->g.s2; a.chunk = L.chunk;
+        return a;
+    }
+
+    // z = M^-1 r with `sweeps` Jacobi sweeps, into w->z (returns the buffer).
+    double* jacobi(const double* r, int sweeps, const KrylovState* guard)
+    {
+        const Geom& g = m->g;
+        double* za = w->z;
+        double* zb = w->z2;
+        dispatch_bs(g.bs, [&](auto B) {
+            hipLaunchKernelGGL((k_jacobi_first<B.value>), dim3(L.grid), dim3(kBlock), 0, st, m->dinv, g.ld, g.N,
+                               L.chunk, r, za, guard);
+        });
+        for (int s = 1; s < sweeps; ++s) {
+            dispatch(g, m->symmetric, [&](auto D, auto B, auto S) {
+                hipLaunchKernelGGL((k_jacobi_sweep<D.value, B.value, S.value>), dim3(L.grid), dim3(kBlock), 0, st,
+                                   m->val, m->dinv, g.ld, g.N, g.s1, g.s2, L.chunk, r, za, zb, guard);
+            });
+            std::swap(za, zb);
+        }
+        RG_HIP(hipGetLastError());
+        return za;
+    }
+
+    // One iteration's launches of the PASO residual-smoothed PCG.
+    void paso_iteration(int sweeps)
+    {
+        const Geom& g = m->g;
+        double* zb = (sweeps > 1) ? w->z : nullptr;
+        PcgArgs a = args(zb);
+        dispatch(g, m->symmetric, [&](auto D, auto B, auto S) {
+            hipLaunchKernelGGL((k_paso_p<B.value, double>), dim3(L.grid), dim3(kBlock), 0, st, a, m->dinv);
+            hipLaunchKernelGGL((k_paso_spmv<D.value, B.value, S.value, double>), dim3(L.grid), dim3(kBlock), 0, st,
+                               a, m->val);
+        });
+        PcgArgs ar = a;
+        ar.N = g.N * g.bs;
+        const Launch Ls = chunked_launch(ar.N);
+        ar.chunk = Ls.chunk;
+        hipLaunchKernelGGL(k_paso_r, dim3(Ls.grid), dim3(kBlock), 0, st, ar);
+        if (sweeps > 1) {
+            double* z = jacobi(w->r, sweeps, w->st);
+            if (z != w->z) RG_HIP(hipMemcpyAsync(w->z, z, sizeof(double) * w->n, hipMemcpyDeviceToDevice, st));
+        }
+        dispatch_bs(g.bs, [&](auto B) {
+            hipLaunchKernelGGL((k_paso_x<B.value, double>), dim3(L.grid), dim3(kBlock), 0, st, a, m->dinv);
+        });
+        RG_HIP(hipGetLastError());
+    }
+
+    // One iteration's launches of the textbook PCG (VT: value precision).
+    template <typename VT, typename DT>
+    void cg_iteration(const VT* val, const DT* dinv, int sweeps)
+    {
+        const Geom& g = m->g;
+        double* zb = (sweeps > 1) ? w->z : nullptr;
+        PcgArgs a = args(zb);
+        dispatch(g, m->symmetric, [&](auto D, auto B, auto S) {
+            hipLaunchKernelGGL((k_cg_spmv<D.value, B.value, S.value, VT>), dim3(L.grid), dim3(kBlock), 0, st, a, val);
+            hipLaunchKernelGGL((k_cg_update<B.value, DT>), dim3(L.grid), dim3(kBlock), 0, st, a, dinv);
+        });
+        if (sweeps > 1) {
+            double* z = jacobi(w->r, sweeps, w->st);
+            if (z != w->z) RG_HIP(hipMemcpyAsync(w->z, z, sizeof(double) * w->n, hipMemcpyDeviceToDevice, st));
+            hipLaunchKernelGGL(k_cg_rz, dim3(L.grid), dim3(kBlock), 0, st, a, w->n);
+        }
+        dispatch_bs(g.bs, [&](auto B) {
+            hipLaunchKernelGGL((k_cg_p<B.value, DT>), dim3(L.grid), dim3(kBlock), 0, st, a, dinv);
+        });
+        RG_HIP(hipGetLastError());
+    }
+
+    void iteration(int method, int sweeps)
+    {
+        if (method == RG_METHOD_PCG) paso_iteration(sweeps);
+        else if (method == RG_METHOD_PCG_PLAIN) cg_iteration<double, double>(m->val, m->dinv, sweeps);
+        else cg_iteration<float, float>(m->val32, m->dinv32, sweeps);
+    }
+
+    void init(int method, int sweeps)
+    {
+        const Geom& g = m->g;
+        double* zb = nullptr;
+        if (sweeps > 1) {
+            double* z = jacobi(w->r, sweeps, nullptr);
+            if (z != w->z) RG_HIP(hipMemcpyAsync(w->z, z, sizeof(double) * w->n, hipMemcpyDeviceToDevice, st));
+            zb = w->z;
+        }
+        PcgArgs a = args(zb);
+        dispatch_bs(g.bs, [&](auto B) {
+            if (method == RG_METHOD_PCG)
+                hipLaunchKernelGGL((k_paso_init<B.value, double>), dim3(L.grid), dim3(kBlock), 0, st, a, m->dinv);
+            else if (method == RG_METHOD_PCG_PLAIN)
+                hipLaunchKernelGGL((k_cg_init<B.value, double>), dim3(L.grid), dim3(kBlock), 0, st, a, m->dinv);
+            else
+                hipLaunchKernelGGL((k_cg_init<B.value, float>), dim3(L.grid), dim3(kBlock), 0, st, a, m->dinv32);
+        });
+        RG_HIP(hipGetLastError());
+    }
+
+    // A chunk of `iters` iterations, replayed from a cached hipGraph.
+    void chunk(int method, int sweeps, int iters, bool use_graph)
+    {
+        if (!use_graph) {
+            for (int k = 0; k < iters; ++k) iteration(method, sweeps);
+            return;
+        }
+        if (!w->graph || w->graph_method != method || w->graph_sweeps != sweeps || w->graph_iters != iters) {
+            if (w->graph) {
+                RG_HIP(hipGraphExecDestroy(w->graph));
+                w->graph = nullptr;
+            }
+            if (!w->capture) RG_HIP(hipStreamCreateWithFlags(&w->capture, hipStreamNonBlocking));
+            hipStream_t keep = st;
+            st = w->capture;
+            hipGraph_t graph;
+            RG_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            try {
+                for (int k = 0; k < iters; ++k) iteration(method, sweeps);
+            } catch (...) {
+                hipStreamEndCapture(st, &graph);
+                st = keep;
+                throw;
+            }
+            RG_HIP(hipStreamEndCapture(st, &graph));
+            st = keep;
+            RG_HIP(hipGraphInstantiate(&w->graph, graph, nullptr, nullptr, 0));
+            RG_HIP(hipGraphDestroy(graph));
+            w->graph_method = method;
+            w->graph_sweeps = sweeps;
+            w->graph_iters = iters;
+        }
+        RG_HIP(hipGraphLaunch(w->graph, st));
+    }
+
+    // Runs the Krylov loop from w->r, w->xw until the state reports done.
+    // Returns (status, iterations, final norm).
+    void krylov(int method, int sweeps, int maxit, double tol, int check, bool use_graph, int* status,
+                int* iters, double* norm)
+    {
+        KrylovState h;
+        std::memset(&h, 0, sizeof(h));
+        h.maxit = maxit;
+        h.tol = tol;
+        KrylovState* pin = reinterpret_cast<KrylovState*>(m->ctx->pinned);
+        *pin = h;
+        RG_HIP(hipMemcpyAsync(w->st, pin, sizeof(KrylovState), hipMemcpyHostToDevice, st));
+        // the copy above reads pinned memory asynchronously: wait before reusing it
+        RG_HIP(hipStreamSynchronize(st));
+        init(method, sweeps);
+        KrylovState* slots = pin + 1;   // two polling slots
+        hipEvent_t evs[2];
+        RG_HIP(hipEventCreateWithFlags(&evs[0], hipEventDisableTiming));
+        RG_HIP(hipEventCreateWithFlags(&evs[1], hipEventDisableTiming));
+        int launched = 0;
+        bool done = false;
+        try {
+            // keep two chunks in flight; poll the older one
+            while (!done) {
+                chunk(method, sweeps, check, use_graph);
+                RG_HIP(hipMemcpyAsync(&slots[launched % 2], w->st, sizeof(KrylovState), hipMemcpyDeviceToHost, st));
+                RG_HIP(hipEventRecord(evs[launched % 2], st));
+                ++launched;
+                if (launched >= 2) {
+                    RG_HIP(hipEventSynchronize(evs[(launched - 2) % 2]));
+                    done = slots[(launched - 2) % 2].done != 0;
+                }
+            }
+        } catch (...) {
+            hipEventDestroy(evs[0]);
+            hipEventDestroy(evs[1]);
+            throw;
+        }
+        RG_HIP(hipStreamSynchronize(st));
+        RG_HIP(hipEventDestroy(evs[0]));
+        RG_HIP(hipEventDestroy(evs[1]));
+        RG_HIP(hipMemcpy(pin, w->st, sizeof(KrylovState), hipMemcpyDeviceToHost));
+        *status = pin->status;
+        *iters = pin->num_iter;
+        *norm = pin->norm;
+    }
+
+    void norms(const double* v, double* n2, double* nmax)
+    {
+        const int grid = 1024;
+        hipLaunchKernelGGL(k_norms, dim3(grid), dim3(kBlock), 0, st, v, w->n, m->ctx->partials,
+                           m->ctx->counters + 1, w->norms);
+        RG_HIP(hipGetLastError());
+        RG_HIP(hipMemcpyAsync(m->ctx->pinned, w->norms, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
+        RG_HIP(hipStreamSynchronize(st));
+        *n2 = std::sqrt(m->ctx->pinned[0]);
+        *nmax = m->ctx->pinned[1];
+    }
+};
+
+void ensure_work(rg_matrix* m)
+{
+    if (m->work) return;
+    rg_solver_work* w = new rg_solver_work();
+    w->n = m->g.N * m->g.bs;
+    const size_t bytes = sizeof(double) * (size_t)w->n;
+    try {
+        RG_HIP(hipMalloc(&w->r, bytes));
+        RG_HIP(hipMalloc(&w->xw, bytes));
+        RG_HIP(hipMalloc(&w->rs, bytes));
+        RG_HIP(hipMalloc(&w->p, bytes));
+        RG_HIP(hipMalloc(&w->v, bytes));
+        RG_HIP(hipMalloc(&w->x2, bytes));
+        RG_HIP(hipMalloc(&w->z, bytes));
+        RG_HIP(hipMalloc(&w->z2, bytes));
+        RG_HIP(hipMalloc(&w->norms, 4 * sizeof(double)));
+        RG_HIP(hipMalloc(&w->st, sizeof(KrylovState)));
+        RG_HIP(hipMalloc(&w->flag, sizeof(int)));
+    } catch (...) {
+        m->work = w;
+        throw;
+    }
+    m->work = w;
+}
+
+void free_work(rg_solver_work* w)
+{
+    if (!w) return;
+    if (w->graph) hipGraphExecDestroy(w->graph);
+    if (w->capture) hipStreamDestroy(w->capture);
+    double* bufs[] = {w->r, w->xw, w->rs, w->p, w->v, w->x2, w->z, w->z2, w->norms};
+    for (double* b : bufs)
+        if (b) hipFree(b);
+    if (w->st) hipFree(w->st);
+    if (w->flag) hipFree(w->flag);
+    delete w;
+}
+
+// SystemMatrix::balance (once per set of values) and the Jacobi inverse of
+// the balanced matrix (Preconditioner_alloc -> invMain; kept until the values
+// are reset without preserving solver data).
+void prepare_solve(rg_matrix* m)
+{
+    const Geom& g = m->g;
+    hipStream_t st = m->ctx->stream;
+    const int grid = (int)((g.N + kBlock - 1) / kBlock);
+    if (!m->is_balanced) {
+        dispatch(g, m->symmetric, [&](auto D, auto B, auto S) {
+            hipLaunchKernelGGL((k_balance_vector<D.value, B.value, S.value>), dim3(grid), dim3(kBlock), 0, st,
+                               m->val, g.ld, g.N, g.s1, g.s2, m->balance);
+            hipLaunchKernelGGL((k_apply_balance<D.value, B.value, S.value>), dim3(grid), dim3(kBlock), 0, st,
+                               m->val, g.ld, g.N, g.s1, g.s2, m->balance);
+        });
+        RG_HIP(hipGetLastError());
+        m->is_balanced = true;
+        m->val32_valid = false;
+    }
+    if (!m->have_prec) {
+        RG_CHECK(g.bs <= 3, RG_VALUE_ERROR,
+                 "SparseMatrix::invMain: block size > 3 needs LAPACK (not supported)");
+        RG_HIP(hipMemsetAsync(m->work->flag, 0, sizeof(int), st));
+        dispatch(g, m->symmetric, [&](auto D, auto B, auto S) {
+            hipLaunchKernelGGL((k_inv_main<B.value, S.value, D.value>), dim3(grid), dim3(kBlock), 0, st, m->val,
+                               g.ld, g.N, m->dinv, m->dinv32, m->work->flag);
+        });
+        RG_HIP(hipGetLastError());
+        int failed = 0;
+        RG_HIP(hipMemcpyAsync(&failed, m->work->flag, sizeof(int), hipMemcpyDeviceToHost, st));
+        RG_HIP(hipStreamSynchronize(st));
+        RG_CHECK(!failed, RG_VALUE_ERROR, "SparseMatrix::invMain: non-regular main diagonal block.");
+        m->have_prec = true;
+    }
+}
+
+// paso::Solver (paso/src/Solver.cpp:46-331) on device vectors.
+int solve(rg_matrix* m, double* x, const double* b, const rg_options& opt, rg_diagnostics& dg)
+{
+    const double EPS = 2.220446049250313e-16;
+    const double t0 = now_s();
+    dg.num_iter = 0; dg.converged = 0; dg.status = 0; dg.outer_steps = 0;
+    dg.residual_norm = -1; dg.set_up_time = -1; dg.net_time = -1; dg.time = -1;
+    const double tolerance = opt.tolerance;
+    RG_CHECK(tolerance >= 100. * EPS, RG_VALUE_ERROR, "Solver: Tolerance is too small.");
+    RG_CHECK(tolerance <= 1., RG_VALUE_ERROR, "Solver: Tolerance must be less than one.");
+    RG_CHECK(opt.method >= RG_METHOD_PCG && opt.method <= RG_METHOD_PCG_MIXED, RG_VALUE_ERROR,
+             "Solver: unknown solver method.");
+    ensure_work(m);
+    rg_solver_work* w = m->work;
+    hipStream_t st = m->ctx->stream;
+    SolverImpl S(m);
+    const int64_t n = w->n;
+    const int vgrid = 2048;
+    prepare_solve(m);
+    if (opt.method == RG_METHOD_PCG_MIXED && !m->val32_valid) {
+        if (!m->val32) {
+            const size_t cnt = (size_t)m->g.nstore * m->g.bs * m->g.bs * m->g.ld;
+            RG_HIP(hipMalloc(&m->val32, cnt * sizeof(float)));
+        }
+        launch_to_fp32(m);
+        m->val32_valid = true;
+    }
+    // r = D b, norms of D b (Solver.cpp:104-130)
+    hipLaunchKernelGGL(k_mul, dim3(vgrid), dim3(kBlock), 0, st, w->r, b, m->balance, n);
+    double norm2_of_b, norm_max_of_b;
+    S.norms(w->r, &norm2_of_b, &norm_max_of_b);
+    RG_CHECK(!std::isnan(norm2_of_b) && !std::isnan(norm_max_of_b), RG_VALUE_ERROR,
+             "Solver: Matrix or right hand side contains undefined values.");
+    int status = RG_OK;
+    const int check = opt.check_interval > 0 ? opt.check_interval : 32;
+    const bool use_graph = true;
+    if (norm2_of_b <= 0.) {
+        hipLaunchKernelGGL(k_fill, dim3(vgrid), dim3(kBlock), 0, st, x, 0.0, n);
+        RG_HIP(hipGetLastError());
+        if (opt.verbose) printf("right hand side is identical to zero.\n");
+    } else {
+        dg.set_up_time = now_s() - t0;
+        // initial guess x = M^-1 D b (Solver.cpp:181)
+        {
+            double* z = S.jacobi(w->r, std::max(1, opt.sweeps), nullptr);
+            RG_HIP(hipMemcpyAsync(w->xw, z, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+        }
+        int totIter = 1;
+        bool finalize = false;
+        double last2 = norm2_of_b, lastmax = norm_max_of_b;
+        const double tn = now_s();
+        while (!finalize) {
+            const int cntIter = opt.iter_max - totIter;
+            finalize = true;
+            if (totIter > 1) hipLaunchKernelGGL(k_mul, dim3(vgrid), dim3(kBlock), 0, st, w->r, b, m->balance, n);
+            launch_spmv(m, -1.0, w->xw, 1.0, w->r);
+            double norm2, normmax;
+            S.norms(w->r, &norm2, &normmax);
+            dg.residual_norm = norm2;
+            dg.outer_steps++;
+            if (opt.verbose)
+                printf("Solver: Step %d: l2/lmax-norm of residual is %g/%g\n", totIter, norm2, normmax);
+            if (totIter > 1 && norm2 >= last2 && normmax >= lastmax) {
+                throw Error(RG_DIVERGENCE, "Solver: No improvement during iteration. Iterative solver gives up.");
+            }
+            if (norm2 > tolerance * norm2_of_b || normmax > tolerance * norm_max_of_b) {
+                double tol = tolerance * std::min(norm2_of_b, 0.1 * norm2 / normmax * norm_max_of_b);
+                if (opt.method == RG_METHOD_PCG_MIXED) {
+                    const double itol = opt.inner_tolerance > 0 ? opt.inner_tolerance : 1e-4;
+                    tol = std::max(tol, itol * norm2);
+                }
+                if (opt.verbose) printf("Solver: new tolerance = %g\n", tol);
+                last2 = norm2;
+                lastmax = normmax;
+                int ks = 0, it = 0;
+                double kn = 0;
+                if (opt.method == RG_METHOD_PCG_MIXED) {
+                    // inner correction solve A32 d = r from d = 0; then x += d
+                    RG_HIP(hipMemcpyAsync(w->x2, w->xw, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+                    hipLaunchKernelGGL(k_fill, dim3(vgrid), dim3(kBlock), 0, st, w->xw, 0.0, n);
+                    S.krylov(opt.method, std::max(1, opt.sweeps), cntIter, tol, check, use_graph, &ks, &it, &kn);
+                    hipLaunchKernelGGL(k_axpy, dim3(vgrid), dim3(kBlock), 0, st, w->xw, 1.0, w->x2, n);
+                } else {
+                    S.krylov(opt.method, std::max(1, opt.sweeps), cntIter, tol, check, use_graph, &ks, &it, &kn);
+                }
+                RG_HIP(hipGetLastError());
+                totIter += it;
+                if (ks == KS_CONVERGED) {
+                    finalize = false;
+                } else if (ks == KS_MAXITER) {
+                    status = RG_MAX_ITER_REACHED;
+                    break;
+                } else if (ks == KS_BREAKDOWN) {
+                    if (it > 1) {
+                        finalize = false;
+                    } else {
+                        status = RG_BREAKDOWN;
+                        break;
+                    }
+                }
+            } else {
+                dg.converged = 1;
+            }
+        }
+        dg.net_time = now_s() - tn;
+        dg.num_iter = totIter;
+        hipLaunchKernelGGL(k_mul, dim3(vgrid), dim3(kBlock), 0, st, x, w->xw, m->balance, n);
+        RG_HIP(hipGetLastError());
+    }
+    RG_HIP(hipStreamSynchronize(st));
+    dg.status = status;
+    dg.time = now_s() - t0;
+    return status;
+}
+
+}  // namespace rg

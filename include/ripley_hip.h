@@ -1,0 +1,215 @@
+/*
+ * ripley_hip -- C ABI of the MI355X (gfx950) implementation of escript's
+ * structured-grid FEM hot path: ripley Rectangle/Brick Q1 assembly into a
+ * PASO-compatible system matrix, Dirichlet constraints, SpMV and PASO's
+ * Jacobi-preconditioned CG driver, at one MPI rank.
+ *
+ * Every entry point names the reference interface it replaces.  All entry
+ * points return an int status: 0 on success, 1..6 the paso::SolverResult codes
+ * (paso/src/Paso.h:44-52), >= 20 an argument, state or device error; the
+ * message of the last failure on the calling thread is rg_last_error().
+ *
+ * Pointers: arguments documented "device" are HBM pointers (hipMalloc /
+ * torch.cuda memory) on the context's device; "host" arguments are ordinary
+ * host memory (escript Data buffers: DOF-major, then component, as
+ * paso::SystemMatrix reads them, paso/src/SystemMatrix.h:637-641).  Host
+ * arrays are borrowed for the duration of the call.  Device memory owned by a
+ * handle is released by its destroy call.  Work is issued on the context's
+ * stream; functions returning host results synchronise that stream.
+ */
+#ifndef RIPLEY_HIP_H
+#define RIPLEY_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------ status codes */
+enum {
+    RG_OK = 0,
+    RG_MAX_ITER_REACHED = 1, /* paso::MaxIterReached */
+    RG_INPUT_ERROR = 2,      /* paso::InputError */
+    RG_MEMORY_ERROR = 3,     /* paso::MemoryError */
+    RG_BREAKDOWN = 4,        /* paso::Breakdown */
+    RG_NEGATIVE_NORM = 5,    /* paso::NegativeNormError */
+    RG_DIVERGENCE = 6,       /* paso::Divergence */
+    RG_VALUE_ERROR = 20,     /* bad argument (PasoException/ValueError in the reference) */
+    RG_STATE_ERROR = 21,     /* e.g. MatrixVector on a balanced matrix */
+    RG_DEVICE_ERROR = 22     /* HIP runtime failure */
+};
+
+typedef struct rg_context rg_context;
+typedef struct rg_matrix rg_matrix;
+
+/* Message of the last failure on this thread (empty string if none). */
+const char* rg_last_error(void);
+
+/* Library version string. */
+const char* rg_version(void);
+
+/* ------------------------------------------------------------------ context
+ * One device and one stream.  stream may be NULL (the null stream) or a
+ * hipStream_t owned by the caller (e.g. torch.cuda.current_stream()).
+ * Replaces nothing in the reference (whose path never leaves the host); it
+ * carries what escript keeps in JMPI + OpenMP state. */
+int rg_context_create(int device, void* stream, rg_context** out);
+int rg_context_destroy(rg_context* ctx);
+int rg_context_set_stream(rg_context* ctx, void* stream);
+int rg_context_synchronize(rg_context* ctx);
+
+/* ------------------------------------------------------------------- matrix
+ * rg_matrix_create replaces RipleyDomain::newSystemMatrix ->
+ * getPasoMatrixPattern -> paso::SystemMatrix<double> ctor
+ * (ripley/src/RipleyDomain.cpp:950-1022, ripley/src/Brick.cpp:2546-2748,
+ * paso/src/SystemMatrix.h:404-492) for a Rectangle (dim 2) or Brick (dim 3)
+ * with ne[d] elements and side length[d] per axis, at one rank.  Values are
+ * zero.  block_size = numEq = numComp (1..4).  flags: RG_MATRIX_SYMMETRIC
+ * stores only the main and upper stencil diagonals (the matrix is then
+ * symmetric by construction).  The pattern is PASO's CSR pattern; it is held
+ * as a diagonal-offset stencil (9 or 27 slots per row in
+ * getDiagonalIndices order, Brick.cpp:2484-2507). */
+enum { RG_MATRIX_SYMMETRIC = 1 };
+int rg_matrix_create(rg_context* ctx, int dim, const int32_t* ne, const double* length,
+                     int block_size, int flags, rg_matrix** out);
+int rg_matrix_destroy(rg_matrix* m);
+
+/* num_rows = number of block rows (nodes); nnz = PASO block nnz. */
+int rg_matrix_get_info(const rg_matrix* m, int64_t* num_rows, int64_t* nnz, int* num_slots,
+                       int* block_size);
+
+/* paso::SystemMatrix::setValues (paso/src/SystemMatrix.h:273-279): every
+ * entry of the pattern := value; clears the balanced state. */
+int rg_matrix_set_values(rg_matrix* m, double value);
+
+/* AbstractSystemMatrix::resetValues(preserveSolverData)
+ * (paso/src/SystemMatrix.h:607-613): setValues(0) and, unless
+ * preserve_solver_data, drops the preconditioner. */
+int rg_matrix_reset_values(rg_matrix* m, int preserve_solver_data);
+
+/* RipleyDomain::addToSystemMatrix<real_t> (ripley/src/RipleyDomain.cpp:
+ * 1433-1459; its intended hook for a non-PASO matrix is the dead CUDA branch
+ * :1444-1450 -> ripley::SystemMatrix::add, RipleySystemMatrix.cu:111-155).
+ * Adds num_elements element matrices.  nodes (host) holds
+ * nodes_per_element (4 in 2-D, 8 in 3-D) node ids per element in the
+ * element's local order; em (host) holds per element
+ * EM[k + n*(m + n*(a + nN*b))] with n = num_eq = block size.  The stencil slot
+ * of a node pair follows from the local positions a, b, as in the reference
+ * stencil matrix.  Elements are applied in order (deterministic). */
+int rg_matrix_add_element_blocks(rg_matrix* m, int64_t num_elements, int nodes_per_element,
+                                 const int32_t* nodes, int num_eq, const double* em);
+
+/* PDE coefficients (device pointers, escript data-point layout, column
+ * major within a data point):
+ *   single PDE: A[i + dim*j], B[i], C[j], D, X[i], Y
+ *   system (n = block size): A[k + n*(i + dim*(m + n*j))], B[k + n*(i + dim*m)],
+ *                            C[k + n*(m + n*j)], D[k + n*m], X[k + n*i], Y[k]
+ * mode RG_COEF_ELEMENT: one data point per element, elem_stride doubles apart
+ * (0: one point shared by all elements) -- the reference's "constant data"
+ * branch; RG_COEF_EXPANDED: one point per quadrature point (4 in 2-D, 8 in
+ * 3-D, x fastest), elem_stride = points_per_element * point size.
+ * isotropic (A of a single PDE only): each data point is one scalar a and
+ * means A = a * identity. */
+enum { RG_COEF_EMPTY = 0, RG_COEF_ELEMENT = 1, RG_COEF_EXPANDED = 2 };
+typedef struct {
+    const double* data;
+    int32_t mode;
+    int32_t elem_stride;
+    int32_t isotropic;
+    int32_t reserved;
+} rg_coef;
+
+typedef struct {
+    rg_coef A, B, C, D, X, Y;
+} rg_pde_coefs;
+
+/* RipleyDomain::addToSystem -> assemblePDE -> DefaultAssembler{2D,3D}::
+ * assemblePDESingle / assemblePDESystem (+ *Reduced when reduced != 0)
+ * -> addToMatrixAndRHS (ripley/src/RipleyDomain.cpp:1024-1035, :1618-1688;
+ * ripley/src/DefaultAssembler3D.cpp:153-2229, :2891-3270, :3631-5745;
+ * ripley/src/DefaultAssembler2D.cpp:152-597, :842-1011, :1186-1679).
+ * Adds the element contributions to the matrix and rhs (device, may be NULL;
+ * num_rows*block_size doubles).  Each node gathers its own contributions; no
+ * atomics; deterministic. */
+int rg_matrix_assemble(rg_matrix* m, const rg_pde_coefs* coefs, int reduced, double* rhs);
+
+/* paso::SystemMatrix::MatrixVector (paso/src/SystemMatrix_MatrixVector.cpp:
+ * 62-91): y = alpha*A*x + beta*y (device x, y).  Fails with RG_STATE_ERROR
+ * on a balanced matrix ("MatrixVector: balanced matrix is not supported."). */
+int rg_matrix_vector(rg_matrix* m, double alpha, const double* x, double beta, double* y);
+
+/* AbstractSystemMatrix::ypAx (paso/src/SystemMatrix.h:646-671): y += A*x. */
+int rg_matrix_ypAx(rg_matrix* m, double* y, const double* x);
+int rg_matrix_ypAx_host(rg_matrix* m, double* y, const double* x);
+
+/* AbstractSystemMatrix::nullifyRowsAndCols (paso/src/SystemMatrix.h:544-605,
+ * SparseMatrix.h:494-535): a_ij := (i==j ? mdv : 0) where row_mask[i] > 0 or
+ * col_mask[j] > 0 (device masks, num_rows*block_size doubles). */
+int rg_matrix_nullify_rows_and_cols(rg_matrix* m, const double* row_mask, const double* col_mask,
+                                    double mdv);
+int rg_matrix_nullify_rows_and_cols_host(rg_matrix* m, const double* row_mask,
+                                         const double* col_mask, double mdv);
+
+/* The PASO CSR form of the matrix (host buffers): ptr[num_rows+1],
+ * index[nnz], val[nnz*bs*bs] with val[k*bs*bs + ir + bs*ic]
+ * (paso/src/SystemMatrix.h:404-492).  Any of the three may be NULL. */
+int rg_matrix_export_csr(rg_matrix* m, int32_t* ptr, int32_t* index, double* val);
+
+/* AbstractSystemMatrix::saveMM (paso/src/SparseMatrix.h:79): MatrixMarket
+ * coordinate file of the scalar matrix. */
+int rg_matrix_save_mm(rg_matrix* m, const char* filename);
+
+/* ------------------------------------------------------------------ solver
+ * Options mirror the escript::SolverBuddy fields that PASO reads
+ * (paso/src/Options.cpp:31-56; defaults escriptcore/src/SolverOptions.cpp:
+ * 40-72).  method:
+ *   RG_METHOD_PCG        PASO Solver() + Solver_PCG (residual-smoothed PCG)
+ *   RG_METHOD_PCG_PLAIN  same outer loop, textbook Jacobi-PCG inside
+ *   RG_METHOD_PCG_MIXED  FP64 outer loop (true residual, norms), inner
+ *                        Jacobi-PCG with FP32 matrix values and FP32 SpMV,
+ *                        FP64 dot products and solution update  */
+enum { RG_METHOD_PCG = 0, RG_METHOD_PCG_PLAIN = 1, RG_METHOD_PCG_MIXED = 2 };
+typedef struct {
+    double tolerance;            /* relative; default 1e-8 */
+    double inner_tolerance;      /* RG_METHOD_PCG_MIXED inner relative reduction (default 1e-4) */
+    int32_t iter_max;            /* default 100000 */
+    int32_t sweeps;              /* Jacobi sweeps, default 1 */
+    int32_t method;
+    int32_t verbose;
+    int32_t accept_failed_convergence;
+    int32_t check_interval;      /* iterations queued between host convergence checks (0: auto) */
+} rg_options;
+
+/* Diagnostics keys of paso::Options::updateEscriptDiagnostics
+ * (paso/src/Options.cpp:441-459) plus the number of outer steps. */
+typedef struct {
+    int32_t num_iter;
+    int32_t converged;
+    int32_t status;
+    int32_t outer_steps;
+    double residual_norm;
+    double set_up_time;
+    double net_time;
+    double time;
+} rg_diagnostics;
+
+void rg_options_default(rg_options* opt);
+
+/* AbstractSystemMatrix::solve -> paso::SystemMatrix::setToSolution ->
+ * SystemMatrix<double>::solve -> Solver (paso/src/SystemMatrix.h:615-644,
+ * paso/src/solve.cpp:41-145, paso/src/Solver.cpp:46-331, PCG.cpp:62-368).
+ * x, b: device, num_rows*block_size doubles.  As in PASO the matrix is left
+ * balanced (MatrixVector then fails until set/reset values).  Returns the
+ * SolverResult code mapped as paso/src/solve.cpp:119-143 does (with
+ * accept_failed_convergence, MaxIterReached/Divergence return RG_OK). */
+int rg_solve(rg_matrix* m, double* x, const double* b, const rg_options* opt,
+             rg_diagnostics* diag);
+int rg_solve_host(rg_matrix* m, double* x, const double* b, const rg_options* opt,
+                  rg_diagnostics* diag);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RIPLEY_HIP_H */
