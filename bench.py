@@ -1,0 +1,265 @@
+"""Benchmark: time to relative residual 1e-8 on the ripley Brick 128^3
+heterogeneous Poisson problem (log10 kappa a Gaussian random field, mean 0,
+sigma 1, correlation length 8 cells, seed 20261202; Y = 1; u = 0 on z = 0).
+
+A step is one solve of the whole problem from its coefficients: reset the
+operator, assemble (GPU gather kernel), insert the Dirichlet constraint,
+and solve to rel. residual 1e-8 (PASO's dual-norm criterion, then verified
+in FP64 as ||b - A x||_2 / ||b||_2).  Inputs are resident in HBM before the
+timed region.  Prints one JSON line.
+
+    python bench.py [--steps K] [--warmup W] [--method mixed|plain|paso|all]
+                    [--ne 128] [--no-cpu]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "esys-escript.github.io_amd"))
+
+HBM_PEAK_GBPS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--ne", type=int, default=128)
+    ap.add_argument("--method", default="all")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-iters", type=int, default=40)
+    ap.add_argument("--spmv-reps", type=int, default=50)
+    ap.add_argument("--quiet", action="store_true")
+    return ap.parse_args()
+
+
+def log(args, *a):
+    if not args.quiet:
+        print(*a, file=sys.stderr, flush=True)
+
+
+class Problem:
+    """The C2 system on the GPU: kappa, operator, rhs, Dirichlet mask."""
+
+    def __init__(self, ne, R, torch, np, fields):
+        self.R, self.torch = R, torch
+        self.dom = R.Brick(ne, ne, ne)
+        self.ne = (ne, ne, ne)
+        self.kappa_np = fields.log10_normal_kappa(self.ne, corr_len=8.0, sigma=1.0, seed=20261202)
+        self.kappa = torch.as_tensor(self.kappa_np, device="cuda")
+        self.q = self.dom.whereZero(2)
+        self.op = self.dom.newOperator(1)
+        E, Q = self.dom.getNumElements(), self.dom.getNumQuadraturePoints()
+        self.cA = R.Coefficient(self.kappa, (3, 3), E, Q, self.dom.device, isotropic=True)
+        self.cY = R.Coefficient(1.0, (), E, Q, self.dom.device)
+        self.rhs = torch.zeros(self.op.size, dtype=torch.float64, device="cuda")
+        self.x = torch.zeros_like(self.rhs)
+
+    def assemble(self):
+        torch = self.torch
+        self.op.resetValues(preserveSolverData=False)
+        self.rhs.zero_()
+        self.op.assemble({"A": self.cA, "Y": self.cY}, self.rhs)
+
+    def constrain(self):
+        # insertConstraint with r = 0 (linearPDEs.py:2084-2112)
+        self.op.nullifyRowsAndCols(self.q, self.q, 1.0)
+        self.rhs.masked_fill_(self.q > 0, 0.0)
+
+    def solve(self, method):
+        R = self.R
+        o = R.SolverOptions()
+        o.setSymmetryOn()
+        o.setTolerance(1e-8)
+        o.setTrueResidualCriterion(True)
+        o.setSolverMethod({"paso": R.SolverOptions.PCG, "plain": R.SolverOptions.PCG_PLAIN,
+                           "mixed": R.SolverOptions.PCG_MIXED}[method])
+        self.op.setToSolution(self.x, self.rhs, o)
+        return o
+
+    def step(self, method):
+        self.assemble()
+        self.constrain()
+        return self.solve(method)
+
+
+def true_residual(P):
+    """FP64 ||b - A x|| / ||b|| on a fresh (unbalanced) copy of the system."""
+    torch = P.torch
+    P.assemble()
+    P.constrain()
+    r = P.rhs.clone()
+    P.op.MatrixVector(-1.0, P.x, 1.0, r)
+    return (torch.linalg.norm(r) / torch.linalg.norm(P.rhs)).item()
+
+
+def time_steps(P, method, steps, warmup, torch):
+    for _ in range(warmup):
+        P.step(method)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    opts = None
+    for _ in range(steps):
+        opts = P.step(method)
+    torch.cuda.synchronize()
+    t = (time.perf_counter() - t0) / steps
+    return t, opts
+
+
+def breakdown(P, method, torch):
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    P.assemble()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    P.constrain()
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    o = P.solve(method)
+    torch.cuda.synchronize()
+    t3 = time.perf_counter()
+    return {"assemble_s": t1 - t0, "constrain_s": t2 - t1, "solve_s": t3 - t2,
+            "num_iter": o.getDiagnostics("num_iter"), "outer_steps": o.getDiagnostics("outer_steps")}
+
+
+def spmv_roofline(P, reps, torch):
+    """Average duration of the FP64 stencil SpMV (MatrixVector, one kernel per
+    call) over `reps` back-to-back launches, timed with HIP events on the
+    stream the library launches on."""
+    P.assemble()
+    P.constrain()
+    N = P.op.num_rows
+    x = torch.rand(N, dtype=torch.float64, device="cuda")
+    y = torch.zeros(N, dtype=torch.float64, device="cuda")
+    for _ in range(5):
+        P.op.MatrixVector(1.0, x, 0.0, y)
+    s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    s.record()
+    for _ in range(reps):
+        P.op.MatrixVector(1.0, x, 0.0, y)
+    e.record()
+    torch.cuda.synchronize()
+    dt = s.elapsed_time(e) / 1000.0 / reps
+    nnz = P.op.nnz
+    alg_bytes = 8 * nnz + 8 * N + 8 * N          # values + x + y (SpMV algorithmic bytes)
+    return dt, alg_bytes
+
+
+def copy_bandwidth(torch):
+    n = 1 << 27   # 1 GiB of float64
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    s.record()
+    for _ in range(10):
+        b.copy_(a)
+    e.record()
+    torch.cuda.synchronize()
+    dt = s.elapsed_time(e) / 1000.0 / 10
+    del a, b
+    return 2 * 8 * n / dt / 1e9
+
+
+def cpu_baseline(P, gpu_paso_iters, iters, np):
+    """The oracle's PASO restatement (OpenMP) on the host: assembly timed in
+    full, PASO PCG timed over a bounded number of iterations and scaled to
+    the iteration count the same algorithm needs on this problem."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    nthreads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
+    O.set_num_threads(nthreads)
+    ne = P.ne
+    kap = P.kappa_np
+    t0 = time.perf_counter()
+    ptr, idx = O.pattern(3, ne)
+    t1 = time.perf_counter()
+    # isotropic kappa I per element, passed as a full tensor (the oracle has no compact form)
+    Afull = kap[:, None, None] * np.eye(3)[None]
+    _, _, val, rhs = O.assemble(3, ne, (1.0, 1.0, 1.0), ptr=ptr, index=idx, A=Afull, Y=1.0)
+    del Afull
+    t2 = time.perf_counter()
+    q = P.q.cpu().numpy()
+    O.nullify(1, ptr, idx, val, q, q, 1.0)
+    rhs = np.where(q > 0, 0.0, rhs)
+    t3 = time.perf_counter()
+    tit = O.time_pcg_iterations(1, ptr, idx, val, rhs, iters, plain=False)
+    per_iter = tit / iters
+    est = (t1 - t0) + (t2 - t1) + (t3 - t2) + per_iter * gpu_paso_iters
+    return {"value": est, "unit": "s", "cores": nthreads, "kind": "port",
+            "sample": ("oracle PASO path on %d host threads: pattern %.2fs + assembly %.2fs + constraints "
+                       "%.2fs timed in full; %d PASO PCG iterations timed (%.4f s/iter) and scaled to the "
+                       "%d iterations PASO PCG needs on this problem" %
+                       (nthreads, t1 - t0, t2 - t1, t3 - t2, iters, per_iter, gpu_paso_iters)),
+            "per_iter_s": per_iter, "assemble_s": t2 - t1}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build runs one rank's share on one MI355X "
+              "(--gpus %d is not supported)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+    import ripley_hip as R
+    from ripley_hip import fields
+
+    log(args, "building C2 problem ne=%d" % args.ne)
+    P = Problem(args.ne, R, torch, np, fields)
+    methods = ["plain", "paso", "mixed"] if args.method == "all" else [args.method]
+    results = {}
+    for m in methods:
+        log(args, "method", m)
+        bd = breakdown(P, m, torch)
+        rel = true_residual(P)
+        t, opts = time_steps(P, m, args.steps, args.warmup, torch)
+        results[m] = dict(bd, step_s=t, rel_residual_fp64=rel, converged=bool(opts.hasConverged()))
+        log(args, m, json.dumps(results[m]))
+    ok = {m: r for m, r in results.items() if r["converged"] and r["rel_residual_fp64"] <= 1e-8 * 1.01}
+    head = min(ok, key=lambda m: ok[m]["step_s"]) if ok else methods[0]
+    dt, alg = spmv_roofline(P, args.spmv_reps, torch)
+    achieved = alg / dt / 1e9
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "spmv_fp64_brick128_traffic.json")
+    if os.path.exists(tfile):
+        traffic = json.load(open(tfile)).get("hbm_bytes_per_launch")
+    roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": traffic,
+            "kernel": "k_spmv<3,1,false,double> (FP64 27-point stencil SpMV)",
+            "alg_bytes_per_launch": alg, "avg_launch_s": dt}
+    copy_bw = copy_bandwidth(torch)
+    cpu = None
+    if not args.no_cpu:
+        paso_iters = results.get("paso", {}).get("num_iter") or results[head]["num_iter"]
+        log(args, "cpu baseline")
+        cpu = cpu_baseline(P, paso_iters, args.cpu_iters, np)
+    N = P.op.num_rows
+    hr = results[head]
+    line = {
+        "metric": "time to rel. residual 1e-8 (s), Brick 128^3: plain PCG vs learned-PC FCG",
+        "value": round(hr["step_s"], 6), "unit": "s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(hr["step_s"] * 1000, 3), "higher_is_better": False,
+        "scaling": "strong", "vs_baseline": None, "dtype": "f64" if head != "mixed" else "f32-values/f64",
+        "data": "synthetic (seeded Gaussian random field, seed 20261202)",
+        "config": {"workload": "ripley Brick %d^3 Poisson, log10 kappa GRF (sigma 1, corr. 8 cells), Y=1, "
+                               "u=0 on z=0, rtol 1e-8" % args.ne,
+                   "unknowns": N, "nnz": P.op.nnz, "method": head},
+        "methods": results,
+        "speedup_vs_plain": round(results["plain"]["step_s"] / hr["step_s"], 3) if "plain" in results else None,
+        "dof_iter_per_s": round(N * hr["num_iter"] / hr["solve_s"], 1),
+        "roofline": roof,
+        "device_copy_GBps": round(copy_bw, 1),
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

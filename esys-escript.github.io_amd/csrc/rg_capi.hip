@@ -433,6 +433,8 @@ void rg_options_default(rg_options* opt)
     opt->verbose = 0;
     opt->accept_failed_convergence = 0;
     opt->check_interval = 0;
+    opt->criterion = RG_CRITERION_PASO;
+    opt->reserved = 0;
 }
 
 // Error mapping of SystemMatrix<double>::solve (paso/src/solve.cpp:119-143).

@@ -72,39 +72,33 @@ __device__ __forceinline__ void block_sum(double (&v)[NV])
     __syncthreads();
 }
 
-// Last-workgroup reduction.  Every block publishes its NV block sums to
-// partials[k*G + b]; the block that takes the last ticket of *counter sums
-// all partials in a fixed order (so the result does not depend on timing),
-// leaves the totals in tot[] (valid in all its threads), resets the counter
-// to zero and returns true.  Other blocks return false.
+// Per-workgroup partial sums of a reducing kernel: the block sums of NV
+// values go to partials[k*gridDim.x + blockIdx.x] with plain stores.  A
+// one-workgroup stage kernel launched next on the stream reduces them in a
+// fixed order (reduce_partials); the kernel boundary makes the stores
+// visible, so no device-scope fence or atomic is needed in the hot kernel.
 template <int NV>
-__device__ __forceinline__ bool last_block_sum(double (&v)[NV], double* partials,
-                                               unsigned* counter, double (&tot)[NV])
+__device__ __forceinline__ void publish_block_sums(double (&v)[NV], double* partials)
 {
-    __shared__ bool s_last;
     block_sum<NV>(v);
-    const unsigned G = gridDim.x;
     if (threadIdx.x == 0) {
 #pragma unroll
-        for (int k = 0; k < NV; ++k)
-            __hip_atomic_store(&partials[k * G + blockIdx.x], v[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __threadfence();
-        const unsigned t = atomicAdd(counter, 1u);
-        s_last = (t == G - 1);
+        for (int k = 0; k < NV; ++k) partials[k * gridDim.x + blockIdx.x] = v[k];
     }
-    __syncthreads();
-    if (!s_last) return false;
-    __threadfence();
+}
+
+// Sum of the G partials of each of NV rows, by one 256-thread workgroup in a
+// fixed order; valid in all threads.
+template <int NV>
+__device__ __forceinline__ void reduce_partials(const double* partials, int G, double (&tot)[NV])
+{
 #pragma unroll
     for (int k = 0; k < NV; ++k) {
         double s = 0.0;
-        for (unsigned b = threadIdx.x; b < G; b += kBlock)
-            s += __hip_atomic_load(&partials[k * G + b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        for (int b = threadIdx.x; b < G; b += kBlock) s += partials[k * G + b];
         tot[k] = s;
     }
     block_sum<NV>(tot);
-    if (threadIdx.x == 0) __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    return true;
 }
 
 // Full-stencil slot s -> (dx, dy, dz) in {-1,0,1}, getDiagonalIndices order.

@@ -39,21 +39,23 @@ void dispatch(const Geom& g, bool sym, F&& f)
     });
 }
 
-// Grid of a chunked streaming kernel over n rows: a multiple of the XCD count,
-// at most kMaxGrid blocks; chunk is a multiple of the block size.
+// Grid of a chunked streaming kernel over n rows: each block owns one
+// contiguous chunk, a multiple of kBlock*rows_per_thread rows; the grid is a
+// multiple of the XCD count and at most max_grid blocks.
 struct Launch {
     int grid;
     int64_t chunk;
 };
-inline Launch chunked_launch(int64_t n, int max_grid = kMaxGrid)
+inline Launch chunked_launch(int64_t n, int rows_per_thread = 1, int max_grid = kMaxGrid)
 {
-    int64_t blocks = (n + kBlock - 1) / kBlock;
-    if (blocks < 1) blocks = 1;
-    int64_t grid = blocks < max_grid ? blocks : max_grid;
+    const int64_t unit = (int64_t)kBlock * rows_per_thread;
+    const int64_t cap = (int64_t)max_grid - kNumXcd;
+    int64_t k = (n + unit * cap - 1) / (unit * cap);
+    if (k < 1) k = 1;
+    const int64_t chunk = unit * k;
+    int64_t grid = (n + chunk - 1) / chunk;
+    if (grid < 1) grid = 1;
     grid = ((grid + kNumXcd - 1) / kNumXcd) * kNumXcd;
-    int64_t chunk = (n + grid - 1) / grid;
-    chunk = ((chunk + kBlock - 1) / kBlock) * kBlock;
-    if (chunk < kBlock) chunk = kBlock;
     return {(int)grid, chunk};
 }
 

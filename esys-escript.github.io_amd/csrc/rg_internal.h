@@ -44,7 +44,7 @@ struct Error : std::runtime_error {
 
 constexpr int kBlock = 256;          // threads per workgroup of every streaming kernel
 constexpr int kNumXcd = 8;           // MI355X: 8 XCDs, blocks dispatched round-robin
-constexpr int kMaxGrid = 2048;       // grid cap of grid-stride streaming kernels
+constexpr int kMaxGrid = 8192;       // grid cap of chunked streaming kernels (partials per reduction)
 constexpr int kNumCounters = 64;     // last-block ticket counters per context
 
 // Geometry of one matrix (one rank's share).
@@ -69,9 +69,10 @@ struct Geom {
 // updated by the last workgroup of each reducing kernel.
 struct KrylovState {
     double tau, tau_new, delta, alpha, beta, gamma1, norm, tol;
-    double rho, rnorm0, pad0, pad1;
+    double rho, rnorm0, rho_prev, pad1;
     int32_t num_iter, maxit, done, status;
-    int32_t pad[4];
+    int32_t replace_mode;   // 1: convergence is only declared on a replaced (true) residual
+    int32_t pad[3];
 };
 enum { KS_RUNNING = 0, KS_CONVERGED = 1, KS_MAXITER = 2, KS_BREAKDOWN = 3 };
 
@@ -92,6 +93,7 @@ struct rg_solver_work {
     int64_t n = 0;   // N*bs
     double *r = nullptr, *xw = nullptr, *rs = nullptr, *p = nullptr, *v = nullptr, *x2 = nullptr;
     double *z = nullptr, *z2 = nullptr;
+    double* bb = nullptr;           // balanced right-hand side D b
     double* norms = nullptr;        // [0] sum of squares, [1] max abs
     rg::KrylovState* st = nullptr;
     int* flag = nullptr;

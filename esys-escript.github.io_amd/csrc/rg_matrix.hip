@@ -296,23 +296,19 @@ __global__ __launch_bounds__(kBlock) void k_spmv(const VT* __restrict__ val, int
                                                  double alpha, double beta)
 {
     const RowRange rr = block_rows(N, chunk);
-    for (int64_t i = rr.r0 + threadIdx.x; i < rr.r1; i += kBlock) {
-        double acc[BS];
-#pragma unroll
-        for (int r = 0; r < BS; ++r) acc[r] = 0.0;
-        stencil_row<DIM, BS, SYM>(val, ld, N, s1, s2, i, x, acc);
+    stencil_block<DIM, BS, SYM, VT>(val, ld, N, s1, s2, rr.r0, rr.r1, x, [&](int64_t i, const double (&acc)[BS]) {
 #pragma unroll
         for (int r = 0; r < BS; ++r) {
             const double yo = (beta == 0.0) ? 0.0 : ((beta == 1.0) ? y[i * BS + r] : beta * y[i * BS + r]);
             y[i * BS + r] = fma(alpha, acc[r], yo);
         }
-    }
+    });
 }
 
 void launch_spmv(rg_matrix* m, double alpha, const double* x, double beta, double* y)
 {
     const Geom& g = m->g;
-    const Launch L = chunked_launch(g.N);
+    const Launch L = chunked_launch(g.N, 2);
     dispatch(g, m->symmetric, [&](auto D, auto B, auto S) {
         hipLaunchKernelGGL((k_spmv<D.value, B.value, S.value, double>), dim3(L.grid), dim3(kBlock), 0,
                            m->ctx->stream, m->val, g.ld, g.N, g.s1, g.s2, L.chunk, x, y, alpha, beta);
@@ -323,7 +319,7 @@ void launch_spmv(rg_matrix* m, double alpha, const double* x, double beta, doubl
 void launch_spmv32(rg_matrix* m, const double* x, double* y)
 {
     const Geom& g = m->g;
-    const Launch L = chunked_launch(g.N);
+    const Launch L = chunked_launch(g.N, 4);
     dispatch(g, m->symmetric, [&](auto D, auto B, auto S) {
         hipLaunchKernelGGL((k_spmv<D.value, B.value, S.value, float>), dim3(L.grid), dim3(kBlock), 0,
                            m->ctx->stream, m->val32, g.ld, g.N, g.s1, g.s2, L.chunk, x, y, 1.0, 0.0);

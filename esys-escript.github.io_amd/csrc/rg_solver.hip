@@ -206,41 +206,46 @@ __global__ void k_fill(double* __restrict__ y, double v, int64_t n)
         y[i] = v;
 }
 
-// ||v||_2^2 and ||v||_inf in one pass (Solver.cpp:106-130, :201-225).
-__global__ __launch_bounds__(kBlock) void k_norms(const double* __restrict__ v, int64_t n, double* partials,
-                                                  unsigned* counter, double* out)
+// ||v||_2^2 and ||v||_inf in one pass (Solver.cpp:106-130, :201-225); with
+// d given, of v / d (the unbalanced residual of a balanced one).  Block sums
+// go to partial row 0, block maxima to row 1; k_norms_stage combines them.
+__global__ __launch_bounds__(kBlock) void k_norms(const double* __restrict__ v, const double* __restrict__ d,
+                                                  int64_t n, double* partials)
 {
-    double s[2] = {0.0, 0.0};
+    double s[1] = {0.0};
+    double mx = 0.0;
     for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
-        const double a = v[i];
+        const double a = d ? v[i] / d[i] : v[i];
         s[0] += a * a;
-        s[1] = fmax(s[1], fabs(a));
+        mx = fmax(mx, fabs(a));
     }
-    double mx = wave_max(s[1]);
+    mx = wave_max(mx);
     __shared__ double smax[kBlock / 64];
     if ((threadIdx.x & 63) == 0) smax[threadIdx.x >> 6] = mx;
     __syncthreads();
-    mx = 0.0;
-    for (int w = 0; w < kBlock / 64; ++w) mx = fmax(mx, smax[w]);
-    // the block maxima go to partial row 2, the sums through last_block_sum
-    double v1[1] = {s[0]}, t1[1];
-    const unsigned G = gridDim.x;
-    if (threadIdx.x == 0)
-        __hip_atomic_store(&partials[2 * G + blockIdx.x], mx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (last_block_sum<1>(v1, partials, counter, t1)) {
-        double m2 = 0.0;
-        for (unsigned b = threadIdx.x; b < G; b += kBlock)
-            m2 = fmax(m2, __hip_atomic_load(&partials[2 * G + b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-        m2 = wave_max(m2);
-        __shared__ double sm2[kBlock / 64];
-        if ((threadIdx.x & 63) == 0) sm2[threadIdx.x >> 6] = m2;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            double mm = 0.0;
-            for (int w = 0; w < kBlock / 64; ++w) mm = fmax(mm, sm2[w]);
-            out[0] = t1[0];
-            out[1] = mm;
-        }
+    if (threadIdx.x == 0) {
+        double m = 0.0;
+        for (int w = 0; w < kBlock / 64; ++w) m = fmax(m, smax[w]);
+        partials[gridDim.x + blockIdx.x] = m;
+    }
+    publish_block_sums<1>(s, partials);
+}
+
+__global__ __launch_bounds__(kBlock) void k_norms_stage(const double* __restrict__ partials, int G, double* out)
+{
+    double t[1];
+    reduce_partials<1>(partials, G, t);
+    double m = 0.0;
+    for (int b = threadIdx.x; b < G; b += kBlock) m = fmax(m, partials[G + b]);
+    m = wave_max(m);
+    __shared__ double sm[kBlock / 64];
+    if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double mm = 0.0;
+        for (int w = 0; w < kBlock / 64; ++w) mm = fmax(mm, sm[w]);
+        out[0] = t[0];
+        out[1] = mm;
     }
 }
 
@@ -254,109 +259,156 @@ __global__ __launch_bounds__(kBlock) void k_norms(const double* __restrict__ v, 
 //           x = (1-gamma) x + gamma x2, s5 = rs.rs              k_paso_x
 //   (8)     stop if sqrt(s5) <= tol, iter > maxit or tau == 0
 // tau of iteration k+1 is reduced in k_paso_x of iteration k (r is final
-// there), so each iteration is four launches.
+// there), so each iteration is four launches.  Streaming kernels of block
+// size 1 move two entries per thread with 16-byte accesses.
 struct PcgArgs {
     double *r, *x, *rs, *p, *v, *x2;
     const double* z;        // stored M^-1 r (Jacobi sweeps > 1) or nullptr
+    const double* b;        // right-hand side of a residual replacement
     KrylovState* st;
     double* partials;
     unsigned* counter;
     int64_t N, ld, s1, s2, chunk;
 };
 
+// Loop over the node range of this block, two nodes per thread for BS == 1.
+// f(i, two) is called with the first node of a pair (two == true) or a single
+// node (two == false).
+template <int BS, class F>
+__device__ __forceinline__ void node_loop(const RowRange& rr, F&& f)
+{
+    if (BS == 1) {
+        for (int64_t i = rr.r0 + 2 * (int64_t)threadIdx.x; i < rr.r1; i += 2 * (int64_t)kBlock)
+            f(i, i + 1 < rr.r1);
+    } else {
+        for (int64_t i = rr.r0 + threadIdx.x; i < rr.r1; i += kBlock) f(i, false);
+    }
+}
+
+template <typename T>
+struct Two {
+    T a, b;
+};
+__device__ __forceinline__ Two<double> ld2(const double* p, bool two)
+{
+    if (two) {
+        const double2 v = *reinterpret_cast<const double2*>(p);
+        return {v.x, v.y};
+    }
+    return {p[0], 0.0};
+}
+__device__ __forceinline__ Two<double> ld2(const float* p, bool two)
+{
+    if (two) {
+        const float2 v = *reinterpret_cast<const float2*>(p);
+        return {(double)v.x, (double)v.y};
+    }
+    return {(double)p[0], 0.0};
+}
+__device__ __forceinline__ void st2(double* p, Two<double> v, bool two)
+{
+    if (two) *reinterpret_cast<double2*>(p) = make_double2(v.a, v.b);
+    else p[0] = v.a;
+}
+
 template <int BS, typename DT>
 __global__ __launch_bounds__(kBlock) void k_paso_init(PcgArgs a, const DT* __restrict__ dinv)
 {
     const RowRange rr = block_rows(a.N, a.chunk);
-    double t[1] = {0.0}, tot[1];
+    double* __restrict__ r = a.r;
+    double* __restrict__ rs = a.rs;
+    double* __restrict__ x2 = a.x2;
+    double* __restrict__ p = a.p;
+    const double* __restrict__ x = a.x;
+    double t[1] = {0.0};
     for (int64_t i = rr.r0 + threadIdx.x; i < rr.r1; i += kBlock) {
         double z[BS];
-        prec_node<BS>(dinv, a.ld, i, a.r, a.z, z);
+        prec_node<BS>(dinv, a.ld, i, r, a.z, z);
 #pragma unroll
         for (int c = 0; c < BS; ++c) {
             const int64_t k = i * BS + c;
-            const double rk = a.r[k];
-            a.rs[k] = rk;
-            a.x2[k] = a.x[k];
-            a.p[k] = 0.0;
+            const double rk = r[k];
+            rs[k] = rk;
+            x2[k] = x[k];
+            p[k] = 0.0;
             t[0] += z[c] * rk;
         }
     }
-    if (last_block_sum<1>(t, a.partials, a.counter, tot) && threadIdx.x == 0) {
-        KrylovState* s = a.st;
-        s->tau = tot[0];
-        s->beta = 0.0;
-        s->num_iter = 1;
-        s->done = 0;
-        s->status = KS_RUNNING;
-    }
+    publish_block_sums<1>(t, a.partials);
 }
 
+// p = M^-1 r + beta p (paso k_paso_p and textbook k_cg_p).
 template <int BS, typename DT>
-__global__ __launch_bounds__(kBlock) void k_paso_p(PcgArgs a, const DT* __restrict__ dinv)
+__global__ __launch_bounds__(kBlock) void k_p_update(PcgArgs a, const DT* __restrict__ dinv)
 {
     if (a.st->done) return;
     const double beta = a.st->beta;
     const RowRange rr = block_rows(a.N, a.chunk);
-    for (int64_t i = rr.r0 + threadIdx.x; i < rr.r1; i += kBlock) {
-        double z[BS];
-        prec_node<BS>(dinv, a.ld, i, a.r, a.z, z);
+    const double* __restrict__ r = a.r;
+    const double* __restrict__ zb = a.z;
+    double* __restrict__ p = a.p;
+    node_loop<BS>(rr, [&](int64_t i, bool two) {
+        if (BS == 1) {
+            Two<double> z;
+            if (zb) z = ld2(zb + i, two);
+            else {
+                const Two<double> rv = ld2(r + i, two), dv = ld2(dinv + i, two);
+                z = {dv.a * rv.a, dv.b * rv.b};
+            }
+            const Two<double> pv = ld2(p + i, two);
+            st2(p + i, {fma(beta, pv.a, z.a), fma(beta, pv.b, z.b)}, two);
+        } else {
+            double z[BS];
+            prec_node<BS>(dinv, a.ld, i, r, zb, z);
 #pragma unroll
-        for (int c = 0; c < BS; ++c) {
-            const int64_t k = i * BS + c;
-            a.p[k] = fma(beta, a.p[k], z[c]);
+            for (int c = 0; c < BS; ++c) p[i * BS + c] = fma(beta, p[i * BS + c], z[c]);
         }
-    }
+    });
 }
 
-template <int DIM, int BS, bool SYM, typename VT>
-__global__ __launch_bounds__(kBlock) void k_paso_spmv(PcgArgs a, const VT* __restrict__ val)
+// v = A p and p.v; the last block forms alpha = (tau or rho) / delta.
+template <int DIM, int BS, bool SYM, typename VT, bool PASO>
+__global__ __launch_bounds__(kBlock) void k_spmv_dot(PcgArgs a, const VT* __restrict__ val)
 {
     if (a.st->done) return;
     const RowRange rr = block_rows(a.N, a.chunk);
-    double dl[1] = {0.0}, tot[1];
-    for (int64_t i = rr.r0 + threadIdx.x; i < rr.r1; i += kBlock) {
-        double acc[BS];
+    const double* __restrict__ p = a.p;
+    double* __restrict__ v = a.v;
+    double dl[1] = {0.0};
+    stencil_block<DIM, BS, SYM, VT>(val, a.ld, a.N, a.s1, a.s2, rr.r0, rr.r1, p,
+                                    [&](int64_t i, const double (&acc)[BS]) {
 #pragma unroll
-        for (int r = 0; r < BS; ++r) acc[r] = 0.0;
-        stencil_row<DIM, BS, SYM>(val, a.ld, a.N, a.s1, a.s2, i, a.p, acc);
-#pragma unroll
-        for (int r = 0; r < BS; ++r) {
-            a.v[i * BS + r] = acc[r];
-            dl[0] += acc[r] * a.p[i * BS + r];
-        }
-    }
-    if (last_block_sum<1>(dl, a.partials, a.counter, tot) && threadIdx.x == 0) {
-        KrylovState* s = a.st;
-        s->delta = tot[0];
-        s->alpha = s->tau / tot[0];
-        if (fabs(tot[0]) <= 0.0) {   // PCG.cpp:262 breakFlag
-            s->done = 1;
-            s->status = KS_BREAKDOWN;
-        }
-    }
+                                        for (int r = 0; r < BS; ++r) {
+                                            v[i * BS + r] = acc[r];
+                                            dl[0] = fma(acc[r], p[i * BS + r], dl[0]);
+                                        }
+                                    });
+    publish_block_sums<1>(dl, a.partials);
 }
 
+// a.N counts scalar entries here (nodes * block size).
 __global__ __launch_bounds__(kBlock) void k_paso_r(PcgArgs a)
 {
     if (a.st->done) return;
     const double alpha = a.st->alpha;
-    // a.N counts scalar entries here (nodes * block size); the update is
-    // independent of the block structure.
     const RowRange rr = block_rows(a.N, a.chunk);
-    double s[2] = {0.0, 0.0}, tot[2];
-    for (int64_t i = rr.r0 + threadIdx.x; i < rr.r1; i += kBlock) {
-        const double ri = fma(-alpha, a.v[i], a.r[i]);
-        a.r[i] = ri;
-        const double d = ri - a.rs[i];
-        s[0] += d * d;
-        s[1] += d * a.rs[i];
-        a.x2[i] = fma(alpha, a.p[i], a.x2[i]);
-    }
-    if (last_block_sum<2>(s, a.partials, a.counter, tot) && threadIdx.x == 0) {
-        KrylovState* st = a.st;
-        st->gamma1 = (fabs(tot[0]) <= 0.0) ? 0.0 : -tot[1] / tot[0];
-    }
+    double* __restrict__ r = a.r;
+    double* __restrict__ x2 = a.x2;
+    const double* __restrict__ rs = a.rs;
+    const double* __restrict__ v = a.v;
+    const double* __restrict__ p = a.p;
+    double s[2] = {0.0, 0.0};
+    node_loop<1>(rr, [&](int64_t i, bool two) {
+        const Two<double> rv = ld2(r + i, two), vv = ld2(v + i, two), rsv = ld2(rs + i, two);
+        const Two<double> pv = ld2(p + i, two), xv = ld2(x2 + i, two);
+        const Two<double> rn = {fma(-alpha, vv.a, rv.a), fma(-alpha, vv.b, rv.b)};
+        st2(r + i, rn, two);
+        st2(x2 + i, {fma(alpha, pv.a, xv.a), fma(alpha, pv.b, xv.b)}, two);
+        const double d0 = rn.a - rsv.a, d1 = two ? rn.b - rsv.b : 0.0;
+        s[0] += d0 * d0 + d1 * d1;
+        s[1] += d0 * rsv.a + (two ? d1 * rsv.b : 0.0);
+    });
+    publish_block_sums<2>(s, a.partials);
 }
 
 template <int BS, typename DT>
@@ -365,23 +417,199 @@ __global__ __launch_bounds__(kBlock) void k_paso_x(PcgArgs a, const DT* __restri
     if (a.st->done) return;
     const double g1 = a.st->gamma1, g2 = 1.0 - g1;
     const RowRange rr = block_rows(a.N, a.chunk);
-    double s[2] = {0.0, 0.0}, tot[2];
+    const double* __restrict__ r = a.r;
+    const double* __restrict__ x2 = a.x2;
+    const double* __restrict__ zb = a.z;
+    double* __restrict__ rs = a.rs;
+    double* __restrict__ x = a.x;
+    double s[2] = {0.0, 0.0};
+    node_loop<BS>(rr, [&](int64_t i, bool two) {
+        if (BS == 1) {
+            const Two<double> rv = ld2(r + i, two), rsv = ld2(rs + i, two), xv = ld2(x + i, two),
+                              x2v = ld2(x2 + i, two);
+            Two<double> z;
+            if (zb) z = ld2(zb + i, two);
+            else {
+                const Two<double> dv = ld2(dinv + i, two);
+                z = {dv.a * rv.a, dv.b * rv.b};
+            }
+            const Two<double> rsn = {g2 * rsv.a + g1 * rv.a, g2 * rsv.b + g1 * rv.b};
+            st2(rs + i, rsn, two);
+            st2(x + i, {g2 * xv.a + g1 * x2v.a, g2 * xv.b + g1 * x2v.b}, two);
+            s[0] += rsn.a * rsn.a + (two ? rsn.b * rsn.b : 0.0);
+            s[1] += z.a * rv.a + (two ? z.b * rv.b : 0.0);
+        } else {
+            double z[BS];
+            prec_node<BS>(dinv, a.ld, i, r, zb, z);
+#pragma unroll
+            for (int c = 0; c < BS; ++c) {
+                const int64_t k = i * BS + c;
+                const double rk = r[k];
+                const double rsk = g2 * rs[k] + g1 * rk;
+                rs[k] = rsk;
+                x[k] = g2 * x[k] + g1 * x2[k];
+                s[0] += rsk * rsk;
+                s[1] += z[c] * rk;
+            }
+        }
+    });
+    publish_block_sums<2>(s, a.partials);
+}
+
+// ------------------------------------------------------ textbook Jacobi-PCG
+//   q = A p, delta = p.q, alpha = rho/delta                  k_spmv_dot
+//   x += alpha p, r -= alpha q, rho' = r.M^-1 r, ||r||       k_cg_update
+//   p = M^-1 r + (rho'/rho) p                                k_p_update
+// Stops when ||r||_2 <= tol (or iter >= maxit, or a zero scalar).
+template <int BS, typename DT>
+__global__ __launch_bounds__(kBlock) void k_cg_init(PcgArgs a, const DT* __restrict__ dinv)
+{
+    const RowRange rr = block_rows(a.N, a.chunk);
+    const double* __restrict__ r = a.r;
+    double* __restrict__ p = a.p;
+    double s[2] = {0.0, 0.0};
     for (int64_t i = rr.r0 + threadIdx.x; i < rr.r1; i += kBlock) {
         double z[BS];
-        prec_node<BS>(dinv, a.ld, i, a.r, a.z, z);
+        prec_node<BS>(dinv, a.ld, i, r, a.z, z);
 #pragma unroll
         for (int c = 0; c < BS; ++c) {
             const int64_t k = i * BS + c;
-            const double rk = a.r[k];
-            const double rsk = g2 * a.rs[k] + g1 * rk;
-            a.rs[k] = rsk;
-            a.x[k] = g2 * a.x[k] + g1 * a.x2[k];
-            s[0] += rsk * rsk;
-            s[1] += z[c] * rk;
+            const double rk = r[k];
+            p[k] = z[c];
+            s[0] += rk * z[c];
+            s[1] += rk * rk;
         }
     }
-    if (last_block_sum<2>(s, a.partials, a.counter, tot) && threadIdx.x == 0) {
-        KrylovState* st = a.st;
+    publish_block_sums<2>(s, a.partials);
+}
+
+// End-of-iteration logic of the textbook PCG, shared by k_cg_update and the
+// residual replacement.
+__device__ __forceinline__ void cg_end(KrylovState* st, double rz, double rr, bool have_rz)
+{
+    st->norm = sqrt(rr);
+    st->tau_new = rz;
+    if (st->norm <= st->tol && !st->replace_mode) { st->done = 1; st->status = KS_CONVERGED; }
+    else if (st->num_iter >= st->maxit) { st->done = 1; st->status = KS_MAXITER; }
+    else if (have_rz) {
+        if (fabs(rz) <= 0.0) { st->done = 1; st->status = KS_BREAKDOWN; }
+        else { st->beta = rz / st->rho; st->rho_prev = st->rho; st->rho = rz; }
+    }
+}
+
+template <int BS, typename DT>
+__global__ __launch_bounds__(kBlock) void k_cg_update(PcgArgs a, const DT* __restrict__ dinv)
+{
+    if (a.st->done) return;
+    const double alpha = a.st->alpha;
+    const RowRange rr = block_rows(a.N, a.chunk);
+    double* __restrict__ x = a.x;
+    double* __restrict__ r = a.r;
+    const double* __restrict__ p = a.p;
+    const double* __restrict__ v = a.v;
+    const bool have_z = a.z != nullptr;
+    double s[2] = {0.0, 0.0};
+    node_loop<BS>(rr, [&](int64_t i, bool two) {
+        if (BS == 1) {
+            const Two<double> xv = ld2(x + i, two), pv = ld2(p + i, two), rv = ld2(r + i, two),
+                              vv = ld2(v + i, two), dv = ld2(dinv + i, two);
+            st2(x + i, {fma(alpha, pv.a, xv.a), fma(alpha, pv.b, xv.b)}, two);
+            const Two<double> rn = {fma(-alpha, vv.a, rv.a), fma(-alpha, vv.b, rv.b)};
+            st2(r + i, rn, two);
+            s[0] += dv.a * rn.a * rn.a + (two ? dv.b * rn.b * rn.b : 0.0);
+            s[1] += rn.a * rn.a + (two ? rn.b * rn.b : 0.0);
+        } else {
+#pragma unroll
+            for (int c = 0; c < BS; ++c) {
+                const int64_t k = i * BS + c;
+                x[k] = fma(alpha, p[k], x[k]);
+                r[k] = fma(-alpha, v[k], r[k]);
+            }
+            double z[BS];
+            jacobi_node<BS>(dinv, a.ld, i, r, z);
+#pragma unroll
+            for (int c = 0; c < BS; ++c) {
+                const double rk = r[i * BS + c];
+                s[0] += rk * z[c];
+                s[1] += rk * rk;
+            }
+        }
+    });
+    publish_block_sums<2>(s, a.partials);
+}
+
+// r.z for the sweeps > 1 path of the textbook PCG (z stored).
+__global__ __launch_bounds__(kBlock) void k_cg_rz(PcgArgs a, int64_t n)
+{
+    if (a.st->done) return;
+    double s[1] = {0.0};
+    for (int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x; k < n; k += (int64_t)gridDim.x * kBlock)
+        s[0] += a.r[k] * a.z[k];
+    publish_block_sums<1>(s, a.partials);
+}
+
+// Residual replacement of the mixed-precision PCG: r = b - A x with the FP64
+// matrix, then r.M^-1 r and ||r|| as at the end of an iteration.
+template <int DIM, int BS, bool SYM, typename DT>
+__global__ __launch_bounds__(kBlock) void k_cg_replace(PcgArgs a, const double* __restrict__ val,
+                                                       const DT* __restrict__ dinv)
+{
+    if (a.st->done) return;
+    const RowRange rr = block_rows(a.N, a.chunk);
+    const double* __restrict__ b = a.b;
+    double* __restrict__ r = a.r;
+    double s[2] = {0.0, 0.0};
+    stencil_block<DIM, BS, SYM, double>(val, a.ld, a.N, a.s1, a.s2, rr.r0, rr.r1, a.x,
+                                        [&](int64_t i, const double (&acc)[BS]) {
+                                            double rv[BS];
+#pragma unroll
+                                            for (int c = 0; c < BS; ++c) {
+                                                rv[c] = b[i * BS + c] - acc[c];
+                                                r[i * BS + c] = rv[c];
+                                            }
+                                            double z[BS];
+                                            jacobi_node<BS>(dinv, a.ld, i, r, z);
+#pragma unroll
+                                            for (int c = 0; c < BS; ++c) {
+                                                s[0] += rv[c] * z[c];
+                                                s[1] += rv[c] * rv[c];
+                                            }
+                                        });
+    publish_block_sums<2>(s, a.partials);
+}
+
+// ------------------------------------------------------------ stage kernels
+// One workgroup after each reducing kernel: reduces its partials in a fixed
+// order and advances the Krylov scalars (the steps of PCG.cpp:156-347 that
+// follow each reduction).
+enum Stage {
+    ST_PASO_INIT, ST_PASO_SPMV, ST_PASO_R, ST_PASO_X,
+    ST_CG_INIT, ST_CG_SPMV, ST_CG_UPDATE, ST_CG_UPDATE_NORZ, ST_CG_RZ, ST_CG_REPLACE
+};
+
+template <int STAGE, int NV>
+__global__ __launch_bounds__(kBlock) void k_stage(KrylovState* st, const double* __restrict__ partials, int G)
+{
+    if (STAGE != ST_PASO_INIT && STAGE != ST_CG_INIT && st->done) return;
+    double tot[NV];
+    reduce_partials<NV>(partials, G, tot);
+    if (threadIdx.x != 0) return;
+    if (STAGE == ST_PASO_INIT) {
+        st->tau = tot[0];
+        st->beta = 0.0;
+        st->num_iter = 1;
+        st->done = 0;
+        st->status = KS_RUNNING;
+    } else if (STAGE == ST_PASO_SPMV) {
+        st->delta = tot[0];
+        st->alpha = st->tau / tot[0];
+        if (fabs(tot[0]) <= 0.0) {   // PCG.cpp:262 breakFlag
+            st->done = 1;
+            st->status = KS_BREAKDOWN;
+        }
+    } else if (STAGE == ST_PASO_R) {
+        st->gamma1 = (fabs(tot[0]) <= 0.0) ? 0.0 : -tot[1] / tot[0];
+    } else if (STAGE == ST_PASO_X) {
         const double norm = sqrt(tot[0]);
         st->norm = norm;
         const bool conv = norm <= st->tol;
@@ -395,34 +623,9 @@ __global__ __launch_bounds__(kBlock) void k_paso_x(PcgArgs a, const DT* __restri
             st->tau = tot[1];
             st->num_iter += 1;
         }
-    }
-}
-
-// ------------------------------------------------------ textbook Jacobi-PCG
-//   q = A p, delta = p.q, alpha = rho/delta                  k_cg_spmv
-//   x += alpha p, r -= alpha q, rho' = r.M^-1 r, ||r||       k_cg_update
-//   p = M^-1 r + (rho'/rho) p                                k_cg_p
-// Stops when ||r||_2 <= tol (or iter >= maxit, or a zero scalar).
-template <int BS, typename DT>
-__global__ __launch_bounds__(kBlock) void k_cg_init(PcgArgs a, const DT* __restrict__ dinv)
-{
-    const RowRange rr = block_rows(a.N, a.chunk);
-    double s[2] = {0.0, 0.0}, tot[2];
-    for (int64_t i = rr.r0 + threadIdx.x; i < rr.r1; i += kBlock) {
-        double z[BS];
-        prec_node<BS>(dinv, a.ld, i, a.r, a.z, z);
-#pragma unroll
-        for (int c = 0; c < BS; ++c) {
-            const int64_t k = i * BS + c;
-            const double rk = a.r[k];
-            a.p[k] = z[c];
-            s[0] += rk * z[c];
-            s[1] += rk * rk;
-        }
-    }
-    if (last_block_sum<2>(s, a.partials, a.counter, tot) && threadIdx.x == 0) {
-        KrylovState* st = a.st;
+    } else if (STAGE == ST_CG_INIT) {
         st->rho = tot[0];
+        st->rho_prev = tot[0];
         st->norm = sqrt(tot[1]);
         st->num_iter = 0;
         st->done = 0;
@@ -430,107 +633,23 @@ __global__ __launch_bounds__(kBlock) void k_cg_init(PcgArgs a, const DT* __restr
         if (st->norm <= st->tol) { st->done = 1; st->status = KS_CONVERGED; }
         else if (st->maxit <= 0) { st->done = 1; st->status = KS_MAXITER; }
         else if (fabs(tot[0]) <= 0.0) { st->done = 1; st->status = KS_BREAKDOWN; }
-    }
-}
-
-template <int DIM, int BS, bool SYM, typename VT>
-__global__ __launch_bounds__(kBlock) void k_cg_spmv(PcgArgs a, const VT* __restrict__ val)
-{
-    if (a.st->done) return;
-    const RowRange rr = block_rows(a.N, a.chunk);
-    double dl[1] = {0.0}, tot[1];
-    for (int64_t i = rr.r0 + threadIdx.x; i < rr.r1; i += kBlock) {
-        double acc[BS];
-#pragma unroll
-        for (int r = 0; r < BS; ++r) acc[r] = 0.0;
-        stencil_row<DIM, BS, SYM>(val, a.ld, a.N, a.s1, a.s2, i, a.p, acc);
-#pragma unroll
-        for (int r = 0; r < BS; ++r) {
-            a.v[i * BS + r] = acc[r];
-            dl[0] += acc[r] * a.p[i * BS + r];
-        }
-    }
-    if (last_block_sum<1>(dl, a.partials, a.counter, tot) && threadIdx.x == 0) {
-        KrylovState* st = a.st;
+    } else if (STAGE == ST_CG_SPMV) {
         st->delta = tot[0];
         if (fabs(tot[0]) <= 0.0) { st->done = 1; st->status = KS_BREAKDOWN; }
         else st->alpha = st->rho / tot[0];
-    }
-}
-
-template <int BS, typename DT>
-__global__ __launch_bounds__(kBlock) void k_cg_update(PcgArgs a, const DT* __restrict__ dinv)
-{
-    if (a.st->done) return;
-    const double alpha = a.st->alpha;
-    const RowRange rr = block_rows(a.N, a.chunk);
-    double s[2] = {0.0, 0.0}, tot[2];
-    for (int64_t i = rr.r0 + threadIdx.x; i < rr.r1; i += kBlock) {
-#pragma unroll
-        for (int c = 0; c < BS; ++c) {
-            const int64_t k = i * BS + c;
-            a.x[k] = fma(alpha, a.p[k], a.x[k]);
-            a.r[k] = fma(-alpha, a.v[k], a.r[k]);
-        }
-        if (a.z == nullptr) {
-            double z[BS];
-            jacobi_node<BS>(dinv, a.ld, i, a.r, z);
-#pragma unroll
-            for (int c = 0; c < BS; ++c) {
-                const double rk = a.r[i * BS + c];
-                s[0] += rk * z[c];
-                s[1] += rk * rk;
-            }
-        } else {
-#pragma unroll
-            for (int c = 0; c < BS; ++c) {
-                const double rk = a.r[i * BS + c];
-                s[1] += rk * rk;
-            }
-        }
-    }
-    if (last_block_sum<2>(s, a.partials, a.counter, tot) && threadIdx.x == 0) {
-        KrylovState* st = a.st;
+    } else if (STAGE == ST_CG_UPDATE || STAGE == ST_CG_UPDATE_NORZ) {
         st->num_iter += 1;
-        st->norm = sqrt(tot[1]);
-        st->tau_new = tot[0];
-        if (st->norm <= st->tol) { st->done = 1; st->status = KS_CONVERGED; }
-        else if (st->num_iter >= st->maxit) { st->done = 1; st->status = KS_MAXITER; }
-        else if (a.z == nullptr) {
-            if (fabs(tot[0]) <= 0.0) { st->done = 1; st->status = KS_BREAKDOWN; }
-            else { st->beta = tot[0] / st->rho; st->rho = tot[0]; }
-        }
-    }
-}
-
-// r.z for the sweeps > 1 path of the textbook PCG (z stored).
-__global__ __launch_bounds__(kBlock) void k_cg_rz(PcgArgs a, int64_t n)
-{
-    if (a.st->done) return;
-    double s[1] = {0.0}, tot[1];
-    for (int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x; k < n; k += (int64_t)gridDim.x * kBlock)
-        s[0] += a.r[k] * a.z[k];
-    if (last_block_sum<1>(s, a.partials, a.counter, tot) && threadIdx.x == 0) {
-        KrylovState* st = a.st;
+        cg_end(st, tot[0], tot[1], STAGE == ST_CG_UPDATE);
+    } else if (STAGE == ST_CG_RZ) {
         if (fabs(tot[0]) <= 0.0) { st->done = 1; st->status = KS_BREAKDOWN; }
-        else { st->beta = tot[0] / st->rho; st->rho = tot[0]; }
-    }
-}
-
-template <int BS, typename DT>
-__global__ __launch_bounds__(kBlock) void k_cg_p(PcgArgs a, const DT* __restrict__ dinv)
-{
-    if (a.st->done) return;
-    const double beta = a.st->beta;
-    const RowRange rr = block_rows(a.N, a.chunk);
-    for (int64_t i = rr.r0 + threadIdx.x; i < rr.r1; i += kBlock) {
-        double z[BS];
-        prec_node<BS>(dinv, a.ld, i, a.r, a.z, z);
-#pragma unroll
-        for (int c = 0; c < BS; ++c) {
-            const int64_t k = i * BS + c;
-            a.p[k] = fma(beta, a.p[k], z[c]);
-        }
+        else { st->beta = tot[0] / st->rho; st->rho_prev = st->rho; st->rho = tot[0]; }
+    } else if (STAGE == ST_CG_REPLACE) {
+        // k_cg_update's stage set beta = rho_rec/rho_prev and rho = rho_rec;
+        // redo that step with the replaced (true) residual
+        st->norm = sqrt(tot[1]);
+        if (st->norm <= st->tol) { st->done = 1; st->status = KS_CONVERGED; }
+        else if (fabs(tot[0]) <= 0.0) { st->done = 1; st->status = KS_BREAKDOWN; }
+        else { st->beta = tot[0] / st->rho_prev; st->rho = tot[0]; }
     }
 }
 
@@ -545,29 +664,18 @@ __global__ __launch_bounds__(kBlock) void k_jacobi_sweep(const double* __restric
 {
     if (st && st->done) return;
     const RowRange rr = block_rows(N, chunk);
-    for (int64_t i = rr.r0 + threadIdx.x; i < rr.r1; i += kBlock) {
-        double acc[BS];
-#pragma unroll
-        for (int q = 0; q < BS; ++q) acc[q] = 0.0;
-        stencil_row<DIM, BS, SYM>(val, ld, N, s1, s2, i, zin, acc);
+    stencil_block<DIM, BS, SYM, double>(val, ld, N, s1, s2, rr.r0, rr.r1, zin, [&](int64_t i, const double (&acc)[BS]) {
         double res[BS];
 #pragma unroll
         for (int q = 0; q < BS; ++q) res[q] = r[i * BS + q] - acc[q];
-        double z[BS];
-        if (BS == 1) {
-            z[0] = dinv[i] * res[0];
-        } else {
 #pragma unroll
-            for (int q = 0; q < BS; ++q) {
-                double s = 0.0;
+        for (int q = 0; q < BS; ++q) {
+            double sacc = 0.0;
 #pragma unroll
-                for (int c = 0; c < BS; ++c) s += dinv[(int64_t)(q + BS * c) * ld + i] * res[c];
-                z[q] = s;
-            }
+            for (int c = 0; c < BS; ++c) sacc += dinv[(int64_t)(q + BS * c) * ld + i] * res[c];
+            zout[i * BS + q] = zin[i * BS + q] + sacc;
         }
-#pragma unroll
-        for (int q = 0; q < BS; ++q) zout[i * BS + q] = zin[i * BS + q] + z[q];
-    }
+    });
 }
 
 template <int BS>
@@ -593,21 +701,32 @@ struct SolverImpl {
     rg_matrix* m;
     rg_solver_work* w;
     hipStream_t st;
-    Launch L;
+    Launch L;     // node kernels without pairing (init, Jacobi sweeps)
 
     explicit SolverImpl(rg_matrix* mm) : m(mm), w(mm->work), st(mm->ctx->stream), L(chunked_launch(mm->g.N)) {}
 
-    PcgArgs args(const double* zbuf)
+    PcgArgs args(const double* zbuf, const Launch& l)
     {
         PcgArgs a;
         a.r = w->r; a.x = w->xw; a.rs = w->rs; a.p = w->p; a.v = w->v; a.x2 = w->x2;
         a.z = zbuf;
+        a.b = w->bb;
         a.st = w->st;
         a.partials = m->ctx->partials;
         a.counter = m->ctx->counters;
-        a.N = m->g.N; a.ld = m->g.ld; a.s1 = m->g.s1; a.s2 = m->g.s2; a.chunk = L.chunk;
+        a.N = m->g.N; a.ld = m->g.ld; a.s1 = m->g.s1; a.s2 = m->g.s2; a.chunk = l.chunk;
         return a;
     }
+    template <int STAGE, int NV>
+    void stage(int G)
+    {
+        hipLaunchKernelGGL((k_stage<STAGE, NV>), dim3(1), dim3(kBlock), 0, st, w->st, m->ctx->partials, G);
+    }
+
+    // launch shapes: stencil kernels (R rows per thread), paired node kernels
+    template <typename VT>
+    Launch spmv_launch() const { return chunked_launch(m->g.N, 16 / (int)sizeof(VT)); }
+    Launch node_launch() const { return chunked_launch(m->g.N, m->g.bs == 1 ? 2 : 1); }
 
     // z = M^-1 r with `sweeps` Jacobi sweeps, into w->z (returns the buffer).
     double* jacobi(const double* r, int sweeps, const KrylovState* guard)
@@ -630,59 +749,77 @@ struct SolverImpl {
         return za;
     }
 
+    void sweeps_into_z(int sweeps, const KrylovState* guard)
+    {
+        double* z = jacobi(w->r, sweeps, guard);
+        if (z != w->z) RG_HIP(hipMemcpyAsync(w->z, z, sizeof(double) * w->n, hipMemcpyDeviceToDevice, st));
+    }
+
     // One iteration's launches of the PASO residual-smoothed PCG.
     void paso_iteration(int sweeps)
     {
         const Geom& g = m->g;
-        double* zb = (sweeps > 1) ? w->z : nullptr;
-        PcgArgs a = args(zb);
-        dispatch(g, m->symmetric, [&](auto D, auto B, auto S) {
-            hipLaunchKernelGGL((k_paso_p<B.value, double>), dim3(L.grid), dim3(kBlock), 0, st, a, m->dinv);
-            hipLaunchKernelGGL((k_paso_spmv<D.value, B.value, S.value, double>), dim3(L.grid), dim3(kBlock), 0, st,
-                               a, m->val);
-        });
-        PcgArgs ar = a;
+        const double* zb = (sweeps > 1) ? w->z : nullptr;
+        const Launch Lsp = spmv_launch<double>(), Ln = node_launch();
+        const PcgArgs an = args(zb, Ln), asp = args(zb, Lsp);
+        PcgArgs ar = args(zb, chunked_launch(g.N * g.bs, 2));
         ar.N = g.N * g.bs;
-        const Launch Ls = chunked_launch(ar.N);
-        ar.chunk = Ls.chunk;
-        hipLaunchKernelGGL(k_paso_r, dim3(Ls.grid), dim3(kBlock), 0, st, ar);
-        if (sweeps > 1) {
-            double* z = jacobi(w->r, sweeps, w->st);
-            if (z != w->z) RG_HIP(hipMemcpyAsync(w->z, z, sizeof(double) * w->n, hipMemcpyDeviceToDevice, st));
-        }
-        dispatch_bs(g.bs, [&](auto B) {
-            hipLaunchKernelGGL((k_paso_x<B.value, double>), dim3(L.grid), dim3(kBlock), 0, st, a, m->dinv);
+        const Launch Lr = chunked_launch(ar.N, 2);
+        dispatch(g, m->symmetric, [&](auto D, auto B, auto S) {
+            hipLaunchKernelGGL((k_p_update<B.value, double>), dim3(Ln.grid), dim3(kBlock), 0, st, an, m->dinv);
+            hipLaunchKernelGGL((k_spmv_dot<D.value, B.value, S.value, double, true>), dim3(Lsp.grid), dim3(kBlock), 0,
+                               st, asp, m->val);
         });
+        stage<ST_PASO_SPMV, 1>(Lsp.grid);
+        hipLaunchKernelGGL(k_paso_r, dim3(Lr.grid), dim3(kBlock), 0, st, ar);
+        stage<ST_PASO_R, 2>(Lr.grid);
+        if (sweeps > 1) sweeps_into_z(sweeps, w->st);
+        dispatch_bs(g.bs, [&](auto B) {
+            hipLaunchKernelGGL((k_paso_x<B.value, double>), dim3(Ln.grid), dim3(kBlock), 0, st, an, m->dinv);
+        });
+        stage<ST_PASO_X, 2>(Ln.grid);
         RG_HIP(hipGetLastError());
     }
 
-    // One iteration's launches of the textbook PCG (VT: value precision).
+    // One iteration's launches of the textbook PCG (VT: value precision, DT:
+    // Jacobi precision); with `replace` the iteration ends with a residual
+    // replacement r = D b - A x in FP64.
     template <typename VT, typename DT>
-    void cg_iteration(const VT* val, const DT* dinv, int sweeps)
+    void cg_iteration(const VT* val, const DT* dinv, int sweeps, bool replace)
     {
         const Geom& g = m->g;
-        double* zb = (sweeps > 1) ? w->z : nullptr;
-        PcgArgs a = args(zb);
+        const double* zb = (sweeps > 1) ? w->z : nullptr;
+        const Launch Lsp = spmv_launch<VT>(), Ln = node_launch(), L64 = spmv_launch<double>();
+        const PcgArgs an = args(zb, Ln), asp = args(zb, Lsp), a64 = args(nullptr, L64);
         dispatch(g, m->symmetric, [&](auto D, auto B, auto S) {
-            hipLaunchKernelGGL((k_cg_spmv<D.value, B.value, S.value, VT>), dim3(L.grid), dim3(kBlock), 0, st, a, val);
-            hipLaunchKernelGGL((k_cg_update<B.value, DT>), dim3(L.grid), dim3(kBlock), 0, st, a, dinv);
+            hipLaunchKernelGGL((k_spmv_dot<D.value, B.value, S.value, VT, false>), dim3(Lsp.grid), dim3(kBlock), 0,
+                               st, asp, val);
+            stage<ST_CG_SPMV, 1>(Lsp.grid);
+            hipLaunchKernelGGL((k_cg_update<B.value, DT>), dim3(Ln.grid), dim3(kBlock), 0, st, an, dinv);
+            if (sweeps > 1) stage<ST_CG_UPDATE_NORZ, 2>(Ln.grid);
+            else stage<ST_CG_UPDATE, 2>(Ln.grid);
+            if (replace) {
+                hipLaunchKernelGGL((k_cg_replace<D.value, B.value, S.value, DT>), dim3(L64.grid), dim3(kBlock), 0,
+                                   st, a64, m->val, dinv);
+                stage<ST_CG_REPLACE, 2>(L64.grid);
+            }
         });
         if (sweeps > 1) {
-            double* z = jacobi(w->r, sweeps, w->st);
-            if (z != w->z) RG_HIP(hipMemcpyAsync(w->z, z, sizeof(double) * w->n, hipMemcpyDeviceToDevice, st));
-            hipLaunchKernelGGL(k_cg_rz, dim3(L.grid), dim3(kBlock), 0, st, a, w->n);
+            sweeps_into_z(sweeps, w->st);
+            hipLaunchKernelGGL(k_cg_rz, dim3(L.grid), dim3(kBlock), 0, st, args(zb, L), w->n);
+            stage<ST_CG_RZ, 1>(L.grid);
         }
         dispatch_bs(g.bs, [&](auto B) {
-            hipLaunchKernelGGL((k_cg_p<B.value, DT>), dim3(L.grid), dim3(kBlock), 0, st, a, dinv);
+            hipLaunchKernelGGL((k_p_update<B.value, DT>), dim3(Ln.grid), dim3(kBlock), 0, st, an, dinv);
         });
         RG_HIP(hipGetLastError());
     }
 
-    void iteration(int method, int sweeps)
+    void iteration(int method, int sweeps, bool last)
     {
         if (method == RG_METHOD_PCG) paso_iteration(sweeps);
-        else if (method == RG_METHOD_PCG_PLAIN) cg_iteration<double, double>(m->val, m->dinv, sweeps);
-        else cg_iteration<float, float>(m->val32, m->dinv32, sweeps);
+        else if (method == RG_METHOD_PCG_PLAIN) cg_iteration<double, double>(m->val, m->dinv, sweeps, false);
+        else cg_iteration<float, float>(m->val32, m->dinv32, 1, last);
     }
 
     void init(int method, int sweeps)
@@ -690,18 +827,21 @@ struct SolverImpl {
         const Geom& g = m->g;
         double* zb = nullptr;
         if (sweeps > 1) {
-            double* z = jacobi(w->r, sweeps, nullptr);
-            if (z != w->z) RG_HIP(hipMemcpyAsync(w->z, z, sizeof(double) * w->n, hipMemcpyDeviceToDevice, st));
+            sweeps_into_z(sweeps, nullptr);
             zb = w->z;
         }
-        PcgArgs a = args(zb);
+        PcgArgs a = args(zb, L);
         dispatch_bs(g.bs, [&](auto B) {
-            if (method == RG_METHOD_PCG)
+            if (method == RG_METHOD_PCG) {
                 hipLaunchKernelGGL((k_paso_init<B.value, double>), dim3(L.grid), dim3(kBlock), 0, st, a, m->dinv);
-            else if (method == RG_METHOD_PCG_PLAIN)
+                stage<ST_PASO_INIT, 1>(L.grid);
+            } else if (method == RG_METHOD_PCG_PLAIN) {
                 hipLaunchKernelGGL((k_cg_init<B.value, double>), dim3(L.grid), dim3(kBlock), 0, st, a, m->dinv);
-            else
+                stage<ST_CG_INIT, 2>(L.grid);
+            } else {
                 hipLaunchKernelGGL((k_cg_init<B.value, float>), dim3(L.grid), dim3(kBlock), 0, st, a, m->dinv32);
+                stage<ST_CG_INIT, 2>(L.grid);
+            }
         });
         RG_HIP(hipGetLastError());
     }
@@ -710,7 +850,7 @@ struct SolverImpl {
     void chunk(int method, int sweeps, int iters, bool use_graph)
     {
         if (!use_graph) {
-            for (int k = 0; k < iters; ++k) iteration(method, sweeps);
+            for (int k = 0; k < iters; ++k) iteration(method, sweeps, k == iters - 1);
             return;
         }
         if (!w->graph || w->graph_method != method || w->graph_sweeps != sweeps || w->graph_iters != iters) {
@@ -724,7 +864,7 @@ struct SolverImpl {
             hipGraph_t graph;
             RG_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
             try {
-                for (int k = 0; k < iters; ++k) iteration(method, sweeps);
+                for (int k = 0; k < iters; ++k) iteration(method, sweeps, k == iters - 1);
             } catch (...) {
                 hipStreamEndCapture(st, &graph);
                 st = keep;
@@ -750,6 +890,7 @@ struct SolverImpl {
         std::memset(&h, 0, sizeof(h));
         h.maxit = maxit;
         h.tol = tol;
+        h.replace_mode = (method == RG_METHOD_PCG_MIXED) ? 1 : 0;
         KrylovState* pin = reinterpret_cast<KrylovState*>(m->ctx->pinned);
         *pin = h;
         RG_HIP(hipMemcpyAsync(w->st, pin, sizeof(KrylovState), hipMemcpyHostToDevice, st));
@@ -788,11 +929,11 @@ struct SolverImpl {
         *norm = pin->norm;
     }
 
-    void norms(const double* v, double* n2, double* nmax)
+    void norms(const double* v, double* n2, double* nmax, const double* d = nullptr)
     {
         const int grid = 1024;
-        hipLaunchKernelGGL(k_norms, dim3(grid), dim3(kBlock), 0, st, v, w->n, m->ctx->partials,
-                           m->ctx->counters + 1, w->norms);
+        hipLaunchKernelGGL(k_norms, dim3(grid), dim3(kBlock), 0, st, v, d, w->n, m->ctx->partials);
+        hipLaunchKernelGGL(k_norms_stage, dim3(1), dim3(kBlock), 0, st, m->ctx->partials, grid, w->norms);
         RG_HIP(hipGetLastError());
         RG_HIP(hipMemcpyAsync(m->ctx->pinned, w->norms, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
         RG_HIP(hipStreamSynchronize(st));
@@ -816,6 +957,7 @@ void ensure_work(rg_matrix* m)
         RG_HIP(hipMalloc(&w->x2, bytes));
         RG_HIP(hipMalloc(&w->z, bytes));
         RG_HIP(hipMalloc(&w->z2, bytes));
+        RG_HIP(hipMalloc(&w->bb, bytes));
         RG_HIP(hipMalloc(&w->norms, 4 * sizeof(double)));
         RG_HIP(hipMalloc(&w->st, sizeof(KrylovState)));
         RG_HIP(hipMalloc(&w->flag, sizeof(int)));
@@ -831,7 +973,7 @@ void free_work(rg_solver_work* w)
     if (!w) return;
     if (w->graph) hipGraphExecDestroy(w->graph);
     if (w->capture) hipStreamDestroy(w->capture);
-    double* bufs[] = {w->r, w->xw, w->rs, w->p, w->v, w->x2, w->z, w->z2, w->norms};
+    double* bufs[] = {w->r, w->xw, w->rs, w->p, w->v, w->x2, w->z, w->z2, w->bb, w->norms};
     for (double* b : bufs)
         if (b) hipFree(b);
     if (w->st) hipFree(w->st);
@@ -876,6 +1018,10 @@ void prepare_solve(rg_matrix* m)
 }
 
 // paso::Solver (paso/src/Solver.cpp:46-331) on device vectors.
+// With RG_CRITERION_TRUE_RESIDUAL the outer loop stops on the unbalanced
+// FP64 residual, ||b - A x||_2 <= rtol ||b||_2, instead of PASO's balanced
+// two-norm/max-norm pair; the inner tolerance is scaled by the observed
+// ratio of balanced to unbalanced residual norms.
 int solve(rg_matrix* m, double* x, const double* b, const rg_options& opt, rg_diagnostics& dg)
 {
     const double EPS = 2.220446049250313e-16;
@@ -887,6 +1033,9 @@ int solve(rg_matrix* m, double* x, const double* b, const rg_options& opt, rg_di
     RG_CHECK(tolerance <= 1., RG_VALUE_ERROR, "Solver: Tolerance must be less than one.");
     RG_CHECK(opt.method >= RG_METHOD_PCG && opt.method <= RG_METHOD_PCG_MIXED, RG_VALUE_ERROR,
              "Solver: unknown solver method.");
+    RG_CHECK(opt.criterion == RG_CRITERION_PASO || opt.criterion == RG_CRITERION_TRUE_RESIDUAL, RG_VALUE_ERROR,
+             "Solver: unknown stopping criterion.");
+    const bool true_res = opt.criterion == RG_CRITERION_TRUE_RESIDUAL;
     ensure_work(m);
     rg_solver_work* w = m->work;
     hipStream_t st = m->ctx->stream;
@@ -903,11 +1052,14 @@ int solve(rg_matrix* m, double* x, const double* b, const rg_options& opt, rg_di
         m->val32_valid = true;
     }
     // r = D b, norms of D b (Solver.cpp:104-130)
-    hipLaunchKernelGGL(k_mul, dim3(vgrid), dim3(kBlock), 0, st, w->r, b, m->balance, n);
+    hipLaunchKernelGGL(k_mul, dim3(vgrid), dim3(kBlock), 0, st, w->bb, b, m->balance, n);
+    RG_HIP(hipMemcpyAsync(w->r, w->bb, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
     double norm2_of_b, norm_max_of_b;
     S.norms(w->r, &norm2_of_b, &norm_max_of_b);
     RG_CHECK(!std::isnan(norm2_of_b) && !std::isnan(norm_max_of_b), RG_VALUE_ERROR,
              "Solver: Matrix or right hand side contains undefined values.");
+    double true2_of_b = 0.0, truemax_of_b = 0.0;
+    if (true_res) S.norms(b, &true2_of_b, &truemax_of_b);
     int status = RG_OK;
     const int check = opt.check_interval > 0 ? opt.check_interval : 32;
     const bool use_graph = true;
@@ -929,10 +1081,12 @@ int solve(rg_matrix* m, double* x, const double* b, const rg_options& opt, rg_di
         while (!finalize) {
             const int cntIter = opt.iter_max - totIter;
             finalize = true;
-            if (totIter > 1) hipLaunchKernelGGL(k_mul, dim3(vgrid), dim3(kBlock), 0, st, w->r, b, m->balance, n);
+            if (totIter > 1) RG_HIP(hipMemcpyAsync(w->r, w->bb, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
             launch_spmv(m, -1.0, w->xw, 1.0, w->r);
             double norm2, normmax;
             S.norms(w->r, &norm2, &normmax);
+            double true2 = 0.0, truemax = 0.0;
+            if (true_res) S.norms(w->r, &true2, &truemax, m->balance);
             dg.residual_norm = norm2;
             dg.outer_steps++;
             if (opt.verbose)
@@ -940,26 +1094,17 @@ int solve(rg_matrix* m, double* x, const double* b, const rg_options& opt, rg_di
             if (totIter > 1 && norm2 >= last2 && normmax >= lastmax) {
                 throw Error(RG_DIVERGENCE, "Solver: No improvement during iteration. Iterative solver gives up.");
             }
-            if (norm2 > tolerance * norm2_of_b || normmax > tolerance * norm_max_of_b) {
+            const bool more = true_res ? (true2 > tolerance * true2_of_b)
+                                       : (norm2 > tolerance * norm2_of_b || normmax > tolerance * norm_max_of_b);
+            if (more) {
                 double tol = tolerance * std::min(norm2_of_b, 0.1 * norm2 / normmax * norm_max_of_b);
-                if (opt.method == RG_METHOD_PCG_MIXED) {
-                    const double itol = opt.inner_tolerance > 0 ? opt.inner_tolerance : 1e-4;
-                    tol = std::max(tol, itol * norm2);
-                }
+                if (true_res) tol = 0.5 * tolerance * true2_of_b * (norm2 / std::max(true2, 1e-300));
                 if (opt.verbose) printf("Solver: new tolerance = %g\n", tol);
                 last2 = norm2;
                 lastmax = normmax;
                 int ks = 0, it = 0;
                 double kn = 0;
-                if (opt.method == RG_METHOD_PCG_MIXED) {
-                    // inner correction solve A32 d = r from d = 0; then x += d
-                    RG_HIP(hipMemcpyAsync(w->x2, w->xw, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
-                    hipLaunchKernelGGL(k_fill, dim3(vgrid), dim3(kBlock), 0, st, w->xw, 0.0, n);
-                    S.krylov(opt.method, std::max(1, opt.sweeps), cntIter, tol, check, use_graph, &ks, &it, &kn);
-                    hipLaunchKernelGGL(k_axpy, dim3(vgrid), dim3(kBlock), 0, st, w->xw, 1.0, w->x2, n);
-                } else {
-                    S.krylov(opt.method, std::max(1, opt.sweeps), cntIter, tol, check, use_graph, &ks, &it, &kn);
-                }
+                S.krylov(opt.method, std::max(1, opt.sweeps), cntIter, tol, check, use_graph, &ks, &it, &kn);
                 RG_HIP(hipGetLastError());
                 totIter += it;
                 if (ks == KS_CONVERGED) {

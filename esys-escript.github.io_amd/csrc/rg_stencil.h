@@ -1,4 +1,12 @@
 // The diagonal-offset stencil row product shared by every SpMV-shaped kernel.
+//
+// Rows are processed R at a time per thread (R = 16 bytes / value size, so
+// one dwordx4 load fetches a slot's values for all R rows), and x is read
+// through aligned 16-byte windows: for each (dy, dz) pair the R+2 values
+// x[b-1 .. b+R] serve all 3*R products of the three dx slots.  Every load of
+// a row group is issued before the first multiply; there is no per-slot
+// branch.  Blocks whose rows lie within one stencil reach of either end of
+// the vector take a checked, branch-free path (clamped indices + select).
 #pragma once
 
 #include "rg_device.h"
@@ -11,49 +19,6 @@ struct StencilShape {
     static constexpr int CENTER = NS / 2;            // slot of the main diagonal
 };
 
-// acc[r] += sum over slots s of block A(i, i+off_s) x[(i+off_s)*BS + :]
-// in ascending column order (the order of PASO's CSR row,
-// paso/src/SparseMatrix_MatrixVector.cpp:262-320).  Columns outside [0, N)
-// are skipped, as the reference stencil SpMV does
-// (cusplibrary/cusp/detail/host/spmv.h:240-437).  With SYM only the main and
-// upper slots are stored; slot s < CENTER of row i is the transpose of the
-// stored slot CENTER-s... of row i+off_s.
-template <int DIM, int BS, bool SYM, typename VT, typename XT>
-__device__ __forceinline__ void stencil_row(const VT* __restrict__ val, int64_t ld, int64_t N,
-                                            int64_t s1, int64_t s2, int64_t i,
-                                            const XT* __restrict__ x, double (&acc)[BS])
-{
-    constexpr int NS = StencilShape<DIM>::NS;
-    constexpr int C = StencilShape<DIM>::CENTER;
-#pragma unroll
-    for (int s = 0; s < NS; ++s) {
-        int dx, dy, dz;
-        slot_delta<DIM>(s, dx, dy, dz);
-        const int64_t j = i + dx + dy * s1 + dz * s2;
-        if ((uint64_t)j < (uint64_t)N) {
-            double xv[BS];
-#pragma unroll
-            for (int c = 0; c < BS; ++c) xv[c] = (double)x[j * BS + c];
-            if (!SYM || s >= C) {
-                const int t = SYM ? s - C : s;
-#pragma unroll
-                for (int c = 0; c < BS; ++c)
-#pragma unroll
-                    for (int r = 0; r < BS; ++r)
-                        acc[r] = fma((double)val[(int64_t)(t * BS * BS + r + BS * c) * ld + i], xv[c], acc[r]);
-            } else {
-                const int t = C - s;   // stored slot of the mirrored offset
-#pragma unroll
-                for (int c = 0; c < BS; ++c)
-#pragma unroll
-                    for (int r = 0; r < BS; ++r)
-                        acc[r] = fma((double)val[(int64_t)(t * BS * BS + c + BS * r) * ld + j], xv[c], acc[r]);
-            }
-        }
-    }
-}
-
-// Number of stored slots.
 template <int DIM, bool SYM>
 __host__ __device__ constexpr int stored_slots()
 {
@@ -71,6 +36,175 @@ __device__ __forceinline__ bool slot_in_grid(int s, int64_t x, int64_t y, int64_
     bool ok = (xx >= 0 && xx < n0 && yy >= 0 && yy < n1);
     if (DIM == 3) ok = ok && (zz >= 0 && zz < n2);
     return ok;
+}
+
+template <typename T>
+__device__ __forceinline__ T ldg(const T* p)
+{
+    return __builtin_nontemporal_load(p);
+}
+
+// acc[r] += sum over slots s of block A(i, i+off_s) x[(i+off_s)*BS + :] in
+// ascending column order (the order of PASO's CSR row,
+// paso/src/SparseMatrix_MatrixVector.cpp:262-320).  Columns outside [0, N)
+// contribute nothing, as in the reference stencil SpMV
+// (cusplibrary/cusp/detail/host/spmv.h:240-437); with CHECK the indices are
+// clamped and the products selected away (branch free), without CHECK the
+// caller guarantees all columns are in range.  With SYM only the main and
+// upper slots are stored; slot s < CENTER of row i is the transpose of
+// stored slot CENTER-s of row i+off_s.
+template <int DIM, int BS, bool SYM, bool CHECK, typename VT, typename XT>
+__device__ __forceinline__ void stencil_row(const VT* __restrict__ val, int64_t ld, int64_t N,
+                                            int64_t s1, int64_t s2, int64_t i,
+                                            const XT* __restrict__ x, double (&acc)[BS])
+{
+    constexpr int NS = StencilShape<DIM>::NS;
+    constexpr int C = StencilShape<DIM>::CENTER;
+    double xv[NS][BS];
+    double av[NS][BS * BS];
+    bool ok[NS];
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+        int dx, dy, dz;
+        slot_delta<DIM>(s, dx, dy, dz);
+        int64_t j = i + dx + dy * s1 + dz * s2;
+        ok[s] = true;
+        if (CHECK) {
+            ok[s] = (uint64_t)j < (uint64_t)N;
+            j = ok[s] ? j : i;
+        }
+#pragma unroll
+        for (int c = 0; c < BS; ++c) xv[s][c] = (double)x[j * BS + c];
+        if (!SYM || s >= C) {
+            const int t = SYM ? s - C : s;
+#pragma unroll
+            for (int e = 0; e < BS * BS; ++e) av[s][e] = (double)ldg(&val[(int64_t)(t * BS * BS + e) * ld + i]);
+        } else {
+            const int t = C - s;
+#pragma unroll
+            for (int r = 0; r < BS; ++r)
+#pragma unroll
+                for (int c = 0; c < BS; ++c)   // transposed block of row j
+                    av[s][r + BS * c] = (double)val[(int64_t)(t * BS * BS + c + BS * r) * ld + j];
+        }
+    }
+#pragma unroll
+    for (int s = 0; s < NS; ++s)
+#pragma unroll
+        for (int c = 0; c < BS; ++c)
+#pragma unroll
+            for (int r = 0; r < BS; ++r) {
+                const double t = fma(av[s][r + BS * c], xv[s][c], acc[r]);
+                acc[r] = (!CHECK || ok[s]) ? t : acc[r];
+            }
+}
+
+// x[b-1+k], k = 0..R+1, from aligned 16-byte loads starting at the even
+// address at or below b-1 (R/2 + 2 loads; no branch: the parity of b is the
+// same for the whole wave and only selects which loaded values are used).
+template <int R>
+__device__ __forceinline__ void x_window(const double* __restrict__ x, int64_t b, double (&w)[R + 2])
+{
+    const int64_t base = (b - 1) & ~(int64_t)1;
+    const bool odd = (b - 1) != base;
+    const double2* p = reinterpret_cast<const double2*>(x + base);
+    double t[R + 4];
+#pragma unroll
+    for (int k = 0; k < (R + 4) / 2; ++k) {
+        const double2 v = p[k];
+        t[2 * k] = v.x;
+        t[2 * k + 1] = v.y;
+    }
+#pragma unroll
+    for (int k = 0; k < R + 2; ++k) w[k] = odd ? t[k + 1] : t[k];
+}
+
+typedef double rg_dbl2 __attribute__((ext_vector_type(2)));
+typedef float rg_flt4 __attribute__((ext_vector_type(4)));
+
+template <typename VT, int R>
+struct VecOf;
+template <>
+struct VecOf<double, 2> {
+    using T = rg_dbl2;
+};
+template <>
+struct VecOf<float, 4> {
+    using T = rg_flt4;
+};
+
+template <typename V>
+__device__ __forceinline__ double vget(const V& v, int k)
+{
+    return (double)v[k];
+}
+
+// Scalar (bs = 1) full-storage stencil for R consecutive rows i0..i0+R-1 of
+// an interior row group (all columns in range, x 16-byte aligned, i0 % R == 0).
+template <int DIM, typename VT, int R>
+__device__ __forceinline__ void stencil_rows_s1(const VT* __restrict__ val, int64_t ld, int64_t s1, int64_t s2,
+                                                int64_t i0, const double* __restrict__ x, double (&acc)[R])
+{
+    using V = typename VecOf<VT, R>::T;
+    constexpr int NPL = (DIM == 3) ? 9 : 3;   // (dy, dz) pairs
+    V av[NPL][3];
+    double w[NPL][R + 2];
+#pragma unroll
+    for (int pl = 0; pl < NPL; ++pl) {
+        const int dy = pl % 3 - 1, dz = (DIM == 3) ? pl / 3 - 1 : 0;
+        const int64_t b = i0 + dy * s1 + dz * s2;
+        x_window<R>(x, b, w[pl]);
+#pragma unroll
+        for (int dx = 0; dx < 3; ++dx) {
+            const int s = dx + 3 * pl;
+            av[pl][dx] = ldg(reinterpret_cast<const V*>(val + (int64_t)s * ld + i0));
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) acc[r] = 0.0;
+#pragma unroll
+    for (int pl = 0; pl < NPL; ++pl)
+#pragma unroll
+        for (int dx = 0; dx < 3; ++dx)
+#pragma unroll
+            for (int r = 0; r < R; ++r) acc[r] = fma(vget(av[pl][dx], r), w[pl][r + dx], acc[r]);
+}
+
+// Rows of one block of a chunked SpMV-shaped kernel.  Calls epi(i, acc) for
+// every row i of [r0, r1) (acc: BS values of row i).  For bs = 1 with full
+// storage, FP64/FP32 values and an aligned x, interior row groups take the
+// vectorised path.
+template <int DIM, int BS, bool SYM, typename VT, class Epi>
+__device__ __forceinline__ void stencil_block(const VT* __restrict__ val, int64_t ld, int64_t N, int64_t s1,
+                                              int64_t s2, int64_t r0, int64_t r1, const double* __restrict__ x,
+                                              Epi&& epi)
+{
+    constexpr int R = 16 / (int)sizeof(VT);
+    const int64_t reach = s2 * (DIM == 3) + s1 + R + 2;
+    const bool fast = (BS == 1) && !SYM && ((reinterpret_cast<uintptr_t>(x) & 15) == 0) &&
+                      (r0 % R == 0) && ((r1 - r0) % R == 0) && (r0 >= reach) && (r1 + reach <= N);
+    if (fast) {
+        for (int64_t i0 = r0 + (int64_t)threadIdx.x * R; i0 < r1; i0 += (int64_t)kBlock * R) {
+            double acc[R];
+            stencil_rows_s1<DIM, VT, R>(val, ld, s1, s2, i0, x, acc);
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                double a1[BS];
+                a1[0] = acc[r];
+                epi(i0 + r, a1);
+            }
+        }
+    } else {
+        const bool interior = (r0 >= reach) && (r1 + reach <= N);
+        for (int64_t i = r0 + threadIdx.x; i < r1; i += kBlock) {
+            double acc[BS];
+#pragma unroll
+            for (int r = 0; r < BS; ++r) acc[r] = 0.0;
+            if (interior) stencil_row<DIM, BS, SYM, false>(val, ld, N, s1, s2, i, x, acc);
+            else stencil_row<DIM, BS, SYM, true>(val, ld, N, s1, s2, i, x, acc);
+            epi(i, acc);
+        }
+    }
 }
 
 }  // namespace rg

@@ -23,6 +23,7 @@ RG_DEVICE_ERROR = 22
 RG_MATRIX_SYMMETRIC = 1
 RG_COEF_EMPTY, RG_COEF_ELEMENT, RG_COEF_EXPANDED = 0, 1, 2
 RG_METHOD_PCG, RG_METHOD_PCG_PLAIN, RG_METHOD_PCG_MIXED = 0, 1, 2
+RG_CRITERION_PASO, RG_CRITERION_TRUE_RESIDUAL = 0, 1
 
 # every function the header declares (checked by the CPU test suite)
 EXPORTS = [
@@ -49,7 +50,8 @@ class RgOptions(ctypes.Structure):
     _fields_ = [("tolerance", ctypes.c_double), ("inner_tolerance", ctypes.c_double),
                 ("iter_max", ctypes.c_int32), ("sweeps", ctypes.c_int32), ("method", ctypes.c_int32),
                 ("verbose", ctypes.c_int32), ("accept_failed_convergence", ctypes.c_int32),
-                ("check_interval", ctypes.c_int32)]
+                ("check_interval", ctypes.c_int32), ("criterion", ctypes.c_int32),
+                ("reserved", ctypes.c_int32)]
 
 
 class RgDiagnostics(ctypes.Structure):

@@ -101,6 +101,7 @@ class SolverOptions:
         self._accept = False
         self._inner_tolerance = 1e-4
         self._check_interval = 0
+        self._true_residual = False
         self.resetDiagnostics(all=True)
 
     # setters / getters named as SolverBuddy's python binding
@@ -192,6 +193,11 @@ class SolverOptions:
     def setCheckInterval(self, iters=0):
         self._check_interval = int(iters)
 
+    def setTrueResidualCriterion(self, flag=True):
+        """Stop on the FP64 unbalanced ||b - A x||_2 <= tol ||b||_2 instead of
+        PASO's balanced two-norm / max-norm test."""
+        self._true_residual = bool(flag)
+
     # diagnostics (paso/src/Options.cpp:441-459)
     _DIAG_KEYS = ("num_iter", "num_level", "num_inner_iter", "time", "set_up_time", "net_time",
                   "residual_norm", "converged", "cum_num_iter", "cum_time", "cum_set_up_time",
@@ -237,6 +243,7 @@ class SolverOptions:
         o.verbose = int(self._verbose)
         o.accept_failed_convergence = int(self._accept)
         o.check_interval = self._check_interval
+        o.criterion = _lib.RG_CRITERION_TRUE_RESIDUAL if self._true_residual else _lib.RG_CRITERION_PASO
         return o
 
 

@@ -166,19 +166,28 @@ int rg_matrix_save_mm(rg_matrix* m, const char* filename);
  * 40-72).  method:
  *   RG_METHOD_PCG        PASO Solver() + Solver_PCG (residual-smoothed PCG)
  *   RG_METHOD_PCG_PLAIN  same outer loop, textbook Jacobi-PCG inside
- *   RG_METHOD_PCG_MIXED  FP64 outer loop (true residual, norms), inner
- *                        Jacobi-PCG with FP32 matrix values and FP32 SpMV,
- *                        FP64 dot products and solution update  */
+ *   RG_METHOD_PCG_MIXED  same outer loop; inner Jacobi-PCG whose SpMV and
+ *                        preconditioner use FP32 matrix values while dot
+ *                        products, scalars and vector updates stay FP64; every
+ *                        check_interval iterations the residual is replaced by
+ *                        the FP64 true residual D b - A x, and convergence is
+ *                        only accepted on such a replaced residual
+ * criterion:
+ *   RG_CRITERION_PASO           balanced ||r||_2 and ||r||_inf tests (Solver.cpp:241)
+ *   RG_CRITERION_TRUE_RESIDUAL  unbalanced FP64 ||b - A x||_2 <= tolerance ||b||_2 */
 enum { RG_METHOD_PCG = 0, RG_METHOD_PCG_PLAIN = 1, RG_METHOD_PCG_MIXED = 2 };
+enum { RG_CRITERION_PASO = 0, RG_CRITERION_TRUE_RESIDUAL = 1 };
 typedef struct {
     double tolerance;            /* relative; default 1e-8 */
-    double inner_tolerance;      /* RG_METHOD_PCG_MIXED inner relative reduction (default 1e-4) */
+    double inner_tolerance;      /* reserved (default 1e-4) */
     int32_t iter_max;            /* default 100000 */
     int32_t sweeps;              /* Jacobi sweeps, default 1 */
     int32_t method;
     int32_t verbose;
     int32_t accept_failed_convergence;
-    int32_t check_interval;      /* iterations queued between host convergence checks (0: auto) */
+    int32_t check_interval;      /* iterations per replayed graph / host check (0: 32) */
+    int32_t criterion;
+    int32_t reserved;
 } rg_options;
 
 /* Diagnostics keys of paso::Options::updateEscriptDiagnostics

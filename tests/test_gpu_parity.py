@@ -223,6 +223,21 @@ def test_solve_mixed_precision_reaches_fp64_tolerance():
     assert np.linalg.norm(x - xo) <= 1e-6 * np.linalg.norm(xo)
 
 
+@pytest.mark.parametrize("method", ["PCG", "PCG_PLAIN", "PCG_MIXED"])
+def test_true_residual_criterion(method):
+    # stop on the FP64 unbalanced ||b - A x||_2 <= tol ||b||_2
+    op, rhs, ptr, idx, oval, orhs = _poisson_problem(3, (14, 12, 16), sigma=2.0)
+    opts = R.SolverOptions()
+    opts.setSymmetryOn()
+    opts.setSolverMethod(getattr(R.SolverOptions, method))
+    opts.setTolerance(1e-9)
+    opts.setTrueResidualCriterion(True)
+    x = op.solve(rhs, opts).cpu().numpy()
+    assert opts.hasConverged()
+    res = np.linalg.norm(orhs - O.spmv(1, ptr, idx, oval, x)) / np.linalg.norm(orhs)
+    assert res <= 1e-9 * 1.001
+
+
 def test_solve_elasticity_parity():
     rng = np.random.default_rng(8)
     dom, length = grid(3, (6, 5, 6))
