@@ -34,6 +34,9 @@ def parse():
     ap.add_argument("--cpu-iters", type=int, default=40)
     ap.add_argument("--spmv-reps", type=int, default=50)
     ap.add_argument("--quiet", action="store_true")
+    ap.add_argument("--probe", action="store_true",
+                    help="only time the FP64 SpMV (full and symmetric storage) and a 1 GiB device copy; "
+                         "used for the rocprofv3 counter runs")
     return ap.parse_args()
 
 
@@ -150,6 +153,37 @@ def spmv_roofline(P, reps, torch):
     return dt, alg_bytes
 
 
+def probe(args, R, torch, np, fields):
+    """Standalone kernels for counter collection: 20 FP64 SpMVs with full
+    storage, 20 with symmetric storage, 5 copies of 1 GiB."""
+    P = Problem(args.ne, R, torch, np, fields)
+    out = {}
+    for sym in (False, True):
+        op = P.dom.newOperator(1, symmetric=sym)
+        rhs = torch.zeros(op.size, dtype=torch.float64, device="cuda")
+        op.assemble({"A": P.cA, "Y": P.cY}, rhs)
+        N = op.num_rows
+        x = torch.rand(N, dtype=torch.float64, device="cuda")
+        y = torch.zeros(N, dtype=torch.float64, device="cuda")
+        for _ in range(3):
+            op.MatrixVector(1.0, x, 0.0, y)
+        s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        s.record()
+        for _ in range(20):
+            op.MatrixVector(1.0, x, 0.0, y)
+        e.record()
+        torch.cuda.synchronize()
+        dt = s.elapsed_time(e) / 1000.0 / 20
+        stored = op.num_slots * op.num_rows * 8
+        out["symmetric" if sym else "full"] = {"avg_launch_s": dt, "stored_value_bytes": stored,
+                                                "alg_bytes": 8 * op.nnz + 16 * N,
+                                                "GBps_alg": (8 * op.nnz + 16 * N) / dt / 1e9}
+        del op
+    out["copy_GBps"] = copy_bandwidth(torch)
+    print(json.dumps(out), flush=True)
+
+
 def copy_bandwidth(torch):
     n = 1 << 27   # 1 GiB of float64
     a = torch.ones(n, dtype=torch.float64, device="cuda")
@@ -211,6 +245,9 @@ def main():
     import ripley_hip as R
     from ripley_hip import fields
 
+    if args.probe:
+        probe(args, R, torch, np, fields)
+        return
     log(args, "building C2 problem ne=%d" % args.ne)
     P = Problem(args.ne, R, torch, np, fields)
     methods = ["plain", "paso", "mixed"] if args.method == "all" else [args.method]

@@ -60,6 +60,10 @@ void check_matrix(const rg_matrix* m)
 
 }  // namespace
 
+namespace rg {
+void set_last_error(const std::string& msg) { g_error = msg; }
+}  // namespace rg
+
 extern "C" {
 
 const char* rg_last_error(void) { return g_error.c_str(); }
@@ -477,6 +481,28 @@ int rg_solve(rg_matrix* m, double* x, const double* b, const rg_options* opt, rg
     if (diag) *diag = dg;
     if (res >= RG_VALUE_ERROR) return res;
     return map_solver_result(res, o);
+}
+
+int rg_matrix_set_learned_preconditioner(rg_matrix* m, rg_lpc* net, const float* logk)
+{
+    return guarded([&] {
+        check_matrix(m);
+        if (!net) {
+            m->lpc = nullptr;
+            m->lpc_logk = nullptr;
+            return RG_OK;
+        }
+        RG_CHECK(m->g.dim == 3 && m->g.bs == 1, RG_VALUE_ERROR,
+                 "set_learned_preconditioner: scalar problems on a Brick only");
+        int32_t nodes[3];
+        rg::lpc_nodes(net, nodes);
+        for (int d = 0; d < 3; ++d)
+            RG_CHECK(nodes[d] == m->g.nn[d], RG_VALUE_ERROR,
+                     "set_learned_preconditioner: network node grid does not match the matrix");
+        m->lpc = net;
+        m->lpc_logk = logk;
+        return RG_OK;
+    });
 }
 
 int rg_solve_host(rg_matrix* m, double* x, const double* b, const rg_options* opt, rg_diagnostics* diag)

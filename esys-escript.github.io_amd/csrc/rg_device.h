@@ -94,8 +94,18 @@ __device__ __forceinline__ void reduce_partials(const double* partials, int G, d
 {
 #pragma unroll
     for (int k = 0; k < NV; ++k) {
+        const double* p = partials + (int64_t)k * G;
         double s = 0.0;
-        for (int b = threadIdx.x; b < G; b += kBlock) s += partials[k * G + b];
+        int b = threadIdx.x;
+        // batches of independent loads, summed in index order
+        for (; b + 15 * kBlock < G; b += 16 * kBlock) {
+            double v[16];
+#pragma unroll
+            for (int u = 0; u < 16; ++u) v[u] = p[b + u * kBlock];
+#pragma unroll
+            for (int u = 0; u < 16; ++u) s += v[u];
+        }
+        for (; b < G; b += kBlock) s += p[b];
         tot[k] = s;
     }
     block_sum<NV>(tot);

@@ -74,7 +74,7 @@ struct KrylovState {
     int32_t replace_mode;   // 1: convergence is only declared on a replaced (true) residual
     int32_t pad[3];
 };
-enum { KS_RUNNING = 0, KS_CONVERGED = 1, KS_MAXITER = 2, KS_BREAKDOWN = 3 };
+enum { KS_RUNNING = 0, KS_CONVERGED = 1, KS_MAXITER = 2, KS_BREAKDOWN = 3, KS_STAGNATED = 4 };
 
 }  // namespace rg
 
@@ -116,6 +116,8 @@ struct rg_matrix {
     bool is_balanced = false;
     bool have_prec = false;
     rg_solver_work* work = nullptr;
+    rg_lpc* lpc = nullptr;         // learned preconditioner (borrowed)
+    const float* lpc_logk = nullptr;   // log10(kappa) per node (device, borrowed)
 };
 
 namespace rg {
@@ -132,6 +134,12 @@ void launch_nullify(rg_matrix* m, const double* row_mask, const double* col_mask
                     int* d_flag);
 void launch_export_csr_values(rg_matrix* m, const int32_t* d_ptr, int32_t* d_index, double* d_val);
 void launch_to_fp32(rg_matrix* m);
+
+// Learned preconditioner (rg_learned.hip): node grid of a network and its
+// forward pass on the balanced solver vectors, z = D^-1 Net(D dinv r).
+void lpc_nodes(const rg_lpc* net, int32_t nodes[3]);
+void lpc_forward_balanced(rg_lpc* net, const double* r, const double* dinv, const double* bal, const float* logk,
+                          double* z, hipStream_t st);
 
 // Host-side pattern helpers.
 void csr_row_ptr(const Geom& g, int32_t* ptr);   // ptr[N+1] on host

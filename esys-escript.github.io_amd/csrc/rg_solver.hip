@@ -578,19 +578,80 @@ __global__ __launch_bounds__(kBlock) void k_cg_replace(PcgArgs a, const double* 
     publish_block_sums<2>(s, a.partials);
 }
 
+// ------------------------------------------------------------ flexible CG
+// Truncated flexible CG (one previous direction) with a variable
+// preconditioner z = M(r) (the learned network):
+//   beta = -(z.q_prev)/(p_prev.q_prev)                 k_fcg_zq      -> ST_FCG_BETA
+//   p = z + beta p                                     k_p_update
+//   q = A p, alpha = (p.r)/(p.q)                       k_fcg_spmv    -> ST_FCG_ALPHA
+//   x += alpha p, r -= alpha q, ||r||                  k_fcg_update  -> ST_FCG_UPDATE
+__global__ __launch_bounds__(kBlock) void k_fcg_zq(PcgArgs a, int64_t n)
+{
+    if (a.st->done) return;
+    const double* __restrict__ z = a.z;
+    const double* __restrict__ q = a.v;
+    double s[1] = {0.0};
+    for (int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x; k < n; k += (int64_t)gridDim.x * kBlock)
+        s[0] += z[k] * q[k];
+    publish_block_sums<1>(s, a.partials);
+}
+
+template <int DIM, int BS, bool SYM, typename VT>
+__global__ __launch_bounds__(kBlock) void k_fcg_spmv(PcgArgs a, const VT* __restrict__ val)
+{
+    if (a.st->done) return;
+    const RowRange rr = block_rows(a.N, a.chunk);
+    const double* __restrict__ p = a.p;
+    const double* __restrict__ r = a.r;
+    double* __restrict__ q = a.v;
+    double s[2] = {0.0, 0.0};
+    stencil_block<DIM, BS, SYM, VT>(val, a.ld, a.N, a.s1, a.s2, rr.r0, rr.r1, p,
+                                    [&](int64_t i, const double (&acc)[BS]) {
+#pragma unroll
+                                        for (int c = 0; c < BS; ++c) {
+                                            q[i * BS + c] = acc[c];
+                                            s[0] = fma(acc[c], p[i * BS + c], s[0]);
+                                            s[1] = fma(r[i * BS + c], p[i * BS + c], s[1]);
+                                        }
+                                    });
+    publish_block_sums<2>(s, a.partials);
+}
+
+// a.N counts scalar entries here
+__global__ __launch_bounds__(kBlock) void k_fcg_update(PcgArgs a)
+{
+    if (a.st->done) return;
+    const double alpha = a.st->alpha;
+    const RowRange rr = block_rows(a.N, a.chunk);
+    double* __restrict__ x = a.x;
+    double* __restrict__ r = a.r;
+    const double* __restrict__ p = a.p;
+    const double* __restrict__ q = a.v;
+    double s[1] = {0.0};
+    node_loop<1>(rr, [&](int64_t i, bool two) {
+        const Two<double> xv = ld2(x + i, two), pv = ld2(p + i, two), rv = ld2(r + i, two), qv = ld2(q + i, two);
+        st2(x + i, {fma(alpha, pv.a, xv.a), fma(alpha, pv.b, xv.b)}, two);
+        const Two<double> rn = {fma(-alpha, qv.a, rv.a), fma(-alpha, qv.b, rv.b)};
+        st2(r + i, rn, two);
+        s[0] += rn.a * rn.a + (two ? rn.b * rn.b : 0.0);
+    });
+    publish_block_sums<1>(s, a.partials);
+}
+
 // ------------------------------------------------------------ stage kernels
 // One workgroup after each reducing kernel: reduces its partials in a fixed
 // order and advances the Krylov scalars (the steps of PCG.cpp:156-347 that
 // follow each reduction).
 enum Stage {
     ST_PASO_INIT, ST_PASO_SPMV, ST_PASO_R, ST_PASO_X,
-    ST_CG_INIT, ST_CG_SPMV, ST_CG_UPDATE, ST_CG_UPDATE_NORZ, ST_CG_RZ, ST_CG_REPLACE
+    ST_CG_INIT, ST_CG_SPMV, ST_CG_UPDATE, ST_CG_UPDATE_NORZ, ST_CG_RZ, ST_CG_REPLACE,
+    ST_FCG_INIT, ST_FCG_BETA, ST_FCG_ALPHA, ST_FCG_UPDATE, ST_FCG_REPLACE
 };
 
 template <int STAGE, int NV>
 __global__ __launch_bounds__(kBlock) void k_stage(KrylovState* st, const double* __restrict__ partials, int G)
 {
-    if (STAGE != ST_PASO_INIT && STAGE != ST_CG_INIT && st->done) return;
+    if (STAGE != ST_PASO_INIT && STAGE != ST_CG_INIT && STAGE != ST_FCG_INIT && st->done) return;
     double tot[NV];
     reduce_partials<NV>(partials, G, tot);
     if (threadIdx.x != 0) return;
@@ -650,6 +711,27 @@ __global__ __launch_bounds__(kBlock) void k_stage(KrylovState* st, const double*
         if (st->norm <= st->tol) { st->done = 1; st->status = KS_CONVERGED; }
         else if (fabs(tot[0]) <= 0.0) { st->done = 1; st->status = KS_BREAKDOWN; }
         else { st->beta = tot[0] / st->rho_prev; st->rho = tot[0]; }
+    } else if (STAGE == ST_FCG_INIT) {
+        st->norm = sqrt(tot[1]);
+        st->num_iter = 0;
+        st->done = 0;
+        st->status = KS_RUNNING;
+        st->delta = 1.0;
+        if (st->norm <= st->tol) { st->done = 1; st->status = KS_CONVERGED; }
+        else if (st->maxit <= 0) { st->done = 1; st->status = KS_MAXITER; }
+    } else if (STAGE == ST_FCG_BETA) {
+        st->beta = (st->num_iter == 0) ? 0.0 : -tot[0] / st->delta;
+    } else if (STAGE == ST_FCG_ALPHA) {
+        if (fabs(tot[0]) <= 0.0) { st->done = 1; st->status = KS_BREAKDOWN; }
+        else { st->alpha = tot[1] / tot[0]; st->delta = tot[0]; }
+    } else if (STAGE == ST_FCG_UPDATE) {
+        st->num_iter += 1;
+        st->norm = sqrt(tot[0]);
+        if (st->norm <= st->tol && !st->replace_mode) { st->done = 1; st->status = KS_CONVERGED; }
+        else if (st->num_iter >= st->maxit) { st->done = 1; st->status = KS_MAXITER; }
+    } else if (STAGE == ST_FCG_REPLACE) {
+        st->norm = sqrt(tot[1]);
+        if (st->norm <= st->tol) { st->done = 1; st->status = KS_CONVERGED; }
     }
 }
 
@@ -815,9 +897,46 @@ struct SolverImpl {
         RG_HIP(hipGetLastError());
     }
 
+    // One iteration of the learned-preconditioner flexible CG (FP32 SpMV,
+    // FP64 vectors and scalars; `replace` ends it with an FP64 residual
+    // replacement).
+    void fcg_iteration(bool replace)
+    {
+        const Geom& g = m->g;
+        RG_CHECK(m->lpc != nullptr, RG_STATE_ERROR, "FCG: no learned preconditioner attached");
+        lpc_forward_balanced(m->lpc, w->r, m->dinv, m->balance, m->lpc_logk, w->z, st);
+        const Launch Lsp = spmv_launch<float>(), L64 = spmv_launch<double>();
+        const Launch Ln = node_launch();
+        PcgArgs an = args(w->z, Ln);
+        const PcgArgs asp = args(nullptr, Lsp), a64 = args(nullptr, L64);
+        PcgArgs ar = args(nullptr, chunked_launch(g.N * g.bs, 2));
+        ar.N = g.N * g.bs;
+        const Launch Lr = chunked_launch(ar.N, 2);
+        const int zg = 1024;
+        hipLaunchKernelGGL(k_fcg_zq, dim3(zg), dim3(kBlock), 0, st, an, w->n);
+        stage<ST_FCG_BETA, 1>(zg);
+        hipLaunchKernelGGL((k_p_update<1, double>), dim3(Ln.grid), dim3(kBlock), 0, st, an, m->dinv);
+        dispatch(g, m->symmetric, [&](auto D, auto B, auto S) {
+            if constexpr (B.value == 1) {
+                hipLaunchKernelGGL((k_fcg_spmv<D.value, 1, S.value, float>), dim3(Lsp.grid), dim3(kBlock), 0, st, asp,
+                                   m->val32);
+                stage<ST_FCG_ALPHA, 2>(Lsp.grid);
+                hipLaunchKernelGGL(k_fcg_update, dim3(Lr.grid), dim3(kBlock), 0, st, ar);
+                stage<ST_FCG_UPDATE, 1>(Lr.grid);
+                if (replace) {
+                    hipLaunchKernelGGL((k_cg_replace<D.value, 1, S.value, float>), dim3(L64.grid), dim3(kBlock), 0,
+                                       st, a64, m->val, m->dinv32);
+                    stage<ST_FCG_REPLACE, 2>(L64.grid);
+                }
+            }
+        });
+        RG_HIP(hipGetLastError());
+    }
+
     void iteration(int method, int sweeps, bool last)
     {
-        if (method == RG_METHOD_PCG) paso_iteration(sweeps);
+        if (method == RG_METHOD_FCG_LEARNED) fcg_iteration(last);
+        else if (method == RG_METHOD_PCG) paso_iteration(sweeps);
         else if (method == RG_METHOD_PCG_PLAIN) cg_iteration<double, double>(m->val, m->dinv, sweeps, false);
         else cg_iteration<float, float>(m->val32, m->dinv32, 1, last);
     }
@@ -838,6 +957,9 @@ struct SolverImpl {
             } else if (method == RG_METHOD_PCG_PLAIN) {
                 hipLaunchKernelGGL((k_cg_init<B.value, double>), dim3(L.grid), dim3(kBlock), 0, st, a, m->dinv);
                 stage<ST_CG_INIT, 2>(L.grid);
+            } else if (method == RG_METHOD_FCG_LEARNED) {
+                hipLaunchKernelGGL((k_cg_init<B.value, double>), dim3(L.grid), dim3(kBlock), 0, st, a, m->dinv);
+                stage<ST_FCG_INIT, 2>(L.grid);
             } else {
                 hipLaunchKernelGGL((k_cg_init<B.value, float>), dim3(L.grid), dim3(kBlock), 0, st, a, m->dinv32);
                 stage<ST_CG_INIT, 2>(L.grid);
@@ -890,7 +1012,7 @@ struct SolverImpl {
         std::memset(&h, 0, sizeof(h));
         h.maxit = maxit;
         h.tol = tol;
-        h.replace_mode = (method == RG_METHOD_PCG_MIXED) ? 1 : 0;
+        h.replace_mode = (method == RG_METHOD_PCG_MIXED || method == RG_METHOD_FCG_LEARNED) ? 1 : 0;
         KrylovState* pin = reinterpret_cast<KrylovState*>(m->ctx->pinned);
         *pin = h;
         RG_HIP(hipMemcpyAsync(w->st, pin, sizeof(KrylovState), hipMemcpyHostToDevice, st));
@@ -903,6 +1025,8 @@ struct SolverImpl {
         RG_HIP(hipEventCreateWithFlags(&evs[1], hipEventDisableTiming));
         int launched = 0;
         bool done = false;
+        std::vector<double> hist;
+        bool stagnated = false;
         try {
             // keep two chunks in flight; poll the older one
             while (!done) {
@@ -912,7 +1036,22 @@ struct SolverImpl {
                 ++launched;
                 if (launched >= 2) {
                     RG_HIP(hipEventSynchronize(evs[(launched - 2) % 2]));
-                    done = slots[(launched - 2) % 2].done != 0;
+                    const KrylovState& sl = slots[(launched - 2) % 2];
+                    done = sl.done != 0;
+                    // flexible CG with a learned preconditioner: stop on
+                    // stagnation (no 1% decrease over two polls)
+                    if (!done && method == RG_METHOD_FCG_LEARNED && !stagnated) {
+                        hist.push_back(sl.norm);
+                        const size_t k = hist.size();
+                        if (k >= 3 && !(hist[k - 1] < 0.99 * hist[k - 3])) {
+                            stagnated = true;
+                            int32_t flag[2] = {1, KS_STAGNATED};
+                            int32_t* pin32 = reinterpret_cast<int32_t*>(m->ctx->pinned + 128);
+                            pin32[0] = flag[0];
+                            pin32[1] = flag[1];
+                            RG_HIP(hipMemcpyAsync(&w->st->done, pin32, 2 * sizeof(int32_t), hipMemcpyHostToDevice, st));
+                        }
+                    }
                 }
             }
         } catch (...) {
@@ -1026,13 +1165,15 @@ int solve(rg_matrix* m, double* x, const double* b, const rg_options& opt, rg_di
 {
     const double EPS = 2.220446049250313e-16;
     const double t0 = now_s();
-    dg.num_iter = 0; dg.converged = 0; dg.status = 0; dg.outer_steps = 0;
+    dg.num_iter = 0; dg.converged = 0; dg.status = 0; dg.outer_steps = 0; dg.fallbacks = 0; dg.reserved = 0;
     dg.residual_norm = -1; dg.set_up_time = -1; dg.net_time = -1; dg.time = -1;
     const double tolerance = opt.tolerance;
     RG_CHECK(tolerance >= 100. * EPS, RG_VALUE_ERROR, "Solver: Tolerance is too small.");
     RG_CHECK(tolerance <= 1., RG_VALUE_ERROR, "Solver: Tolerance must be less than one.");
-    RG_CHECK(opt.method >= RG_METHOD_PCG && opt.method <= RG_METHOD_PCG_MIXED, RG_VALUE_ERROR,
+    RG_CHECK(opt.method >= RG_METHOD_PCG && opt.method <= RG_METHOD_FCG_LEARNED, RG_VALUE_ERROR,
              "Solver: unknown solver method.");
+    RG_CHECK(opt.method != RG_METHOD_FCG_LEARNED || (m->lpc && m->g.bs == 1), RG_VALUE_ERROR,
+             "Solver: the flexible CG needs a learned preconditioner attached to a scalar matrix.");
     RG_CHECK(opt.criterion == RG_CRITERION_PASO || opt.criterion == RG_CRITERION_TRUE_RESIDUAL, RG_VALUE_ERROR,
              "Solver: unknown stopping criterion.");
     const bool true_res = opt.criterion == RG_CRITERION_TRUE_RESIDUAL;
@@ -1043,7 +1184,8 @@ int solve(rg_matrix* m, double* x, const double* b, const rg_options& opt, rg_di
     const int64_t n = w->n;
     const int vgrid = 2048;
     prepare_solve(m);
-    if (opt.method == RG_METHOD_PCG_MIXED && !m->val32_valid) {
+    int method = opt.method;
+    if ((method == RG_METHOD_PCG_MIXED || method == RG_METHOD_FCG_LEARNED) && !m->val32_valid) {
         if (!m->val32) {
             const size_t cnt = (size_t)m->g.nstore * m->g.bs * m->g.bs * m->g.ld;
             RG_HIP(hipMalloc(&m->val32, cnt * sizeof(float)));
@@ -1104,10 +1246,17 @@ int solve(rg_matrix* m, double* x, const double* b, const rg_options& opt, rg_di
                 lastmax = normmax;
                 int ks = 0, it = 0;
                 double kn = 0;
-                S.krylov(opt.method, std::max(1, opt.sweeps), cntIter, tol, check, use_graph, &ks, &it, &kn);
+                S.krylov(method, std::max(1, opt.sweeps), cntIter, tol, check, use_graph, &ks, &it, &kn);
                 RG_HIP(hipGetLastError());
                 totIter += it;
                 if (ks == KS_CONVERGED) {
+                    finalize = false;
+                } else if (ks == KS_STAGNATED) {
+                    // the learned preconditioner stopped helping: continue
+                    // from the current iterate with the Jacobi-PCG
+                    if (opt.verbose) printf("Solver: learned preconditioner stagnated, falling back to PCG\n");
+                    method = RG_METHOD_PCG_MIXED;
+                    dg.fallbacks++;
                     finalize = false;
                 } else if (ks == KS_MAXITER) {
                     status = RG_MAX_ITER_REACHED;

@@ -38,6 +38,16 @@ __device__ __forceinline__ bool slot_in_grid(int s, int64_t x, int64_t y, int64_
     return ok;
 }
 
+// Between the z-groups of a row group's loads: with RG_ZGROUP_BARRIER the
+// loads of one z-plane group are not hoisted above the multiplies of the
+// previous one (fewer live registers, higher occupancy).
+__device__ __forceinline__ void zgroup_fence()
+{
+#ifdef RG_ZGROUP_BARRIER
+    __builtin_amdgcn_sched_barrier(0);
+#endif
+}
+
 template <typename T>
 __device__ __forceinline__ T ldg(const T* p)
 {
@@ -60,43 +70,49 @@ __device__ __forceinline__ void stencil_row(const VT* __restrict__ val, int64_t 
 {
     constexpr int NS = StencilShape<DIM>::NS;
     constexpr int C = StencilShape<DIM>::CENTER;
-    double xv[NS][BS];
-    double av[NS][BS * BS];
-    bool ok[NS];
+    constexpr int G = 9;                       // slots per z-plane group
 #pragma unroll
-    for (int s = 0; s < NS; ++s) {
-        int dx, dy, dz;
-        slot_delta<DIM>(s, dx, dy, dz);
-        int64_t j = i + dx + dy * s1 + dz * s2;
-        ok[s] = true;
-        if (CHECK) {
-            ok[s] = (uint64_t)j < (uint64_t)N;
-            j = ok[s] ? j : i;
-        }
+    for (int g0 = 0; g0 < NS; g0 += G) {
+        double xv[G][BS];
+        double av[G][BS * BS];
+        bool ok[G];
 #pragma unroll
-        for (int c = 0; c < BS; ++c) xv[s][c] = (double)x[j * BS + c];
-        if (!SYM || s >= C) {
-            const int t = SYM ? s - C : s;
-#pragma unroll
-            for (int e = 0; e < BS * BS; ++e) av[s][e] = (double)ldg(&val[(int64_t)(t * BS * BS + e) * ld + i]);
-        } else {
-            const int t = C - s;
-#pragma unroll
-            for (int r = 0; r < BS; ++r)
-#pragma unroll
-                for (int c = 0; c < BS; ++c)   // transposed block of row j
-                    av[s][r + BS * c] = (double)val[(int64_t)(t * BS * BS + c + BS * r) * ld + j];
-        }
-    }
-#pragma unroll
-    for (int s = 0; s < NS; ++s)
-#pragma unroll
-        for (int c = 0; c < BS; ++c)
-#pragma unroll
-            for (int r = 0; r < BS; ++r) {
-                const double t = fma(av[s][r + BS * c], xv[s][c], acc[r]);
-                acc[r] = (!CHECK || ok[s]) ? t : acc[r];
+        for (int q = 0; q < G; ++q) {
+            const int s = g0 + q;
+            int dx, dy, dz;
+            slot_delta<DIM>(s, dx, dy, dz);
+            int64_t j = i + dx + dy * s1 + dz * s2;
+            ok[q] = true;
+            if (CHECK) {
+                ok[q] = (uint64_t)j < (uint64_t)N;
+                j = ok[q] ? j : i;
             }
+#pragma unroll
+            for (int c = 0; c < BS; ++c) xv[q][c] = (double)x[j * BS + c];
+            if (!SYM || s >= C) {
+                const int t = SYM ? s - C : s;
+#pragma unroll
+                for (int e = 0; e < BS * BS; ++e) av[q][e] = (double)ldg(&val[(int64_t)(t * BS * BS + e) * ld + i]);
+            } else {
+                const int t = C - s;
+#pragma unroll
+                for (int r = 0; r < BS; ++r)
+#pragma unroll
+                    for (int c = 0; c < BS; ++c)   // transposed block of row j
+                        av[q][r + BS * c] = (double)val[(int64_t)(t * BS * BS + c + BS * r) * ld + j];
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < G; ++q)
+#pragma unroll
+            for (int c = 0; c < BS; ++c)
+#pragma unroll
+                for (int r = 0; r < BS; ++r) {
+                    const double t = fma(av[q][r + BS * c], xv[q][c], acc[r]);
+                    acc[r] = (!CHECK || ok[q]) ? t : acc[r];
+                }
+        zgroup_fence();
+    }
 }
 
 // x[b-1+k], k = 0..R+1, from aligned 16-byte loads starting at the even
@@ -146,28 +162,97 @@ __device__ __forceinline__ void stencil_rows_s1(const VT* __restrict__ val, int6
                                                 int64_t i0, const double* __restrict__ x, double (&acc)[R])
 {
     using V = typename VecOf<VT, R>::T;
-    constexpr int NPL = (DIM == 3) ? 9 : 3;   // (dy, dz) pairs
-    V av[NPL][3];
-    double w[NPL][R + 2];
+    constexpr int NZ = (DIM == 3) ? 3 : 1;   // z-plane groups of three (dy) planes
+#pragma unroll
+    for (int r = 0; r < R; ++r) acc[r] = 0.0;
+#pragma unroll
+    for (int zg = 0; zg < NZ; ++zg) {
+        V av[3][3];
+        double w[3][R + 2];
+#pragma unroll
+        for (int py = 0; py < 3; ++py) {
+            const int pl = py + 3 * zg;
+            const int dy = py - 1, dz = (DIM == 3) ? zg - 1 : 0;
+            const int64_t b = i0 + dy * s1 + dz * s2;
+            x_window<R>(x, b, w[py]);
+#pragma unroll
+            for (int dx = 0; dx < 3; ++dx) {
+                const int s = dx + 3 * pl;
+                av[py][dx] = ldg(reinterpret_cast<const V*>(val + (int64_t)s * ld + i0));
+            }
+        }
+#pragma unroll
+        for (int py = 0; py < 3; ++py)
+#pragma unroll
+            for (int dx = 0; dx < 3; ++dx)
+#pragma unroll
+                for (int r = 0; r < R; ++r) acc[r] = fma(vget(av[py][dx], r), w[py][r + dx], acc[r]);
+        zgroup_fence();
+    }
+}
+
+// R consecutive values v[j0 .. j0+R-1] of one value stream from two aligned
+// 16-byte loads at the aligned address at or below j0 (the misalignment is
+// the same for the whole wave).
+template <typename VT, int R>
+__device__ __forceinline__ void value_window(const VT* __restrict__ v, int64_t j0, double (&w)[R])
+{
+    using V = typename VecOf<VT, R>::T;
+    const int64_t base = j0 & ~(int64_t)(R - 1);
+    const int o = (int)(j0 - base);
+    const V a = *reinterpret_cast<const V*>(v + base);
+    const V b = *reinterpret_cast<const V*>(v + base + R);
+    double t[2 * R];
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+        t[k] = (double)a[k];
+        t[k + R] = (double)b[k];
+    }
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+        double r = t[k];
+#pragma unroll
+        for (int q = 1; q < R; ++q) r = (o == q) ? t[k + q] : r;
+        w[k] = r;
+    }
+}
+
+// Scalar symmetric-storage stencil for R consecutive rows of an interior row
+// group: the main and upper slots of row i are stored at row i; slot s below
+// the main diagonal is the stored slot CENTER-s of row i+off_s.
+template <int DIM, typename VT, int R>
+__device__ __forceinline__ void stencil_rows_s1_sym(const VT* __restrict__ val, int64_t ld, int64_t s1, int64_t s2,
+                                                    int64_t i0, const double* __restrict__ x, double (&acc)[R])
+{
+    using V = typename VecOf<VT, R>::T;
+    constexpr int NPL = (DIM == 3) ? 9 : 3;
+    constexpr int C = StencilShape<DIM>::CENTER;
+#pragma unroll
+    for (int r = 0; r < R; ++r) acc[r] = 0.0;
 #pragma unroll
     for (int pl = 0; pl < NPL; ++pl) {
         const int dy = pl % 3 - 1, dz = (DIM == 3) ? pl / 3 - 1 : 0;
         const int64_t b = i0 + dy * s1 + dz * s2;
-        x_window<R>(x, b, w[pl]);
+        double w[R + 2];
+        x_window<R>(x, b, w);
+        double av[3][R];
 #pragma unroll
         for (int dx = 0; dx < 3; ++dx) {
             const int s = dx + 3 * pl;
-            av[pl][dx] = ldg(reinterpret_cast<const V*>(val + (int64_t)s * ld + i0));
+            if (s >= C) {
+                const V v = ldg(reinterpret_cast<const V*>(val + (int64_t)(s - C) * ld + i0));
+#pragma unroll
+                for (int r = 0; r < R; ++r) av[dx][r] = (double)v[r];
+            } else {
+                value_window<VT, R>(val + (int64_t)(C - s) * ld, b + dx - 1, av[dx]);
+            }
         }
-    }
-#pragma unroll
-    for (int r = 0; r < R; ++r) acc[r] = 0.0;
-#pragma unroll
-    for (int pl = 0; pl < NPL; ++pl)
 #pragma unroll
         for (int dx = 0; dx < 3; ++dx)
 #pragma unroll
-            for (int r = 0; r < R; ++r) acc[r] = fma(vget(av[pl][dx], r), w[pl][r + dx], acc[r]);
+            for (int r = 0; r < R; ++r) acc[r] = fma(av[dx][r], w[r + dx], acc[r]);
+        if (pl % 3 == 2) zgroup_fence();
+    }
 }
 
 // Rows of one block of a chunked SpMV-shaped kernel.  Calls epi(i, acc) for
@@ -181,12 +266,13 @@ __device__ __forceinline__ void stencil_block(const VT* __restrict__ val, int64_
 {
     constexpr int R = 16 / (int)sizeof(VT);
     const int64_t reach = s2 * (DIM == 3) + s1 + R + 2;
-    const bool fast = (BS == 1) && !SYM && ((reinterpret_cast<uintptr_t>(x) & 15) == 0) &&
+    const bool fast = (BS == 1) && ((reinterpret_cast<uintptr_t>(x) & 15) == 0) &&
                       (r0 % R == 0) && ((r1 - r0) % R == 0) && (r0 >= reach) && (r1 + reach <= N);
     if (fast) {
         for (int64_t i0 = r0 + (int64_t)threadIdx.x * R; i0 < r1; i0 += (int64_t)kBlock * R) {
             double acc[R];
-            stencil_rows_s1<DIM, VT, R>(val, ld, s1, s2, i0, x, acc);
+            if (SYM) stencil_rows_s1_sym<DIM, VT, R>(val, ld, s1, s2, i0, x, acc);
+            else stencil_rows_s1<DIM, VT, R>(val, ld, s1, s2, i0, x, acc);
 #pragma unroll
             for (int r = 0; r < R; ++r) {
                 double a1[BS];

@@ -22,7 +22,7 @@ RG_DEVICE_ERROR = 22
 
 RG_MATRIX_SYMMETRIC = 1
 RG_COEF_EMPTY, RG_COEF_ELEMENT, RG_COEF_EXPANDED = 0, 1, 2
-RG_METHOD_PCG, RG_METHOD_PCG_PLAIN, RG_METHOD_PCG_MIXED = 0, 1, 2
+RG_METHOD_PCG, RG_METHOD_PCG_PLAIN, RG_METHOD_PCG_MIXED, RG_METHOD_FCG_LEARNED = 0, 1, 2, 3
 RG_CRITERION_PASO, RG_CRITERION_TRUE_RESIDUAL = 0, 1
 
 # every function the header declares (checked by the CPU test suite)
@@ -34,6 +34,8 @@ EXPORTS = [
     "rg_matrix_vector", "rg_matrix_ypAx", "rg_matrix_ypAx_host",
     "rg_matrix_nullify_rows_and_cols", "rg_matrix_nullify_rows_and_cols_host",
     "rg_matrix_export_csr", "rg_matrix_save_mm", "rg_options_default", "rg_solve", "rg_solve_host",
+    "rg_lpc_num_params", "rg_lpc_create", "rg_lpc_destroy", "rg_lpc_set_params", "rg_lpc_get_params",
+    "rg_lpc_apply", "rg_lpc_gradients", "rg_lpc_train_step", "rg_matrix_set_learned_preconditioner",
 ]
 
 
@@ -57,7 +59,8 @@ class RgOptions(ctypes.Structure):
 class RgDiagnostics(ctypes.Structure):
     _fields_ = [("num_iter", ctypes.c_int32), ("converged", ctypes.c_int32), ("status", ctypes.c_int32),
                 ("outer_steps", ctypes.c_int32), ("residual_norm", ctypes.c_double),
-                ("set_up_time", ctypes.c_double), ("net_time", ctypes.c_double), ("time", ctypes.c_double)]
+                ("set_up_time", ctypes.c_double), ("net_time", ctypes.c_double), ("time", ctypes.c_double),
+                ("fallbacks", ctypes.c_int32), ("reserved", ctypes.c_int32)]
 
 
 class RipleyHipError(RuntimeError):
@@ -108,6 +111,15 @@ def load():
         "rg_options_default": (None, [ctypes.POINTER(RgOptions)]),
         "rg_solve": (I, [P, P, P, ctypes.POINTER(RgOptions), ctypes.POINTER(RgDiagnostics)]),
         "rg_solve_host": (I, [P, P, P, ctypes.POINTER(RgOptions), ctypes.POINTER(RgDiagnostics)]),
+        "rg_lpc_num_params": (I64, []),
+        "rg_lpc_create": (I, [P, P, I, PP]),
+        "rg_lpc_destroy": (I, [P]),
+        "rg_lpc_set_params": (I, [P, P]),
+        "rg_lpc_get_params": (I, [P, P]),
+        "rg_lpc_apply": (I, [P, P, P, P, P]),
+        "rg_lpc_gradients": (I, [P, P, P, P, P, P, P]),
+        "rg_lpc_train_step": (I, [P, P, P, P, P, ctypes.c_float, P]),
+        "rg_matrix_set_learned_preconditioner": (I, [P, P, P]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

@@ -88,6 +88,7 @@ class SolverOptions:
     # methods of this implementation beyond the reference's enum
     PCG_PLAIN = 1001
     PCG_MIXED = 1002
+    FCG_LEARNED = 1003
 
     def __init__(self):
         self._method = SolverOptions.DEFAULT
@@ -106,7 +107,7 @@ class SolverOptions:
 
     # setters / getters named as SolverBuddy's python binding
     def setSolverMethod(self, method):
-        if method not in (self.DEFAULT, self.PCG, self.PCG_PLAIN, self.PCG_MIXED, self.BICGSTAB):
+        if method not in (self.DEFAULT, self.PCG, self.PCG_PLAIN, self.PCG_MIXED, self.FCG_LEARNED, self.BICGSTAB):
             raise ValueError("unknown solver method %r" % (method,))
         self._method = method
 
@@ -201,12 +202,12 @@ class SolverOptions:
     # diagnostics (paso/src/Options.cpp:441-459)
     _DIAG_KEYS = ("num_iter", "num_level", "num_inner_iter", "time", "set_up_time", "net_time",
                   "residual_norm", "converged", "cum_num_iter", "cum_time", "cum_set_up_time",
-                  "cum_net_time", "outer_steps")
+                  "cum_net_time", "outer_steps", "fallbacks")
 
     def resetDiagnostics(self, all=False):
         keep = {} if all else {k: v for k, v in getattr(self, "_diag", {}).items() if k.startswith("cum_")}
         self._diag = {"num_iter": 0, "num_level": 0, "num_inner_iter": 0, "time": 0.0, "set_up_time": 0.0,
-                      "net_time": 0.0, "residual_norm": 0.0, "converged": False, "outer_steps": 0,
+                      "net_time": 0.0, "residual_norm": 0.0, "converged": False, "outer_steps": 0, "fallbacks": 0,
                       "cum_num_iter": 0, "cum_time": 0.0, "cum_set_up_time": 0.0, "cum_net_time": 0.0}
         self._diag.update(keep)
 
@@ -239,7 +240,7 @@ class SolverOptions:
         if m == self.BICGSTAB:
             raise NotImplementedError("BiCGStab is outside this path (PASO PCG only)")
         o.method = {self.PCG: _lib.RG_METHOD_PCG, self.PCG_PLAIN: _lib.RG_METHOD_PCG_PLAIN,
-                    self.PCG_MIXED: _lib.RG_METHOD_PCG_MIXED}[m]
+                    self.PCG_MIXED: _lib.RG_METHOD_PCG_MIXED, self.FCG_LEARNED: _lib.RG_METHOD_FCG_LEARNED}[m]
         o.verbose = int(self._verbose)
         o.accept_failed_convergence = int(self._accept)
         o.check_interval = self._check_interval
@@ -406,6 +407,23 @@ class SystemMatrix:
             raise ValueError("rhs must be a float64 GPU tensor of %d entries" % self.size)
         check(load().rg_matrix_assemble(self.handle, ctypes.byref(pc), int(bool(reduced)), _ptr(rhs)))
 
+    def setLearnedPreconditioner(self, net, logk=None):
+        """Attach a LearnedPreconditioner (built for this domain's node grid)
+        and log10(kappa) per node for SolverOptions.FCG_LEARNED; net=None
+        detaches.  Both are borrowed: keep them alive while solving."""
+        self._bind()
+        if net is None:
+            check(load().rg_matrix_set_learned_preconditioner(self.handle, None, None))
+            self._lpc = None
+            return
+        lk = None
+        if logk is not None:
+            lk = torch.as_tensor(logk).to(device=self.device, dtype=torch.float32).contiguous().reshape(-1)
+            if lk.numel() != self.num_rows:
+                raise ValueError("logk needs one value per node")
+        check(load().rg_matrix_set_learned_preconditioner(self.handle, net.handle, _ptr(lk)))
+        self._lpc = (net, lk)
+
     def exportCSR(self):
         """(ptr, index, val) numpy arrays of the PASO CSR form."""
         ptr = np.zeros(self.num_rows + 1, dtype=np.int32)
@@ -432,7 +450,7 @@ class SystemMatrix:
             raise ValueError("out must be a contiguous float64 GPU tensor of %d entries" % self.size)
         dg = _lib.RgDiagnostics()
         rc = load().rg_solve(self.handle, _ptr(out), _ptr(b), ctypes.byref(o), ctypes.byref(dg))
-        for k in ("num_iter", "set_up_time", "net_time", "time", "residual_norm", "outer_steps"):
+        for k in ("num_iter", "set_up_time", "net_time", "time", "residual_norm", "outer_steps", "fallbacks"):
             options._updateDiagnostics(k, getattr(dg, k))
         options._updateDiagnostics("converged", bool(dg.converged))
         check(rc)

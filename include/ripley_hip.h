@@ -42,6 +42,7 @@ enum {
 
 typedef struct rg_context rg_context;
 typedef struct rg_matrix rg_matrix;
+typedef struct rg_lpc rg_lpc;
 
 /* Message of the last failure on this thread (empty string if none). */
 const char* rg_last_error(void);
@@ -172,10 +173,18 @@ int rg_matrix_save_mm(rg_matrix* m, const char* filename);
  *                        check_interval iterations the residual is replaced by
  *                        the FP64 true residual D b - A x, and convergence is
  *                        only accepted on such a replaced residual
+ *   RG_METHOD_FCG_LEARNED flexible CG (truncated to one direction) whose
+ *                        preconditioner is the learned network attached with
+ *                        rg_matrix_set_learned_preconditioner; SpMV on FP32
+ *                        values with FP64 residual replacement as above.  If
+ *                        the residual stagnates (no 1% decrease over two
+ *                        check intervals) the solve falls back to
+ *                        RG_METHOD_PCG_MIXED (Jacobi) from the current iterate.
+ *                        Extension: the reference has no counterpart.
  * criterion:
  *   RG_CRITERION_PASO           balanced ||r||_2 and ||r||_inf tests (Solver.cpp:241)
  *   RG_CRITERION_TRUE_RESIDUAL  unbalanced FP64 ||b - A x||_2 <= tolerance ||b||_2 */
-enum { RG_METHOD_PCG = 0, RG_METHOD_PCG_PLAIN = 1, RG_METHOD_PCG_MIXED = 2 };
+enum { RG_METHOD_PCG = 0, RG_METHOD_PCG_PLAIN = 1, RG_METHOD_PCG_MIXED = 2, RG_METHOD_FCG_LEARNED = 3 };
 enum { RG_CRITERION_PASO = 0, RG_CRITERION_TRUE_RESIDUAL = 1 };
 typedef struct {
     double tolerance;            /* relative; default 1e-8 */
@@ -191,7 +200,8 @@ typedef struct {
 } rg_options;
 
 /* Diagnostics keys of paso::Options::updateEscriptDiagnostics
- * (paso/src/Options.cpp:441-459) plus the number of outer steps. */
+ * (paso/src/Options.cpp:441-459) plus the number of outer steps and of
+ * switches from the learned preconditioner to Jacobi-PCG. */
 typedef struct {
     int32_t num_iter;
     int32_t converged;
@@ -201,6 +211,8 @@ typedef struct {
     double set_up_time;
     double net_time;
     double time;
+    int32_t fallbacks;
+    int32_t reserved;
 } rg_diagnostics;
 
 void rg_options_default(rg_options* opt);
@@ -216,6 +228,35 @@ int rg_solve(rg_matrix* m, double* x, const double* b, const rg_options* opt,
              rg_diagnostics* diag);
 int rg_solve_host(rg_matrix* m, double* x, const double* b, const rg_options* opt,
                   rg_diagnostics* diag);
+
+/* --------------------------------------------------- learned preconditioner
+ * No counterpart in the reference (its preconditioners are PASO's Jacobi,
+ * Gauss-Seidel, ILU0, RILU, AMG).  A 3-level 3-D convolutional U-Net
+ * (16 -> 32 -> 64 channels, 3x3x3 kernels, BF16 weights, FP32 accumulate)
+ * on a node grid: input channels are the Jacobi-scaled residual (normalised
+ * to unit RMS) and log10(kappa) per node; the output is an error correction
+ * e ~ A^-1 r.  Parameters: per layer W[co][27][ci] then b[co] (FP32), 9
+ * layers, rg_lpc_num_params() floats.  A network is created for one node grid
+ * (nodes[0..2]); `training` allocates the backward-pass buffers. */
+int64_t rg_lpc_num_params(void);
+int rg_lpc_create(rg_context* ctx, const int32_t* nodes, int training, rg_lpc** out);
+int rg_lpc_destroy(rg_lpc* net);
+int rg_lpc_set_params(rg_lpc* net, const float* params);   /* host */
+int rg_lpc_get_params(rg_lpc* net, float* params);         /* host */
+/* z = Net(r) (device vectors on the node grid; dinv = 1/diag(A), logk
+ * float per node, either may be NULL). */
+int rg_lpc_apply(rg_lpc* net, const double* r, const double* dinv, const float* logk, double* z);
+/* Cosine loss 1 - (y.e)/(|y||e|) of Net(r) against the error e (device) and
+ * its parameter gradient (host, may be NULL). */
+int rg_lpc_gradients(rg_lpc* net, const double* r, const double* dinv, const float* logk, const double* e,
+                     double* loss, float* grads);
+/* One Adam step (lr) on that loss. */
+int rg_lpc_train_step(rg_lpc* net, const double* r, const double* dinv, const float* logk, const double* e,
+                      float lr, double* loss);
+/* Attach a network (created for this matrix's node grid, scalar 3-D
+ * problems) and log10(kappa) per node (device, kept borrowed) for
+ * RG_METHOD_FCG_LEARNED.  net = NULL detaches. */
+int rg_matrix_set_learned_preconditioner(rg_matrix* m, rg_lpc* net, const float* logk);
 
 #ifdef __cplusplus
 }

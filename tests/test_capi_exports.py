@@ -38,3 +38,20 @@ def test_host_only_entry_points():
     o = _lib.RgOptions()
     L.rg_options_default(ctypes.byref(o))
     assert o.tolerance == 1e-8 and o.iter_max == 100000 and o.sweeps == 1 and o.method == 0
+
+
+def test_cpp_header_compiles_against_library(tmp_path):
+    import shutil
+    import subprocess
+    from ripley_hip import _lib
+    gxx = shutil.which("g++")
+    if gxx is None:
+        pytest.skip("no g++")
+    src = tmp_path / "use.cpp"
+    src.write_text('#include "ripley_hip.hpp"\n'
+                   'int main() { ripley_hip::SolverOptions o; return o.iter_max == 100000 ? 0 : 1; }\n')
+    libdir = os.path.dirname(_lib.LIB_PATH)
+    exe = tmp_path / "use"
+    subprocess.check_call([gxx, "-std=c++17", "-I", os.path.join(ROOT, "include"), str(src), "-L", libdir,
+                           "-lripley_hip", "-Wl,-rpath," + libdir, "-o", str(exe)])
+    assert subprocess.call([str(exe)]) == 0
