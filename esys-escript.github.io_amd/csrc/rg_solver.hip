@@ -1251,9 +1251,10 @@ int solve(rg_matrix* m, double* x, const double* b, const rg_options& opt, rg_di
                 totIter += it;
                 if (ks == KS_CONVERGED) {
                     finalize = false;
-                } else if (ks == KS_STAGNATED) {
-                    // the learned preconditioner stopped helping: continue
-                    // from the current iterate with the Jacobi-PCG
+                } else if (ks == KS_STAGNATED || (ks == KS_BREAKDOWN && method == RG_METHOD_FCG_LEARNED)) {
+                    // the learned preconditioner stopped helping (or gave a
+                    // zero direction): continue from the current iterate
+                    // with the Jacobi-PCG
                     if (opt.verbose) printf("Solver: learned preconditioner stagnated, falling back to PCG\n");
                     method = RG_METHOD_PCG_MIXED;
                     dg.fallbacks++;

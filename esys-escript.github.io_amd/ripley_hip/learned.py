@@ -1,0 +1,319 @@
+"""Learned preconditioner for the flexible CG (SolverOptions.FCG_LEARNED).
+
+The network (forward, backward and Adam kernels are the library's own BF16
+MFMA kernels, FP32 accumulate) is a 3-level 3-D U-Net on the node grid of a
+Brick:
+
+    level 1 (node grid):      2 -> 16 -> 16          skip s1
+    level 2 (1/2, avg pool):  16 -> 32 -> 32         skip s2
+    level 3 (1/4, avg pool):  32 -> 64 -> 64
+    decoder:  up(64) ++ s2 -> 32,  up(32) ++ s1 -> 16,  16 -> 1 (linear)
+
+Input channels: u / rms(u) with u = r / diag(A) (the Jacobi-scaled
+residual) and log10(kappa); output: a correction proportional to the error
+e = A^-1 r.  It is trained with the cosine loss 1 - <y, e>/(|y||e|) (the
+flexible CG is invariant to the scale of its preconditioner) on
+residual/error pairs that the plain Jacobi-PCG produces: e_k = x* - x_k and
+r_k = b - A x_k for iterates x_k of the plain solver.
+
+The reference has no learned preconditioner (PASO offers Jacobi,
+Gauss-Seidel, ILU0, RILU and AMG, paso/src/Preconditioner.cpp:46-107); the
+plain Jacobi-PCG stays the fallback: the solver switches to it when the
+flexible CG stops reducing the residual.
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import check, load
+from .core import SolverOptions, _ptr, get_context
+
+# (cin, cout, relu) of the 9 convolutions, in parameter order
+LAYERS = ((16, 16, True), (16, 16, True), (16, 32, True), (32, 32, True), (32, 64, True), (64, 64, True),
+          (96, 32, True), (48, 16, True), (16, 16, False))
+
+
+def num_params():
+    return int(load().rg_lpc_num_params())
+
+
+def split_params(flat):
+    """[(W[co][27][ci], b[co]), ...] views of a flat parameter vector."""
+    out, o = [], 0
+    for cin, cout, _ in LAYERS:
+        ws = 27 * cin * cout
+        out.append((flat[o:o + ws].reshape(cout, 27, cin), flat[o + ws:o + ws + cout]))
+        o += ws + cout
+    return out
+
+
+def init_params(seed=0):
+    """He-normal weights over the real fan-in (layer 0 sees 2 channels),
+    zero biases."""
+    rng = np.random.default_rng(seed)
+    parts = []
+    for l, (cin, cout, relu) in enumerate(LAYERS):
+        fan = 27 * (2 if l == 0 else cin)
+        std = np.sqrt((2.0 if relu else 1.0) / fan)
+        w = rng.standard_normal((cout, 27, cin)) * std
+        if l == 0:
+            w[:, :, 2:] = 0.0
+        parts.append(w.reshape(-1))
+        parts.append(np.zeros(cout))
+    return np.concatenate(parts).astype(np.float32)
+
+
+class LearnedPreconditioner:
+    """One network instance for a node grid nodes = (n0, n1, n2)."""
+
+    def __init__(self, nodes, training=False, params=None, seed=0, device=0):
+        self.ctx = get_context(device)
+        self.device = torch.device("cuda", device)
+        self.nodes = tuple(int(n) for n in nodes)
+        self.training = bool(training)
+        nd = (ctypes.c_int32 * 3)(*self.nodes)
+        h = ctypes.c_void_p()
+        check(load().rg_lpc_create(self.ctx.handle, nd, int(self.training), ctypes.byref(h)))
+        self.handle = h
+        self.set_params(init_params(seed) if params is None else params)
+
+    def __del__(self):
+        try:
+            if self.handle:
+                load().rg_lpc_destroy(self.handle)
+                self.handle = None
+        except Exception:
+            pass
+
+    @property
+    def num_nodes(self):
+        return self.nodes[0] * self.nodes[1] * self.nodes[2]
+
+    def set_params(self, params):
+        p = np.ascontiguousarray(params, dtype=np.float32).reshape(-1)
+        if p.size != num_params():
+            raise ValueError("expected %d parameters, got %d" % (num_params(), p.size))
+        self.ctx.bind_stream()
+        check(load().rg_lpc_set_params(self.handle, p.ctypes.data_as(ctypes.c_void_p)))
+
+    def get_params(self):
+        p = np.zeros(num_params(), dtype=np.float32)
+        self.ctx.bind_stream()
+        check(load().rg_lpc_get_params(self.handle, p.ctypes.data_as(ctypes.c_void_p)))
+        return p
+
+    def save(self, path):
+        np.save(path, self.get_params())
+
+    def load(self, path):
+        self.set_params(np.load(path, allow_pickle=False))
+
+    def _vec(self, v, dtype=torch.float64):
+        if v is None:
+            return None
+        t = torch.as_tensor(v).to(device=self.device, dtype=dtype).contiguous().reshape(-1)
+        if t.numel() != self.num_nodes:
+            raise ValueError("vector has %d entries, expected %d" % (t.numel(), self.num_nodes))
+        return t
+
+    def apply(self, r, dinv=None, logk=None):
+        """z = Net(r) on device vectors over the node grid."""
+        r, dinv, lk = self._vec(r), self._vec(dinv), self._vec(logk, torch.float32)
+        z = torch.empty(self.num_nodes, dtype=torch.float64, device=self.device)
+        self.ctx.bind_stream()
+        check(load().rg_lpc_apply(self.handle, _ptr(r), _ptr(dinv), _ptr(lk), _ptr(z)))
+        return z
+
+    def gradients(self, r, dinv, logk, e):
+        """(loss, flat FP32 gradient) of the cosine loss of Net(r) against e."""
+        r, dinv, lk, e = self._vec(r), self._vec(dinv), self._vec(logk, torch.float32), self._vec(e)
+        g = np.zeros(num_params(), dtype=np.float32)
+        loss = ctypes.c_double()
+        self.ctx.bind_stream()
+        check(load().rg_lpc_gradients(self.handle, _ptr(r), _ptr(dinv), _ptr(lk), _ptr(e), ctypes.byref(loss),
+                                      g.ctypes.data_as(ctypes.c_void_p)))
+        return loss.value, g
+
+    def train_step(self, r, dinv, logk, e, lr=1e-3):
+        r, dinv, lk, e = self._vec(r), self._vec(dinv), self._vec(logk, torch.float32), self._vec(e)
+        loss = ctypes.c_double()
+        self.ctx.bind_stream()
+        check(load().rg_lpc_train_step(self.handle, _ptr(r), _ptr(dinv), _ptr(lk), _ptr(e), ctypes.c_float(lr),
+                                       ctypes.byref(loss)))
+        return loss.value
+
+
+# --------------------------------------------------------------------------
+# training data from the plain solver
+
+
+def operator_diagonal(op):
+    """diag(A) of an assembled, not yet balanced SystemMatrix (bs 1), via
+    the PASO CSR export."""
+    ptr, idx, val = op.exportCSR()
+    rows = np.repeat(np.arange(op.num_rows, dtype=np.int64), np.diff(ptr))
+    d = np.zeros(op.num_rows)
+    m = idx == rows
+    d[rows[m]] = val[m]
+    return d
+
+
+def plain_pcg_pairs(make_operator, b, iterations, tol=1e-10):
+    """Residual/error pairs from the plain FP64 Jacobi-PCG.
+
+    make_operator() returns a freshly assembled and constrained SystemMatrix
+    (it is called once for the reference solve, once per sampled iterate and
+    once for an unbalanced copy used for products).  For each k in
+    `iterations`, x_k is the plain solver's iterate after k iterations, and
+    the pair is (r_k = b - A x_k, e_k = x* - x_k), with x* solved to `tol`.
+    Returns (dinv, [(r_k, e_k), ...]) as device tensors."""
+    A = make_operator()
+    dinv = torch.as_tensor(1.0 / operator_diagonal(A), device=A.device)
+    S = make_operator()
+    opt = SolverOptions()
+    opt.setSolverMethod(SolverOptions.PCG_PLAIN)
+    opt.setTolerance(tol)
+    opt.setTrueResidualCriterion(True)
+    xs = S.solve(b, opt)
+    del S
+    pairs = []
+    for k in iterations:
+        S = make_operator()
+        o = SolverOptions()
+        o.setSolverMethod(SolverOptions.PCG_PLAIN)
+        o.setTolerance(tol)
+        o.setIterMax(int(k))
+        o.setAcceptanceConvergenceFailureOn()
+        o.setCheckInterval(max(1, min(32, int(k))))
+        xk = S.solve(b, o)
+        del S
+        r = b.clone()
+        A.MatrixVector(-1.0, xk, 1.0, r)
+        pairs.append((r, xs - xk))
+    return dinv, pairs
+
+
+def nodal_log10(kappa_elem, ne):
+    """log10(kappa) per node: the mean over the (up to 8) adjacent elements
+    (element-constant kappa, x fastest), float32 device tensor."""
+    n0, n1, n2 = ne
+    k = torch.as_tensor(kappa_elem, dtype=torch.float64).reshape(1, 1, n2, n1, n0)
+    lk = torch.log10(k)
+    lk = torch.nn.functional.pad(lk, (1, 1, 1, 1, 1, 1), mode="replicate")
+    nod = torch.nn.functional.avg_pool3d(lk, 2, stride=1)
+    return nod.reshape(-1).to(torch.float32)
+
+
+class BrickPoisson:
+    """-div(kappa grad u) = f on Brick(ne) (unit cube), u = 0 on z = 0, kappa
+    constant per element (isotropic A = kappa I on Function, Y = f) -- the
+    problem family of the learned preconditioner."""
+
+    def __init__(self, ne, kappa_elem, f=1.0, device=0, symmetric=False):
+        from .core import Coefficient
+        from .domain import Brick
+        self.ne = tuple(int(n) for n in ne)
+        self.dom = Brick(*self.ne, device=device)
+        self.device = self.dom.device
+        self.symmetric = symmetric
+        self.kappa = torch.as_tensor(kappa_elem, dtype=torch.float64).to(self.device).reshape(-1)
+        E, Q = self.dom.getNumElements(), self.dom.getNumQuadraturePoints()
+        self.cA = Coefficient(self.kappa, (3, 3), E, Q, self.device, isotropic=True)
+        self.cY = Coefficient(float(f), (), E, Q, self.device)
+        self.q = self.dom.whereZero(2)
+        self.nodes = tuple(n + 1 for n in self.ne)
+        self.logk = nodal_log10(self.kappa.cpu(), self.ne).to(self.device)
+        self.rhs = None
+
+    def operator(self):
+        """A freshly assembled, constrained operator (the rhs is kept)."""
+        op = self.dom.newOperator(1, symmetric=self.symmetric)
+        rhs = torch.zeros(op.size, dtype=torch.float64, device=self.device)
+        op.assemble({"A": self.cA, "Y": self.cY}, rhs)
+        op.nullifyRowsAndCols(self.q, self.q, 1.0)
+        rhs.masked_fill_(self.q > 0, 0.0)
+        self.rhs = rhs
+        return op
+
+
+DEFAULT_ITERATES = (1, 2, 4, 8, 16, 32, 64, 128, 256, 512)
+
+
+def generate_training_set(ne, num_fields, kappa_fn, iterations=DEFAULT_ITERATES, first_seed=1, log=None):
+    """[(r, e, dinv, logk), ...] on the device: for each seed s in
+    first_seed .. first_seed+num_fields-1, kappa = kappa_fn(ne, s) and the
+    plain Jacobi-PCG's residual/error pairs at the given iteration counts."""
+    data = []
+    for s in range(first_seed, first_seed + num_fields):
+        prob = BrickPoisson(ne, kappa_fn(ne, s))
+        prob.operator()
+        b = prob.rhs
+        dinv, pairs = plain_pcg_pairs(prob.operator, b, iterations)
+        for r, e in pairs:
+            data.append((r, e, dinv, prob.logk))
+        if log:
+            log("training set: field %d/%d (%d pairs)" % (s - first_seed + 1, num_fields, len(data)))
+    return data
+
+
+def train(net, data, steps, lr=1e-3, lr_final=1e-4, seed=0, log=None, log_every=100):
+    """Adam on the cosine loss over random pairs of `data`; the learning rate
+    decays geometrically from lr to lr_final.  Returns the loss history."""
+    rng = np.random.default_rng(seed)
+    hist = []
+    for k in range(steps):
+        r, e, dinv, logk = data[int(rng.integers(len(data)))]
+        a = lr * (lr_final / lr) ** (k / max(1, steps - 1))
+        hist.append(net.train_step(r, dinv, logk, e, a))
+        if log and (k % log_every == 0 or k == steps - 1):
+            log("step %d loss %.4f (mean of last %d: %.4f)" % (k, hist[-1], min(len(hist), log_every),
+                                                               float(np.mean(hist[-log_every:]))))
+    return hist
+
+
+def _main():
+    import argparse
+    import sys
+    import time
+    from . import fields
+    ap = argparse.ArgumentParser(description="train the learned preconditioner on the GPU")
+    ap.add_argument("--ne", type=int, default=64)
+    ap.add_argument("--fields", type=int, default=32)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--family", default="log10grf", choices=["log10grf", "lognormal"])
+    ap.add_argument("--sigma", type=float, default=1.0)
+    ap.add_argument("--corr", type=float, default=8.0)
+    ap.add_argument("--init", default=None)
+    ap.add_argument("--out", required=True)
+    a = ap.parse_args()
+
+    def kfn(ne, s):
+        if a.family == "log10grf":
+            return fields.log10_normal_kappa(ne, corr_len=a.corr, sigma=a.sigma, seed=s)
+        return fields.lognormal_kappa(ne, corr_len=a.corr, sigma_log=a.sigma, seed=s)
+
+    def log(msg):
+        print(msg, flush=True)
+
+    ne = (a.ne,) * 3
+    t0 = time.perf_counter()
+    data = generate_training_set(ne, a.fields, kfn, log=log)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    net = LearnedPreconditioner(tuple(n + 1 for n in ne), training=True)
+    if a.init:
+        net.load(a.init)
+    hist = train(net, data, a.steps, lr=a.lr, log=log)
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    net.save(a.out)
+    log("data %.1fs, training %.1fs (%.2f ms/step), final loss %.4f" %
+        (t1 - t0, t2 - t1, 1e3 * (t2 - t1) / max(1, a.steps), float(np.mean(hist[-100:]))))
+    sys.exit(0)
+
+
+if __name__ == "__main__":
+    _main()
