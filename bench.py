@@ -34,6 +34,9 @@ def parse():
     ap.add_argument("--cpu-iters", type=int, default=40)
     ap.add_argument("--spmv-reps", type=int, default=50)
     ap.add_argument("--quiet", action="store_true")
+    ap.add_argument("--extras", default="c1,c3,c4",
+                    help="comma list of the other configurations to measure (c1, c3, c4, c5); '' for none")
+    ap.add_argument("--c4-rhs", type=int, default=64)
     ap.add_argument("--probe", action="store_true",
                     help="only time the FP64 SpMV (full and symmetric storage) and a 1 GiB device copy; "
                          "used for the rocprofv3 counter runs")
@@ -234,6 +237,237 @@ def cpu_baseline(P, gpu_paso_iters, iters, np):
             "per_iter_s": per_iter, "assemble_s": t2 - t1}
 
 
+# ------------------------------------------------------------------ C1, C3, C4
+def _opts(R, method, tol=1e-8, true_res=True):
+    o = R.SolverOptions()
+    o.setSymmetryOn()
+    o.setTolerance(tol)
+    o.setTrueResidualCriterion(true_res)
+    o.setSolverMethod(getattr(R.SolverOptions, method))
+    return o
+
+
+def _timed(torch, f):
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    r = f()
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0, r
+
+
+def _rel_res(torch, op_fresh, x, b):
+    r = b.clone()
+    op_fresh.MatrixVector(-1.0, x, 1.0, r)
+    return (torch.linalg.norm(r) / torch.linalg.norm(b)).item()
+
+
+def bench_c1(R, torch, np, args):
+    """Rectangle 256x256, A = I, Y = 1, u = 0 on x0 = 0, PASO PCG rtol 1e-8
+    (the reference's CPU-runnable case), GPU and oracle on the host."""
+    dom = R.Rectangle(256, 256)
+    E, Q = dom.getNumElements(), dom.getNumQuadraturePoints()
+    cA = R.Coefficient(torch.eye(2, dtype=torch.float64), (2, 2), E, Q, dom.device)
+    cY = R.Coefficient(1.0, (), E, Q, dom.device)
+    q = dom.whereZero(0)
+
+    def system():
+        op = dom.newOperator(1)
+        rhs = torch.zeros(op.size, dtype=torch.float64, device="cuda")
+        op.assemble({"A": cA, "Y": cY}, rhs)
+        op.nullifyRowsAndCols(q, q, 1.0)
+        rhs.masked_fill_(q > 0, 0.0)
+        return op, rhs
+
+    out = {"workload": "ripley Rectangle 256^2 Poisson, A=I, Y=1, u=0 on x0=0, PASO PCG (Jacobi), rtol 1e-8",
+           "unknowns": dom.getNumNodes()}
+    for method in ("PCG", "PCG_MIXED"):
+        op, rhs = system()
+        o = _opts(R, method, true_res=(method != "PCG"))
+        op.solve(rhs, o)   # warm-up (graph capture)
+        ts = []
+        for _ in range(3):
+            op, rhs = system()
+            o = _opts(R, method, true_res=(method != "PCG"))
+            t, x = _timed(torch, lambda: op.solve(rhs, o))
+            ts.append(t)
+        op2, _ = system()
+        out[method] = {"solve_s": min(ts), "num_iter": o.getDiagnostics("num_iter"),
+                       "rel_residual_fp64": _rel_res(torch, op2, x, rhs)}
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        nth = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
+        O.set_num_threads(nth)
+        t0 = time.perf_counter()
+        ptr, idx = O.pattern(2, (256, 256))
+        _, _, val, orhs = O.assemble(2, (256, 256), (1.0, 1.0), ptr=ptr, index=idx, A=np.eye(2), Y=1.0)
+        qn = q.cpu().numpy()
+        O.nullify(1, ptr, idx, val, qn, qn, 1.0)
+        orhs = np.where(qn > 0, 0.0, orhs)
+        t1 = time.perf_counter()
+        _, dg, _, _ = O.solve(1, ptr, idx, val, orhs, tolerance=1e-8)
+        t2 = time.perf_counter()
+        out["cpu_oracle_paso"] = {"assemble_s": t1 - t0, "solve_s": t2 - t1, "num_iter": dg["num_iter"],
+                                  "threads": nth}
+    return out
+
+
+def bench_c3(R, torch, np, args):
+    """Brick 96^3 linear elasticity (3x3 blocks), Lame parameters in 8
+    random horizontal layers U[1e9, 5e10] Pa (seed 20261203), gravity
+    rho g = 2700*9.81, u = 0 on z = 0: PASO PCG in FP64 vs FP32 block SpMV
+    with FP64 refinement."""
+    from ripley_hip import fields
+    ne = (96, 96, 96)
+    dom = R.Brick(*ne)
+    E, Q = dom.getNumElements(), dom.getNumQuadraturePoints()
+    lam, mu = fields.layered_lame(ne)
+    eye = torch.eye(3, dtype=torch.float64, device="cuda")
+    bl = torch.einsum("ij,kl->ijkl", eye, eye)
+    bm = torch.einsum("ik,jl->ijkl", eye, eye) + torch.einsum("il,jk->ijkl", eye, eye)
+    lam_t = torch.as_tensor(lam, device="cuda")
+    mu_t = torch.as_tensor(mu, device="cuda")
+    A = lam_t[:, None, None, None, None] * bl + mu_t[:, None, None, None, None] * bm
+    cA = R.Coefficient(A, (3, 3, 3, 3), E, Q, dom.device)
+    del A
+    cY = R.Coefficient(torch.tensor([0.0, 0.0, -2700.0 * 9.81], dtype=torch.float64), (3,), E, Q, dom.device)
+    q = dom.whereZero(2).repeat_interleave(3)
+
+    def system():
+        op = dom.newOperator(3)
+        rhs = torch.zeros(op.size, dtype=torch.float64, device="cuda")
+        op.assemble({"A": cA, "Y": cY}, rhs)
+        op.nullifyRowsAndCols(q, q, 1.0)
+        rhs.masked_fill_(q > 0, 0.0)
+        return op, rhs
+
+    op, _ = system()
+    out = {"workload": "ripley Brick 96^3 Lame elasticity, 3x3 blocks, 8 layers lambda/mu U[1e9,5e10] Pa, "
+                       "gravity, u=0 on z=0, rtol 1e-8", "unknowns": op.size, "block_nnz": op.nnz}
+    # FP64 3x3 block SpMV: algorithmic bytes 72 nnzb + 2*24 N
+    N = op.num_rows
+    x = torch.rand(op.size, dtype=torch.float64, device="cuda")
+    y = torch.zeros_like(x)
+    for _ in range(3):
+        op.MatrixVector(1.0, x, 0.0, y)
+    s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    s.record()
+    for _ in range(20):
+        op.MatrixVector(1.0, x, 0.0, y)
+    e.record()
+    torch.cuda.synchronize()
+    dt = s.elapsed_time(e) / 1e3 / 20
+    alg = 72 * op.nnz + 48 * N
+    out["spmv_fp64"] = {"avg_launch_s": dt, "alg_bytes": alg, "GBps": alg / dt / 1e9,
+                        "frac_of_8TBps": alg / dt / 1e9 / HBM_PEAK_GBPS}
+    del op
+    for method in ("PCG", "PCG_MIXED"):
+        op, rhs = system()
+        o = _opts(R, method)
+        t, x = _timed(torch, lambda: op.solve(rhs, o))
+        op2, _ = system()
+        out[method] = {"solve_s": t, "num_iter": o.getDiagnostics("num_iter"),
+                       "converged": o.hasConverged(), "rel_residual_fp64": _rel_res(torch, op2, x, rhs)}
+        del op, op2
+    return out
+
+
+def bench_c4(R, torch, np, args):
+    """(i) Brick 128^3, kappa = exp(GRF), sigma_log 1.5, 64 unit point loads
+    at random top-surface nodes (seed 20261204), u = 0 on z = 0: one batched
+    solve of all right-hand sides vs one right-hand side alone.
+    (ii) 32 Rectangles n0 = n1 = n ~ U{64..512} (seed 20261205), kappa =
+    exp(GRF) sigma_log 1, Y = 1, u = 0 on x0 = 0: one batched solve vs the
+    32 systems solved one after the other.  FP32 SpMV + FP64 refinement,
+    rtol 1e-8 on the FP64 residual of every column / system."""
+    from ripley_hip import batch as RB
+    from ripley_hip import fields
+    out = {}
+    K = args.c4_rhs
+    ne = (128, 128, 128)
+    dom = R.Brick(*ne)
+    E, Q = dom.getNumElements(), dom.getNumQuadraturePoints()
+    kap = torch.as_tensor(fields.lognormal_kappa(ne, corr_len=8.0, sigma_log=1.5, seed=20261204), device="cuda")
+    cA = R.Coefficient(kap, (3, 3), E, Q, dom.device, isotropic=True)
+    q = dom.whereZero(2)
+    op = dom.newOperator(1)
+    op.assemble({"A": cA}, None)
+    op.nullifyRowsAndCols(q, q, 1.0)
+    rng = np.random.Generator(np.random.PCG64(20261204))
+    n0 = ne[0] + 1
+    top = (ne[2]) * n0 * n0
+    nodes = top + rng.choice(n0 * n0, size=K, replace=False)
+    B = torch.zeros((op.size, K), dtype=torch.float64, device="cuda")
+    B[torch.as_tensor(nodes, device="cuda"), torch.arange(K, device="cuda")] = 1.0
+    o = _opts(R, "PCG_MIXED")
+    best, times = RB.autotune_many(op, K, fp32=True, reps=3)
+    t, (X, diags) = _timed(torch, lambda: RB.solve_many(op, B, o))
+    Bsub = B[:, :1].contiguous()
+    t1, (X1, d1) = _timed(torch, lambda: RB.solve_many(op, Bsub, _opts(R, "PCG_MIXED")))
+    R_ = B - RB.matrix_vector_many(op, X)
+    rel = (torch.linalg.norm(R_, dim=0) / torch.linalg.norm(B, dim=0)).max().item()
+    its = [d["num_iter"] for d in diags]
+    out["many_rhs"] = {"workload": "Brick 128^3, ln kappa GRF sigma 1.5, %d point loads on the top surface" % K,
+                       "unknowns": op.size, "rhs": K, "batched_solve_s": t, "single_rhs_solve_s": t1,
+                       "rhs_per_s_batched": K / t, "rhs_per_s_single": 1.0 / t1,
+                       "speedup_vs_one_at_a_time": (K * t1) / t, "iters_min_max": [min(its), max(its)],
+                       "max_rel_residual_fp64": rel, "all_converged": all(d["converged"] for d in diags),
+                       "autotune_rows_per_thread": best, "autotune_s_per_spmm": times}
+    del X, B, R_, op
+    # (ii) uneven 2-D instances
+    rng = np.random.Generator(np.random.PCG64(20261205))
+    sizes = [int(v) for v in rng.integers(64, 513, size=32)]
+    ops, rhss = [], []
+
+    def build(n, j):
+        d2 = R.Rectangle(n, n)
+        E2, Q2 = d2.getNumElements(), d2.getNumQuadraturePoints()
+        k2 = torch.as_tensor(fields.lognormal_kappa((n, n), corr_len=8.0, sigma_log=1.0, seed=20261205 + j),
+                             device="cuda")
+        c = R.Coefficient(k2, (2, 2), E2, Q2, d2.device, isotropic=True)
+        cy = R.Coefficient(1.0, (), E2, Q2, d2.device)
+        o2 = d2.newOperator(1)
+        rhs = torch.zeros(o2.size, dtype=torch.float64, device="cuda")
+        o2.assemble({"A": c, "Y": cy}, rhs)
+        qq = d2.whereZero(0)
+        o2.nullifyRowsAndCols(qq, qq, 1.0)
+        rhs.masked_fill_(qq > 0, 0.0)
+        return o2, rhs
+
+    for j, n in enumerate(sizes):
+        o2, rhs = build(n, j)
+        ops.append(o2)
+        rhss.append(rhs)
+    bs = RB.BatchSolver(ops)
+    tile, ttimes = bs.autotune(fp32=True, reps=3)
+    b = bs.join(rhss)
+    bs.solve(b, _opts(R, "PCG_MIXED"))   # warm-up (graph capture)
+    t, (x, diags) = _timed(torch, lambda: bs.solve(b, _opts(R, "PCG_MIXED")))
+    xs = bs.split(x)
+    worst = 0.0
+    for o2, rhs, xj in zip(ops, rhss, xs):
+        r = rhs.clone()
+        o2.MatrixVector(-1.0, xj, 1.0, r)
+        worst = max(worst, (torch.linalg.norm(r) / torch.linalg.norm(rhs)).item())
+    # the same 32 systems one after the other with the single-system solver
+    seq = 0.0
+    seq_iters = 0
+    for j, n in enumerate(sizes):
+        o2, rhs = build(n, j)
+        oo = _opts(R, "PCG_MIXED")
+        tj, _ = _timed(torch, lambda: o2.solve(rhs, oo))
+        seq += tj
+        seq_iters += oo.getDiagnostics("num_iter")
+    its = [d["num_iter"] for d in diags]
+    out["uneven_2d"] = {"workload": "32 Rectangles n^2, n ~ U{64..512}, ln kappa GRF sigma 1, Y=1",
+                        "sizes": sizes, "unknowns_total": int(sum((n + 1) ** 2 for n in sizes)),
+                        "batched_solve_s": t, "sequential_solve_s": seq, "speedup": seq / t,
+                        "iters_min_max": [min(its), max(its)], "max_rel_residual_fp64": worst,
+                        "all_converged": all(d["converged"] for d in diags),
+                        "autotune_tile_mult": tile, "autotune_s_per_spmv": ttimes}
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -295,6 +529,20 @@ def main():
         "device_copy_GBps": round(copy_bw, 1),
         "cpu_baseline": cpu,
     }
+    extras = {}
+    for name in [e.strip() for e in args.extras.split(",") if e.strip()]:
+        fn = {"c1": bench_c1, "c3": bench_c3, "c4": bench_c4}.get(name)
+        if fn is None:
+            continue
+        log(args, "config", name)
+        try:
+            t0 = time.perf_counter()
+            extras[name] = fn(R, torch, np, args)
+            extras[name]["wall_s"] = time.perf_counter() - t0
+        except Exception as ex:   # a failing extra configuration is reported, not fatal
+            extras[name] = {"error": "%s: %s" % (type(ex).__name__, ex)}
+        log(args, name, json.dumps(extras[name]))
+    line["configs"] = extras
     print(json.dumps(line), flush=True)
 
 

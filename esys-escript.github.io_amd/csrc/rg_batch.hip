@@ -1,0 +1,1214 @@
+// Batched Jacobi-PCG: (1) many right-hand sides on one operator -- the
+// stencil SpMV becomes an SpMM over an N x K row-major block, so the matrix
+// is streamed once per iteration for all K columns; (2) many independent
+// scalar systems of uneven sizes -- every kernel spans all systems through
+// a tile table (tile -> system, row range), so one iteration is one set of
+// launches for the whole batch.  Each column / system has its own Krylov
+// scalars (a KrylovState each, rg_krylov.h), its own convergence and its
+// own done flag; the iterations are captured once into a hipGraph and
+// replayed check_interval iterations at a time.
+//
+// Both solve the unbalanced system from x = 0 with the textbook Jacobi-PCG
+// (PCG.cpp:62-368 without PASO's residual smoothing), the SpMV in FP64 or
+// FP32 values (RG_METHOD_PCG_PLAIN / RG_METHOD_PCG_MIXED), all vectors,
+// dots and updates in FP64, and an FP64 residual replacement r = b - A x at
+// the end of every chunk; a column converges only on that FP64 residual:
+// ||b - A x||_2 <= tol ||b||_2 per column.  Tile shapes (rows per thread of
+// the SpMM, rows per tile of the batch) are picked by timing candidates on
+// the actual operator (autotune).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "rg_device.h"
+#include "rg_dispatch.h"
+#include "rg_krylov.h"
+#include "rg_stencil.h"
+
+namespace rg {
+
+namespace {
+
+double now_s()
+{
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+// ============================================================ many RHS (SpMM)
+// Thread t of a block handles column k = t % KP of rows (t / KP) * R ..
+// + R - 1 of each pass; KP = K rounded up to a power of two (lanes k >= K
+// idle).  Vectors are row-major N x K (column index fastest).
+struct MArgs {
+    const double* dinv;      // FP64 Jacobi inverse, ld stride (unused slots)
+    const double* b;
+    double* x;
+    double* r;
+    double* p;
+    double* q;
+    KrylovState* st;         // K states
+    double* partials;        // [quantity][K][G]
+    int64_t N, ld, s1, s2, chunk;
+    int K;
+};
+
+// Per-column block sums of NV values, thread t holding CPT consecutive
+// columns k = (t % KPC) * CPT + c in v[q][c]; partials[(q*K + k)*gridDim.x +
+// blockIdx.x] (fixed order).
+template <int KPC, int CPT, int NV>
+__device__ __forceinline__ void publish_column_sums(double (&v)[NV][CPT], double* partials, int K)
+{
+    __shared__ double red[NV][kBlock * CPT];
+#pragma unroll
+    for (int q = 0; q < NV; ++q)
+#pragma unroll
+        for (int c = 0; c < CPT; ++c) red[q][threadIdx.x * CPT + c] = v[q][c];
+    __syncthreads();
+    if ((int)threadIdx.x < K) {
+        const int kk = threadIdx.x / CPT, c = threadIdx.x % CPT;
+#pragma unroll
+        for (int q = 0; q < NV; ++q) {
+            double s = 0.0;
+            for (int t = kk; t < kBlock; t += KPC) s += red[q][t * CPT + c];
+            partials[((int64_t)q * K + threadIdx.x) * gridDim.x + blockIdx.x] = s;
+        }
+    }
+}
+
+template <int CPT>
+struct ColVec {
+    double v[CPT];
+};
+template <int CPT>
+__device__ __forceinline__ ColVec<CPT> load_cols(const double* p)
+{
+    ColVec<CPT> r;
+    if constexpr (CPT == 1) {
+        r.v[0] = p[0];
+    } else if constexpr (CPT == 2) {
+        const double2 t = *reinterpret_cast<const double2*>(p);
+        r.v[0] = t.x;
+        r.v[1] = t.y;
+    } else {
+        const double2 t0 = *reinterpret_cast<const double2*>(p);
+        const double2 t1 = *reinterpret_cast<const double2*>(p + 2);
+        r.v[0] = t0.x;
+        r.v[1] = t0.y;
+        r.v[2] = t1.x;
+        r.v[3] = t1.y;
+    }
+    return r;
+}
+
+// Y = A X (stencil) for R consecutive rows and CPT consecutive columns per
+// thread (KPC threads per row; x windows shared by the three dx slots of
+// each (dy, dz) pair), then epi(row, first column, acc[CPT]).
+template <int DIM, typename VT, int KPC, int R, int CPT, class Epi>
+__device__ __forceinline__ void spmm_rows(const VT* __restrict__ val, const MArgs& a, const double* __restrict__ X,
+                                          Epi&& epi)
+{
+    constexpr int NPL = (DIM == 3) ? 9 : 3;
+    constexpr int RPP = (kBlock / KPC) * R;
+    const int k0 = (threadIdx.x % KPC) * CPT;
+    const bool active = k0 < a.K;
+    const int64_t sub = (int64_t)(threadIdx.x / KPC) * R;
+    const RowRange rr = block_rows(a.N, a.chunk);
+    for (int64_t i0 = rr.r0 + sub; i0 < rr.r1; i0 += RPP) {
+        double acc[R][CPT];
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+#pragma unroll
+            for (int c = 0; c < CPT; ++c) acc[r][c] = 0.0;
+#pragma unroll
+        for (int pl = 0; pl < NPL; ++pl) {
+            const int dy = pl % 3 - 1, dz = (DIM == 3) ? pl / 3 - 1 : 0;
+            const int64_t b = i0 + dy * a.s1 + dz * a.s2;
+            ColVec<CPT> w[R + 2];
+#pragma unroll
+            for (int t = 0; t < R + 2; ++t) {
+                const int64_t j = b - 1 + t;
+                const bool ok = active && j >= 0 && j < a.N;
+                w[t] = load_cols<CPT>(X + (ok ? j : 0) * a.K + (active ? k0 : 0));
+                if (!ok)
+#pragma unroll
+                    for (int c = 0; c < CPT; ++c) w[t].v[c] = 0.0;
+            }
+#pragma unroll
+            for (int dx = 0; dx < 3; ++dx) {
+                const int s = dx + 3 * pl;
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    const int64_t row = i0 + r;
+                    const double v = (row < rr.r1) ? (double)val[(int64_t)s * a.ld + row] : 0.0;
+#pragma unroll
+                    for (int c = 0; c < CPT; ++c) acc[r][c] = fma(v, w[r + dx].v[c], acc[r][c]);
+                }
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+            if (active && i0 + r < rr.r1) epi(i0 + r, k0, acc[r]);
+    }
+}
+
+// q = A p, partial p.q per column
+template <int DIM, typename VT, int KPC, int R, int CPT>
+__global__ __launch_bounds__(kBlock) void k_mspmv_dot(MArgs a, const VT* __restrict__ val)
+{
+    double s[1][CPT] = {};
+    spmm_rows<DIM, VT, KPC, R, CPT>(val, a, a.p, [&](int64_t i, int k0, const double (&acc)[CPT]) {
+#pragma unroll
+        for (int c = 0; c < CPT; ++c) {
+            const int64_t e = i * a.K + k0 + c;
+            a.q[e] = acc[c];
+            s[0][c] = fma(acc[c], a.p[e], s[0][c]);
+        }
+    });
+    publish_column_sums<KPC, CPT, 1>(s, a.partials, a.K);
+}
+
+// y = A x (plain SpMM, for the autotuner and rg_matrix_vector_multi)
+template <int DIM, typename VT, int KPC, int R, int CPT>
+__global__ __launch_bounds__(kBlock) void k_mspmv(MArgs a, const VT* __restrict__ val, const double* __restrict__ X,
+                                                  double* __restrict__ Y)
+{
+    spmm_rows<DIM, VT, KPC, R, CPT>(val, a, X, [&](int64_t i, int k0, const double (&acc)[CPT]) {
+#pragma unroll
+        for (int c = 0; c < CPT; ++c) Y[i * a.K + k0 + c] = acc[c];
+    });
+}
+
+// r = b - A x (FP64 values), partial r.D^-1 r and r.r per column
+template <int DIM, int KPC, int R, int CPT>
+__global__ __launch_bounds__(kBlock) void k_mreplace(MArgs a, const double* __restrict__ val)
+{
+    double s[2][CPT] = {};
+    spmm_rows<DIM, double, KPC, R, CPT>(val, a, a.x, [&](int64_t i, int k0, const double (&acc)[CPT]) {
+#pragma unroll
+        for (int c = 0; c < CPT; ++c) {
+            if (a.st[k0 + c].done) continue;
+            const int64_t e = i * a.K + k0 + c;
+            const double rv = a.b[e] - acc[c];
+            a.r[e] = rv;
+            s[0][c] += rv * a.dinv[i] * rv;
+            s[1][c] += rv * rv;
+        }
+    });
+    publish_column_sums<KPC, CPT, 2>(s, a.partials, a.K);
+}
+
+// element loop of the vector kernels: (row i, column k) pairs of this block
+template <int KP, class F>
+__device__ __forceinline__ void column_loop(const MArgs& a, F&& f)
+{
+    const int k = threadIdx.x % KP;
+    if (k >= a.K) return;
+    const RowRange rr = block_rows(a.N, a.chunk);
+    for (int64_t i = rr.r0 + threadIdx.x / KP; i < rr.r1; i += kBlock / KP) f(i, k);
+}
+
+// r = b, p = D^-1 r, partial r.D^-1 r and r.r (x = 0)
+template <int KP>
+__global__ __launch_bounds__(kBlock) void k_minit(MArgs a)
+{
+    double s[2][1] = {};
+    column_loop<KP>(a, [&](int64_t i, int k) {
+        const int64_t e = i * a.K + k;
+        const double rv = a.b[e];
+        const double z = a.dinv[i] * rv;
+        a.r[e] = rv;
+        a.x[e] = 0.0;
+        a.p[e] = z;
+        s[0][0] += rv * z;
+        s[1][0] += rv * rv;
+    });
+    publish_column_sums<KP, 1, 2>(s, a.partials, a.K);
+}
+
+// x += alpha p, r -= alpha q, partial r.D^-1 r and r.r per column
+template <int KP>
+__global__ __launch_bounds__(kBlock) void k_mupdate(MArgs a)
+{
+    double s[2][1] = {};
+    const int k = threadIdx.x % KP;
+    const bool live = k < a.K && !a.st[k < a.K ? k : 0].done;
+    const double alpha = live ? a.st[k].alpha : 0.0;
+    column_loop<KP>(a, [&](int64_t i, int kk) {
+        if (!live) return;
+        const int64_t e = i * a.K + kk;
+        a.x[e] = fma(alpha, a.p[e], a.x[e]);
+        const double rv = fma(-alpha, a.q[e], a.r[e]);
+        a.r[e] = rv;
+        s[0][0] += rv * a.dinv[i] * rv;
+        s[1][0] += rv * rv;
+    });
+    publish_column_sums<KP, 1, 2>(s, a.partials, a.K);
+}
+
+// p = D^-1 r + beta p
+template <int KP>
+__global__ __launch_bounds__(kBlock) void k_mpupdate(MArgs a)
+{
+    const int k = threadIdx.x % KP;
+    const bool live = k < a.K && !a.st[k < a.K ? k : 0].done;
+    const double beta = live ? a.st[k].beta : 0.0;
+    column_loop<KP>(a, [&](int64_t i, int kk) {
+        if (!live) return;
+        const int64_t e = i * a.K + kk;
+        a.p[e] = fma(beta, a.p[e], a.dinv[i] * a.r[e]);
+    });
+}
+
+// one workgroup per column: reduce its partials, apply the stage
+template <int STAGE, int NV>
+__global__ __launch_bounds__(kBlock) void k_mstage(KrylovState* sts, const double* __restrict__ partials, int G, int K)
+{
+    KrylovState* st = sts + blockIdx.x;
+    if (!stage_starts<STAGE>() && st->done) return;
+    double tot[NV];
+#pragma unroll
+    for (int q = 0; q < NV; ++q) {
+        const double* p = partials + ((int64_t)q * K + blockIdx.x) * G;
+        double s = 0.0;
+        for (int b = threadIdx.x; b < G; b += kBlock) s += p[b];
+        tot[q] = s;
+    }
+    block_sum<NV>(tot);
+    if (threadIdx.x != 0) return;
+    stage_apply<STAGE, NV>(st, tot);
+}
+
+// dinv[i] = 1 / a_ii of a scalar stencil matrix (main slot), unbalanced
+__global__ void k_diag_inv(const double* __restrict__ val, int64_t ld, int64_t N, int center, double* __restrict__ d,
+                           int* __restrict__ flag)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (int64_t)gridDim.x * blockDim.x) {
+        const double a = val[(int64_t)center * ld + i];
+        if (a == 0.0) flag[0] = 1;
+        d[i] = a != 0.0 ? 1.0 / a : 0.0;
+    }
+}
+
+// ========================================================= batch of systems
+struct Inst {
+    const double* val;
+    const float* val32;
+    const double* dinv;
+    int64_t N, ld, s1, s2;
+    int64_t off;    // offset of the system's vectors in the batch vectors
+};
+struct Tile {
+    int32_t inst;
+    int32_t pad;
+    int64_t r0, r1;
+};
+struct BArgs {
+    const Inst* inst;
+    const Tile* tiles;
+    const double* b;
+    double* x;
+    double* r;
+    double* p;
+    double* q;
+    KrylovState* st;
+    double* partials;   // [quantity][G]
+};
+
+template <int NV>
+__device__ __forceinline__ void publish_tile_sums(double (&v)[NV], double* partials)
+{
+    publish_block_sums<NV>(v, partials);
+}
+
+template <int DIM, typename VT>
+__global__ __launch_bounds__(kBlock) void k_bspmv_dot(BArgs a)
+{
+    const Tile t = a.tiles[blockIdx.x];
+    const Inst in = a.inst[t.inst];
+    double s[1] = {0.0};
+    if (!a.st[t.inst].done) {
+        const VT* val = std::is_same<VT, float>::value ? (const VT*)in.val32 : (const VT*)in.val;
+        const double* p = a.p + in.off;
+        double* q = a.q + in.off;
+        stencil_block<DIM, 1, false, VT>(val, in.ld, in.N, in.s1, in.s2, t.r0, t.r1, p,
+                                         [&](int64_t i, const double (&acc)[1]) {
+                                             q[i] = acc[0];
+                                             s[0] = fma(acc[0], p[i], s[0]);
+                                         });
+    }
+    publish_tile_sums<1>(s, a.partials);
+}
+
+template <int DIM, typename VT>
+__global__ __launch_bounds__(kBlock) void k_bspmv(BArgs a, const double* __restrict__ X, double* __restrict__ Y)
+{
+    const Tile t = a.tiles[blockIdx.x];
+    const Inst in = a.inst[t.inst];
+    const VT* val = std::is_same<VT, float>::value ? (const VT*)in.val32 : (const VT*)in.val;
+    const double* x = X + in.off;
+    double* y = Y + in.off;
+    stencil_block<DIM, 1, false, VT>(val, in.ld, in.N, in.s1, in.s2, t.r0, t.r1, x,
+                                     [&](int64_t i, const double (&acc)[1]) { y[i] = acc[0]; });
+}
+
+template <int DIM>
+__global__ __launch_bounds__(kBlock) void k_breplace(BArgs a)
+{
+    const Tile t = a.tiles[blockIdx.x];
+    const Inst in = a.inst[t.inst];
+    double s[2] = {0.0, 0.0};
+    if (!a.st[t.inst].done) {
+        const double* x = a.x + in.off;
+        const double* b = a.b + in.off;
+        double* r = a.r + in.off;
+        stencil_block<DIM, 1, false, double>(in.val, in.ld, in.N, in.s1, in.s2, t.r0, t.r1, x,
+                                             [&](int64_t i, const double (&acc)[1]) {
+                                                 const double rv = b[i] - acc[0];
+                                                 r[i] = rv;
+                                                 s[0] += rv * in.dinv[i] * rv;
+                                                 s[1] += rv * rv;
+                                             });
+    }
+    publish_tile_sums<2>(s, a.partials);
+}
+
+__global__ __launch_bounds__(kBlock) void k_binit(BArgs a)
+{
+    const Tile t = a.tiles[blockIdx.x];
+    const Inst in = a.inst[t.inst];
+    double s[2] = {0.0, 0.0};
+    for (int64_t i = t.r0 + threadIdx.x; i < t.r1; i += kBlock) {
+        const int64_t e = in.off + i;
+        const double rv = a.b[e];
+        const double z = in.dinv[i] * rv;
+        a.r[e] = rv;
+        a.x[e] = 0.0;
+        a.p[e] = z;
+        s[0] += rv * z;
+        s[1] += rv * rv;
+    }
+    publish_tile_sums<2>(s, a.partials);
+}
+
+__global__ __launch_bounds__(kBlock) void k_bupdate(BArgs a)
+{
+    const Tile t = a.tiles[blockIdx.x];
+    const Inst in = a.inst[t.inst];
+    double s[2] = {0.0, 0.0};
+    if (!a.st[t.inst].done) {
+        const double alpha = a.st[t.inst].alpha;
+        for (int64_t i = t.r0 + threadIdx.x; i < t.r1; i += kBlock) {
+            const int64_t e = in.off + i;
+            a.x[e] = fma(alpha, a.p[e], a.x[e]);
+            const double rv = fma(-alpha, a.q[e], a.r[e]);
+            a.r[e] = rv;
+            s[0] += rv * in.dinv[i] * rv;
+            s[1] += rv * rv;
+        }
+    }
+    publish_tile_sums<2>(s, a.partials);
+}
+
+__global__ __launch_bounds__(kBlock) void k_bpupdate(BArgs a)
+{
+    const Tile t = a.tiles[blockIdx.x];
+    const Inst in = a.inst[t.inst];
+    if (a.st[t.inst].done) return;
+    const double beta = a.st[t.inst].beta;
+    for (int64_t i = t.r0 + threadIdx.x; i < t.r1; i += kBlock) {
+        const int64_t e = in.off + i;
+        a.p[e] = fma(beta, a.p[e], in.dinv[i] * a.r[e]);
+    }
+}
+
+// one workgroup per system: reduce the partials of its tiles [tb[j], tb[j+1])
+template <int STAGE, int NV>
+__global__ __launch_bounds__(kBlock) void k_bstage(KrylovState* sts, const double* __restrict__ partials, int G,
+                                                   const int* __restrict__ tb)
+{
+    KrylovState* st = sts + blockIdx.x;
+    if (!stage_starts<STAGE>() && st->done) return;
+    const int b0 = tb[blockIdx.x], b1 = tb[blockIdx.x + 1];
+    double tot[NV];
+#pragma unroll
+    for (int q = 0; q < NV; ++q) {
+        const double* p = partials + (int64_t)q * G;
+        double s = 0.0;
+        for (int b = b0 + threadIdx.x; b < b1; b += kBlock) s += p[b];
+        tot[q] = s;
+    }
+    block_sum<NV>(tot);
+    if (threadIdx.x != 0) return;
+    stage_apply<STAGE, NV>(st, tot);
+}
+
+// ================================================================ host side
+template <class T>
+struct DevArray {
+    T* p = nullptr;
+    size_t n = 0;
+    void alloc(size_t count)
+    {
+        if (count <= n && p) return;
+        if (p) hipFree(p);
+        p = nullptr;
+        n = 0;
+        RG_HIP(hipMalloc(&p, sizeof(T) * std::max<size_t>(count, 1)));
+        n = count;
+    }
+    ~DevArray()
+    {
+        if (p) hipFree(p);
+    }
+};
+
+// The chunked, graph-captured iteration loop shared by both batched
+// solvers.  iteration(s, last) issues one iteration on stream s.
+struct ChunkRunner {
+    hipStream_t stream = nullptr;      // caller's stream
+    hipStream_t capture = nullptr;     // internal capture stream
+    hipGraphExec_t graph = nullptr;
+    int graph_iters = -1;
+    std::vector<unsigned char> graph_key;   // everything the captured launches depend on
+
+    ~ChunkRunner()
+    {
+        if (graph) hipGraphExecDestroy(graph);
+        if (capture) hipStreamDestroy(capture);
+    }
+
+    template <class F>
+    void run(const std::vector<unsigned char>& key, int iters, F&& iteration)
+    {
+        if (!graph || graph_iters != iters || graph_key != key) {
+            if (graph) {
+                RG_HIP(hipGraphExecDestroy(graph));
+                graph = nullptr;
+            }
+            if (!capture) RG_HIP(hipStreamCreateWithFlags(&capture, hipStreamNonBlocking));
+            hipGraph_t g;
+            RG_HIP(hipStreamBeginCapture(capture, hipStreamCaptureModeThreadLocal));
+            try {
+                for (int k = 0; k < iters; ++k) iteration(capture, k == iters - 1);
+            } catch (...) {
+                hipStreamEndCapture(capture, &g);
+                throw;
+            }
+            RG_HIP(hipStreamEndCapture(capture, &g));
+            RG_HIP(hipGraphInstantiate(&graph, g, nullptr, nullptr, 0));
+            RG_HIP(hipGraphDestroy(g));
+            graph_iters = iters;
+            graph_key = key;
+        }
+        RG_HIP(hipGraphLaunch(graph, stream));
+    }
+};
+
+// Polls `count` Krylov states every chunk (two chunks in flight) until all
+// are done.
+template <class Chunk>
+void poll_until_done(hipStream_t st, KrylovState* d_states, int count, Chunk&& chunk, std::vector<KrylovState>& out)
+{
+    KrylovState* pin = nullptr;
+    RG_HIP(hipHostMalloc(&pin, sizeof(KrylovState) * 2 * count));
+    hipEvent_t evs[2] = {nullptr, nullptr};
+    auto cleanup = [&]() {
+        if (evs[0]) hipEventDestroy(evs[0]);
+        if (evs[1]) hipEventDestroy(evs[1]);
+        hipHostFree(pin);
+    };
+    try {
+        RG_HIP(hipEventCreateWithFlags(&evs[0], hipEventDisableTiming));
+        RG_HIP(hipEventCreateWithFlags(&evs[1], hipEventDisableTiming));
+        int launched = 0;
+        bool done = false;
+        while (!done) {
+            chunk();
+            KrylovState* slot = pin + (launched % 2) * count;
+            RG_HIP(hipMemcpyAsync(slot, d_states, sizeof(KrylovState) * count, hipMemcpyDeviceToHost, st));
+            RG_HIP(hipEventRecord(evs[launched % 2], st));
+            ++launched;
+            if (launched >= 2) {
+                RG_HIP(hipEventSynchronize(evs[(launched - 2) % 2]));
+                const KrylovState* s = pin + ((launched - 2) % 2) * count;
+                done = true;
+                for (int j = 0; j < count; ++j) done = done && s[j].done;
+            }
+        }
+        RG_HIP(hipStreamSynchronize(st));
+        out.resize(count);
+        RG_HIP(hipMemcpy(out.data(), d_states, sizeof(KrylovState) * count, hipMemcpyDeviceToHost));
+    } catch (...) {
+        cleanup();
+        throw;
+    }
+    cleanup();
+}
+
+void init_states(hipStream_t st, KrylovState* d, int count, double rtol, int maxit)
+{
+    std::vector<KrylovState> h(count);
+    for (auto& s : h) {
+        std::memset(&s, 0, sizeof(s));
+        s.tol = rtol;
+        s.maxit = maxit;
+        s.replace_mode = 1;
+    }
+    RG_HIP(hipMemcpyAsync(d, h.data(), sizeof(KrylovState) * count, hipMemcpyHostToDevice, st));
+    RG_HIP(hipStreamSynchronize(st));
+}
+
+void fill_diagnostics(const std::vector<KrylovState>& s, double t, rg_diagnostics* dg, int count)
+{
+    for (int j = 0; j < count; ++j) {
+        rg_diagnostics& d = dg[j];
+        d.num_iter = s[j].num_iter;
+        d.converged = s[j].status == KS_CONVERGED;
+        d.status = s[j].status == KS_CONVERGED ? RG_OK
+                   : s[j].status == KS_MAXITER ? RG_MAX_ITER_REACHED
+                                               : RG_BREAKDOWN;
+        d.outer_steps = 1;
+        d.residual_norm = s[j].norm;
+        d.set_up_time = 0.0;
+        d.net_time = t;
+        d.time = t;
+        d.fallbacks = 0;
+        d.reserved = 0;
+    }
+}
+
+int batch_status(const rg_diagnostics* dg, int count, bool accept)
+{
+    int worst = RG_OK;
+    for (int j = 0; j < count; ++j)
+        if (dg[j].status != RG_OK && !(accept && dg[j].status == RG_MAX_ITER_REACHED)) worst = dg[j].status;
+    return worst;
+}
+
+// Jacobi inverse (unbalanced) and, for FP32 values, the FP32 copy.
+void prepare_matrix(rg_matrix* m, bool fp32, double* dinv)
+{
+    const Geom& g = m->g;
+    RG_CHECK(!m->is_balanced, RG_STATE_ERROR,
+             "batched solve: the matrix was balanced by an earlier solve; set or reset its values first");
+    hipStream_t st = m->ctx->stream;
+    DevArray<int> flag;
+    flag.alloc(1);
+    RG_HIP(hipMemsetAsync(flag.p, 0, sizeof(int), st));
+    const int center = (g.dim == 3 ? 27 : 9) / 2;
+    hipLaunchKernelGGL(k_diag_inv, dim3(std::min<int64_t>((g.N + 255) / 256, 4096)), dim3(256), 0, st, m->val, g.ld,
+                       g.N, center, dinv, flag.p);
+    RG_HIP(hipGetLastError());
+    int failed = 0;
+    RG_HIP(hipMemcpyAsync(&failed, flag.p, sizeof(int), hipMemcpyDeviceToHost, st));
+    RG_HIP(hipStreamSynchronize(st));
+    RG_CHECK(!failed, RG_VALUE_ERROR, "batched solve: zero on the main diagonal");
+    if (fp32 && !m->val32_valid) {
+        if (!m->val32) {
+            const size_t cnt = (size_t)m->g.nstore * m->g.ld;
+            RG_HIP(hipMalloc(&m->val32, cnt * sizeof(float)));
+        }
+        launch_to_fp32(m);
+        m->val32_valid = true;
+    }
+}
+
+template <class F>
+void dispatch_kp(int K, F&& f)
+{
+    if (K <= 1) f(IC<1>{});
+    else if (K <= 2) f(IC<2>{});
+    else if (K <= 4) f(IC<4>{});
+    else if (K <= 8) f(IC<8>{});
+    else if (K <= 16) f(IC<16>{});
+    else if (K <= 32) f(IC<32>{});
+    else f(IC<64>{});
+}
+
+
+}  // namespace
+
+// ---------------------------------------------------------------- many RHS
+// SpMM tile shapes: (rows per thread R, columns per thread CPT).  A shape
+// applies when CPT divides the number of columns.
+constexpr int kNumShapes = 4;
+constexpr int kShapeR[kNumShapes] = {1, 1, 1, 2};
+constexpr int kShapeC[kNumShapes] = {1, 2, 4, 2};
+
+inline int pow2_at_least(int v)
+{
+    int p = 1;
+    while (p < v) p <<= 1;
+    return p;
+}
+
+// f(IC<KPC>, IC<R>, IC<CPT>) for shape `shape` and K columns
+template <class F>
+void dispatch_shape(int K, int shape, F&& f)
+{
+    const int R = kShapeR[shape], C = kShapeC[shape];
+    RG_CHECK(K % C == 0, RG_VALUE_ERROR, "SpMM shape does not divide the column count");
+    const int KPC = pow2_at_least(K / C);
+    auto go = [&](auto RC, auto CC) {
+        switch (KPC) {
+            case 1: f(IC<1>{}, RC, CC); break;
+            case 2: f(IC<2>{}, RC, CC); break;
+            case 4: f(IC<4>{}, RC, CC); break;
+            case 8: f(IC<8>{}, RC, CC); break;
+            case 16: f(IC<16>{}, RC, CC); break;
+            case 32: f(IC<32>{}, RC, CC); break;
+            default: f(IC<64>{}, RC, CC); break;
+        }
+    };
+    if (R == 1 && C == 1) go(IC<1>{}, IC<1>{});
+    else if (R == 1 && C == 2) go(IC<1>{}, IC<2>{});
+    else if (R == 1 && C == 4) go(IC<1>{}, IC<4>{});
+    else go(IC<2>{}, IC<2>{});
+}
+
+struct MultiSolver {
+    rg_matrix* m;
+    int K, KP, shape;
+    bool fp32;
+    hipStream_t st;
+    MArgs a;
+    Launch L;       // SpMM / vector kernels
+
+    MultiSolver(rg_matrix* mm, int k, int shp, bool f32) : m(mm), K(k), shape(shp), fp32(f32), st(mm->ctx->stream)
+    {
+        KP = pow2_at_least(K);
+        const int kpc = pow2_at_least(K / kShapeC[shape]);
+        const int64_t rpp = (int64_t)(kBlock / kpc) * kShapeR[shape];
+        L = chunked_launch((m->g.N + rpp - 1) / rpp * kBlock, 1, 2048);
+        // chunked_launch counted threads; convert to rows per block
+        L.chunk = L.chunk / kBlock * rpp;
+        std::memset(&a, 0, sizeof(a));
+        a.N = m->g.N; a.ld = m->g.ld; a.s1 = m->g.s1; a.s2 = m->g.s2; a.chunk = L.chunk; a.K = K;
+    }
+
+    void spmm(const double* X, double* Y)
+    {
+        const Geom& g = m->g;
+        dispatch_shape(K, shape, [&](auto KPC, auto RC, auto CC) {
+            constexpr int P = KPC.value, Rv = RC.value, Cv = CC.value;
+            if (g.dim == 3) {
+                if (fp32) hipLaunchKernelGGL((k_mspmv<3, float, P, Rv, Cv>), dim3(L.grid), dim3(kBlock), 0, st, a, m->val32, X, Y);
+                else hipLaunchKernelGGL((k_mspmv<3, double, P, Rv, Cv>), dim3(L.grid), dim3(kBlock), 0, st, a, m->val, X, Y);
+            } else {
+                if (fp32) hipLaunchKernelGGL((k_mspmv<2, float, P, Rv, Cv>), dim3(L.grid), dim3(kBlock), 0, st, a, m->val32, X, Y);
+                else hipLaunchKernelGGL((k_mspmv<2, double, P, Rv, Cv>), dim3(L.grid), dim3(kBlock), 0, st, a, m->val, X, Y);
+            }
+        });
+        RG_HIP(hipGetLastError());
+    }
+
+    template <int STAGE, int NV>
+    void stage(hipStream_t s)
+    {
+        hipLaunchKernelGGL((k_mstage<STAGE, NV>), dim3(K), dim3(kBlock), 0, s, a.st, a.partials, L.grid, K);
+    }
+
+    void init(hipStream_t s)
+    {
+        dispatch_kp(K, [&](auto KPC) {
+            hipLaunchKernelGGL((k_minit<KPC.value>), dim3(L.grid), dim3(kBlock), 0, s, a);
+        });
+        stage<ST_BATCH_INIT, 2>(s);
+        RG_HIP(hipGetLastError());
+    }
+
+    void iteration(hipStream_t s, bool last)
+    {
+        const Geom& g = m->g;
+        dispatch_shape(K, shape, [&](auto KPC, auto RC, auto CC) {
+            constexpr int P = KPC.value, Rv = RC.value, Cv = CC.value;
+            if (g.dim == 3) {
+                if (fp32) hipLaunchKernelGGL((k_mspmv_dot<3, float, P, Rv, Cv>), dim3(L.grid), dim3(kBlock), 0, s, a, m->val32);
+                else hipLaunchKernelGGL((k_mspmv_dot<3, double, P, Rv, Cv>), dim3(L.grid), dim3(kBlock), 0, s, a, m->val);
+            } else {
+                if (fp32) hipLaunchKernelGGL((k_mspmv_dot<2, float, P, Rv, Cv>), dim3(L.grid), dim3(kBlock), 0, s, a, m->val32);
+                else hipLaunchKernelGGL((k_mspmv_dot<2, double, P, Rv, Cv>), dim3(L.grid), dim3(kBlock), 0, s, a, m->val);
+            }
+        });
+        stage<ST_CG_SPMV, 1>(s);
+        dispatch_kp(K, [&](auto KPC) {
+            hipLaunchKernelGGL((k_mupdate<KPC.value>), dim3(L.grid), dim3(kBlock), 0, s, a);
+        });
+        stage<ST_CG_UPDATE, 2>(s);
+        if (last) {
+            dispatch_shape(K, shape, [&](auto KPC, auto RC, auto CC) {
+                constexpr int P = KPC.value, Rv = RC.value, Cv = CC.value;
+                if (g.dim == 3) hipLaunchKernelGGL((k_mreplace<3, P, Rv, Cv>), dim3(L.grid), dim3(kBlock), 0, s, a, m->val);
+                else hipLaunchKernelGGL((k_mreplace<2, P, Rv, Cv>), dim3(L.grid), dim3(kBlock), 0, s, a, m->val);
+            });
+            stage<ST_CG_REPLACE, 2>(s);
+        }
+        dispatch_kp(K, [&](auto KPC) {
+            hipLaunchKernelGGL((k_mpupdate<KPC.value>), dim3(L.grid), dim3(kBlock), 0, s, a);
+        });
+        RG_HIP(hipGetLastError());
+    }
+};
+
+// Times `reps` SpMMs for each applicable shape; returns the best.
+int autotune_multi(rg_matrix* m, int K, bool fp32, const double* X, double* Y, int reps, double* times)
+{
+    hipStream_t st = m->ctx->stream;
+    hipEvent_t e0, e1;
+    RG_HIP(hipEventCreate(&e0));
+    RG_HIP(hipEventCreate(&e1));
+    int best = 0;
+    float bt = 1e30f;
+    for (int c = 0; c < kNumShapes; ++c) {
+        if (times) times[c] = -1.0;
+        if (K % kShapeC[c]) continue;
+        MultiSolver S(m, K, c, fp32);
+        S.spmm(X, Y);   // warm-up
+        RG_HIP(hipEventRecord(e0, st));
+        for (int k = 0; k < reps; ++k) S.spmm(X, Y);
+        RG_HIP(hipEventRecord(e1, st));
+        RG_HIP(hipEventSynchronize(e1));
+        float ms = 0.f;
+        RG_HIP(hipEventElapsedTime(&ms, e0, e1));
+        if (times) times[c] = ms / 1e3 / reps;
+        if (ms < bt) {
+            bt = ms;
+            best = c;
+        }
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    return best;
+}
+
+int solve_multi(rg_matrix* m, double* X, const double* B, int K, const rg_options& opt, rg_diagnostics* dg)
+{
+    const double t0 = now_s();
+    const Geom& g = m->g;
+    RG_CHECK(g.bs == 1 && !m->symmetric, RG_VALUE_ERROR, "solve_multi: scalar matrices with full storage only");
+    RG_CHECK(K >= 1 && K <= 64, RG_VALUE_ERROR, "solve_multi: 1 to 64 right-hand sides");
+    RG_CHECK(opt.method == RG_METHOD_PCG_PLAIN || opt.method == RG_METHOD_PCG_MIXED, RG_VALUE_ERROR,
+             "solve_multi: method must be RG_METHOD_PCG_PLAIN or RG_METHOD_PCG_MIXED");
+    RG_CHECK(opt.tolerance > 0.0 && opt.tolerance <= 1.0, RG_VALUE_ERROR, "solve_multi: tolerance must be in (0, 1]");
+    const bool fp32 = opt.method == RG_METHOD_PCG_MIXED;
+    hipStream_t st = m->ctx->stream;
+    const size_t nv = (size_t)g.N * K;
+    DevArray<double> dinv, r, p, q, part;
+    DevArray<KrylovState> sts;
+    dinv.alloc(g.ld);
+    r.alloc(nv);
+    p.alloc(nv);
+    q.alloc(nv);
+    sts.alloc(K);
+    prepare_matrix(m, fp32, dinv.p);
+    int shape = m->multi_shape - 1;
+    if (shape < 0 || K % kShapeC[shape]) {
+        shape = autotune_multi(m, K, fp32, B, q.p, 5, nullptr);
+        m->multi_shape = shape + 1;
+    }
+    MultiSolver S(m, K, shape, fp32);
+    part.alloc((size_t)2 * K * S.L.grid);
+    S.a.dinv = dinv.p; S.a.b = B; S.a.x = X; S.a.r = r.p; S.a.p = p.p; S.a.q = q.p;
+    S.a.st = sts.p; S.a.partials = part.p;
+    init_states(st, sts.p, K, opt.tolerance, opt.iter_max);
+    S.init(st);
+    const int check = opt.check_interval > 0 ? opt.check_interval : 32;
+    ChunkRunner run;
+    run.stream = st;
+    std::vector<KrylovState> fin;
+    const std::vector<unsigned char> key;
+    poll_until_done(st, sts.p, K, [&]() { run.run(key, check, [&](hipStream_t s, bool last) { S.iteration(s, last); }); },
+                    fin);
+    fill_diagnostics(fin, now_s() - t0, dg, K);
+    return batch_status(dg, K, opt.accept_failed_convergence != 0);
+}
+
+void multi_vector(rg_matrix* m, const double* X, double* Y, int K)
+{
+    RG_CHECK(m->g.bs == 1 && !m->symmetric, RG_VALUE_ERROR, "matrix_vector_multi: scalar matrices with full storage only");
+    RG_CHECK(K >= 1 && K <= 64, RG_VALUE_ERROR, "matrix_vector_multi: 1 to 64 columns");
+    RG_CHECK(!m->is_balanced, RG_STATE_ERROR, "matrix_vector_multi: the matrix is balanced (set or reset values first)");
+    int shape = m->multi_shape - 1;
+    if (shape < 0 || K % kShapeC[shape]) shape = 0;
+    MultiSolver S(m, K, shape, false);
+    S.spmm(X, Y);
+}
+
+}  // namespace rg
+
+// ---------------------------------------------------------- batch of systems
+struct rg_batch {
+    rg_context* ctx = nullptr;
+    std::vector<rg_matrix*> mats;
+    int dim = 2;
+    std::vector<int64_t> offsets;   // count + 1 (aligned to 64 doubles)
+    int tile_mult = 0;              // rows per tile = kBlock * R * tile_mult (0: autotune)
+    rg::DevArray<rg::Inst> inst;
+    rg::DevArray<rg::Tile> tiles;
+    rg::DevArray<int> tile_begin;
+    rg::DevArray<double> dinv;
+    rg::DevArray<double> r, p, q, part;
+    rg::DevArray<rg::KrylovState> sts;
+    int G = 0;
+    int built_mult = -1;
+    bool built_fp32 = false;
+    rg::ChunkRunner run;
+};
+
+namespace rg {
+
+namespace {
+
+constexpr int kTileCands[4] = {1, 2, 4, 8};
+
+void build_tiles(rg_batch* B, int mult, bool fp32)
+{
+    const int R = fp32 ? spmv_rows_per_thread<float>() : spmv_rows_per_thread<double>();
+    const int64_t tile = (int64_t)kBlock * R * mult;
+    std::vector<Tile> t;
+    std::vector<int> tb;
+    for (size_t j = 0; j < B->mats.size(); ++j) {
+        tb.push_back((int)t.size());
+        const int64_t N = B->mats[j]->g.N;
+        for (int64_t r0 = 0; r0 < N; r0 += tile) t.push_back(Tile{(int32_t)j, 0, r0, std::min(N, r0 + tile)});
+    }
+    tb.push_back((int)t.size());
+    B->G = (int)t.size();
+    hipStream_t st = B->ctx->stream;
+    B->tiles.alloc(t.size());
+    B->tile_begin.alloc(tb.size());
+    RG_HIP(hipMemcpyAsync(B->tiles.p, t.data(), sizeof(Tile) * t.size(), hipMemcpyHostToDevice, st));
+    RG_HIP(hipMemcpyAsync(B->tile_begin.p, tb.data(), sizeof(int) * tb.size(), hipMemcpyHostToDevice, st));
+    RG_HIP(hipStreamSynchronize(st));
+    B->built_mult = mult;
+    B->built_fp32 = fp32;
+}
+
+void build_inst(rg_batch* B)
+{
+    std::vector<Inst> h;
+    for (size_t j = 0; j < B->mats.size(); ++j) {
+        const rg_matrix* m = B->mats[j];
+        Inst in;
+        in.val = m->val;
+        in.val32 = m->val32;
+        in.dinv = B->dinv.p + B->offsets[j];
+        in.N = m->g.N; in.ld = m->g.ld; in.s1 = m->g.s1; in.s2 = m->g.s2;
+        in.off = B->offsets[j];
+        h.push_back(in);
+    }
+    B->inst.alloc(h.size());
+    RG_HIP(hipMemcpyAsync(B->inst.p, h.data(), sizeof(Inst) * h.size(), hipMemcpyHostToDevice, B->ctx->stream));
+    RG_HIP(hipStreamSynchronize(B->ctx->stream));
+}
+
+BArgs bargs(rg_batch* B, const double* b, double* x)
+{
+    BArgs a;
+    a.inst = B->inst.p; a.tiles = B->tiles.p; a.b = b; a.x = x;
+    a.r = B->r.p; a.p = B->p.p; a.q = B->q.p; a.st = B->sts.p; a.partials = B->part.p;
+    return a;
+}
+
+void batch_spmv(rg_batch* B, bool fp32, const double* X, double* Y)
+{
+    BArgs a = bargs(B, nullptr, nullptr);
+    hipStream_t st = B->ctx->stream;
+    if (B->dim == 3) {
+        if (fp32) hipLaunchKernelGGL((k_bspmv<3, float>), dim3(B->G), dim3(kBlock), 0, st, a, X, Y);
+        else hipLaunchKernelGGL((k_bspmv<3, double>), dim3(B->G), dim3(kBlock), 0, st, a, X, Y);
+    } else {
+        if (fp32) hipLaunchKernelGGL((k_bspmv<2, float>), dim3(B->G), dim3(kBlock), 0, st, a, X, Y);
+        else hipLaunchKernelGGL((k_bspmv<2, double>), dim3(B->G), dim3(kBlock), 0, st, a, X, Y);
+    }
+    RG_HIP(hipGetLastError());
+}
+
+template <int STAGE, int NV>
+void bstage(rg_batch* B, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_bstage<STAGE, NV>), dim3((int)B->mats.size()), dim3(kBlock), 0, s, B->sts.p, B->part.p, B->G,
+                       B->tile_begin.p);
+}
+
+void batch_iteration(rg_batch* B, bool fp32, const BArgs& a, hipStream_t s, bool last)
+{
+    const int G = B->G;
+    auto spmv_dot = [&](auto D) {
+        if (fp32) hipLaunchKernelGGL((k_bspmv_dot<D.value, float>), dim3(G), dim3(kBlock), 0, s, a);
+        else hipLaunchKernelGGL((k_bspmv_dot<D.value, double>), dim3(G), dim3(kBlock), 0, s, a);
+    };
+    if (B->dim == 3) spmv_dot(IC<3>{});
+    else spmv_dot(IC<2>{});
+    bstage<ST_CG_SPMV, 1>(B, s);
+    hipLaunchKernelGGL(k_bupdate, dim3(G), dim3(kBlock), 0, s, a);
+    bstage<ST_CG_UPDATE, 2>(B, s);
+    if (last) {
+        if (B->dim == 3) hipLaunchKernelGGL((k_breplace<3>), dim3(G), dim3(kBlock), 0, s, a);
+        else hipLaunchKernelGGL((k_breplace<2>), dim3(G), dim3(kBlock), 0, s, a);
+        bstage<ST_CG_REPLACE, 2>(B, s);
+    }
+    hipLaunchKernelGGL(k_bpupdate, dim3(G), dim3(kBlock), 0, s, a);
+    RG_HIP(hipGetLastError());
+}
+
+void batch_prepare(rg_batch* B, bool fp32)
+{
+    const int64_t tot = B->offsets.back();
+    B->dinv.alloc(tot);
+    for (size_t j = 0; j < B->mats.size(); ++j) prepare_matrix(B->mats[j], fp32, B->dinv.p + B->offsets[j]);
+    B->r.alloc(tot);
+    B->p.alloc(tot);
+    B->q.alloc(tot);
+    B->sts.alloc(B->mats.size());
+    build_inst(B);
+}
+
+}  // namespace
+
+int batch_autotune(rg_batch* B, bool fp32, int reps, double* times)
+{
+    batch_prepare(B, fp32);
+    hipStream_t st = B->ctx->stream;
+    const int64_t tot = B->offsets.back();
+    DevArray<double> x;
+    x.alloc(tot);
+    RG_HIP(hipMemsetAsync(x.p, 0, sizeof(double) * tot, st));
+    hipEvent_t e0, e1;
+    RG_HIP(hipEventCreate(&e0));
+    RG_HIP(hipEventCreate(&e1));
+    int best = 1;
+    float bt = 1e30f;
+    for (int c = 0; c < 4; ++c) {
+        build_tiles(B, kTileCands[c], fp32);
+        batch_spmv(B, fp32, x.p, B->q.p);
+        RG_HIP(hipEventRecord(e0, st));
+        for (int k = 0; k < reps; ++k) batch_spmv(B, fp32, x.p, B->q.p);
+        RG_HIP(hipEventRecord(e1, st));
+        RG_HIP(hipEventSynchronize(e1));
+        float ms = 0.f;
+        RG_HIP(hipEventElapsedTime(&ms, e0, e1));
+        if (times) times[c] = ms / 1e3 / reps;
+        if (ms < bt) {
+            bt = ms;
+            best = kTileCands[c];
+        }
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    B->tile_mult = best;
+    return best;
+}
+
+int batch_solve(rg_batch* B, double* X, const double* b, const rg_options& opt, rg_diagnostics* dg)
+{
+    const double t0 = now_s();
+    const int count = (int)B->mats.size();
+    RG_CHECK(opt.method == RG_METHOD_PCG_PLAIN || opt.method == RG_METHOD_PCG_MIXED, RG_VALUE_ERROR,
+             "batch_solve: method must be RG_METHOD_PCG_PLAIN or RG_METHOD_PCG_MIXED");
+    RG_CHECK(opt.tolerance > 0.0 && opt.tolerance <= 1.0, RG_VALUE_ERROR, "batch_solve: tolerance must be in (0, 1]");
+    const bool fp32 = opt.method == RG_METHOD_PCG_MIXED;
+    if (B->tile_mult <= 0) batch_autotune(B, fp32, 5, nullptr);
+    batch_prepare(B, fp32);
+    if (B->built_mult != B->tile_mult || B->built_fp32 != fp32) build_tiles(B, B->tile_mult, fp32);
+    B->part.alloc((size_t)2 * B->G);
+    hipStream_t st = B->ctx->stream;
+    init_states(st, B->sts.p, count, opt.tolerance, opt.iter_max);
+    const BArgs a = bargs(B, b, X);
+    hipLaunchKernelGGL(k_binit, dim3(B->G), dim3(kBlock), 0, st, a);
+    bstage<ST_BATCH_INIT, 2>(B, st);
+    RG_HIP(hipGetLastError());
+    const int check = opt.check_interval > 0 ? opt.check_interval : 32;
+    B->run.stream = st;
+    // the captured launches depend on the argument block, the tile table and the precision
+    std::vector<unsigned char> key(sizeof(BArgs) + 3 * sizeof(int));
+    const int extra[3] = {fp32 ? 1 : 0, B->tile_mult, B->G};
+    std::memcpy(key.data(), &a, sizeof(BArgs));
+    std::memcpy(key.data() + sizeof(BArgs), extra, sizeof(extra));
+    std::vector<KrylovState> fin;
+    poll_until_done(st, B->sts.p, count,
+                    [&]() { B->run.run(key, check, [&](hipStream_t s, bool last) { batch_iteration(B, fp32, a, s, last); }); },
+                    fin);
+    fill_diagnostics(fin, now_s() - t0, dg, count);
+    return batch_status(dg, count, opt.accept_failed_convergence != 0);
+}
+
+void batch_vector(rg_batch* B, const double* X, double* Y)
+{
+    for (auto* m : B->mats)
+        RG_CHECK(!m->is_balanced, RG_STATE_ERROR, "batch_vector: a matrix is balanced (set or reset values first)");
+    if (B->inst.n == 0) {
+        B->dinv.alloc(B->offsets.back());
+        build_inst(B);
+    }
+    if (B->built_mult < 0) build_tiles(B, B->tile_mult > 0 ? B->tile_mult : 1, false);
+    batch_spmv(B, false, X, Y);
+}
+
+}  // namespace rg
+
+// ==================================================================== C ABI
+namespace rg {
+void set_last_error(const std::string& msg);
+}
+
+namespace {
+template <class F>
+int batch_guard(F&& f)
+{
+    try {
+        rg::set_last_error("");
+        return f();
+    } catch (const rg::Error& e) {
+        rg::set_last_error(e.what());
+        (void)hipGetLastError();
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        rg::set_last_error("out of host memory");
+        return RG_MEMORY_ERROR;
+    } catch (const std::exception& e) {
+        rg::set_last_error(e.what());
+        (void)hipGetLastError();
+        return RG_VALUE_ERROR;
+    }
+}
+}  // namespace
+
+extern "C" {
+
+int rg_matrix_vector_multi(rg_matrix* m, const double* X, double* Y, int nrhs)
+{
+    return batch_guard([&] {
+        RG_CHECK(m && X && Y, RG_VALUE_ERROR, "matrix_vector_multi: NULL argument");
+        RG_HIP(hipSetDevice(m->ctx->device));
+        rg::multi_vector(m, X, Y, nrhs);
+        return RG_OK;
+    });
+}
+
+int rg_solve_multi(rg_matrix* m, double* X, const double* B, int nrhs, const rg_options* opt, rg_diagnostics* diag)
+{
+    return batch_guard([&] {
+        RG_CHECK(m && X && B && opt && diag, RG_VALUE_ERROR, "solve_multi: NULL argument");
+        RG_HIP(hipSetDevice(m->ctx->device));
+        return rg::solve_multi(m, X, B, nrhs, *opt, diag);
+    });
+}
+
+int rg_multi_autotune(rg_matrix* m, int nrhs, int fp32, int reps, int* shape, double* times)
+{
+    return batch_guard([&] {
+        RG_CHECK(m && shape, RG_VALUE_ERROR, "multi_autotune: NULL argument");
+        RG_CHECK(m->g.bs == 1 && !m->symmetric, RG_VALUE_ERROR, "multi_autotune: scalar full-storage matrices only");
+        RG_CHECK(nrhs >= 1 && nrhs <= 64 && reps >= 1, RG_VALUE_ERROR, "multi_autotune: 1..64 columns, reps >= 1");
+        RG_CHECK(!m->is_balanced, RG_STATE_ERROR, "multi_autotune: the matrix is balanced (set or reset values first)");
+        RG_HIP(hipSetDevice(m->ctx->device));
+        rg::DevArray<double> x, y, d;
+        x.alloc((size_t)m->g.N * nrhs);
+        y.alloc((size_t)m->g.N * nrhs);
+        RG_HIP(hipMemsetAsync(x.p, 0, sizeof(double) * m->g.N * nrhs, m->ctx->stream));
+        d.alloc(m->g.ld);
+        if (fp32) rg::prepare_matrix(m, true, d.p);
+        *shape = rg::autotune_multi(m, nrhs, fp32 != 0, x.p, y.p, reps, times);
+        m->multi_shape = *shape + 1;
+        return RG_OK;
+    });
+}
+
+int rg_multi_set_shape(rg_matrix* m, int shape)
+{
+    return batch_guard([&] {
+        RG_CHECK(m, RG_VALUE_ERROR, "multi_set_shape: NULL matrix");
+        RG_CHECK(shape >= -1 && shape < rg::kNumShapes, RG_VALUE_ERROR,
+                 "multi_set_shape: shape must be -1 (autotune) or 0..3");
+        m->multi_shape = shape + 1;
+        return RG_OK;
+    });
+}
+
+int rg_batch_create(rg_context* ctx, int count, rg_matrix* const* mats, rg_batch** out)
+{
+    return batch_guard([&] {
+        RG_CHECK(ctx && mats && out && count >= 1, RG_VALUE_ERROR, "batch_create: NULL argument or empty batch");
+        rg_batch* B = new rg_batch();
+        B->ctx = ctx;
+        int64_t off = 0;
+        for (int j = 0; j < count; ++j) {
+            rg_matrix* m = mats[j];
+            if (!m || m->ctx != ctx || m->g.bs != 1 || m->symmetric || m->g.dim != mats[0]->g.dim) {
+                delete B;
+                throw rg::Error(RG_VALUE_ERROR,
+                                "batch_create: scalar full-storage matrices of one dimension on this context only");
+            }
+            B->mats.push_back(m);
+            B->offsets.push_back(off);
+            off += (m->g.N + 63) / 64 * 64;
+        }
+        B->offsets.push_back(off);
+        B->dim = mats[0]->g.dim;
+        *out = B;
+        return RG_OK;
+    });
+}
+
+int rg_batch_destroy(rg_batch* b)
+{
+    if (b) {
+        hipStreamSynchronize(b->ctx->stream);
+        delete b;
+    }
+    return RG_OK;
+}
+
+int rg_batch_offsets(const rg_batch* b, int64_t* offsets)
+{
+    return batch_guard([&] {
+        RG_CHECK(b && offsets, RG_VALUE_ERROR, "batch_offsets: NULL argument");
+        for (size_t j = 0; j < b->offsets.size(); ++j) offsets[j] = b->offsets[j];
+        return RG_OK;
+    });
+}
+
+int rg_batch_vector(rg_batch* b, const double* x, double* y)
+{
+    return batch_guard([&] {
+        RG_CHECK(b && x && y, RG_VALUE_ERROR, "batch_vector: NULL argument");
+        RG_HIP(hipSetDevice(b->ctx->device));
+        rg::batch_vector(b, x, y);
+        return RG_OK;
+    });
+}
+
+int rg_batch_autotune(rg_batch* b, int fp32, int reps, int* tile_mult, double* times)
+{
+    return batch_guard([&] {
+        RG_CHECK(b && tile_mult && reps >= 1, RG_VALUE_ERROR, "batch_autotune: bad argument");
+        RG_HIP(hipSetDevice(b->ctx->device));
+        *tile_mult = rg::batch_autotune(b, fp32 != 0, reps, times);
+        return RG_OK;
+    });
+}
+
+int rg_batch_set_tile(rg_batch* b, int tile_mult)
+{
+    return batch_guard([&] {
+        RG_CHECK(b && (tile_mult == 0 || tile_mult == 1 || tile_mult == 2 || tile_mult == 4 || tile_mult == 8),
+                 RG_VALUE_ERROR, "batch_set_tile: tile multiplier must be 0 (autotune), 1, 2, 4 or 8");
+        b->tile_mult = tile_mult;
+        return RG_OK;
+    });
+}
+
+int rg_batch_solve(rg_batch* b, double* x, const double* rhs, const rg_options* opt, rg_diagnostics* diag)
+{
+    return batch_guard([&] {
+        RG_CHECK(b && x && rhs && opt && diag, RG_VALUE_ERROR, "batch_solve: NULL argument");
+        RG_HIP(hipSetDevice(b->ctx->device));
+        return rg::batch_solve(b, x, rhs, *opt, diag);
+    });
+}
+
+}  // extern "C"

@@ -32,6 +32,7 @@ int guarded(F&& f)
         return f();
     } catch (const rg::Error& e) {
         g_error = e.what();
+        if (e.code == RG_DEVICE_ERROR) (void)hipGetLastError();   // clear the failed call's error state
         return e.code;
     } catch (const std::bad_alloc&) {
         g_error = "out of host memory";
@@ -291,6 +292,49 @@ int rg_matrix_assemble(rg_matrix* m, const rg_pde_coefs* coefs, int reduced, dou
     });
 }
 
+int rg_matrix_assemble_boundary(rg_matrix* m, const rg_coef* d, const rg_coef* y, int reduced, double* rhs)
+{
+    return guarded([&] {
+        check_matrix(m);
+        use_device(m->ctx);
+        rg::launch_assemble_boundary(m, d, y, reduced, rhs);
+        return RG_OK;
+    });
+}
+
+int rg_matrix_add_dirac(rg_matrix* m, int64_t count, const int64_t* nodes, const double* d, const double* y,
+                        double* rhs)
+{
+    return guarded([&] {
+        check_matrix(m);
+        RG_CHECK(count >= 0 && (count == 0 || nodes), RG_VALUE_ERROR, "add_dirac: NULL node list");
+        RG_CHECK(!(y && !rhs), RG_VALUE_ERROR, "add_dirac: y given without a right-hand side");
+        if (count == 0 || (!d && !y)) return RG_OK;
+        const int bs = m->g.bs;
+        for (int64_t p = 0; p < count; ++p)
+            RG_CHECK(nodes[p] >= 0 && nodes[p] < m->g.N, RG_VALUE_ERROR, "add_dirac: node index out of range");
+        use_device(m->ctx);
+        hipStream_t st = m->ctx->stream;
+        struct Buf {
+            void* p = nullptr;
+            ~Buf() { if (p) hipFree(p); }
+        } dn, dd, dy;
+        RG_HIP(hipMalloc(&dn.p, sizeof(int64_t) * count));
+        RG_HIP(hipMemcpyAsync(dn.p, nodes, sizeof(int64_t) * count, hipMemcpyHostToDevice, st));
+        if (d) {
+            RG_HIP(hipMalloc(&dd.p, sizeof(double) * count * bs * bs));
+            RG_HIP(hipMemcpyAsync(dd.p, d, sizeof(double) * count * bs * bs, hipMemcpyHostToDevice, st));
+        }
+        if (y) {
+            RG_HIP(hipMalloc(&dy.p, sizeof(double) * count * bs));
+            RG_HIP(hipMemcpyAsync(dy.p, y, sizeof(double) * count * bs, hipMemcpyHostToDevice, st));
+        }
+        rg::launch_add_dirac(m, count, (const int64_t*)dn.p, (const double*)dd.p, (const double*)dy.p, rhs);
+        RG_HIP(hipStreamSynchronize(st));
+        return RG_OK;
+    });
+}
+
 int rg_matrix_vector(rg_matrix* m, double alpha, const double* x, double beta, double* y)
 {
     return guarded([&] {
@@ -490,6 +534,7 @@ int rg_matrix_set_learned_preconditioner(rg_matrix* m, rg_lpc* net, const float*
         if (!net) {
             m->lpc = nullptr;
             m->lpc_logk = nullptr;
+            if (m->work) m->work->graph_method = -1;
             return RG_OK;
         }
         RG_CHECK(m->g.dim == 3 && m->g.bs == 1, RG_VALUE_ERROR,
@@ -501,6 +546,7 @@ int rg_matrix_set_learned_preconditioner(rg_matrix* m, rg_lpc* net, const float*
                      "set_learned_preconditioner: network node grid does not match the matrix");
         m->lpc = net;
         m->lpc_logk = logk;
+        if (m->work) m->work->graph_method = -1;   // captured iterations refer to the old network
         return RG_OK;
     });
 }

@@ -116,6 +116,7 @@ struct rg_matrix {
     bool is_balanced = false;
     bool have_prec = false;
     rg_solver_work* work = nullptr;
+    int multi_shape = 0;           // SpMM tile shape + 1 of the many-RHS solver (0: autotune)
     rg_lpc* lpc = nullptr;         // learned preconditioner (borrowed)
     const float* lpc_logk = nullptr;   // log10(kappa) per node (device, borrowed)
 };
@@ -128,6 +129,9 @@ void launch_set_values(rg_matrix* m, double value);
 void launch_add_element_blocks(rg_matrix* m, int64_t ne, int npe, const int32_t* d_nodes,
                                const double* d_em);
 void launch_assemble(rg_matrix* m, const rg_pde_coefs& c, int reduced, double* rhs);
+void launch_assemble_boundary(rg_matrix* m, const rg_coef* d, const rg_coef* y, int reduced, double* rhs);
+void launch_add_dirac(rg_matrix* m, int64_t count, const int64_t* d_nodes, const double* d_d, const double* d_y,
+                      double* rhs);
 void launch_spmv(rg_matrix* m, double alpha, const double* x, double beta, double* y);
 void launch_spmv32(rg_matrix* m, const double* x, double* y);   // FP32 values, y = A32 x
 void launch_nullify(rg_matrix* m, const double* row_mask, const double* col_mask, double mdv,

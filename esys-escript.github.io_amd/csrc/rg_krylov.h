@@ -1,0 +1,139 @@
+// Scalar recurrences of the Krylov solvers, shared by the single-system
+// solver and the batched solvers: after each reducing kernel one workgroup
+// reduces the block partials in a fixed order and applies stage_apply.
+#pragma once
+
+#include "rg_device.h"
+
+namespace rg {
+
+// End-of-iteration logic of the textbook PCG, shared by k_cg_update and the
+// residual replacement.
+__device__ __forceinline__ void cg_end(KrylovState* st, double rz, double rr, bool have_rz)
+{
+    st->norm = sqrt(rr);
+    st->tau_new = rz;
+    if (st->norm <= st->tol && !st->replace_mode) { st->done = 1; st->status = KS_CONVERGED; }
+    else if (st->num_iter >= st->maxit) { st->done = 1; st->status = KS_MAXITER; }
+    else if (have_rz) {
+        if (fabs(rz) <= 0.0) { st->done = 1; st->status = KS_BREAKDOWN; }
+        else { st->beta = rz / st->rho; st->rho_prev = st->rho; st->rho = rz; }
+    }
+}
+
+// ------------------------------------------------------------ stage kernels
+// One workgroup after each reducing kernel: reduces its partials in a fixed
+// order and advances the Krylov scalars (the steps of PCG.cpp:156-347 that
+// follow each reduction).
+enum Stage {
+    ST_PASO_INIT, ST_PASO_SPMV, ST_PASO_R, ST_PASO_X,
+    ST_CG_INIT, ST_CG_SPMV, ST_CG_UPDATE, ST_CG_UPDATE_NORZ, ST_CG_RZ, ST_CG_REPLACE,
+    ST_FCG_INIT, ST_FCG_BETA, ST_FCG_ALPHA, ST_FCG_UPDATE, ST_FCG_REPLACE,
+    ST_BATCH_INIT
+};
+
+// Stages that start a solve run even when the state says done.
+template <int STAGE>
+__device__ __forceinline__ bool stage_starts()
+{
+    return STAGE == ST_PASO_INIT || STAGE == ST_CG_INIT || STAGE == ST_FCG_INIT || STAGE == ST_BATCH_INIT;
+}
+
+// The scalar recurrence of one stage, given the reduced totals (one thread).
+template <int STAGE, int NV>
+__device__ __forceinline__ void stage_apply(KrylovState* st, const double (&tot)[NV])
+{
+    if constexpr (STAGE == ST_PASO_INIT) {
+        st->tau = tot[0];
+        st->beta = 0.0;
+        st->num_iter = 1;
+        st->done = 0;
+        st->status = KS_RUNNING;
+    } else if constexpr (STAGE == ST_PASO_SPMV) {
+        st->delta = tot[0];
+        st->alpha = st->tau / tot[0];
+        if (fabs(tot[0]) <= 0.0) {   // PCG.cpp:262 breakFlag
+            st->done = 1;
+            st->status = KS_BREAKDOWN;
+        }
+    } else if constexpr (STAGE == ST_PASO_R) {
+        st->gamma1 = (fabs(tot[0]) <= 0.0) ? 0.0 : -tot[1] / tot[0];
+    } else if constexpr (STAGE == ST_PASO_X) {
+        const double norm = sqrt(tot[0]);
+        st->norm = norm;
+        const bool conv = norm <= st->tol;
+        const bool maxi = st->num_iter > st->maxit;
+        const bool brk = fabs(st->tau) <= 0.0;
+        if (conv || maxi || brk) {
+            st->done = 1;
+            st->status = maxi ? KS_MAXITER : (brk ? KS_BREAKDOWN : KS_CONVERGED);
+        } else {
+            st->beta = tot[1] / st->tau;
+            st->tau = tot[1];
+            st->num_iter += 1;
+        }
+    } else if constexpr (STAGE == ST_BATCH_INIT) {
+        // textbook PCG from x = 0 (r = b): the tolerance arrives relative
+        // and becomes absolute, tol * ||b||
+        st->tol = st->tol * sqrt(tot[1]);
+        st->rho = tot[0];
+        st->rho_prev = tot[0];
+        st->norm = sqrt(tot[1]);
+        st->num_iter = 0;
+        st->done = 0;
+        st->status = KS_RUNNING;
+        if (st->norm <= st->tol) { st->done = 1; st->status = KS_CONVERGED; }
+        else if (st->maxit <= 0) { st->done = 1; st->status = KS_MAXITER; }
+        else if (fabs(tot[0]) <= 0.0) { st->done = 1; st->status = KS_BREAKDOWN; }
+    } else if constexpr (STAGE == ST_CG_INIT) {
+        st->rho = tot[0];
+        st->rho_prev = tot[0];
+        st->norm = sqrt(tot[1]);
+        st->num_iter = 0;
+        st->done = 0;
+        st->status = KS_RUNNING;
+        if (st->norm <= st->tol) { st->done = 1; st->status = KS_CONVERGED; }
+        else if (st->maxit <= 0) { st->done = 1; st->status = KS_MAXITER; }
+        else if (fabs(tot[0]) <= 0.0) { st->done = 1; st->status = KS_BREAKDOWN; }
+    } else if constexpr (STAGE == ST_CG_SPMV) {
+        st->delta = tot[0];
+        if (fabs(tot[0]) <= 0.0) { st->done = 1; st->status = KS_BREAKDOWN; }
+        else st->alpha = st->rho / tot[0];
+    } else if constexpr (STAGE == ST_CG_UPDATE || STAGE == ST_CG_UPDATE_NORZ) {
+        st->num_iter += 1;
+        cg_end(st, tot[0], tot[1], STAGE == ST_CG_UPDATE);
+    } else if constexpr (STAGE == ST_CG_RZ) {
+        if (fabs(tot[0]) <= 0.0) { st->done = 1; st->status = KS_BREAKDOWN; }
+        else { st->beta = tot[0] / st->rho; st->rho_prev = st->rho; st->rho = tot[0]; }
+    } else if constexpr (STAGE == ST_CG_REPLACE) {
+        // k_cg_update's stage set beta = rho_rec/rho_prev and rho = rho_rec;
+        // redo that step with the replaced (true) residual
+        st->norm = sqrt(tot[1]);
+        if (st->norm <= st->tol) { st->done = 1; st->status = KS_CONVERGED; }
+        else if (fabs(tot[0]) <= 0.0) { st->done = 1; st->status = KS_BREAKDOWN; }
+        else { st->beta = tot[0] / st->rho_prev; st->rho = tot[0]; }
+    } else if constexpr (STAGE == ST_FCG_INIT) {
+        st->norm = sqrt(tot[1]);
+        st->num_iter = 0;
+        st->done = 0;
+        st->status = KS_RUNNING;
+        st->delta = 1.0;
+        if (st->norm <= st->tol) { st->done = 1; st->status = KS_CONVERGED; }
+        else if (st->maxit <= 0) { st->done = 1; st->status = KS_MAXITER; }
+    } else if constexpr (STAGE == ST_FCG_BETA) {
+        st->beta = (st->num_iter == 0) ? 0.0 : -tot[0] / st->delta;
+    } else if constexpr (STAGE == ST_FCG_ALPHA) {
+        if (fabs(tot[0]) <= 0.0) { st->done = 1; st->status = KS_BREAKDOWN; }
+        else { st->alpha = tot[1] / tot[0]; st->delta = tot[0]; }
+    } else if constexpr (STAGE == ST_FCG_UPDATE) {
+        st->num_iter += 1;
+        st->norm = sqrt(tot[0]);
+        if (st->norm <= st->tol && !st->replace_mode) { st->done = 1; st->status = KS_CONVERGED; }
+        else if (st->num_iter >= st->maxit) { st->done = 1; st->status = KS_MAXITER; }
+    } else if constexpr (STAGE == ST_FCG_REPLACE) {
+        st->norm = sqrt(tot[1]);
+        if (st->norm <= st->tol) { st->done = 1; st->status = KS_CONVERGED; }
+    }
+}
+
+}  // namespace rg

@@ -47,69 +47,97 @@ __device__ __forceinline__ int64_t tap_offset(const Grid3& g, int tap)
 
 // ------------------------------------------------------------- conv (fwd)
 // out[v][co] = act( bias[co] + sum_{tap,ci} in[v+off(tap)][ci] W[co][tap][ci] )
-// One wave computes 32 voxels x COUT channels; 4 waves per block.
+// A workgroup computes a 16 x 4 x 4 brick of output voxels: it stages the
+// 18 x 6 x 6 input brick (all CIN channels) in LDS once, then each of its 4
+// waves multiplies 4 rows of 16 voxels (M tiles) by all COUT/16 channel
+// tiles, reading the A operand (16 voxels x 8 channels of one tap per lane
+// group) from LDS and the weights (B) from global memory (shared by every
+// workgroup, cache resident).
+constexpr int kTX = 16, kTY = 4, kTZ = 4;                 // output brick
+constexpr int kHX = kTX + 2, kHY = kTY + 2, kHZ = kTZ + 2;  // input brick with halo
+constexpr int kHalo = kHX * kHY * kHZ;
+
+struct ConvTiles {
+    int nbx, nby, nbz;
+};
+inline ConvTiles conv_tiles(const Grid3& g)
+{
+    return {(g.d0 + kTX - 1) / kTX, (g.d1 + kTY - 1) / kTY, (g.d2 + kTZ - 1) / kTZ};
+}
+
 template <int CIN, int COUT, bool RELU>
-__global__ __launch_bounds__(256) void k_conv_fwd(Grid3 g, const bf16* __restrict__ in, const bf16* __restrict__ w,
-                                                  const float* __restrict__ bias, bf16* __restrict__ out)
+__global__ __launch_bounds__(256) void k_conv_fwd(Grid3 g, ConvTiles T, const bf16* __restrict__ in,
+                                                  const bf16* __restrict__ w, const float* __restrict__ bias,
+                                                  bf16* __restrict__ out)
 {
     constexpr int NT = COUT / 16;
-    constexpr int G8 = 27 * CIN / 8;          // groups of 8 along K
+    constexpr int C8 = CIN / 8;             // 16-byte channel groups per voxel
+    constexpr int G8 = 27 * C8;             // groups of 8 along K = 27 * CIN
     constexpr int STEPS = (G8 + 3) / 4;
+    __shared__ bf16x8 tile[kHalo * C8];
+    const int bx = blockIdx.x % T.nbx, by = (blockIdx.x / T.nbx) % T.nby, bz = blockIdx.x / (T.nbx * T.nby);
+    const int x0 = bx * kTX, y0 = by * kTY, z0 = bz * kTZ;
+    // stage the input brick: padded coordinates (x0 + hx, y0 + hy, z0 + hz)
+    const int p2 = g.d2 + 2;
+    for (int e = threadIdx.x; e < kHalo * C8; e += 256) {
+        const int v = e / C8, c8 = e % C8;
+        const int hx = v % kHX, hy = (v / kHX) % kHY, hz = v / (kHX * kHY);
+        const int px = x0 + hx, py = y0 + hy, pz = z0 + hz;
+        bf16x8 val = bf16x8{};
+        if (px < g.p0 && py < g.p1 && pz < p2)
+            val = *reinterpret_cast<const bf16x8*>(in + (((int64_t)pz * g.p1 + py) * g.p0 + px) * CIN + c8 * 8);
+        tile[e] = val;
+    }
+    __syncthreads();
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int q = lane >> 4, n = lane & 15;
-    const int64_t vbase = ((int64_t)blockIdx.x * 4 + wave) * 32;
-    int64_t pv[2];
-    bool valid[2];
+    f32x4 acc[4][NT];
 #pragma unroll
-    for (int t = 0; t < 2; ++t) {
-        const int64_t v = vbase + 16 * t + n;
-        valid[t] = v < g.nvox;
-        pv[t] = padded_index(g, valid[t] ? v : 0);
-    }
-    f32x4 acc[2][NT];
+    for (int m = 0; m < 4; ++m)
 #pragma unroll
-    for (int t = 0; t < 2; ++t)
-#pragma unroll
-        for (int j = 0; j < NT; ++j) acc[t][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+        for (int j = 0; j < NT; ++j) acc[m][j] = f32x4{0.f, 0.f, 0.f, 0.f};
     for (int s = 0; s < STEPS; ++s) {
         const int gi = 4 * s + q;
         const bool gok = gi < G8;
         const int gg = gok ? gi : 0;
-        const int tap = gg / (CIN / 8), cig = gg % (CIN / 8);
-        const int64_t off = tap_offset(g, tap);
-        bf16x8 a[2], b[NT];
-#pragma unroll
-        for (int t = 0; t < 2; ++t) {
-            a[t] = *reinterpret_cast<const bf16x8*>(in + (pv[t] + off) * CIN + cig * 8);
-            if (!gok) a[t] = bf16x8{};
-        }
+        const int tap = gg / C8, cig = gg % C8;
+        const int dx = tap % 3, dy = (tap / 3) % 3, dz = tap / 9;   // offsets + 1
+        bf16x8 b[NT];
 #pragma unroll
         for (int j = 0; j < NT; ++j) {
             b[j] = *reinterpret_cast<const bf16x8*>(w + ((int64_t)(16 * j + n) * 27 + tap) * CIN + cig * 8);
             if (!gok) b[j] = bf16x8{};
         }
 #pragma unroll
-        for (int t = 0; t < 2; ++t)
+        for (int m = 0; m < 4; ++m) {
+            const int row = 4 * wave + m;                 // (ty, tz) row of the brick
+            const int ty = row % kTY, tz = row / kTY;
+            bf16x8 a = tile[(((tz + dz) * kHY + (ty + dy)) * kHX + (n + dx)) * C8 + cig];
+            if (!gok) a = bf16x8{};
 #pragma unroll
-            for (int j = 0; j < NT; ++j)
-                acc[t][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[t], b[j], acc[t][j], 0, 0, 0);
+            for (int j = 0; j < NT; ++j) acc[m][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[j], acc[m][j], 0, 0, 0);
+        }
     }
-    // D rows 4q..4q+3 are voxels, column n is the channel
+    // D rows 4q..4q+3 are the voxels x0 + 4q + r of the row, column n the channel
 #pragma unroll
-    for (int t = 0; t < 2; ++t)
+    for (int m = 0; m < 4; ++m) {
+        const int row = 4 * wave + m;
+        const int y = y0 + row % kTY, z = z0 + row / kTY;
+        if (y >= g.d1 || z >= g.d2) continue;
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-            const int64_t v = vbase + 16 * t + 4 * q + r;
-            if (v >= g.nvox) continue;
-            const int64_t p = padded_index(g, v);
+            const int x = x0 + 4 * q + r;
+            if (x >= g.d0) continue;
+            const int64_t p = (((int64_t)z + 1) * g.p1 + (y + 1)) * g.p0 + (x + 1);
 #pragma unroll
             for (int j = 0; j < NT; ++j) {
                 const int co = 16 * j + n;
-                float o = acc[t][j][r] + (bias ? bias[co] : 0.f);
+                float o = acc[m][j][r] + (bias ? bias[co] : 0.f);
                 if (RELU) o = fmaxf(o, 0.f);
                 out[p * COUT + co] = (bf16)o;
             }
         }
+    }
 }
 
 // ------------------------------------------------- conv (weight gradient)
@@ -427,9 +455,10 @@ inline int64_t padded_size(const Grid3& g) { return (int64_t)(g.d0 + 2) * (g.d1 
 template <int CIN, int COUT>
 void conv_fwd_t(hipStream_t st, const Grid3& g, const bf16* in, const bf16* w, const float* b, bf16* out, bool relu)
 {
-    const int grid = (int)((g.nvox + 127) / 128);
-    if (relu) hipLaunchKernelGGL((k_conv_fwd<CIN, COUT, true>), dim3(grid), dim3(256), 0, st, g, in, w, b, out);
-    else hipLaunchKernelGGL((k_conv_fwd<CIN, COUT, false>), dim3(grid), dim3(256), 0, st, g, in, w, b, out);
+    const ConvTiles T = conv_tiles(g);
+    const int grid = T.nbx * T.nby * T.nbz;
+    if (relu) hipLaunchKernelGGL((k_conv_fwd<CIN, COUT, true>), dim3(grid), dim3(256), 0, st, g, T, in, w, b, out);
+    else hipLaunchKernelGGL((k_conv_fwd<CIN, COUT, false>), dim3(grid), dim3(256), 0, st, g, T, in, w, b, out);
 }
 
 void conv_fwd(hipStream_t st, const Grid3& g, int cin, int cout, const bf16* in, const bf16* w, const float* b,
@@ -549,7 +578,7 @@ void net_init(Net& N, rg_context* ctx, const int32_t* nodes, bool training)
         N.part = (float*)N.alloc(sizeof(float) * N.part_elems);
     }
     N.scale = (double*)N.alloc(sizeof(double) * 4);
-    N.red = (double*)N.alloc(sizeof(double) * 3 * 1024 + 8);
+    N.red = (double*)N.alloc(sizeof(double) * (3 * 1024 + 8));   // partials, then 3 loss dots at 3*1024
 }
 
 __global__ void k_to_bf16(const float* __restrict__ a, bf16* __restrict__ b, int64_t n)
@@ -851,9 +880,11 @@ static int lpc_guard(F&& f)
         return f();
     } catch (const rg::Error& e) {
         rg::set_last_error(e.what());
+        (void)hipGetLastError();   // do not leave a failed call's error for the next caller
         return e.code;
     } catch (const std::exception& e) {
         rg::set_last_error(e.what());
+        (void)hipGetLastError();
         return RG_VALUE_ERROR;
     }
 }

@@ -254,6 +254,172 @@ __global__ __launch_bounds__(kBlock) void k_assemble(AsmArgs a, double* __restri
     if (doF) rhs[i * NB + k] += f;
 }
 
+// Boundary terms, colour free: one thread per (node i, equation k,
+// component m) visits the face elements around node i on every face of the
+// domain the node lies on and adds row a (node i's local face index) of
+//   EM_S[k,m,a,b] = sum_q w d[k + n m] N_a N_b,   EM_F[k,a] = sum_q w y[k] N_a
+// -- the face integrals of DefaultAssembler2D/3D::assemblePDEBoundarySingle/
+// System (2-point Gauss rule along each tangential axis, weight face measure
+// / points; DefaultAssembler2D.cpp:601-840, DefaultAssembler3D.cpp:2233-2890)
+// or, reduced, the one-point rule at the face centre (:1014-1200,
+// :3274-3630).  Face elements are indexed as the reference's
+// FunctionOnBoundary samples: faces x0=0, x0=L, x1=0, x1=L[, x2=0, x2=L],
+// within a face the lower tangential element index fastest (m_faceOffset).
+struct BndArgs {
+    CoefDev d, y;
+    int64_t n[3];        // nodes per axis
+    int ne[3];           // elements per axis
+    double h[3];
+    int64_t face_off[6];
+    int reduced;
+};
+
+template <int DIM, int NB, bool SYM>
+__global__ __launch_bounds__(kBlock) void k_assemble_boundary(BndArgs a, double* __restrict__ val, int64_t ld,
+                                                              int64_t N, double* __restrict__ rhs)
+{
+    constexpr int NS = StencilShape<DIM>::NS;
+    constexpr int CEN = StencilShape<DIM>::CENTER;
+    constexpr int NST = stored_slots<DIM, SYM>();
+    constexpr int FIRST = SYM ? CEN : 0;
+    constexpr int NFN = 1 << (DIM - 1);                // nodes per face element
+    const int km = blockIdx.y;
+    const int k = km % NB, m = km / NB;
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= N) return;
+    const int64_t c[3] = {i % a.n[0], (i / a.n[0]) % a.n[1], DIM == 3 ? i / (a.n[0] * a.n[1]) : 0};
+    const bool doS = a.d.mode != 0;
+    const bool doF = rhs != nullptr && m == 0 && a.y.mode != 0;
+    const int nq = a.reduced ? 1 : NFN;
+    const double g0 = 0.5 - 0.5 / 1.73205080756887719318, g1 = 0.5 + 0.5 / 1.73205080756887719318;
+    double acc[NS];
+#pragma unroll
+    for (int s = 0; s < NS; ++s) acc[s] = 0.0;
+    double f = 0.0;
+    bool any = false;
+#pragma unroll
+    for (int face = 0; face < 2 * DIM; ++face) {
+        const int ax = face / 2, side = face % 2;
+        if (c[ax] != (side ? a.n[ax] - 1 : 0)) continue;
+        any = true;
+        const int t0 = (ax == 0) ? 1 : 0;
+        const int t1 = (DIM == 3) ? ((ax == 2) ? 1 : 2) : -1;
+        const double meas = a.h[t0] * (DIM == 3 ? a.h[t1] : 1.0);
+        const double w = meas / nq;
+#pragma unroll
+        for (int e1 = 0; e1 < (DIM == 3 ? 2 : 1); ++e1)
+#pragma unroll
+            for (int e0 = 0; e0 < 2; ++e0) {
+                const int64_t j0 = c[t0] - 1 + e0;
+                const int64_t j1 = (DIM == 3) ? c[t1] - 1 + e1 : 0;
+                if (j0 < 0 || j0 >= a.ne[t0]) continue;
+                if (DIM == 3 && (j1 < 0 || j1 >= a.ne[t1])) continue;
+                const int64_t fe = a.face_off[face] + j0 + (int64_t)a.ne[t0] * j1;
+                const int ab0 = 1 - e0, ab1 = 1 - e1;      // node i's local face bits
+                for (int q = 0; q < nq; ++q) {
+                    const double xi0 = a.reduced ? 0.5 : ((q & 1) ? g1 : g0);
+                    const double xi1 = a.reduced ? 0.5 : (((q >> 1) & 1) ? g1 : g0);
+                    const double Na = (ab0 ? xi0 : 1.0 - xi0) * (DIM == 3 ? (ab1 ? xi1 : 1.0 - xi1) : 1.0);
+                    if (doS) {
+                        const double dv = coef_pt(a.d, fe, q, NB * NB)[k + NB * m];
+#pragma unroll
+                        for (int b = 0; b < NFN; ++b) {
+                            const int bb0 = b & 1, bb1 = (b >> 1) & 1;
+                            const double Nb = (bb0 ? xi0 : 1.0 - xi0) * (DIM == 3 ? (bb1 ? xi1 : 1.0 - xi1) : 1.0);
+                            int off[3] = {1, 1, 1};
+                            off[t0] = e0 + bb0;
+                            if (DIM == 3) off[t1] = e1 + bb1;
+                            const int s = off[0] + 3 * off[1] + (DIM == 3 ? 9 * off[2] : 0);
+                            acc[s] += w * dv * Na * Nb;
+                        }
+                    }
+                    if (doF) f += w * coef_pt(a.y, fe, q, NB)[k] * Na;
+                }
+            }
+    }
+    if (!any) return;
+    if (doS) {
+#pragma unroll
+        for (int t = 0; t < NST; ++t)
+            val[(int64_t)(t * NB * NB + k + NB * m) * ld + i] += acc[FIRST + t];
+    }
+    if (doF) rhs[i * NB + k] += f;
+}
+
+// Dirac points in order (RipleyDomain::assemblePDEDirac, RipleyDomain.cpp:
+// 1756-1800): rhs[k + n*node] += y[p][k]; diagonal block += d[p] (k + n*m).
+template <int NB>
+__global__ void k_add_dirac(int64_t count, const int64_t* __restrict__ nodes, const double* __restrict__ d,
+                            const double* __restrict__ y, double* __restrict__ val, int64_t ld, int diag_slot,
+                            double* __restrict__ rhs)
+{
+    if (threadIdx.x >= NB * NB) return;
+    const int k = threadIdx.x % NB, m = threadIdx.x / NB;
+    for (int64_t p = 0; p < count; ++p) {
+        const int64_t i = nodes[p];
+        if (d) val[(int64_t)(diag_slot * NB * NB + k + NB * m) * ld + i] += d[p * NB * NB + k + NB * m];
+        if (y && rhs && m == 0) rhs[i * NB + k] += y[p * NB + k];
+    }
+}
+
+void launch_assemble_boundary(rg_matrix* m, const rg_coef* d, const rg_coef* y, int reduced, double* rhs)
+{
+    const Geom& g = m->g;
+    auto cv = [](const rg_coef* x) {
+        return x ? CoefDev{x->data, x->data ? x->mode : 0, x->elem_stride, 0} : CoefDev{nullptr, 0, 0, 0};
+    };
+    BndArgs a;
+    a.d = cv(d);
+    a.y = cv(y);
+    for (const rg_coef* x : {d, y}) {
+        if (!x) continue;
+        RG_CHECK(x->mode >= RG_COEF_EMPTY && x->mode <= RG_COEF_EXPANDED, RG_VALUE_ERROR,
+                 "assemble_boundary: unknown coefficient mode");
+        RG_CHECK(x->mode == RG_COEF_EMPTY || x->data != nullptr, RG_VALUE_ERROR,
+                 "assemble_boundary: coefficient data is NULL");
+        RG_CHECK(!(reduced && x->mode == RG_COEF_EXPANDED), RG_VALUE_ERROR,
+                 "assemble_boundary: reduced-order coefficients cannot be expanded");
+        RG_CHECK(!x->isotropic, RG_VALUE_ERROR, "assemble_boundary: isotropic is defined for A only");
+    }
+    for (int t = 0; t < 3; ++t) {
+        a.n[t] = g.nn[t];
+        a.ne[t] = g.ne[t];
+        a.h[t] = g.h[t];
+    }
+    int64_t off = 0;
+    for (int f = 0; f < 6; ++f) {
+        a.face_off[f] = off;
+        if (f >= 2 * g.dim) continue;
+        const int ax = f / 2;
+        int64_t cnt = 1;
+        for (int t = 0; t < g.dim; ++t)
+            if (t != ax) cnt *= g.ne[t];
+        off += cnt;
+    }
+    a.reduced = reduced;
+    if (!a.d.mode && !(a.y.mode && rhs)) return;
+    const dim3 grid((unsigned)((g.N + kBlock - 1) / kBlock), (unsigned)(g.bs * g.bs));
+    dispatch(g, m->symmetric, [&](auto D, auto B, auto S) {
+        hipLaunchKernelGGL((k_assemble_boundary<D.value, B.value, S.value>), grid, dim3(kBlock), 0, m->ctx->stream, a,
+                           m->val, g.ld, g.N, rhs);
+    });
+    RG_HIP(hipGetLastError());
+    m->val32_valid = false;
+}
+
+void launch_add_dirac(rg_matrix* m, int64_t count, const int64_t* d_nodes, const double* d_d, const double* d_y,
+                      double* rhs)
+{
+    const Geom& g = m->g;
+    const int diag_slot = m->symmetric ? 0 : (g.dim == 3 ? 13 : 4);
+    dispatch_bs(g.bs, [&](auto B) {
+        hipLaunchKernelGGL((k_add_dirac<B.value>), dim3(1), dim3(64), 0, m->ctx->stream, count, d_nodes, d_d, d_y,
+                           m->val, g.ld, diag_slot, rhs);
+    });
+    RG_HIP(hipGetLastError());
+    m->val32_valid = false;
+}
+
 void launch_assemble(rg_matrix* m, const rg_pde_coefs& c, int reduced, double* rhs)
 {
     const Geom& g = m->g;
@@ -308,7 +474,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv(const VT* __restrict__ val, int
 void launch_spmv(rg_matrix* m, double alpha, const double* x, double beta, double* y)
 {
     const Geom& g = m->g;
-    const Launch L = chunked_launch(g.N, 2);
+    const Launch L = chunked_launch(g.N, spmv_rows_per_thread<double>());
     dispatch(g, m->symmetric, [&](auto D, auto B, auto S) {
         hipLaunchKernelGGL((k_spmv<D.value, B.value, S.value, double>), dim3(L.grid), dim3(kBlock), 0,
                            m->ctx->stream, m->val, g.ld, g.N, g.s1, g.s2, L.chunk, x, y, alpha, beta);
@@ -319,7 +485,7 @@ void launch_spmv(rg_matrix* m, double alpha, const double* x, double beta, doubl
 void launch_spmv32(rg_matrix* m, const double* x, double* y)
 {
     const Geom& g = m->g;
-    const Launch L = chunked_launch(g.N, 4);
+    const Launch L = chunked_launch(g.N, spmv_rows_per_thread<float>());
     dispatch(g, m->symmetric, [&](auto D, auto B, auto S) {
         hipLaunchKernelGGL((k_spmv<D.value, B.value, S.value, float>), dim3(L.grid), dim3(kBlock), 0,
                            m->ctx->stream, m->val32, g.ld, g.N, g.s1, g.s2, L.chunk, x, y, 1.0, 0.0);

@@ -16,9 +16,11 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <limits>
 
 #include "rg_device.h"
 #include "rg_dispatch.h"
+#include "rg_krylov.h"
 #include "rg_stencil.h"
 
 namespace rg {
@@ -483,20 +485,6 @@ __global__ __launch_bounds__(kBlock) void k_cg_init(PcgArgs a, const DT* __restr
     publish_block_sums<2>(s, a.partials);
 }
 
-// End-of-iteration logic of the textbook PCG, shared by k_cg_update and the
-// residual replacement.
-__device__ __forceinline__ void cg_end(KrylovState* st, double rz, double rr, bool have_rz)
-{
-    st->norm = sqrt(rr);
-    st->tau_new = rz;
-    if (st->norm <= st->tol && !st->replace_mode) { st->done = 1; st->status = KS_CONVERGED; }
-    else if (st->num_iter >= st->maxit) { st->done = 1; st->status = KS_MAXITER; }
-    else if (have_rz) {
-        if (fabs(rz) <= 0.0) { st->done = 1; st->status = KS_BREAKDOWN; }
-        else { st->beta = rz / st->rho; st->rho_prev = st->rho; st->rho = rz; }
-    }
-}
-
 template <int BS, typename DT>
 __global__ __launch_bounds__(kBlock) void k_cg_update(PcgArgs a, const DT* __restrict__ dinv)
 {
@@ -639,100 +627,15 @@ __global__ __launch_bounds__(kBlock) void k_fcg_update(PcgArgs a)
 }
 
 // ------------------------------------------------------------ stage kernels
-// One workgroup after each reducing kernel: reduces its partials in a fixed
-// order and advances the Krylov scalars (the steps of PCG.cpp:156-347 that
-// follow each reduction).
-enum Stage {
-    ST_PASO_INIT, ST_PASO_SPMV, ST_PASO_R, ST_PASO_X,
-    ST_CG_INIT, ST_CG_SPMV, ST_CG_UPDATE, ST_CG_UPDATE_NORZ, ST_CG_RZ, ST_CG_REPLACE,
-    ST_FCG_INIT, ST_FCG_BETA, ST_FCG_ALPHA, ST_FCG_UPDATE, ST_FCG_REPLACE
-};
-
+// One workgroup after each reducing kernel (recurrences in rg_krylov.h).
 template <int STAGE, int NV>
 __global__ __launch_bounds__(kBlock) void k_stage(KrylovState* st, const double* __restrict__ partials, int G)
 {
-    if (STAGE != ST_PASO_INIT && STAGE != ST_CG_INIT && STAGE != ST_FCG_INIT && st->done) return;
+    if (!stage_starts<STAGE>() && st->done) return;
     double tot[NV];
     reduce_partials<NV>(partials, G, tot);
     if (threadIdx.x != 0) return;
-    if (STAGE == ST_PASO_INIT) {
-        st->tau = tot[0];
-        st->beta = 0.0;
-        st->num_iter = 1;
-        st->done = 0;
-        st->status = KS_RUNNING;
-    } else if (STAGE == ST_PASO_SPMV) {
-        st->delta = tot[0];
-        st->alpha = st->tau / tot[0];
-        if (fabs(tot[0]) <= 0.0) {   // PCG.cpp:262 breakFlag
-            st->done = 1;
-            st->status = KS_BREAKDOWN;
-        }
-    } else if (STAGE == ST_PASO_R) {
-        st->gamma1 = (fabs(tot[0]) <= 0.0) ? 0.0 : -tot[1] / tot[0];
-    } else if (STAGE == ST_PASO_X) {
-        const double norm = sqrt(tot[0]);
-        st->norm = norm;
-        const bool conv = norm <= st->tol;
-        const bool maxi = st->num_iter > st->maxit;
-        const bool brk = fabs(st->tau) <= 0.0;
-        if (conv || maxi || brk) {
-            st->done = 1;
-            st->status = maxi ? KS_MAXITER : (brk ? KS_BREAKDOWN : KS_CONVERGED);
-        } else {
-            st->beta = tot[1] / st->tau;
-            st->tau = tot[1];
-            st->num_iter += 1;
-        }
-    } else if (STAGE == ST_CG_INIT) {
-        st->rho = tot[0];
-        st->rho_prev = tot[0];
-        st->norm = sqrt(tot[1]);
-        st->num_iter = 0;
-        st->done = 0;
-        st->status = KS_RUNNING;
-        if (st->norm <= st->tol) { st->done = 1; st->status = KS_CONVERGED; }
-        else if (st->maxit <= 0) { st->done = 1; st->status = KS_MAXITER; }
-        else if (fabs(tot[0]) <= 0.0) { st->done = 1; st->status = KS_BREAKDOWN; }
-    } else if (STAGE == ST_CG_SPMV) {
-        st->delta = tot[0];
-        if (fabs(tot[0]) <= 0.0) { st->done = 1; st->status = KS_BREAKDOWN; }
-        else st->alpha = st->rho / tot[0];
-    } else if (STAGE == ST_CG_UPDATE || STAGE == ST_CG_UPDATE_NORZ) {
-        st->num_iter += 1;
-        cg_end(st, tot[0], tot[1], STAGE == ST_CG_UPDATE);
-    } else if (STAGE == ST_CG_RZ) {
-        if (fabs(tot[0]) <= 0.0) { st->done = 1; st->status = KS_BREAKDOWN; }
-        else { st->beta = tot[0] / st->rho; st->rho_prev = st->rho; st->rho = tot[0]; }
-    } else if (STAGE == ST_CG_REPLACE) {
-        // k_cg_update's stage set beta = rho_rec/rho_prev and rho = rho_rec;
-        // redo that step with the replaced (true) residual
-        st->norm = sqrt(tot[1]);
-        if (st->norm <= st->tol) { st->done = 1; st->status = KS_CONVERGED; }
-        else if (fabs(tot[0]) <= 0.0) { st->done = 1; st->status = KS_BREAKDOWN; }
-        else { st->beta = tot[0] / st->rho_prev; st->rho = tot[0]; }
-    } else if (STAGE == ST_FCG_INIT) {
-        st->norm = sqrt(tot[1]);
-        st->num_iter = 0;
-        st->done = 0;
-        st->status = KS_RUNNING;
-        st->delta = 1.0;
-        if (st->norm <= st->tol) { st->done = 1; st->status = KS_CONVERGED; }
-        else if (st->maxit <= 0) { st->done = 1; st->status = KS_MAXITER; }
-    } else if (STAGE == ST_FCG_BETA) {
-        st->beta = (st->num_iter == 0) ? 0.0 : -tot[0] / st->delta;
-    } else if (STAGE == ST_FCG_ALPHA) {
-        if (fabs(tot[0]) <= 0.0) { st->done = 1; st->status = KS_BREAKDOWN; }
-        else { st->alpha = tot[1] / tot[0]; st->delta = tot[0]; }
-    } else if (STAGE == ST_FCG_UPDATE) {
-        st->num_iter += 1;
-        st->norm = sqrt(tot[0]);
-        if (st->norm <= st->tol && !st->replace_mode) { st->done = 1; st->status = KS_CONVERGED; }
-        else if (st->num_iter >= st->maxit) { st->done = 1; st->status = KS_MAXITER; }
-    } else if (STAGE == ST_FCG_REPLACE) {
-        st->norm = sqrt(tot[1]);
-        if (st->norm <= st->tol) { st->done = 1; st->status = KS_CONVERGED; }
-    }
+    stage_apply<STAGE, NV>(st, tot);
 }
 
 // Jacobi sweeps 2..n (Smoother.cpp:100-127): z += inv(D) (r - A z), into
@@ -807,7 +710,7 @@ struct SolverImpl {
 
     // launch shapes: stencil kernels (R rows per thread), paired node kernels
     template <typename VT>
-    Launch spmv_launch() const { return chunked_launch(m->g.N, 16 / (int)sizeof(VT)); }
+    Launch spmv_launch() const { return chunked_launch(m->g.N, spmv_rows_per_thread<VT>()); }
     Launch node_launch() const { return chunked_launch(m->g.N, m->g.bs == 1 ? 2 : 1); }
 
     // z = M^-1 r with `sweeps` Jacobi sweeps, into w->z (returns the buffer).
@@ -1039,11 +942,19 @@ struct SolverImpl {
                     const KrylovState& sl = slots[(launched - 2) % 2];
                     done = sl.done != 0;
                     // flexible CG with a learned preconditioner: stop on
-                    // stagnation (no 1% decrease over two polls)
+                    // stagnation -- no new best residual (1% below the best
+                    // so far) for max(4 polls, 64 iterations)
                     if (!done && method == RG_METHOD_FCG_LEARNED && !stagnated) {
                         hist.push_back(sl.norm);
                         const size_t k = hist.size();
-                        if (k >= 3 && !(hist[k - 1] < 0.99 * hist[k - 3])) {
+                        const size_t window = std::max<size_t>(4, (size_t)((64 + check - 1) / check));
+                        bool improving = k <= window;
+                        if (!improving) {
+                            double best_before = hist[0];
+                            for (size_t j = 1; j + window < k; ++j) best_before = std::min(best_before, hist[j]);
+                            for (size_t j = k - window; j < k; ++j) improving = improving || hist[j] < 0.99 * best_before;
+                        }
+                        if (!improving) {
                             stagnated = true;
                             int32_t flag[2] = {1, KS_STAGNATED};
                             int32_t* pin32 = reinterpret_cast<int32_t*>(m->ctx->pinned + 128);
@@ -1259,6 +1170,9 @@ int solve(rg_matrix* m, double* x, const double* b, const rg_options& opt, rg_di
                     method = RG_METHOD_PCG_MIXED;
                     dg.fallbacks++;
                     finalize = false;
+                    // the switch is the remedy for a lack of progress: the
+                    // next outer step is not judged against this one
+                    last2 = lastmax = std::numeric_limits<double>::infinity();
                 } else if (ks == KS_MAXITER) {
                     status = RG_MAX_ITER_REACHED;
                     break;

@@ -36,6 +36,9 @@ EXPORTS = [
     "rg_matrix_export_csr", "rg_matrix_save_mm", "rg_options_default", "rg_solve", "rg_solve_host",
     "rg_lpc_num_params", "rg_lpc_create", "rg_lpc_destroy", "rg_lpc_set_params", "rg_lpc_get_params",
     "rg_lpc_apply", "rg_lpc_gradients", "rg_lpc_train_step", "rg_matrix_set_learned_preconditioner",
+    "rg_solve_multi", "rg_matrix_vector_multi", "rg_multi_autotune", "rg_multi_set_shape",
+    "rg_batch_create", "rg_batch_destroy", "rg_matrix_assemble_boundary", "rg_matrix_add_dirac",
+    "rg_batch_offsets", "rg_batch_vector", "rg_batch_set_tile", "rg_batch_autotune", "rg_batch_solve",
 ]
 
 
@@ -120,6 +123,19 @@ def load():
         "rg_lpc_gradients": (I, [P, P, P, P, P, P, P]),
         "rg_lpc_train_step": (I, [P, P, P, P, P, ctypes.c_float, P]),
         "rg_matrix_set_learned_preconditioner": (I, [P, P, P]),
+        "rg_solve_multi": (I, [P, P, P, I, ctypes.POINTER(RgOptions), P]),
+        "rg_matrix_vector_multi": (I, [P, P, P, I]),
+        "rg_multi_autotune": (I, [P, I, I, I, P, P]),
+        "rg_multi_set_shape": (I, [P, I]),
+        "rg_matrix_assemble_boundary": (I, [P, ctypes.POINTER(RgCoef), ctypes.POINTER(RgCoef), I, P]),
+        "rg_matrix_add_dirac": (I, [P, I64, P, P, P, P]),
+        "rg_batch_create": (I, [P, I, P, PP]),
+        "rg_batch_destroy": (I, [P]),
+        "rg_batch_offsets": (I, [P, P]),
+        "rg_batch_vector": (I, [P, P, P]),
+        "rg_batch_set_tile": (I, [P, I]),
+        "rg_batch_autotune": (I, [P, I, I, P, P]),
+        "rg_batch_solve": (I, [P, P, P, ctypes.POINTER(RgOptions), P]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

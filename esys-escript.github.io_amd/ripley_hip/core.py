@@ -424,6 +424,35 @@ class SystemMatrix:
         check(load().rg_matrix_set_learned_preconditioner(self.handle, net.handle, _ptr(lk)))
         self._lpc = (net, lk)
 
+    def assembleBoundary(self, coefs, rhs=None, reduced=False):
+        """Face terms d (matrix) and y (rhs) of the Coefficient objects under
+        keys "d", "y" (RipleyDomain::assemblePDEBoundary ->
+        DefaultAssembler*::assemblePDEBoundarySingle/System[Reduced])."""
+        self._bind()
+        cd = coefs.get("d")
+        cy = coefs.get("y")
+        if rhs is not None and (rhs.numel() != self.size or rhs.dtype != torch.float64 or not rhs.is_cuda):
+            raise ValueError("rhs must be a float64 GPU tensor of %d entries" % self.size)
+        d = cd.c() if cd is not None else _lib.RgCoef()
+        y = cy.c() if cy is not None else _lib.RgCoef()
+        check(load().rg_matrix_assemble_boundary(self.handle, ctypes.byref(d), ctypes.byref(y), int(bool(reduced)),
+                                                 _ptr(rhs)))
+
+    def addDirac(self, nodes, d=None, y=None, rhs=None):
+        """Dirac point terms (RipleyDomain::assemblePDEDirac): d (count, n, n)
+        added to the diagonal blocks of the nodes, y (count, n) to rhs."""
+        self._bind()
+        nodes = np.ascontiguousarray(nodes, dtype=np.int64).reshape(-1)
+        n = self.block_size
+        dd = None if d is None else np.ascontiguousarray(
+            _to_point_major(torch.as_tensor(np.asarray(d, dtype=np.float64)).reshape(len(nodes), n, n),
+                            (n, n)).numpy())
+        yy = None if y is None else np.ascontiguousarray(np.asarray(y, dtype=np.float64).reshape(len(nodes), n))
+        check(load().rg_matrix_add_dirac(self.handle, len(nodes), nodes.ctypes.data_as(ctypes.c_void_p),
+                                         dd.ctypes.data_as(ctypes.c_void_p) if dd is not None else None,
+                                         yy.ctypes.data_as(ctypes.c_void_p) if yy is not None else None,
+                                         _ptr(rhs)))
+
     def exportCSR(self):
         """(ptr, index, val) numpy arrays of the PASO CSR form."""
         ptr = np.zeros(self.num_rows + 1, dtype=np.int32)

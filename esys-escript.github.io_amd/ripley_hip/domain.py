@@ -5,7 +5,14 @@ ripley/src/Brick.cpp:89-275; python constructors ripley/src/ripleycpp.cpp:
 * Solution / ContinuousFunction: one value per node, (N, *shape);
 * Function: one value per quadrature point, (E, Q, *shape), Q = 2**dim,
   points x fastest at (1 -+ 1/sqrt3)/2 of each element;
-* ReducedFunction: one value per element, (E, *shape).
+* ReducedFunction: one value per element, (E, *shape);
+* FunctionOnBoundary: one value per face quadrature point, (F, 2**(dim-1),
+  *shape), face elements in the reference's order (faces x0=0, x0=L, x1=0,
+  x1=L[, x2=0, x2=L], within a face by the tangential element indices, lower
+  axis fastest; Rectangle.cpp/Brick.cpp m_faceOffset);
+* ReducedFunctionOnBoundary: one value per face element, (F, *shape);
+* DiracDeltaFunctions: one value per Dirac point (nearest node of each point
+  given at construction, RipleyDomain.cpp:1820-1850).
 
 Nodes and elements are numbered lexicographically with x fastest.
 """
@@ -19,13 +26,67 @@ _G1 = 0.5 + 0.5 / np.sqrt(3.0)
 
 
 class _Grid:
-    def __init__(self, ne, length, device=0):
+    def __init__(self, ne, length, device=0, diracPoints=()):
         self._ne = tuple(int(n) for n in ne)
         self._len = tuple(float(x) for x in length)
         if any(n < 1 for n in self._ne):
             raise ValueError("need at least one element per axis")
         self._device = torch.device("cuda", device) if torch.cuda.is_available() else torch.device("cpu")
         self._dev_index = device
+        self._dirac = [self._nearest_node(p) for p in diracPoints]
+
+    def _nearest_node(self, point):
+        h = self.getSpacing()
+        nn = self.getNumNodesPerAxis()
+        idx, stride = 0, 1
+        for d in range(self.getDim()):
+            k = int(np.clip(np.floor(point[d] / h[d] + 0.5), 0, nn[d] - 1))
+            idx += k * stride
+            stride *= nn[d]
+        return idx
+
+    def getDiracNodes(self):
+        """Node index of each Dirac point (DiracDeltaFunctions samples)."""
+        return list(self._dirac)
+
+    # --- boundary (FunctionOnBoundary) ------------------------------------
+    def _faces(self):
+        """[(face, normal axis, side, tangential axes, element count)]."""
+        out = []
+        dim = self.getDim()
+        for f in range(2 * dim):
+            ax, side = f // 2, f % 2
+            t = [a for a in range(dim) if a != ax]
+            out.append((f, ax, side, t, int(np.prod([self._ne[a] for a in t]))))
+        return out
+
+    def getNumFaceElements(self):
+        return sum(c for *_, c in self._faces())
+
+    def getNumBoundaryQuadraturePoints(self):
+        return 1 << (self.getDim() - 1)
+
+    def getBoundaryQuadraturePoints(self, reduced=False):
+        """Coordinates (F, Qb, dim) of the FunctionOnBoundary points (F, dim
+        for reduced: the face element centres)."""
+        dim = self.getDim()
+        h = self.getSpacing()
+        pts = []
+        for f, ax, side, t, cnt in self._faces():
+            ne_t = [self._ne[a] for a in t]
+            j = np.arange(cnt)
+            jt = [j % ne_t[0]] + ([j // ne_t[0]] if dim == 3 else [])
+            nq = 1 if reduced else (1 << (dim - 1))
+            P = np.zeros((cnt, nq, dim))
+            P[:, :, ax] = self._len[ax] if side else 0.0
+            for q in range(nq):
+                for i, a in enumerate(t):
+                    xi = 0.5 if reduced else (_G1 if (q >> i) & 1 else _G0)
+                    P[:, q, a] = (jt[i] + xi) * h[a]
+            pts.append(P)
+        P = np.concatenate(pts, axis=0)
+        P = P[:, 0, :] if reduced else P
+        return torch.as_tensor(P, device=self._device)
 
     # --- sizes (Rectangle.h:352-424, Brick.h:352-416) -------------------
     def getDim(self):
@@ -99,12 +160,12 @@ class _Grid:
 class Rectangle(_Grid):
     """ripley.Rectangle(n0, n1, l0=1, l1=1) at one rank."""
 
-    def __init__(self, n0, n1, l0=1.0, l1=1.0, device=0):
-        super().__init__((n0, n1), (l0, l1), device)
+    def __init__(self, n0, n1, l0=1.0, l1=1.0, device=0, diracPoints=()):
+        super().__init__((n0, n1), (l0, l1), device, diracPoints)
 
 
 class Brick(_Grid):
     """ripley.Brick(n0, n1, n2, l0=1, l1=1, l2=1) at one rank."""
 
-    def __init__(self, n0, n1, n2, l0=1.0, l1=1.0, l2=1.0, device=0):
-        super().__init__((n0, n1, n2), (l0, l1, l2), device)
+    def __init__(self, n0, n1, n2, l0=1.0, l1=1.0, l2=1.0, device=0, diracPoints=()):
+        super().__init__((n0, n1, n2), (l0, l1, l2), device, diracPoints)

@@ -25,7 +25,8 @@ def kronecker_scaled(value):
 
 class LinearPDE:
     COEFFICIENTS = ("A", "B", "C", "D", "X", "Y", "A_reduced", "B_reduced", "C_reduced", "D_reduced",
-                    "X_reduced", "Y_reduced", "q", "r")
+                    "X_reduced", "Y_reduced", "d", "y", "d_reduced", "y_reduced", "d_dirac", "y_dirac",
+                    "q", "r")
 
     def __init__(self, domain, numEquations=1, numSolutions=None):
         if numSolutions is None:
@@ -49,9 +50,10 @@ class LinearPDE:
         d, n = self.dim, self.n
         base = name.split("_")[0]
         if n == 1:
-            return {"A": (d, d), "B": (d,), "C": (d,), "D": (), "X": (d,), "Y": (), "q": (), "r": ()}[base]
+            return {"A": (d, d), "B": (d,), "C": (d,), "D": (), "X": (d,), "Y": (), "d": (), "y": (), "q": (),
+                    "r": ()}[base]
         return {"A": (n, d, n, d), "B": (n, d, n), "C": (n, n, d), "D": (n, n), "X": (n, d), "Y": (n,),
-                "q": (n,), "r": (n,)}[base]
+                "d": (n, n), "y": (n,), "q": (n,), "r": (n,)}[base]
 
     def setValue(self, **coefficients):
         for k, v in coefficients.items():
@@ -61,7 +63,7 @@ class LinearPDE:
             if k in ("q", "r"):
                 self._operator_valid = self._operator_valid and k == "r"
                 self._rhs_valid = False
-            elif k[0] in "XY":
+            elif k[0] in "XYy":
                 self._rhs_valid = False
             else:
                 self._operator_valid = False
@@ -103,6 +105,13 @@ class LinearPDE:
         return Coefficient(v, self._point_shape(name), self.domain.getNumElements(),
                            self.domain.getNumQuadraturePoints(), self.domain.device, isotropic=iso)
 
+    def _boundary_coefficient(self, name):
+        v = self._coef.get(name)
+        if v is None:
+            return None
+        return Coefficient(v, self._point_shape(name), self.domain.getNumFaceElements(),
+                           self.domain.getNumBoundaryQuadraturePoints(), self.domain.device)
+
     def _node_vector(self, v):
         N, n = self.domain.getNumNodes(), self.n
         if v is None:
@@ -128,6 +137,21 @@ class LinearPDE:
                 op.assemble(full, rhs, reduced=False)
             if any(c is not None for c in red.values()):
                 op.assemble(red, rhs, reduced=True)
+            # boundary and Dirac terms (RipleyDomain::addToSystem,
+            # RipleyDomain.cpp:1024-1035)
+            bnd = {k: self._boundary_coefficient(k) for k in ("d", "y")}
+            if any(c is not None for c in bnd.values()):
+                op.assembleBoundary(bnd, rhs, reduced=False)
+            bred = {k: self._boundary_coefficient(k + "_reduced") for k in ("d", "y")}
+            if any(c is not None for c in bred.values()):
+                op.assembleBoundary(bred, rhs, reduced=True)
+            dd, yd = self._coef.get("d_dirac"), self._coef.get("y_dirac")
+            if dd is not None or yd is not None:
+                nodes = self.domain.getDiracNodes()
+                op.addDirac(nodes, d=None if dd is None else np.broadcast_to(
+                    np.asarray(dd, dtype=np.float64), (len(nodes), self.n, self.n)),
+                            y=None if yd is None else np.broadcast_to(
+                    np.asarray(yd, dtype=np.float64), (len(nodes), self.n)), rhs=rhs)
             self._rhs = rhs
             self.insertConstraint()
             self._operator_valid = True

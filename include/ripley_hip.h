@@ -43,6 +43,7 @@ enum {
 typedef struct rg_context rg_context;
 typedef struct rg_matrix rg_matrix;
 typedef struct rg_lpc rg_lpc;
+typedef struct rg_batch rg_batch;
 
 /* Message of the last failure on this thread (empty string if none). */
 const char* rg_last_error(void);
@@ -134,6 +135,27 @@ typedef struct {
  * num_rows*block_size doubles).  Each node gathers its own contributions; no
  * atomics; deterministic. */
 int rg_matrix_assemble(rg_matrix* m, const rg_pde_coefs* coefs, int reduced, double* rhs);
+
+/* RipleyDomain::assemblePDEBoundary -> DefaultAssembler2D/3D::
+ * assemblePDEBoundarySingle/System[Reduced] (DefaultAssembler2D.cpp:601-840,
+ * 1014-1200; DefaultAssembler3D.cpp:2233-2890, 3274-3630): adds the face
+ * terms  int_boundary d u v  to the matrix and  int_boundary y v  to rhs
+ * (device, may be NULL).  d: n x n per data point (d[k + n*m]), y: n per
+ * point; one data point per face element (RG_COEF_ELEMENT) or per face
+ * quadrature point (RG_COEF_EXPANDED: 2 in 2-D, 4 in 3-D, lower tangential
+ * axis fastest).  Face elements are numbered as the reference's
+ * FunctionOnBoundary samples: faces x0=0, x0=L, x1=0, x1=L[, x2=0, x2=L],
+ * within a face by the two tangential element indices, lower axis fastest.
+ * reduced: one-point rule at the face centre (element data only).  Either
+ * coefficient may be NULL. */
+int rg_matrix_assemble_boundary(rg_matrix* m, const rg_coef* d, const rg_coef* y, int reduced, double* rhs);
+
+/* RipleyDomain::assemblePDEDirac (ripley/src/RipleyDomain.cpp:1756-1800):
+ * for Dirac point p (in order) at node nodes[p] (host), rhs[k + n*node] +=
+ * y[p*n + k] and the node's diagonal block += d[p*n*n + k + n*m].  d, y
+ * host (either may be NULL); rhs device (required with y). */
+int rg_matrix_add_dirac(rg_matrix* m, int64_t count, const int64_t* nodes, const double* d, const double* y,
+                        double* rhs);
 
 /* paso::SystemMatrix::MatrixVector (paso/src/SystemMatrix_MatrixVector.cpp:
  * 62-91): y = alpha*A*x + beta*y (device x, y).  Fails with RG_STATE_ERROR
@@ -228,6 +250,54 @@ int rg_solve(rg_matrix* m, double* x, const double* b, const rg_options* opt,
              rg_diagnostics* diag);
 int rg_solve_host(rg_matrix* m, double* x, const double* b, const rg_options* opt,
                   rg_diagnostics* diag);
+
+/* ---------------------------------------------------------- batched solves
+ * No counterpart in the reference (PASO solves one right-hand side of one
+ * system per call, paso/src/solve.cpp:41-145).  Both solvers run the
+ * textbook Jacobi-PCG (PCG.cpp:62-368 without the residual smoothing) on
+ * the UNBALANCED system from x = 0, with FP64 (RG_METHOD_PCG_PLAIN) or
+ * FP32 (RG_METHOD_PCG_MIXED) matrix values in the SpMV, FP64 vectors, dots
+ * and updates, and an FP64 residual replacement r = b - A x every
+ * check_interval iterations; each system / column converges on
+ * ||b - A x||_2 <= tolerance * ||b||_2 of that FP64 residual.  The matrices
+ * must not have been balanced by rg_solve (set or reset their values
+ * first).  Scalar matrices with full storage only.  diag receives one entry
+ * per column / system; the return value is the worst status (with
+ * accept_failed_convergence, MaxIterReached is not an error). */
+
+/* Many right-hand sides on one matrix: X, B device, row-major
+ * num_rows x nrhs (column index fastest), 1 <= nrhs <= 64.  The SpMM reads
+ * the matrix once per iteration for all columns; its tile shape is
+ * autotuned on the first call (rg_multi_autotune sets it explicitly). */
+int rg_solve_multi(rg_matrix* m, double* X, const double* B, int nrhs, const rg_options* opt,
+                   rg_diagnostics* diag);
+/* Y = A X for a row-major num_rows x nrhs block (device). */
+int rg_matrix_vector_multi(rg_matrix* m, const double* X, double* Y, int nrhs);
+/* Times reps SpMMs for each tile shape -- (rows, columns) per thread of
+ * (1,1), (1,2), (1,4), (2,2); shapes whose column count does not divide nrhs
+ * are skipped (time -1) -- keeps the fastest for this matrix and returns its
+ * index in *shape; times (host, may be NULL) receives 4 seconds/SpMM. */
+int rg_multi_autotune(rg_matrix* m, int nrhs, int fp32, int reps, int* shape, double* times);
+/* Fixes the SpMM shape (0..3 as above) or, with -1, autotunes on the next
+ * solve. */
+int rg_multi_set_shape(rg_matrix* m, int shape);
+
+/* A batch of independent systems (count matrices of one dimension, sizes
+ * free) solved together: every kernel launch spans all systems (a tile
+ * table maps workgroups to (system, rows)).  Vectors are the systems'
+ * vectors concatenated at rg_batch_offsets (count + 1 entries, each offset
+ * a multiple of 64).  The matrices are borrowed. */
+int rg_batch_create(rg_context* ctx, int count, rg_matrix* const* mats, rg_batch** out);
+int rg_batch_destroy(rg_batch* b);
+int rg_batch_offsets(const rg_batch* b, int64_t* offsets);
+/* y = A x for every system (device, concatenated). */
+int rg_batch_vector(rg_batch* b, const double* x, double* y);
+/* Rows per tile = 256 * (2 FP64 | 4 FP32) * tile_mult; 0 = autotune on the
+ * next solve.  rg_batch_autotune times reps batched SpMVs per candidate
+ * (1, 2, 4, 8), keeps the fastest; times (host, may be NULL): 4 entries. */
+int rg_batch_set_tile(rg_batch* b, int tile_mult);
+int rg_batch_autotune(rg_batch* b, int fp32, int reps, int* tile_mult, double* times);
+int rg_batch_solve(rg_batch* b, double* x, const double* rhs, const rg_options* opt, rg_diagnostics* diag);
 
 /* --------------------------------------------------- learned preconditioner
  * No counterpart in the reference (its preconditioners are PASO's Jacobi,

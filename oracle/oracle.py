@@ -59,6 +59,8 @@ def lib():
             "orc_element_matrices": (None, [I, P, I, I, CP, CP, CP, CP, CP, CP, P, P]),
             "orc_add_to_csr": (I, [I, I64, P, P, P, I, P, P]),
             "orc_assemble": (None, [I, P, P, I, I, CP, CP, CP, CP, CP, CP, P, P, P, P]),
+            "orc_assemble_boundary": (None, [I, P, P, I, I, CP, CP, P, P, P, P]),
+            "orc_add_dirac": (None, [I, I64, P, P, P, I64, P, P, P, P]),
             "orc_spmv": (None, [I, I64, P, P, P, D, P, D, P]),
             "orc_nullify": (None, [I, I64, P, P, P, P, P, D]),
             "orc_balance": (None, [I, I64, P, P, P, P]),
@@ -193,6 +195,36 @@ def assemble(dim, ne, length, n=1, ptr=None, index=None, expanded=(), reduced=Fa
     lib().orc_assemble(dim, _p(ne_a), _p(ln), n, int(reduced), *[ctypes.byref(cs[k]) for k in "ABCDXY"],
                        _p(ptr), _p(index), _p(val), _p(rhs))
     return ptr, index, val, rhs
+
+
+def assemble_boundary(dim, ne, length, n=1, ptr=None, index=None, reduced=False, val=None, rhs=None,
+                      d=None, y=None, expanded=()):
+    """Boundary terms d (n x n per point) and y (n per point) on the face
+    elements; coefficients per face element (reference FunctionOnBoundary
+    order), or per face element and point with 'd'/'y' in ``expanded``."""
+    if ptr is None:
+        ptr, index = pattern(dim, ne)
+    N = len(ptr) - 1
+    val = np.zeros(len(index) * n * n) if val is None else val
+    rhs = np.zeros(N * n) if rhs is None else rhs
+    ne_a = np.asarray(list(ne) + [1] * (3 - len(ne)), dtype=np.int32)
+    ln = np.asarray(list(length) + [1.0] * (3 - len(length)), dtype=np.float64)
+    pd = () if n == 1 else (n, n)
+    py = () if n == 1 else (n,)
+    cd = Coef(d, pd, expanded="d" in expanded)
+    cy = Coef(y, py, expanded="y" in expanded)
+    lib().orc_assemble_boundary(dim, _p(ne_a), _p(ln), n, int(reduced), ctypes.byref(cd.c()), ctypes.byref(cy.c()),
+                                _p(ptr), _p(index), _p(val), _p(rhs))
+    return ptr, index, val, rhs
+
+
+def add_dirac(n, ptr, index, val, nodes, d=None, y=None, rhs=None):
+    nodes = np.ascontiguousarray(nodes, dtype=np.int64)
+    dd = None if d is None else np.ascontiguousarray(d, dtype=np.float64)
+    yy = None if y is None else np.ascontiguousarray(y, dtype=np.float64)
+    lib().orc_add_dirac(n, len(ptr) - 1, _p(ptr), _p(index), _p(val) if val is not None else None, len(nodes),
+                        _p(nodes), _p(dd) if dd is not None else None, _p(yy) if yy is not None else None,
+                        _p(rhs) if rhs is not None else None)
 
 
 def add_to_csr(bs, ptr, index, val, nodes, EM):

@@ -343,6 +343,106 @@ ORC_API void orc_assemble(int dim, const int* ne, const double* len, int n, int 
     }
 }
 
+// ------------------------------------------------------ boundary, Dirac
+
+// Face-element matrices of assemblePDEBoundarySingle/System and their
+// Reduced variants (DefaultAssembler2D.cpp:601-840, :1014-1200;
+// DefaultAssembler3D.cpp:2233-2890, :3274-3630), and the element loop with
+// addToMatrixAndRHS: faces in the order x0=0, x0=L, x1=0, x1=L[, x2=0,
+// x2=L]; face elements of a face numbered by their two tangential element
+// indices, lower axis fastest, offset by the element counts of the faces
+// before (m_faceOffset); quadrature points on a face element 2 (2-D) or 2x2
+// (3-D), lower tangential axis fastest, at the Gauss points of the element's
+// edge/face, weight = face measure / points; the one-point rule at the face
+// centre for reduced.  Per face element and point:
+//   EM_S[k,m,a,b] += w d[k + n m] N_a N_b,   EM_F[k,a] += w y[k] N_a
+// for the element nodes a, b on the face (the element's other nodes get 0),
+// then added with addToMatrixAndRHS at the element's first node
+// (e.g. DefaultAssembler3D.cpp:2360-2364).
+ORC_API void orc_assemble_boundary(int dim, const int* ne, const double* len, int n, int reduced,
+                                   const orc_coef* d, const orc_coef* y, const index_t* ptr, const index_t* index,
+                                   double* val, double* rhs)
+{
+    double h[3] = {1.0, 1.0, 1.0};
+    for (int a = 0; a < dim; ++a) h[a] = len[a] / ne[a];
+    const int nN = 1 << dim;
+    const int nn[3] = {ne[0] + 1, ne[1] + 1, dim == 3 ? ne[2] + 1 : 1};
+    const int64_t N = (int64_t)nn[0] * nn[1] * nn[2];
+    const bool addS = d->mode && val;
+    const bool addF = y->mode && rhs;
+    const int nq = reduced ? 1 : (1 << (dim - 1));
+    std::vector<double> EM_S(n * n * nN * nN), EM_F(n * nN);
+    int64_t offset = 0;
+    for (int f = 0; f < 2 * dim; ++f) {
+        const int ax = f / 2, side = f % 2;
+        int t[2] = {-1, -1}, nt = 0;
+        for (int a = 0; a < dim; ++a)
+            if (a != ax) t[nt++] = a;
+        const int ne0 = ne[t[0]], ne1 = (dim == 3) ? ne[t[1]] : 1;
+        const double meas = h[t[0]] * (dim == 3 ? h[t[1]] : 1.0);
+        const double w = meas / nq;
+        for (int j1 = 0; j1 < ne1; ++j1)
+            for (int j0 = 0; j0 < ne0; ++j0) {
+                const int64_t e = offset + j0 + (int64_t)ne0 * j1;
+                std::fill(EM_S.begin(), EM_S.end(), 0.0);
+                std::fill(EM_F.begin(), EM_F.end(), 0.0);
+                for (int q = 0; q < nq; ++q) {
+                    const double xi0 = reduced ? 0.5 : gauss_pt(q & 1);
+                    const double xi1 = reduced ? 0.5 : gauss_pt((q >> 1) & 1);
+                    const double* dp = addS ? coef_point(*d, e, q, n * n) : nullptr;
+                    const double* yp = addF ? coef_point(*y, e, q, n) : nullptr;
+                    for (int a = 0; a < nN; ++a) {
+                        const int ab[3] = {a & 1, (a >> 1) & 1, (a >> 2) & 1};
+                        if (ab[ax] != side) continue;
+                        const double Na = (ab[t[0]] ? xi0 : 1.0 - xi0) *
+                                          (dim == 3 ? (ab[t[1]] ? xi1 : 1.0 - xi1) : 1.0);
+                        if (addF)
+                            for (int k = 0; k < n; ++k) EM_F[k + n * a] += w * yp[k] * Na;
+                        if (addS)
+                            for (int b = 0; b < nN; ++b) {
+                                const int bb[3] = {b & 1, (b >> 1) & 1, (b >> 2) & 1};
+                                if (bb[ax] != side) continue;
+                                const double Nb = (bb[t[0]] ? xi0 : 1.0 - xi0) *
+                                                  (dim == 3 ? (bb[t[1]] ? xi1 : 1.0 - xi1) : 1.0);
+                                for (int m = 0; m < n; ++m)
+                                    for (int k = 0; k < n; ++k)
+                                        EM_S[k + n * (m + n * (a + nN * b))] += w * dp[k + n * m] * Na * Nb;
+                            }
+                    }
+                }
+                // the volume element carrying the face element
+                int kk[3] = {0, 0, 0};
+                kk[ax] = side ? ne[ax] - 1 : 0;
+                kk[t[0]] = j0;
+                if (dim == 3) kk[t[1]] = j1;
+                const int64_t first = kk[0] + (int64_t)nn[0] * (kk[1] + (int64_t)nn[1] * kk[2]);
+                index_t nodes[8];
+                for (int a = 0; a < nN; ++a)
+                    nodes[a] = (index_t)(first + (a & 1) + ((a >> 1) & 1) * nn[0] +
+                                         ((a >> 2) & 1) * (int64_t)nn[0] * nn[1]);
+                if (addF)
+                    for (int a = 0; a < nN; ++a)
+                        for (int k = 0; k < n; ++k) rhs[(int64_t)k + (int64_t)n * nodes[a]] += EM_F[k + n * a];
+                if (addS) orc_add_to_csr(n, N, ptr, index, val, nN, nodes, EM_S.data());
+            }
+        offset += (int64_t)ne0 * ne1;
+    }
+}
+
+// RipleyDomain::assemblePDEDirac (RipleyDomain.cpp:1756-1800): for each
+// Dirac point in order, rhs[eq + n*dof] += y_i[eq] and the n x n block d_i
+// added to the diagonal block of row dof (addToSystemMatrix with one node).
+ORC_API void orc_add_dirac(int n, int64_t nrows, const index_t* ptr, const index_t* index, double* val,
+                           int64_t count, const int64_t* nodes, const double* d, const double* y, double* rhs)
+{
+    for (int64_t p = 0; p < count; ++p) {
+        const index_t dof = (index_t)nodes[p];
+        if (y && rhs)
+            for (int k = 0; k < n; ++k) rhs[k + (int64_t)n * dof] += y[p * n + k];
+        if (d && val) orc_add_to_csr(n, nrows, ptr, index, val, 1, &dof, d + p * n * n);
+    }
+}
+
 // ------------------------------------------------------- matrix operations
 
 // SparseMatrix_MatrixVector_CSR_OFFSET0: out = alpha*A*in + beta*out with the
