@@ -34,7 +34,7 @@ def parse():
     ap.add_argument("--cpu-iters", type=int, default=40)
     ap.add_argument("--spmv-reps", type=int, default=50)
     ap.add_argument("--quiet", action="store_true")
-    ap.add_argument("--extras", default="c1,c3,c4",
+    ap.add_argument("--extras", default="c1,c3,c4,c5",
                     help="comma list of the other configurations to measure (c1, c3, c4, c5); '' for none")
     ap.add_argument("--c4-rhs", type=int, default=64)
     ap.add_argument("--probe", action="store_true",
@@ -76,6 +76,15 @@ class Problem:
         self.op.nullifyRowsAndCols(self.q, self.q, 1.0)
         self.rhs.masked_fill_(self.q > 0, 0.0)
 
+    def attach_network(self, weights):
+        """The learned preconditioner for this grid (weights trained at 64^3
+        on the same field family by ripley_hip.learned)."""
+        from ripley_hip import learned
+        torch = self.torch
+        nodes = tuple(n + 1 for n in self.ne)
+        self.net = learned.LearnedPreconditioner(nodes, params=self.np.load(weights, allow_pickle=False))
+        self.logk = learned.nodal_log10(self.kappa.cpu(), self.ne).to("cuda")
+
     def solve(self, method):
         R = self.R
         o = R.SolverOptions()
@@ -83,7 +92,10 @@ class Problem:
         o.setTolerance(1e-8)
         o.setTrueResidualCriterion(True)
         o.setSolverMethod({"paso": R.SolverOptions.PCG, "plain": R.SolverOptions.PCG_PLAIN,
-                           "mixed": R.SolverOptions.PCG_MIXED}[method])
+                           "mixed": R.SolverOptions.PCG_MIXED, "learned": R.SolverOptions.FCG_LEARNED}[method])
+        if method == "learned":
+            self.op.setLearnedPreconditioner(self.net, self.logk)
+            o.setCheckInterval(8)
         self.op.setToSolution(self.x, self.rhs, o)
         return o
 
@@ -403,6 +415,7 @@ def bench_c4(R, torch, np, args):
     best, times = RB.autotune_many(op, K, fp32=True, reps=3)
     t, (X, diags) = _timed(torch, lambda: RB.solve_many(op, B, o))
     Bsub = B[:, :1].contiguous()
+    RB.solve_many(op, Bsub, _opts(R, "PCG_MIXED"))   # warm-up: tile choice for one column, graph capture
     t1, (X1, d1) = _timed(torch, lambda: RB.solve_many(op, Bsub, _opts(R, "PCG_MIXED")))
     R_ = B - RB.matrix_vector_many(op, X)
     rel = (torch.linalg.norm(R_, dim=0) / torch.linalg.norm(B, dim=0)).max().item()
@@ -412,7 +425,7 @@ def bench_c4(R, torch, np, args):
                        "rhs_per_s_batched": K / t, "rhs_per_s_single": 1.0 / t1,
                        "speedup_vs_one_at_a_time": (K * t1) / t, "iters_min_max": [min(its), max(its)],
                        "max_rel_residual_fp64": rel, "all_converged": all(d["converged"] for d in diags),
-                       "autotune_rows_per_thread": best, "autotune_s_per_spmm": times}
+                       "autotune_spmm_shape": RB.SPMM_SHAPES[best], "autotune_s_per_spmm": times}
     del X, B, R_, op
     # (ii) uneven 2-D instances
     rng = np.random.Generator(np.random.PCG64(20261205))
@@ -468,6 +481,51 @@ def bench_c4(R, torch, np, args):
     return out
 
 
+def bench_c5(R, torch, np, args):
+    """Brick 192^3 Poisson, kappa = exp(GRF) with sigma_log 2 and correlation
+    length 12 cells (seed 20261206), Y = 1, u = 0 on z = 0: plain FP64
+    Jacobi-PCG vs FP32-SpMV PCG vs the flexible CG with the learned
+    preconditioner trained on 64^3 fields of the same family."""
+    from ripley_hip import fields, learned
+    ne = (192, 192, 192)
+    dom = R.Brick(*ne)
+    E, Q = dom.getNumElements(), dom.getNumQuadraturePoints()
+    kap_np = fields.lognormal_kappa(ne, corr_len=12.0, sigma_log=2.0, seed=20261206)
+    kap = torch.as_tensor(kap_np, device="cuda")
+    cA = R.Coefficient(kap, (3, 3), E, Q, dom.device, isotropic=True)
+    cY = R.Coefficient(1.0, (), E, Q, dom.device)
+    q = dom.whereZero(2)
+
+    def system():
+        op = dom.newOperator(1)
+        rhs = torch.zeros(op.size, dtype=torch.float64, device="cuda")
+        op.assemble({"A": cA, "Y": cY}, rhs)
+        op.nullifyRowsAndCols(q, q, 1.0)
+        rhs.masked_fill_(q > 0, 0.0)
+        return op, rhs
+
+    out = {"workload": "ripley Brick 192^3 Poisson, ln kappa GRF (sigma 2, corr. 12 cells), Y=1, u=0 on z=0, "
+                       "rtol 1e-8", "unknowns": dom.getNumNodes()}
+    weights = os.path.join(ROOT, "esys-escript.github.io_amd", "ripley_hip", "weights", "lpc_lognormal_s2_l12.npy")
+    methods = ["PCG_PLAIN", "PCG_MIXED"] + (["FCG_LEARNED"] if os.path.exists(weights) else [])
+    net = None
+    for method in methods:
+        op, rhs = system()
+        o = _opts(R, method)
+        if method == "FCG_LEARNED":
+            net = learned.LearnedPreconditioner(tuple(n + 1 for n in ne), params=np.load(weights, allow_pickle=False))
+            op.setLearnedPreconditioner(net, learned.nodal_log10(kap.cpu(), ne).to("cuda"))
+            o.setCheckInterval(8)
+        t, x = _timed(torch, lambda: op.solve(rhs, o))
+        op2, _ = system()
+        out[method] = {"solve_s": t, "num_iter": o.getDiagnostics("num_iter"), "converged": o.hasConverged(),
+                       "fallbacks": o.getDiagnostics("fallbacks"), "rel_residual_fp64": _rel_res(torch, op2, x, rhs)}
+        del op, op2
+    if "FCG_LEARNED" in out:
+        out["learned_speedup_vs_plain"] = out["PCG_PLAIN"]["solve_s"] / out["FCG_LEARNED"]["solve_s"]
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -485,13 +543,20 @@ def main():
     log(args, "building C2 problem ne=%d" % args.ne)
     P = Problem(args.ne, R, torch, np, fields)
     methods = ["plain", "paso", "mixed"] if args.method == "all" else [args.method]
+    weights = os.path.join(ROOT, "esys-escript.github.io_amd", "ripley_hip", "weights", "lpc_log10grf_s1_l8.npy")
+    if args.method == "all" and os.path.exists(weights):
+        methods.append("learned")
+    if "learned" in methods:
+        P.attach_network(weights)
     results = {}
     for m in methods:
         log(args, "method", m)
         bd = breakdown(P, m, torch)
         rel = true_residual(P)
-        t, opts = time_steps(P, m, args.steps, args.warmup, torch)
-        results[m] = dict(bd, step_s=t, rel_residual_fp64=rel, converged=bool(opts.hasConverged()))
+        t, opts = time_steps(P, m, args.steps if m != "learned" else 1, args.warmup if m != "learned" else 0,
+                             torch)
+        results[m] = dict(bd, step_s=t, rel_residual_fp64=rel, converged=bool(opts.hasConverged()),
+                          fallbacks=opts.getDiagnostics("fallbacks"))
         log(args, m, json.dumps(results[m]))
     ok = {m: r for m, r in results.items() if r["converged"] and r["rel_residual_fp64"] <= 1e-8 * 1.01}
     head = min(ok, key=lambda m: ok[m]["step_s"]) if ok else methods[0]
@@ -524,6 +589,8 @@ def main():
                    "unknowns": N, "nnz": P.op.nnz, "method": head},
         "methods": results,
         "speedup_vs_plain": round(results["plain"]["step_s"] / hr["step_s"], 3) if "plain" in results else None,
+        "learned_fcg_speedup_vs_plain": (round(results["plain"]["step_s"] / results["learned"]["step_s"], 3)
+                                         if "plain" in results and "learned" in results else None),
         "dof_iter_per_s": round(N * hr["num_iter"] / hr["solve_s"], 1),
         "roofline": roof,
         "device_copy_GBps": round(copy_bw, 1),
@@ -531,7 +598,7 @@ def main():
     }
     extras = {}
     for name in [e.strip() for e in args.extras.split(",") if e.strip()]:
-        fn = {"c1": bench_c1, "c3": bench_c3, "c4": bench_c4}.get(name)
+        fn = {"c1": bench_c1, "c3": bench_c3, "c4": bench_c4, "c5": bench_c5}.get(name)
         if fn is None:
             continue
         log(args, "config", name)

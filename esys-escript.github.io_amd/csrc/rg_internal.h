@@ -72,7 +72,8 @@ struct KrylovState {
     double rho, rnorm0, rho_prev, pad1;
     int32_t num_iter, maxit, done, status;
     int32_t replace_mode;   // 1: convergence is only declared on a replaced (true) residual
-    int32_t pad[3];
+    int32_t early;          // BiCGStab: converged on ||s|| this iteration (x += alpha phat pending)
+    int32_t pad[2];
 };
 enum { KS_RUNNING = 0, KS_CONVERGED = 1, KS_MAXITER = 2, KS_BREAKDOWN = 3, KS_STAGNATED = 4 };
 
@@ -93,6 +94,7 @@ struct rg_solver_work {
     int64_t n = 0;   // N*bs
     double *r = nullptr, *xw = nullptr, *rs = nullptr, *p = nullptr, *v = nullptr, *x2 = nullptr;
     double *z = nullptr, *z2 = nullptr;
+    double* s = nullptr;            // BiCGStab's s
     double* bb = nullptr;           // balanced right-hand side D b
     double* norms = nullptr;        // [0] sum of squares, [1] max abs
     rg::KrylovState* st = nullptr;

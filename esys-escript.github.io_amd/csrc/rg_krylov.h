@@ -29,14 +29,16 @@ enum Stage {
     ST_PASO_INIT, ST_PASO_SPMV, ST_PASO_R, ST_PASO_X,
     ST_CG_INIT, ST_CG_SPMV, ST_CG_UPDATE, ST_CG_UPDATE_NORZ, ST_CG_RZ, ST_CG_REPLACE,
     ST_FCG_INIT, ST_FCG_BETA, ST_FCG_ALPHA, ST_FCG_UPDATE, ST_FCG_REPLACE,
-    ST_BATCH_INIT
+    ST_BATCH_INIT,
+    ST_BICG_RHO, ST_BICG_ALPHA, ST_BICG_S, ST_BICG_XA, ST_BICG_OMEGA, ST_BICG_END
 };
 
 // Stages that start a solve run even when the state says done.
 template <int STAGE>
 __device__ __forceinline__ bool stage_starts()
 {
-    return STAGE == ST_PASO_INIT || STAGE == ST_CG_INIT || STAGE == ST_FCG_INIT || STAGE == ST_BATCH_INIT;
+    return STAGE == ST_PASO_INIT || STAGE == ST_CG_INIT || STAGE == ST_FCG_INIT || STAGE == ST_BATCH_INIT ||
+           STAGE == ST_BICG_XA;
 }
 
 // The scalar recurrence of one stage, given the reduced totals (one thread).
@@ -112,6 +114,34 @@ __device__ __forceinline__ void stage_apply(KrylovState* st, const double (&tot)
         if (st->norm <= st->tol) { st->done = 1; st->status = KS_CONVERGED; }
         else if (fabs(tot[0]) <= 0.0) { st->done = 1; st->status = KS_BREAKDOWN; }
         else { st->beta = tot[0] / st->rho_prev; st->rho = tot[0]; }
+    } else if constexpr (STAGE == ST_BICG_RHO) {
+        // Solver_BiCGStab (paso/src/BiCGStab.cpp:125-150)
+        st->num_iter += 1;
+        st->tau = tot[0];
+        if (fabs(tot[0]) <= 0.0) { st->done = 1; st->status = KS_BREAKDOWN; }
+        else st->beta = (st->num_iter > 1) ? tot[0] / st->tau_new * (st->alpha / st->gamma1) : 0.0;
+    } else if constexpr (STAGE == ST_BICG_ALPHA) {
+        if (fabs(tot[0]) <= 0.0) { st->done = 1; st->status = KS_BREAKDOWN; }   // :164
+        else st->alpha = st->tau / tot[0];
+    } else if constexpr (STAGE == ST_BICG_S) {
+        st->norm = sqrt(tot[0]);                                                  // :176-186
+        if (st->norm <= st->tol) { st->done = 1; st->status = KS_CONVERGED; st->early = 1; }
+    } else if constexpr (STAGE == ST_BICG_XA) {
+        st->early = 0;
+    } else if constexpr (STAGE == ST_BICG_OMEGA) {
+        if (fabs(tot[1]) <= 0.0) { st->done = 1; st->status = KS_BREAKDOWN; }    // :206
+        else st->gamma1 = tot[0] / tot[1];
+    } else if constexpr (STAGE == ST_BICG_END) {
+        st->norm = sqrt(tot[0]);                                                  // :221-228
+        const bool conv = st->norm <= st->tol;
+        const bool maxi = st->num_iter > st->maxit;
+        const bool brk = fabs(st->gamma1) <= 0.0;
+        if (conv || maxi || brk) {
+            st->done = 1;
+            st->status = conv ? KS_CONVERGED : (maxi ? KS_MAXITER : KS_BREAKDOWN);
+        } else {
+            st->tau_new = st->tau;
+        }
     } else if constexpr (STAGE == ST_FCG_INIT) {
         st->norm = sqrt(tot[1]);
         st->num_iter = 0;

@@ -265,6 +265,7 @@ __global__ __launch_bounds__(kBlock) void k_norms_stage(const double* __restrict
 // size 1 move two entries per thread with 16-byte accesses.
 struct PcgArgs {
     double *r, *x, *rs, *p, *v, *x2;
+    double *zp, *z2p, *s;   // BiCGStab: phat, shat, s
     const double* z;        // stored M^-1 r (Jacobi sweeps > 1) or nullptr
     const double* b;        // right-hand side of a residual replacement
     KrylovState* st;
@@ -626,6 +627,128 @@ __global__ __launch_bounds__(kBlock) void k_fcg_update(PcgArgs a)
     publish_block_sums<1>(s, a.partials);
 }
 
+// ------------------------------------------------------------ BiCGStab
+// Solver_BiCGStab (paso/src/BiCGStab.cpp:77-257), Jacobi preconditioner,
+// vectors: rtld = rs, p, v, t = x2, phat = zp, shat = z2p, s.
+//   rho = rtld.r                                   k_bicg_rho     -> ST_BICG_RHO
+//   p = r + beta (p - omega v), phat = M^-1 p        k_bicg_p
+//   v = A phat, rtld.v                              k_bicg_spmv_v  -> ST_BICG_ALPHA
+//   r -= alpha v, s = r, shat = M^-1 s, ||s||        k_bicg_s       -> ST_BICG_S
+//   (converged on ||s||: x += alpha phat)           k_bicg_xa      -> ST_BICG_XA
+//   t = A shat, t.s, t.t                            k_bicg_spmv_t  -> ST_BICG_OMEGA
+//   x += alpha phat + omega shat, r = s - omega t    k_bicg_update  -> ST_BICG_END
+template <int BS>
+__global__ __launch_bounds__(kBlock) void k_bicg_init(PcgArgs a)
+{
+    const RowRange rr = block_rows(a.N * BS, a.chunk);
+    for (int64_t k = rr.r0 + threadIdx.x; k < rr.r1; k += kBlock) {
+        a.rs[k] = a.r[k];
+        a.p[k] = 0.0;
+        a.v[k] = 0.0;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_bicg_rho(PcgArgs a, int64_t n)
+{
+    if (a.st->done) return;
+    double s[1] = {0.0};
+    for (int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x; k < n; k += (int64_t)gridDim.x * kBlock)
+        s[0] += a.rs[k] * a.r[k];
+    publish_block_sums<1>(s, a.partials);
+}
+
+template <int BS>
+__global__ __launch_bounds__(kBlock) void k_bicg_p(PcgArgs a, const double* __restrict__ dinv)
+{
+    if (a.st->done) return;
+    const double beta = a.st->beta, omega = a.st->gamma1;
+    const bool first = a.st->num_iter <= 1;
+    const RowRange rr = block_rows(a.N, a.chunk);
+    for (int64_t i = rr.r0 + threadIdx.x; i < rr.r1; i += kBlock) {
+#pragma unroll
+        for (int c = 0; c < BS; ++c) {
+            const int64_t k = i * BS + c;
+            a.p[k] = first ? a.r[k] : a.r[k] + beta * (a.p[k] - omega * a.v[k]);
+        }
+        double z[BS];
+        jacobi_node<BS>(dinv, a.ld, i, a.p, z);
+#pragma unroll
+        for (int c = 0; c < BS; ++c) a.zp[i * BS + c] = z[c];
+    }
+}
+
+// out = A in (stencil), partials: MODE 0: rtld.out; MODE 1: out.s, out.out
+template <int DIM, int BS, bool SYM, int MODE>
+__global__ __launch_bounds__(kBlock) void k_bicg_spmv(PcgArgs a, const double* __restrict__ val,
+                                                      const double* __restrict__ in, double* __restrict__ out)
+{
+    if (a.st->done) return;
+    const RowRange rr = block_rows(a.N, a.chunk);
+    double s[2] = {0.0, 0.0};
+    stencil_block<DIM, BS, SYM, double>(val, a.ld, a.N, a.s1, a.s2, rr.r0, rr.r1, in,
+                                        [&](int64_t i, const double (&acc)[BS]) {
+#pragma unroll
+                                            for (int c = 0; c < BS; ++c) {
+                                                const int64_t k = i * BS + c;
+                                                out[k] = acc[c];
+                                                if (MODE == 0) {
+                                                    s[0] += a.rs[k] * acc[c];
+                                                } else {
+                                                    s[0] += acc[c] * a.s[k];
+                                                    s[1] += acc[c] * acc[c];
+                                                }
+                                            }
+                                        });
+    publish_block_sums<2>(s, a.partials);
+}
+
+template <int BS>
+__global__ __launch_bounds__(kBlock) void k_bicg_s(PcgArgs a, const double* __restrict__ dinv)
+{
+    if (a.st->done) return;
+    const double alpha = a.st->alpha;
+    const RowRange rr = block_rows(a.N, a.chunk);
+    double s[1] = {0.0};
+    for (int64_t i = rr.r0 + threadIdx.x; i < rr.r1; i += kBlock) {
+#pragma unroll
+        for (int c = 0; c < BS; ++c) {
+            const int64_t k = i * BS + c;
+            const double rv = a.r[k] - alpha * a.v[k];
+            a.r[k] = rv;
+            a.s[k] = rv;
+            s[0] += rv * rv;
+        }
+        double z[BS];
+        jacobi_node<BS>(dinv, a.ld, i, a.s, z);
+#pragma unroll
+        for (int c = 0; c < BS; ++c) a.z2p[i * BS + c] = z[c];
+    }
+    publish_block_sums<1>(s, a.partials);
+}
+
+// x += alpha phat after convergence on ||s|| (BiCGStab.cpp:181-186)
+__global__ __launch_bounds__(kBlock) void k_bicg_xa(PcgArgs a, int64_t n)
+{
+    if (!a.st->early) return;
+    const double alpha = a.st->alpha;
+    for (int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x; k < n; k += (int64_t)gridDim.x * kBlock)
+        a.x[k] += alpha * a.zp[k];
+}
+
+__global__ __launch_bounds__(kBlock) void k_bicg_update(PcgArgs a, int64_t n)
+{
+    if (a.st->done) return;
+    const double alpha = a.st->alpha, omega = a.st->gamma1;
+    double s[1] = {0.0};
+    for (int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x; k < n; k += (int64_t)gridDim.x * kBlock) {
+        a.x[k] += alpha * a.zp[k] + omega * a.z2p[k];
+        const double rv = a.s[k] - omega * a.x2[k];
+        a.r[k] = rv;
+        s[0] += rv * rv;
+    }
+    publish_block_sums<1>(s, a.partials);
+}
+
 // ------------------------------------------------------------ stage kernels
 // One workgroup after each reducing kernel (recurrences in rg_krylov.h).
 template <int STAGE, int NV>
@@ -694,6 +817,7 @@ struct SolverImpl {
     {
         PcgArgs a;
         a.r = w->r; a.x = w->xw; a.rs = w->rs; a.p = w->p; a.v = w->v; a.x2 = w->x2;
+        a.zp = w->z; a.z2p = w->z2; a.s = w->s;
         a.z = zbuf;
         a.b = w->bb;
         a.st = w->st;
@@ -836,9 +960,37 @@ struct SolverImpl {
         RG_HIP(hipGetLastError());
     }
 
+    // One iteration of PASO's BiCGStab (FP64, one Jacobi sweep).
+    void bicg_iteration()
+    {
+        const Geom& g = m->g;
+        const Launch Lsp = spmv_launch<double>(), Ln = chunked_launch(g.N);
+        const PcgArgs an = args(nullptr, Ln), asp = args(nullptr, Lsp);
+        const int vg = 1024;
+        hipLaunchKernelGGL(k_bicg_rho, dim3(vg), dim3(kBlock), 0, st, an, w->n);
+        stage<ST_BICG_RHO, 1>(vg);
+        dispatch(g, m->symmetric, [&](auto D, auto B, auto S) {
+            hipLaunchKernelGGL((k_bicg_p<B.value>), dim3(Ln.grid), dim3(kBlock), 0, st, an, m->dinv);
+            hipLaunchKernelGGL((k_bicg_spmv<D.value, B.value, S.value, 0>), dim3(Lsp.grid), dim3(kBlock), 0, st, asp,
+                               m->val, w->z, w->v);
+            stage<ST_BICG_ALPHA, 2>(Lsp.grid);
+            hipLaunchKernelGGL((k_bicg_s<B.value>), dim3(Ln.grid), dim3(kBlock), 0, st, an, m->dinv);
+            stage<ST_BICG_S, 1>(Ln.grid);
+            hipLaunchKernelGGL(k_bicg_xa, dim3(vg), dim3(kBlock), 0, st, an, w->n);
+            stage<ST_BICG_XA, 1>(vg);
+            hipLaunchKernelGGL((k_bicg_spmv<D.value, B.value, S.value, 1>), dim3(Lsp.grid), dim3(kBlock), 0, st, asp,
+                               m->val, w->z2, w->x2);
+            stage<ST_BICG_OMEGA, 2>(Lsp.grid);
+        });
+        hipLaunchKernelGGL(k_bicg_update, dim3(vg), dim3(kBlock), 0, st, an, w->n);
+        stage<ST_BICG_END, 1>(vg);
+        RG_HIP(hipGetLastError());
+    }
+
     void iteration(int method, int sweeps, bool last)
     {
-        if (method == RG_METHOD_FCG_LEARNED) fcg_iteration(last);
+        if (method == RG_METHOD_BICGSTAB) bicg_iteration();
+        else if (method == RG_METHOD_FCG_LEARNED) fcg_iteration(last);
         else if (method == RG_METHOD_PCG) paso_iteration(sweeps);
         else if (method == RG_METHOD_PCG_PLAIN) cg_iteration<double, double>(m->val, m->dinv, sweeps, false);
         else cg_iteration<float, float>(m->val32, m->dinv32, 1, last);
@@ -860,6 +1012,11 @@ struct SolverImpl {
             } else if (method == RG_METHOD_PCG_PLAIN) {
                 hipLaunchKernelGGL((k_cg_init<B.value, double>), dim3(L.grid), dim3(kBlock), 0, st, a, m->dinv);
                 stage<ST_CG_INIT, 2>(L.grid);
+            } else if (method == RG_METHOD_BICGSTAB) {
+                PcgArgs ai = a;
+                const Launch Lv = chunked_launch(g.N * B.value);
+                ai.chunk = Lv.chunk;
+                hipLaunchKernelGGL((k_bicg_init<B.value>), dim3(Lv.grid), dim3(kBlock), 0, st, ai);
             } else if (method == RG_METHOD_FCG_LEARNED) {
                 hipLaunchKernelGGL((k_cg_init<B.value, double>), dim3(L.grid), dim3(kBlock), 0, st, a, m->dinv);
                 stage<ST_FCG_INIT, 2>(L.grid);
@@ -1007,6 +1164,7 @@ void ensure_work(rg_matrix* m)
         RG_HIP(hipMalloc(&w->x2, bytes));
         RG_HIP(hipMalloc(&w->z, bytes));
         RG_HIP(hipMalloc(&w->z2, bytes));
+        RG_HIP(hipMalloc(&w->s, bytes));
         RG_HIP(hipMalloc(&w->bb, bytes));
         RG_HIP(hipMalloc(&w->norms, 4 * sizeof(double)));
         RG_HIP(hipMalloc(&w->st, sizeof(KrylovState)));
@@ -1023,7 +1181,7 @@ void free_work(rg_solver_work* w)
     if (!w) return;
     if (w->graph) hipGraphExecDestroy(w->graph);
     if (w->capture) hipStreamDestroy(w->capture);
-    double* bufs[] = {w->r, w->xw, w->rs, w->p, w->v, w->x2, w->z, w->z2, w->bb, w->norms};
+    double* bufs[] = {w->r, w->xw, w->rs, w->p, w->v, w->x2, w->z, w->z2, w->s, w->bb, w->norms};
     for (double* b : bufs)
         if (b) hipFree(b);
     if (w->st) hipFree(w->st);
@@ -1081,8 +1239,10 @@ int solve(rg_matrix* m, double* x, const double* b, const rg_options& opt, rg_di
     const double tolerance = opt.tolerance;
     RG_CHECK(tolerance >= 100. * EPS, RG_VALUE_ERROR, "Solver: Tolerance is too small.");
     RG_CHECK(tolerance <= 1., RG_VALUE_ERROR, "Solver: Tolerance must be less than one.");
-    RG_CHECK(opt.method >= RG_METHOD_PCG && opt.method <= RG_METHOD_FCG_LEARNED, RG_VALUE_ERROR,
+    RG_CHECK(opt.method >= RG_METHOD_PCG && opt.method <= RG_METHOD_BICGSTAB, RG_VALUE_ERROR,
              "Solver: unknown solver method.");
+    RG_CHECK(opt.method != RG_METHOD_BICGSTAB || opt.sweeps <= 1, RG_VALUE_ERROR,
+             "Solver: BiCGStab runs with one Jacobi sweep.");
     RG_CHECK(opt.method != RG_METHOD_FCG_LEARNED || (m->lpc && m->g.bs == 1), RG_VALUE_ERROR,
              "Solver: the flexible CG needs a learned preconditioner attached to a scalar matrix.");
     RG_CHECK(opt.criterion == RG_CRITERION_PASO || opt.criterion == RG_CRITERION_TRUE_RESIDUAL, RG_VALUE_ERROR,

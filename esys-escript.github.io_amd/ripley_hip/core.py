@@ -233,14 +233,11 @@ class SolverOptions:
         o.sweeps = self._sweeps
         m = self._method
         if m == self.DEFAULT:
-            if not self._symmetric:
-                raise NotImplementedError("the default method of a non-symmetric PDE is BiCGStab, which is "
-                                          "outside this path; set SolverOptions.PCG")
-            m = self.PCG
-        if m == self.BICGSTAB:
-            raise NotImplementedError("BiCGStab is outside this path (PASO PCG only)")
+            # PASO's default: PCG for symmetric, BiCGStab otherwise (paso/src/Options.cpp)
+            m = self.PCG if self._symmetric else self.BICGSTAB
         o.method = {self.PCG: _lib.RG_METHOD_PCG, self.PCG_PLAIN: _lib.RG_METHOD_PCG_PLAIN,
-                    self.PCG_MIXED: _lib.RG_METHOD_PCG_MIXED, self.FCG_LEARNED: _lib.RG_METHOD_FCG_LEARNED}[m]
+                    self.PCG_MIXED: _lib.RG_METHOD_PCG_MIXED, self.FCG_LEARNED: _lib.RG_METHOD_FCG_LEARNED,
+                    self.BICGSTAB: _lib.RG_METHOD_BICGSTAB}[m]
         o.verbose = int(self._verbose)
         o.accept_failed_convergence = int(self._accept)
         o.check_interval = self._check_interval

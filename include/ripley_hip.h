@@ -195,6 +195,9 @@ int rg_matrix_save_mm(rg_matrix* m, const char* filename);
  *                        check_interval iterations the residual is replaced by
  *                        the FP64 true residual D b - A x, and convergence is
  *                        only accepted on such a replaced residual
+ *   RG_METHOD_BICGSTAB    PASO's Solver_BiCGStab (paso/src/BiCGStab.cpp:77-257)
+ *                         in FP64 with the Jacobi preconditioner (one sweep),
+ *                         the method PASO uses for non-symmetric systems.
  *   RG_METHOD_FCG_LEARNED flexible CG (truncated to one direction) whose
  *                        preconditioner is the learned network attached with
  *                        rg_matrix_set_learned_preconditioner; SpMV on FP32
@@ -206,7 +209,8 @@ int rg_matrix_save_mm(rg_matrix* m, const char* filename);
  * criterion:
  *   RG_CRITERION_PASO           balanced ||r||_2 and ||r||_inf tests (Solver.cpp:241)
  *   RG_CRITERION_TRUE_RESIDUAL  unbalanced FP64 ||b - A x||_2 <= tolerance ||b||_2 */
-enum { RG_METHOD_PCG = 0, RG_METHOD_PCG_PLAIN = 1, RG_METHOD_PCG_MIXED = 2, RG_METHOD_FCG_LEARNED = 3 };
+enum { RG_METHOD_PCG = 0, RG_METHOD_PCG_PLAIN = 1, RG_METHOD_PCG_MIXED = 2, RG_METHOD_FCG_LEARNED = 3,
+       RG_METHOD_BICGSTAB = 4 };
 enum { RG_CRITERION_PASO = 0, RG_CRITERION_TRUE_RESIDUAL = 1 };
 typedef struct {
     double tolerance;            /* relative; default 1e-8 */

@@ -263,8 +263,11 @@ def inv_main(bs, ptr, index, val):
 
 
 def solve(bs, ptr, index, val, b, tolerance=1e-8, iter_max=100000, sweeps=1, plain=False,
-          verbose=False):
-    """PASO Solver() on a copy of val; returns (x, diagnostics dict, balanced val, d)."""
+          verbose=False, method=None):
+    """PASO Solver() on a copy of val; returns (x, diagnostics dict, balanced val, d).
+    method: "PCG" (PASO's, default), "PCG_PLAIN" (plain=True) or "BICGSTAB"."""
+    if method is not None:
+        plain = {"PCG": 0, "PCG_PLAIN": 1, "BICGSTAB": 2}[method]
     val = np.array(val, dtype=np.float64, copy=True)
     b = np.ascontiguousarray(b, dtype=np.float64)
     N = len(ptr) - 1

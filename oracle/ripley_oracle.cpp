@@ -642,7 +642,7 @@ typedef struct {
     double tolerance;      // SolverBuddy default 1e-8 (SolverOptions.cpp:40-72)
     int iter_max;          // default 100000
     int sweeps;            // Jacobi sweeps, SolverBuddy default 1
-    int plain;             // 0: PASO residual-smoothed PCG; 1: textbook PCG
+    int plain;             // 0: PASO residual-smoothed PCG; 1: textbook PCG; 2: BiCGStab
     int verbose;
 } orc_options;
 
@@ -733,6 +733,74 @@ static int solver_pcg_paso(Sys& S, double* r, double* x, int* iter, double* tole
             maxIterFlag = num_iter > maxit;
             breakFlag = (std::fabs(tau) <= 0.0);
         }
+    }
+    *iter = num_iter;
+    *tolerance = norm_of_residual;
+    if (maxIterFlag) return MaxIterReached;
+    if (breakFlag) return Breakdown;
+    return NoError;
+}
+
+// Solver_BiCGStab (paso/src/BiCGStab.cpp:77-257): right-preconditioned
+// BiCGStab with the Jacobi smoother; on entry r = b - A x; early exit when
+// ||s||_2 <= tol (x += alpha phat), else the stabilising step; stops when
+// ||r||_2 <= tol, num_iter > maxit, or a zero rho, rtld.v, t.t or omega.
+static int solver_bicgstab(Sys& S, double* r, double* x, int* iter, double* tolerance)
+{
+    const int64_t n = S.n;
+    std::vector<double> rtld(r, r + n), p(n, 0.0), v(n, 0.0), t(n, 0.0), phat(n, 0.0), shat(n, 0.0), s(n, 0.0);
+    const int maxit = *iter;
+    const double tol = *tolerance;
+    int num_iter = 0;
+    double alpha = 0, omega = 0, rho = 0, rho1 = 0, norm_of_residual = 0;
+    bool breakFlag = false, maxIterFlag = false, convergeFlag = false;
+    while (true) {
+        ++num_iter;                                                                  // :125
+        rho = par_sum(n, [&](int64_t i) { return rtld[i] * r[i]; });                 // :133
+        if ((breakFlag = (std::fabs(rho) <= 0.0))) break;                            // :139
+        if (num_iter > 1) {                                                          // :142-150
+            const double beta = rho / rho1 * (alpha / omega);
+#pragma omp parallel for schedule(static)
+            for (int64_t i = 0; i < n; ++i) p[i] = r[i] + beta * (p[i] - omega * v[i]);
+        } else {
+#pragma omp parallel for schedule(static)
+            for (int64_t i = 0; i < n; ++i) p[i] = r[i];
+        }
+        S.prec(phat.data(), p.data());                                               // :154
+        S.mv(1.0, phat.data(), 0.0, v.data());                                       // :155
+        const double sum_2 = par_sum(n, [&](int64_t i) { return rtld[i] * v[i]; });  // :158
+        if ((breakFlag = (std::fabs(sum_2) <= 0.0))) break;                          // :164
+        alpha = rho / sum_2;
+#pragma omp parallel for schedule(static)
+        for (int64_t i = 0; i < n; ++i) {                                            // :168-173
+            r[i] -= alpha * v[i];
+            s[i] = r[i];
+        }
+        norm_of_residual = std::sqrt(par_sum(n, [&](int64_t i) { return s[i] * s[i]; }));
+        if ((convergeFlag = (norm_of_residual <= tol))) {                            // :181-186
+#pragma omp parallel for schedule(static)
+            for (int64_t i = 0; i < n; ++i) x[i] += alpha * phat[i];
+            maxIterFlag = false;
+            breakFlag = false;
+            break;
+        }
+        S.prec(shat.data(), s.data());                                               // :189
+        S.mv(1.0, shat.data(), 0.0, t.data());                                       // :190
+        const double omegaNumtr = par_sum(n, [&](int64_t i) { return t[i] * s[i]; }); // :193-196
+        const double omegaDenumtr = par_sum(n, [&](int64_t i) { return t[i] * t[i]; });
+        if ((breakFlag = (std::fabs(omegaDenumtr) <= 0.0))) break;                   // :206
+        omega = omegaNumtr / omegaDenumtr;
+#pragma omp parallel for schedule(static)
+        for (int64_t i = 0; i < n; ++i) {                                            // :209-214
+            x[i] += alpha * phat[i] + omega * shat[i];
+            r[i] = s[i] - omega * t[i];
+        }
+        norm_of_residual = std::sqrt(par_sum(n, [&](int64_t i) { return r[i] * r[i]; }));
+        convergeFlag = norm_of_residual <= tol;                                      // :221-223
+        maxIterFlag = num_iter > maxit;
+        breakFlag = std::fabs(omega) <= 0.0;
+        if (convergeFlag || maxIterFlag || breakFlag) break;
+        rho1 = rho;                                                                  // :228
     }
     *iter = num_iter;
     *tolerance = norm_of_residual;
@@ -844,8 +912,9 @@ ORC_API int orc_solve(int bs, int64_t nrows, const index_t* ptr, const index_t* 
                 double tol = tolerance * std::min(norm2_of_b, 0.1 * norm2 / normmax * norm_max_of_b);
                 last2 = norm2;
                 lastmax = normmax;
-                status = opt->plain ? solver_pcg_plain(S, r.data(), x, &cntIter, &tol)
-                                    : solver_pcg_paso(S, r.data(), x, &cntIter, &tol);
+                status = opt->plain == 2 ? solver_bicgstab(S, r.data(), x, &cntIter, &tol)
+                         : opt->plain ? solver_pcg_plain(S, r.data(), x, &cntIter, &tol)
+                                      : solver_pcg_paso(S, r.data(), x, &cntIter, &tol);
                 totIter += cntIter;
                 if (status == NoError) {
                     finalize = false;

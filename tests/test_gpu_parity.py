@@ -333,3 +333,40 @@ def test_linearpde_facade_linear_exact_solution():
     pde.getSolverOptions().setSolverMethod(R.SolverOptions.PCG)
     u = pde.getSolution()
     assert torch.max(torch.abs(u - u_ex)).item() <= 1e-6 * torch.max(torch.abs(u_ex)).item()
+
+
+def _advection_problem(dim, ne, seed, n=1):
+    """Non-symmetric constrained system on the GPU and in the oracle."""
+    rng = np.random.default_rng(seed)
+    dom, length = grid(dim, ne)
+    E = dom.getNumElements()
+    if n == 1:
+        A = np.tile(np.eye(dim), (E, 1, 1))
+        B = rng.standard_normal((E, dim)) * 3.0
+        Y = np.ones(E)
+    else:
+        A = np.tile(np.einsum("ik,jl->ijkl", np.eye(n), np.eye(dim)), (E, 1, 1, 1, 1))
+        B = rng.standard_normal((E, n, dim, n)) * 2.0
+        Y = np.ones((E, n))
+    op, rhs = gpu_assemble(dom, n, A=A, B=B, Y=Y)
+    q = dirichlet_mask(dom, dim - 1, n)
+    op.nullifyRowsAndCols(to_dev(q), to_dev(q), 1.0)
+    rhs = torch.where(to_dev(q) > 0, torch.zeros_like(rhs), rhs)
+    ptr, idx, oval, orhs = O.assemble(dim, ne, length, n=n, A=A, B=B, Y=Y)
+    O.nullify(n, ptr, idx, oval, q, q, 1.0)
+    orhs = np.where(q > 0, 0.0, orhs)
+    return op, rhs, ptr, idx, oval, orhs
+
+
+@pytest.mark.parametrize("dim,ne,n", [(2, (24, 20), 1), (3, (10, 9, 8), 1), (3, (6, 5, 5), 3), (2, (12, 10), 2)])
+def test_bicgstab_parity(dim, ne, n):
+    op, rhs, ptr, idx, oval, orhs = _advection_problem(dim, ne, 9, n)
+    opts = R.SolverOptions()      # non-symmetric: PASO's default is BiCGStab
+    opts.setTolerance(1e-9)
+    x = op.solve(rhs, opts).cpu().numpy()
+    xo, dg, _, _ = O.solve(n, ptr, idx, oval, orhs, tolerance=1e-9, method="BICGSTAB")
+    assert opts.hasConverged() and dg["converged"] == 1
+    assert abs(opts.getDiagnostics("num_iter") - dg["num_iter"]) <= max(3, 0.05 * dg["num_iter"])
+    res = np.linalg.norm(orhs - O.spmv(n, ptr, idx, oval, x)) / np.linalg.norm(orhs)
+    assert res <= 1e-8
+    assert np.linalg.norm(x - xo) <= 1e-6 * np.linalg.norm(xo)

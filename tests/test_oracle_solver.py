@@ -149,3 +149,53 @@ def test_sweeps_two_converges():
     x2, d2, _, _ = O.solve(1, ptr, idx, val, rhs, sweeps=2)
     assert d1["rc"] == 0 and d2["rc"] == 0
     assert np.linalg.norm(x1 - x2) <= 1e-6 * np.linalg.norm(x1)
+
+
+def _advection_system(dim, ne, seed, n=1):
+    """Non-symmetric constrained system: -lap u + B . grad u = 1 (n = 1) or
+    an elasticity-like system with a skew first-order term (n > 1)."""
+    rng = np.random.default_rng(seed)
+    E = int(np.prod(ne))
+    if n == 1:
+        A = np.tile(np.eye(dim), (E, 1, 1))
+        B = rng.standard_normal((E, dim)) * 3.0
+        ptr, idx, val, rhs = O.assemble(dim, ne, (1.0,) * dim, A=A, B=B, Y=1.0)
+    else:
+        eye = np.eye(dim)
+        A = np.tile(np.einsum("ik,jl->ijkl", np.eye(n), eye), (E, 1, 1, 1, 1))
+        B = rng.standard_normal((E, n, dim, n)) * 2.0
+        ptr, idx, val, rhs = O.assemble(dim, ne, (1.0,) * dim, n=n, A=A, B=B, Y=np.ones((E, n)))
+    N = len(ptr) - 1
+    q = np.zeros(N)
+    q[: ne[0] + 1] = 1.0
+    qn = np.repeat(q, n)
+    O.nullify(n, ptr, idx, val, qn, qn, 1.0)
+    rhs = np.where(qn > 0, 0.0, rhs)
+    return ptr, idx, val, rhs
+
+
+@pytest.mark.parametrize("dim,ne,n", [(2, (12, 10), 1), (3, (6, 5, 4), 1), (2, (8, 7), 2)])
+def test_bicgstab_restatement_solves_nonsymmetric(dim, ne, n):
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spl
+    ptr, idx, val, rhs = _advection_system(dim, ne, 4, n)
+    D = O.csr_to_dense(n, ptr, idx, val)
+    assert np.abs(D - D.T).max() > 1e-3          # genuinely non-symmetric
+    x, dg, _, _ = O.solve(n, ptr, idx, val, rhs, tolerance=1e-10, method="BICGSTAB")
+    assert dg["converged"] == 1 and dg["rc"] == 0
+    xs = spl.spsolve(sp.csr_matrix(D), rhs)
+    assert np.linalg.norm(x - xs) <= 1e-8 * np.linalg.norm(xs)
+    assert np.linalg.norm(rhs - D @ x) <= 1e-9 * np.linalg.norm(rhs)
+
+
+def test_bicgstab_on_spd_matches_pcg_solution():
+    ptr, idx, val, rhs = (lambda r: (r[0], r[1], r[2], r[3]))(O.assemble(2, (10, 9), (1.0, 1.0),
+                                                                           A=np.eye(2), Y=1.0))
+    N = len(ptr) - 1
+    q = np.zeros(N)
+    q[:11] = 1.0
+    O.nullify(1, ptr, idx, val, q, q, 1.0)
+    rhs = np.where(q > 0, 0.0, rhs)
+    xb, db, _, _ = O.solve(1, ptr, idx, val, rhs, tolerance=1e-10, method="BICGSTAB")
+    xp, dp, _, _ = O.solve(1, ptr, idx, val, rhs, tolerance=1e-10)
+    assert np.linalg.norm(xb - xp) <= 1e-8 * np.linalg.norm(xp)
