@@ -52,7 +52,7 @@ class Problem:
     """The C2 system on the GPU: kappa, operator, rhs, Dirichlet mask."""
 
     def __init__(self, ne, R, torch, np, fields):
-        self.R, self.torch = R, torch
+        self.R, self.torch, self.np = R, torch, np
         self.dom = R.Brick(ne, ne, ne)
         self.ne = (ne, ne, ne)
         self.kappa_np = fields.log10_normal_kappa(self.ne, corr_len=8.0, sigma=1.0, seed=20261202)
