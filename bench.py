@@ -58,7 +58,10 @@ class Problem:
         self.kappa_np = fields.log10_normal_kappa(self.ne, corr_len=8.0, sigma=1.0, seed=20261202)
         self.kappa = torch.as_tensor(self.kappa_np, device="cuda")
         self.q = self.dom.whereZero(2)
-        self.op = self.dom.newOperator(1)
+        # full stencil storage, and the upper-half storage of a symmetric
+        # operator (RG_MATRIX_SYMMETRIC) for the "_sym" methods
+        self.ops = {"full": self.dom.newOperator(1), "sym": self.dom.newOperator(1, symmetric=True)}
+        self.op = self.ops["full"]
         E, Q = self.dom.getNumElements(), self.dom.getNumQuadraturePoints()
         self.cA = R.Coefficient(self.kappa, (3, 3), E, Q, self.dom.device, isotropic=True)
         self.cY = R.Coefficient(1.0, (), E, Q, self.dom.device)
@@ -92,14 +95,21 @@ class Problem:
         o.setTolerance(1e-8)
         o.setTrueResidualCriterion(True)
         o.setSolverMethod({"paso": R.SolverOptions.PCG, "plain": R.SolverOptions.PCG_PLAIN,
-                           "mixed": R.SolverOptions.PCG_MIXED, "learned": R.SolverOptions.FCG_LEARNED}[method])
+                           "mixed": R.SolverOptions.PCG_MIXED, "learned": R.SolverOptions.FCG_LEARNED,
+                           "plain_sym": R.SolverOptions.PCG_PLAIN,
+                           "paso_sym": R.SolverOptions.PCG}[method])
         if method == "learned":
             self.op.setLearnedPreconditioner(self.net, self.logk)
             o.setCheckInterval(8)
         self.op.setToSolution(self.x, self.rhs, o)
         return o
 
+    def select(self, method):
+        """The operator storage a method runs on."""
+        self.op = self.ops["sym" if method.endswith("_sym") else "full"]
+
     def step(self, method):
+        self.select(method)
         self.assemble()
         self.constrain()
         return self.solve(method)
@@ -129,6 +139,7 @@ def time_steps(P, method, steps, warmup, torch):
 
 
 def breakdown(P, method, torch):
+    P.select(method)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     P.assemble()
@@ -148,6 +159,7 @@ def spmv_roofline(P, reps, torch):
     """Average duration of the FP64 stencil SpMV (MatrixVector, one kernel per
     call) over `reps` back-to-back launches, timed with HIP events on the
     stream the library launches on."""
+    P.op = P.ops["full"]
     P.assemble()
     P.constrain()
     N = P.op.num_rows
@@ -542,7 +554,7 @@ def main():
         return
     log(args, "building C2 problem ne=%d" % args.ne)
     P = Problem(args.ne, R, torch, np, fields)
-    methods = ["plain", "paso", "mixed"] if args.method == "all" else [args.method]
+    methods = ["plain", "paso", "mixed", "plain_sym", "paso_sym"] if args.method == "all" else [args.method]
     weights = os.path.join(ROOT, "esys-escript.github.io_amd", "ripley_hip", "weights", "lpc_log10grf_s1_l8.npy")
     if args.method == "all" and os.path.exists(weights):
         methods.append("learned")
@@ -568,7 +580,7 @@ def main():
         traffic = json.load(open(tfile)).get("hbm_bytes_per_launch")
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": traffic,
-            "kernel": "k_spmv<3,1,false,double> (FP64 27-point stencil SpMV)",
+            "kernel": "k_spmv<3,1,false,double,0> (FP64 27-point stencil SpMV, y = A x)",
             "alg_bytes_per_launch": alg, "avg_launch_s": dt}
     copy_bw = copy_bandwidth(torch)
     cpu = None

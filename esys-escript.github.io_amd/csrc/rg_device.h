@@ -111,6 +111,41 @@ __device__ __forceinline__ void reduce_partials(const double* partials, int G, d
     block_sum<NV>(tot);
 }
 
+// Sum of the G partials of each of NV rows by one workgroup of NT threads
+// (NT a multiple of 64): thread t adds partials t, t+NT, ... in order, then
+// the waves' sums are combined in a fixed order; valid in all threads.
+// With NT = 1024 a stage reads 4096 partials in a single round trip.
+template <int NV, int NT>
+__device__ __forceinline__ void reduce_partials_wide(const double* partials, int G, double (&tot)[NV])
+{
+    __shared__ double red[NV][NT / 64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+        const double* p = partials + (int64_t)k * G;
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int b = threadIdx.x + u * NT;
+            v[u] = (b < G) ? p[b] : 0.0;
+        }
+        double s = 0.0;
+#pragma unroll
+        for (int u = 0; u < 8; ++u) s += v[u];
+        for (int b = threadIdx.x + 8 * NT; b < G; b += NT) s += p[b];
+        s = wave_sum(s);
+        if (lane == 0) red[k][w] = s;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+        double s = 0.0;
+#pragma unroll
+        for (int j = 0; j < NT / 64; ++j) s += red[k][j];
+        tot[k] = s;
+    }
+}
+
 // Full-stencil slot s -> (dx, dy, dz) in {-1,0,1}, getDiagonalIndices order.
 template <int DIM>
 __host__ __device__ __forceinline__ void slot_delta(int s, int& dx, int& dy, int& dz)

@@ -454,8 +454,10 @@ void launch_assemble(rg_matrix* m, const rg_pde_coefs& c, int reduced, double* r
 // -------------------------------------------------------------------- SpMV
 // paso SystemMatrix::MatrixVector -> SparseMatrix_MatrixVector_CSR_OFFSET0
 // (paso/src/SparseMatrix_MatrixVector.cpp:191-336): out = alpha*A*in +
-// beta*out, out first scaled by beta (zeroed for beta == 0).
-template <int DIM, int BS, bool SYM, typename VT>
+// beta*out, out first scaled by beta (zeroed for beta == 0).  BETA selects
+// the epilogue at compile time (0: beta == 0, 1: beta == 1, 2: general): a
+// run-time branch on beta inside the row loop cost 150 registers.
+template <int DIM, int BS, bool SYM, typename VT, int BETA>
 __global__ __launch_bounds__(kBlock) void k_spmv(const VT* __restrict__ val, int64_t ld, int64_t N,
                                                  int64_t s1, int64_t s2, int64_t chunk,
                                                  const double* __restrict__ x, double* __restrict__ y,
@@ -465,8 +467,9 @@ __global__ __launch_bounds__(kBlock) void k_spmv(const VT* __restrict__ val, int
     stencil_block<DIM, BS, SYM, VT>(val, ld, N, s1, s2, rr.r0, rr.r1, x, [&](int64_t i, const double (&acc)[BS]) {
 #pragma unroll
         for (int r = 0; r < BS; ++r) {
-            const double yo = (beta == 0.0) ? 0.0 : ((beta == 1.0) ? y[i * BS + r] : beta * y[i * BS + r]);
-            y[i * BS + r] = fma(alpha, acc[r], yo);
+            if (BETA == 0) y[i * BS + r] = alpha * acc[r];
+            else if (BETA == 1) y[i * BS + r] = fma(alpha, acc[r], y[i * BS + r]);
+            else y[i * BS + r] = fma(alpha, acc[r], beta * y[i * BS + r]);
         }
     });
 }
@@ -476,8 +479,15 @@ void launch_spmv(rg_matrix* m, double alpha, const double* x, double beta, doubl
     const Geom& g = m->g;
     const Launch L = chunked_launch(g.N, spmv_rows_per_thread<double>());
     dispatch(g, m->symmetric, [&](auto D, auto B, auto S) {
-        hipLaunchKernelGGL((k_spmv<D.value, B.value, S.value, double>), dim3(L.grid), dim3(kBlock), 0,
-                           m->ctx->stream, m->val, g.ld, g.N, g.s1, g.s2, L.chunk, x, y, alpha, beta);
+        if (beta == 0.0)
+            hipLaunchKernelGGL((k_spmv<D.value, B.value, S.value, double, 0>), dim3(L.grid), dim3(kBlock), 0,
+                               m->ctx->stream, m->val, g.ld, g.N, g.s1, g.s2, L.chunk, x, y, alpha, beta);
+        else if (beta == 1.0)
+            hipLaunchKernelGGL((k_spmv<D.value, B.value, S.value, double, 1>), dim3(L.grid), dim3(kBlock), 0,
+                               m->ctx->stream, m->val, g.ld, g.N, g.s1, g.s2, L.chunk, x, y, alpha, beta);
+        else
+            hipLaunchKernelGGL((k_spmv<D.value, B.value, S.value, double, 2>), dim3(L.grid), dim3(kBlock), 0,
+                               m->ctx->stream, m->val, g.ld, g.N, g.s1, g.s2, L.chunk, x, y, alpha, beta);
     });
     RG_HIP(hipGetLastError());
 }
@@ -487,7 +497,7 @@ void launch_spmv32(rg_matrix* m, const double* x, double* y)
     const Geom& g = m->g;
     const Launch L = chunked_launch(g.N, spmv_rows_per_thread<float>());
     dispatch(g, m->symmetric, [&](auto D, auto B, auto S) {
-        hipLaunchKernelGGL((k_spmv<D.value, B.value, S.value, float>), dim3(L.grid), dim3(kBlock), 0,
+        hipLaunchKernelGGL((k_spmv<D.value, B.value, S.value, float, 0>), dim3(L.grid), dim3(kBlock), 0,
                            m->ctx->stream, m->val32, g.ld, g.N, g.s1, g.s2, L.chunk, x, y, 1.0, 0.0);
     });
     RG_HIP(hipGetLastError());

@@ -751,12 +751,14 @@ __global__ __launch_bounds__(kBlock) void k_bicg_update(PcgArgs a, int64_t n)
 
 // ------------------------------------------------------------ stage kernels
 // One workgroup after each reducing kernel (recurrences in rg_krylov.h).
+constexpr int kStageThreads = 1024;
+
 template <int STAGE, int NV>
-__global__ __launch_bounds__(kBlock) void k_stage(KrylovState* st, const double* __restrict__ partials, int G)
+__global__ __launch_bounds__(kStageThreads) void k_stage(KrylovState* st, const double* __restrict__ partials, int G)
 {
     if (!stage_starts<STAGE>() && st->done) return;
     double tot[NV];
-    reduce_partials<NV>(partials, G, tot);
+    reduce_partials_wide<NV, kStageThreads>(partials, G, tot);
     if (threadIdx.x != 0) return;
     stage_apply<STAGE, NV>(st, tot);
 }
@@ -829,7 +831,7 @@ struct SolverImpl {
     template <int STAGE, int NV>
     void stage(int G)
     {
-        hipLaunchKernelGGL((k_stage<STAGE, NV>), dim3(1), dim3(kBlock), 0, st, w->st, m->ctx->partials, G);
+        hipLaunchKernelGGL((k_stage<STAGE, NV>), dim3(1), dim3(kStageThreads), 0, st, w->st, m->ctx->partials, G);
     }
 
     // launch shapes: stencil kernels (R rows per thread), paired node kernels
