@@ -244,20 +244,26 @@ class BrickPoisson:
 DEFAULT_ITERATES = (1, 2, 4, 8, 16, 32, 64, 128, 256, 512)
 
 
-def generate_training_set(ne, num_fields, kappa_fn, iterations=DEFAULT_ITERATES, first_seed=1, log=None):
+def generate_training_set(ne, num_fields, kappa_fn, iterations=DEFAULT_ITERATES, first_seed=1, log=None,
+                          per_field=None):
     """[(r, e, dinv, logk), ...] on the device: for each seed s in
     first_seed .. first_seed+num_fields-1, kappa = kappa_fn(ne, s) and the
-    plain Jacobi-PCG's residual/error pairs at the given iteration counts."""
+    plain Jacobi-PCG's residual/error pairs at the given iteration counts
+    (with per_field = m: m of them per field, chosen by the seed)."""
     data = []
     for s in range(first_seed, first_seed + num_fields):
         prob = BrickPoisson(ne, kappa_fn(ne, s))
         prob.operator()
         b = prob.rhs
-        dinv, pairs = plain_pcg_pairs(prob.operator, b, iterations)
+        its = iterations
+        if per_field:
+            its = [iterations[(7 * s + j) % len(iterations)] for j in range(per_field)]
+        dinv, pairs = plain_pcg_pairs(prob.operator, b, its)
         for r, e in pairs:
             data.append((r, e, dinv, prob.logk))
-        if log:
-            log("training set: field %d/%d (%d pairs)" % (s - first_seed + 1, num_fields, len(data)))
+        n = s - first_seed + 1
+        if log and (n % max(1, num_fields // 20) == 0 or n == num_fields):
+            log("training set: field %d/%d (%d pairs)" % (n, num_fields, len(data)))
     return data
 
 
@@ -284,6 +290,8 @@ def _main():
     ap = argparse.ArgumentParser(description="train the learned preconditioner on the GPU")
     ap.add_argument("--ne", type=int, default=64)
     ap.add_argument("--fields", type=int, default=32)
+    ap.add_argument("--per-field", type=int, default=0,
+                    help="residual/error pairs per field (0: one per iteration count 1, 2, 4, ..., 512)")
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--family", default="log10grf", choices=["log10grf", "lognormal"])
@@ -303,7 +311,7 @@ def _main():
 
     ne = (a.ne,) * 3
     t0 = time.perf_counter()
-    data = generate_training_set(ne, a.fields, kfn, log=log)
+    data = generate_training_set(ne, a.fields, kfn, log=log, per_field=a.per_field or None)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     net = LearnedPreconditioner(tuple(n + 1 for n in ne), training=True)
