@@ -393,6 +393,34 @@ def bench_c3(R, torch, np, args):
         out[method] = {"solve_s": t, "num_iter": o.getDiagnostics("num_iter"),
                        "converged": o.hasConverged(), "rel_residual_fp64": _rel_res(torch, op2, x, rhs)}
         del op, op2
+    if not args.no_cpu:
+        # the oracle's PASO path on the host (FP64 CSR, 3x3 blocks): assembly
+        # timed in full, a bounded number of PCG iterations timed and scaled
+        # to the iteration count PASO PCG needs here
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        nth = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
+        O.set_num_threads(nth)
+        t0 = time.perf_counter()
+        ptr, idx = O.pattern(3, ne)
+        A_np = (lam[:, None, None, None, None] * np.einsum("ij,kl->ijkl", np.eye(3), np.eye(3))
+                + mu[:, None, None, None, None] * (np.einsum("ik,jl->ijkl", np.eye(3), np.eye(3))
+                                                    + np.einsum("il,jk->ijkl", np.eye(3), np.eye(3))))
+        _, _, val, orhs = O.assemble(3, ne, (1.0, 1.0, 1.0), n=3, ptr=ptr, index=idx, A=A_np,
+                                     Y=np.array([0.0, 0.0, -2700.0 * 9.81]))
+        del A_np
+        t1 = time.perf_counter()
+        qn = q.cpu().numpy()
+        O.nullify(3, ptr, idx, val, qn, qn, 1.0)
+        orhs = np.where(qn > 0, 0.0, orhs)
+        iters = 15
+        tit = O.time_pcg_iterations(3, ptr, idx, val, orhs, iters, plain=False)
+        n_gpu = out["PCG"]["num_iter"]
+        out["cpu_oracle_paso"] = {"assemble_s": t1 - t0, "per_iter_s": tit / iters, "threads": nth,
+                                  "estimate_s": (t1 - t0) + tit / iters * n_gpu,
+                                  "sample": "%d PASO PCG iterations timed, scaled to the %d the GPU needed"
+                                            % (iters, n_gpu)}
+        del val, ptr, idx
     return out
 
 

@@ -265,46 +265,68 @@ __global__ void k_flip_weights(const bf16* __restrict__ w, int cout, int cin, bf
 }
 
 // ------------------------------------------------------- elementwise ops
+// All channel counts and offsets are multiples of 8: each thread moves one
+// 16-byte group of 8 channels of one voxel (element e = voxel * c/8 + group).
+__device__ __forceinline__ bf16x8 ld8(const bf16* p) { return *reinterpret_cast<const bf16x8*>(p); }
+__device__ __forceinline__ void st8(bf16* p, const bf16x8& v) { *reinterpret_cast<bf16x8*>(p) = v; }
+
+__device__ __forceinline__ int64_t fine_padded(const Grid3& gf, int64_t x, int64_t y, int64_t z)
+{
+    return ((z + 1) * gf.p1 + (y + 1)) * gf.p0 + (x + 1);
+}
+
 // dpre = dout * (act > 0)  (ReLU backward), interior voxels only
 __global__ void k_relu_bwd(Grid3 g, int c, const bf16* __restrict__ dout, const bf16* __restrict__ act,
                            bf16* __restrict__ dpre)
 {
-    const int64_t total = g.nvox * c;
+    const int c8 = c / 8;
+    const int64_t total = g.nvox * c8;
     for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t v = e / c;
-        const int ch = e % c;
-        const int64_t p = padded_index(g, v) * c + ch;
-        dpre[p] = ((float)act[p] > 0.f) ? dout[p] : (bf16)0.f;
+        const int64_t p = padded_index(g, e / c8) * c + (e % c8) * 8;
+        const bf16x8 a = ld8(act + p), d = ld8(dout + p);
+        bf16x8 o;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) o[k] = ((float)a[k] > 0.f) ? d[k] : (bf16)0.f;
+        st8(dpre + p, o);
     }
 }
 
 // 2x2x2 average pooling (fine grid gf -> coarse grid gc, gc.d = gf.d / 2)
 __global__ void k_pool_fwd(Grid3 gf, Grid3 gc, int c, const bf16* __restrict__ in, bf16* __restrict__ out)
 {
-    const int64_t total = gc.nvox * c;
+    const int c8 = c / 8;
+    const int64_t total = gc.nvox * c8;
     for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t vc = e / c;
-        const int ch = e % c;
+        const int64_t vc = e / c8;
+        const int ch = (int)(e % c8) * 8;
         const int64_t X = vc % gc.d0, Y = (vc / gc.d0) % gc.d1, Z = vc / ((int64_t)gc.d0 * gc.d1);
-        float s = 0.f;
+        float s[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
-            const int64_t x = 2 * X + (k & 1), y = 2 * Y + ((k >> 1) & 1), z = 2 * Z + (k >> 2);
-            s += (float)in[(((z + 1) * gf.p1 + (y + 1)) * gf.p0 + (x + 1)) * c + ch];
+            const bf16x8 v = ld8(in + fine_padded(gf, 2 * X + (k & 1), 2 * Y + ((k >> 1) & 1), 2 * Z + (k >> 2)) * c + ch);
+#pragma unroll
+            for (int j = 0; j < 8; ++j) s[j] += (float)v[j];
         }
-        out[padded_index(gc, vc) * c + ch] = (bf16)(0.125f * s);
+        bf16x8 o;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) o[j] = (bf16)(0.125f * s[j]);
+        st8(out + padded_index(gc, vc) * c + ch, o);
     }
 }
 
 __global__ void k_pool_bwd(Grid3 gf, Grid3 gc, int c, const bf16* __restrict__ dout, bf16* __restrict__ din)
 {
-    const int64_t total = gf.nvox * c;
+    const int c8 = c / 8;
+    const int64_t total = gf.nvox * c8;
     for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t v = e / c;
-        const int ch = e % c;
+        const int64_t v = e / c8;
+        const int ch = (int)(e % c8) * 8;
         const int64_t x = v % gf.d0, y = (v / gf.d0) % gf.d1, z = v / ((int64_t)gf.d0 * gf.d1);
-        const int64_t pc = (((z / 2) + 1) * gc.p1 + (y / 2) + 1) * gc.p0 + (x / 2) + 1;
-        din[padded_index(gf, v) * c + ch] = (bf16)(0.125f * (float)dout[pc * c + ch]);
+        const bf16x8 d = ld8(dout + ((((z / 2) + 1) * gc.p1 + (y / 2) + 1) * gc.p0 + (x / 2) + 1) * c + ch);
+        bf16x8 o;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) o[j] = (bf16)(0.125f * (float)d[j]);
+        st8(din + padded_index(gf, v) * c + ch, o);
     }
 }
 
@@ -314,31 +336,38 @@ __global__ void k_pool_bwd(Grid3 gf, Grid3 gc, int c, const bf16* __restrict__ d
 __global__ void k_up_fwd(Grid3 gf, Grid3 gc, int c, const bf16* __restrict__ in, int cout, int coff,
                          bf16* __restrict__ out)
 {
-    const int64_t total = gf.nvox * c;
+    const int c8 = c / 8;
+    const int64_t total = gf.nvox * c8;
     for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t v = e / c;
-        const int ch = e % c;
+        const int64_t v = e / c8;
+        const int ch = (int)(e % c8) * 8;
         const int64_t x = v % gf.d0, y = (v / gf.d0) % gf.d1, z = v / ((int64_t)gf.d0 * gf.d1);
         const int64_t pc = (((z / 2) + 1) * gc.p1 + (y / 2) + 1) * gc.p0 + (x / 2) + 1;
-        out[padded_index(gf, v) * cout + coff + ch] = in[pc * c + ch];
+        st8(out + padded_index(gf, v) * cout + coff + ch, ld8(in + pc * c + ch));
     }
 }
 
 __global__ void k_up_bwd(Grid3 gf, Grid3 gc, int c, const bf16* __restrict__ dout, int cout, int coff,
                          bf16* __restrict__ din)
 {
-    const int64_t total = gc.nvox * c;
+    const int c8 = c / 8;
+    const int64_t total = gc.nvox * c8;
     for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t vc = e / c;
-        const int ch = e % c;
+        const int64_t vc = e / c8;
+        const int ch = (int)(e % c8) * 8;
         const int64_t X = vc % gc.d0, Y = (vc / gc.d0) % gc.d1, Z = vc / ((int64_t)gc.d0 * gc.d1);
-        float s = 0.f;
+        float s[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
-            const int64_t x = 2 * X + (k & 1), y = 2 * Y + ((k >> 1) & 1), z = 2 * Z + (k >> 2);
-            s += (float)dout[(((z + 1) * gf.p1 + (y + 1)) * gf.p0 + (x + 1)) * cout + coff + ch];
+            const bf16x8 v =
+                ld8(dout + fine_padded(gf, 2 * X + (k & 1), 2 * Y + ((k >> 1) & 1), 2 * Z + (k >> 2)) * cout + coff + ch);
+#pragma unroll
+            for (int j = 0; j < 8; ++j) s[j] += (float)v[j];
         }
-        din[padded_index(gc, vc) * c + ch] = (bf16)s;
+        bf16x8 o;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) o[j] = (bf16)s[j];
+        st8(din + padded_index(gc, vc) * c + ch, o);
     }
 }
 
@@ -346,12 +375,12 @@ __global__ void k_up_bwd(Grid3 gf, Grid3 gc, int c, const bf16* __restrict__ dou
 __global__ void k_copy_channels(Grid3 g, int c, const bf16* __restrict__ in, int cin, int ioff,
                                 bf16* __restrict__ out, int cout, int ooff)
 {
-    const int64_t total = g.nvox * c;
+    const int c8 = c / 8;
+    const int64_t total = g.nvox * c8;
     for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t v = e / c;
-        const int ch = e % c;
-        const int64_t p = padded_index(g, v);
-        out[p * cout + ooff + ch] = in[p * cin + ioff + ch];
+        const int64_t p = padded_index(g, e / c8);
+        const int ch = (int)(e % c8) * 8;
+        st8(out + p * cout + ooff + ch, ld8(in + p * cin + ioff + ch));
     }
 }
 
@@ -359,12 +388,16 @@ __global__ void k_copy_channels(Grid3 g, int c, const bf16* __restrict__ in, int
 __global__ void k_add_channels(Grid3 g, int c, const bf16* __restrict__ src, int cs, int ioff,
                                bf16* __restrict__ dst, int cd, int ooff)
 {
-    const int64_t total = g.nvox * c;
+    const int c8 = c / 8;
+    const int64_t total = g.nvox * c8;
     for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t v = e / c;
-        const int ch = e % c;
-        const int64_t p = padded_index(g, v);
-        dst[p * cd + ooff + ch] = (bf16)((float)dst[p * cd + ooff + ch] + (float)src[p * cs + ioff + ch]);
+        const int64_t p = padded_index(g, e / c8);
+        const int ch = (int)(e % c8) * 8;
+        const bf16x8 a = ld8(dst + p * cd + ooff + ch), b = ld8(src + p * cs + ioff + ch);
+        bf16x8 o;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) o[j] = (bf16)((float)a[j] + (float)b[j]);
+        st8(dst + p * cd + ooff + ch, o);
     }
 }
 
@@ -386,9 +419,11 @@ __global__ void k_make_input(Grid3 g, int n0, int n1, int n2, const double* __re
             lk = logk ? logk[i] : 0.f;
         }
         const int64_t p = padded_index(g, v) * cin;
-        out[p] = (bf16)u;
-        out[p + 1] = (bf16)lk;
-        for (int ch = 2; ch < cin; ++ch) out[p + ch] = (bf16)0.f;
+        bf16x8 first = bf16x8{};
+        first[0] = (bf16)u;
+        first[1] = (bf16)lk;
+        st8(out + p, first);
+        for (int ch = 8; ch < cin; ch += 8) st8(out + p + ch, bf16x8{});
     }
 }
 
@@ -646,18 +681,18 @@ void net_forward(Net& N, const double* r, const double* dinv, const double* bal,
                        bal, logk, N.scale, 16, N.x0.p);
     conv_fwd(st, g1, 16, 16, N.x0.p, N.W16[0], N.B32[0], N.a1.p, true);
     conv_fwd(st, g1, 16, 16, N.a1.p, N.W16[1], N.B32[1], N.s1.p, true);
-    hipLaunchKernelGGL(k_pool_fwd, dim3(ew_grid(g2.nvox * 16)), dim3(256), 0, st, g1, g2, 16, N.s1.p, N.p1.p);
+    hipLaunchKernelGGL(k_pool_fwd, dim3(ew_grid(g2.nvox * 2)), dim3(256), 0, st, g1, g2, 16, N.s1.p, N.p1.p);
     conv_fwd(st, g2, 16, 32, N.p1.p, N.W16[2], N.B32[2], N.a2.p, true);
     conv_fwd(st, g2, 32, 32, N.a2.p, N.W16[3], N.B32[3], N.s2.p, true);
-    hipLaunchKernelGGL(k_pool_fwd, dim3(ew_grid(g3.nvox * 32)), dim3(256), 0, st, g2, g3, 32, N.s2.p, N.p2.p);
+    hipLaunchKernelGGL(k_pool_fwd, dim3(ew_grid(g3.nvox * 4)), dim3(256), 0, st, g2, g3, 32, N.s2.p, N.p2.p);
     conv_fwd(st, g3, 32, 64, N.p2.p, N.W16[4], N.B32[4], N.a3.p, true);
     conv_fwd(st, g3, 64, 64, N.a3.p, N.W16[5], N.B32[5], N.a3b.p, true);
-    hipLaunchKernelGGL(k_up_fwd, dim3(ew_grid(g2.nvox * 64)), dim3(256), 0, st, g2, g3, 64, N.a3b.p, 96, 0, N.cat2.p);
-    hipLaunchKernelGGL(k_copy_channels, dim3(ew_grid(g2.nvox * 32)), dim3(256), 0, st, g2, 32, N.s2.p, 32, 0,
+    hipLaunchKernelGGL(k_up_fwd, dim3(ew_grid(g2.nvox * 8)), dim3(256), 0, st, g2, g3, 64, N.a3b.p, 96, 0, N.cat2.p);
+    hipLaunchKernelGGL(k_copy_channels, dim3(ew_grid(g2.nvox * 4)), dim3(256), 0, st, g2, 32, N.s2.p, 32, 0,
                        N.cat2.p, 96, 64);
     conv_fwd(st, g2, 96, 32, N.cat2.p, N.W16[6], N.B32[6], N.a4.p, true);
-    hipLaunchKernelGGL(k_up_fwd, dim3(ew_grid(g1.nvox * 32)), dim3(256), 0, st, g1, g2, 32, N.a4.p, 48, 0, N.cat1.p);
-    hipLaunchKernelGGL(k_copy_channels, dim3(ew_grid(g1.nvox * 16)), dim3(256), 0, st, g1, 16, N.s1.p, 16, 0,
+    hipLaunchKernelGGL(k_up_fwd, dim3(ew_grid(g1.nvox * 4)), dim3(256), 0, st, g1, g2, 32, N.a4.p, 48, 0, N.cat1.p);
+    hipLaunchKernelGGL(k_copy_channels, dim3(ew_grid(g1.nvox * 2)), dim3(256), 0, st, g1, 16, N.s1.p, 16, 0,
                        N.cat1.p, 48, 32);
     conv_fwd(st, g1, 48, 16, N.cat1.p, N.W16[7], N.B32[7], N.a5.p, true);
     conv_fwd(st, g1, 16, 16, N.a5.p, N.W16[8], N.B32[8], N.y.p, false);
@@ -760,7 +795,7 @@ void layer_dgrad(Net& N, int l, const Grid3& g, const bf16* dpre, bf16* dx)
 
 void relu_bwd(Net& N, const Grid3& g, int c, const bf16* dout, const bf16* act, bf16* dpre)
 {
-    hipLaunchKernelGGL(k_relu_bwd, dim3(ew_grid(g.nvox * c)), dim3(256), 0, N.ctx->stream, g, c, dout, act, dpre);
+    hipLaunchKernelGGL(k_relu_bwd, dim3(ew_grid(g.nvox * c / 8)), dim3(256), 0, N.ctx->stream, g, c, dout, act, dpre);
 }
 
 // Forward + loss + backward; gradients in N.gW / N.gB; returns the loss.
@@ -784,16 +819,16 @@ double net_gradients(Net& N, const double* r, const double* dinv, const float* l
     layer_wgrad(N, 7, g1, N.cat1.p, N.gtmp1.p);
     layer_dgrad(N, 7, g1, N.gtmp1.p, N.gcat1.p);
     // cat1 = [up(a4) (32), s1 (16)]
-    hipLaunchKernelGGL(k_up_bwd, dim3(ew_grid(g2.nvox * 32)), dim3(256), 0, st, g1, g2, 32, N.gcat1.p, 48, 0, N.ga4.p);
-    hipLaunchKernelGGL(k_copy_channels, dim3(ew_grid(g1.nvox * 16)), dim3(256), 0, st, g1, 16, N.gcat1.p, 48, 32,
+    hipLaunchKernelGGL(k_up_bwd, dim3(ew_grid(g2.nvox * 4)), dim3(256), 0, st, g1, g2, 32, N.gcat1.p, 48, 0, N.ga4.p);
+    hipLaunchKernelGGL(k_copy_channels, dim3(ew_grid(g1.nvox * 2)), dim3(256), 0, st, g1, 16, N.gcat1.p, 48, 32,
                        N.gs1.p, 16, 0);
     // level-2 decoder: a4 = relu(c4(cat2))
     relu_bwd(N, g2, 32, N.ga4.p, N.a4.p, N.ga2.p);   // ga2 used as scratch here
     layer_wgrad(N, 6, g2, N.cat2.p, N.ga2.p);
     layer_dgrad(N, 6, g2, N.ga2.p, N.gcat2.p);
     // cat2 = [up(a3b) (64), s2 (32)]
-    hipLaunchKernelGGL(k_up_bwd, dim3(ew_grid(g3.nvox * 64)), dim3(256), 0, st, g2, g3, 64, N.gcat2.p, 96, 0, N.ga3b.p);
-    hipLaunchKernelGGL(k_copy_channels, dim3(ew_grid(g2.nvox * 32)), dim3(256), 0, st, g2, 32, N.gcat2.p, 96, 64,
+    hipLaunchKernelGGL(k_up_bwd, dim3(ew_grid(g3.nvox * 8)), dim3(256), 0, st, g2, g3, 64, N.gcat2.p, 96, 0, N.ga3b.p);
+    hipLaunchKernelGGL(k_copy_channels, dim3(ew_grid(g2.nvox * 4)), dim3(256), 0, st, g2, 32, N.gcat2.p, 96, 64,
                        N.gs2.p, 32, 0);
     // level 3: a3b = relu(c5(a3)), a3 = relu(c4(p2))
     relu_bwd(N, g3, 64, N.ga3b.p, N.a3b.p, N.ga3b.p);
@@ -803,8 +838,8 @@ double net_gradients(Net& N, const double* r, const double* dinv, const float* l
     layer_wgrad(N, 4, g3, N.p2.p, N.ga3.p);
     layer_dgrad(N, 4, g3, N.ga3.p, N.gp2.p);
     // p2 = pool(s2): add the pooled path into gs2
-    hipLaunchKernelGGL(k_pool_bwd, dim3(ew_grid(g2.nvox * 32)), dim3(256), 0, st, g2, g3, 32, N.gp2.p, N.gs2b.p);
-    hipLaunchKernelGGL(k_add_channels, dim3(ew_grid(g2.nvox * 32)), dim3(256), 0, st, g2, 32, N.gs2b.p, 32, 0,
+    hipLaunchKernelGGL(k_pool_bwd, dim3(ew_grid(g2.nvox * 4)), dim3(256), 0, st, g2, g3, 32, N.gp2.p, N.gs2b.p);
+    hipLaunchKernelGGL(k_add_channels, dim3(ew_grid(g2.nvox * 4)), dim3(256), 0, st, g2, 32, N.gs2b.p, 32, 0,
                        N.gs2.p, 32, 0);
     // level 2: s2 = relu(c3(a2)), a2 = relu(c2(p1))
     relu_bwd(N, g2, 32, N.gs2.p, N.s2.p, N.gs2.p);
@@ -813,8 +848,8 @@ double net_gradients(Net& N, const double* r, const double* dinv, const float* l
     relu_bwd(N, g2, 32, N.ga2.p, N.a2.p, N.ga2.p);
     layer_wgrad(N, 2, g2, N.p1.p, N.ga2.p);
     layer_dgrad(N, 2, g2, N.ga2.p, N.gp1.p);
-    hipLaunchKernelGGL(k_pool_bwd, dim3(ew_grid(g1.nvox * 16)), dim3(256), 0, st, g1, g2, 16, N.gp1.p, N.gs1b.p);
-    hipLaunchKernelGGL(k_add_channels, dim3(ew_grid(g1.nvox * 16)), dim3(256), 0, st, g1, 16, N.gs1b.p, 16, 0,
+    hipLaunchKernelGGL(k_pool_bwd, dim3(ew_grid(g1.nvox * 2)), dim3(256), 0, st, g1, g2, 16, N.gp1.p, N.gs1b.p);
+    hipLaunchKernelGGL(k_add_channels, dim3(ew_grid(g1.nvox * 2)), dim3(256), 0, st, g1, 16, N.gs1b.p, 16, 0,
                        N.gs1.p, 16, 0);
     // level 1: s1 = relu(c1(a1)), a1 = relu(c0(x0))
     relu_bwd(N, g1, 16, N.gs1.p, N.s1.p, N.gs1.p);

@@ -298,8 +298,26 @@ def _main():
     ap.add_argument("--sigma", type=float, default=1.0)
     ap.add_argument("--corr", type=float, default=8.0)
     ap.add_argument("--init", default=None)
-    ap.add_argument("--out", required=True)
+    ap.add_argument("--out", default=None)
+    ap.add_argument("--apply-reps", type=int, default=0,
+                    help="only time this many applications on a random residual over (ne+1)^3 nodes")
     a = ap.parse_args()
+    if a.apply_reps > 0:
+        nodes = (a.ne + 1,) * 3
+        net = LearnedPreconditioner(nodes)
+        if a.init:
+            net.load(a.init)
+        r = torch.randn(int(np.prod(nodes)), dtype=torch.float64, device="cuda")
+        net.apply(r)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(a.apply_reps):
+            net.apply(r)
+        torch.cuda.synchronize()
+        print("apply %s: %.3f ms" % (nodes, 1e3 * (time.perf_counter() - t0) / a.apply_reps), flush=True)
+        sys.exit(0)
+    if a.out is None:
+        ap.error("--out is required for training")
 
     def kfn(ne, s):
         if a.family == "log10grf":

@@ -370,3 +370,41 @@ def test_bicgstab_parity(dim, ne, n):
     res = np.linalg.norm(orhs - O.spmv(n, ptr, idx, oval, x)) / np.linalg.norm(orhs)
     assert res <= 1e-8
     assert np.linalg.norm(x - xo) <= 1e-6 * np.linalg.norm(xo)
+
+
+@pytest.mark.parametrize("method", ["PCG", "PCG_PLAIN", "PCG_MIXED"])
+@pytest.mark.parametrize("dim,ne,n", [(3, (14, 12, 10), 1), (2, (40, 30), 1), (3, (6, 5, 5), 3)])
+def test_symmetric_storage_solves(method, dim, ne, n):
+    """Solves on upper-half (symmetric) storage against the oracle's full CSR
+    solve: the storage the benchmark's fastest method uses."""
+    rng = np.random.default_rng(31)
+    dom, length = grid(dim, ne)
+    E = dom.getNumElements()
+    if n == 1:
+        A = np.exp(1.5 * rng.standard_normal(E))[:, None, None] * np.eye(dim)[None]
+        Y = np.ones(E)
+    else:
+        lam, mu = rng.uniform(1, 5, E), rng.uniform(1, 5, E)
+        eye = np.eye(dim)
+        A = (lam[:, None, None, None, None] * np.einsum("ij,kl->ijkl", eye, eye)
+             + mu[:, None, None, None, None] * (np.einsum("ik,jl->ijkl", eye, eye)
+                                                 + np.einsum("il,jk->ijkl", eye, eye)))
+        Y = np.tile([0.0, 0.0, -1.0], (E, 1))
+    op, rhs = gpu_assemble(dom, n, symmetric=True, A=A, Y=Y)
+    q = dirichlet_mask(dom, dim - 1, n)
+    op.nullifyRowsAndCols(to_dev(q), to_dev(q), 1.0)
+    rhs = torch.where(to_dev(q) > 0, torch.zeros_like(rhs), rhs)
+    ptr, idx, oval, orhs = O.assemble(dim, ne, length, n=n, A=A, Y=Y)
+    O.nullify(n, ptr, idx, oval, q, q, 1.0)
+    orhs = np.where(q > 0, 0.0, orhs)
+    opts = R.SolverOptions()
+    opts.setSymmetryOn()
+    opts.setSolverMethod(getattr(R.SolverOptions, method))
+    opts.setTolerance(1e-9)
+    opts.setTrueResidualCriterion(True)
+    x = op.solve(rhs, opts).cpu().numpy()
+    assert opts.hasConverged()
+    res = np.linalg.norm(orhs - O.spmv(n, ptr, idx, oval, x)) / np.linalg.norm(orhs)
+    assert res <= 1e-9 * 1.001
+    xo, dg, _, _ = O.solve(n, ptr, idx, oval, orhs, tolerance=1e-11, plain=(method == "PCG_PLAIN"))
+    assert np.linalg.norm(x - xo) <= 1e-6 * np.linalg.norm(xo)
