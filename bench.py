@@ -97,7 +97,12 @@ class Problem:
         o.setSolverMethod({"paso": R.SolverOptions.PCG, "plain": R.SolverOptions.PCG_PLAIN,
                            "mixed": R.SolverOptions.PCG_MIXED, "learned": R.SolverOptions.FCG_LEARNED,
                            "plain_sym": R.SolverOptions.PCG_PLAIN,
-                           "paso_sym": R.SolverOptions.PCG}[method])
+                           "paso_sym": R.SolverOptions.PCG, "paso_gs": R.SolverOptions.PCG,
+                           "paso_ilu0": R.SolverOptions.PCG}[method])
+        if method == "paso_gs":
+            o.setPreconditioner(R.SolverOptions.GAUSS_SEIDEL)
+        elif method == "paso_ilu0":
+            o.setPreconditioner(R.SolverOptions.ILU0)
         if method == "learned":
             self.op.setLearnedPreconditioner(self.net, self.logk)
             o.setCheckInterval(8)
@@ -582,7 +587,8 @@ def main():
         return
     log(args, "building C2 problem ne=%d" % args.ne)
     P = Problem(args.ne, R, torch, np, fields)
-    methods = ["plain", "paso", "mixed", "plain_sym", "paso_sym"] if args.method == "all" else [args.method]
+    methods = (["plain", "paso", "mixed", "plain_sym", "paso_sym", "paso_gs", "paso_ilu0"] if args.method == "all"
+               else [args.method])
     weights = os.path.join(ROOT, "esys-escript.github.io_amd", "ripley_hip", "weights", "lpc_log10grf_s1_l8.npy")
     if args.method == "all" and os.path.exists(weights):
         methods.append("learned")

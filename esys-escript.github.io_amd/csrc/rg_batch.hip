@@ -792,6 +792,8 @@ int solve_multi(rg_matrix* m, double* X, const double* B, int K, const rg_option
     RG_CHECK(K >= 1 && K <= 64, RG_VALUE_ERROR, "solve_multi: 1 to 64 right-hand sides");
     RG_CHECK(opt.method == RG_METHOD_PCG_PLAIN || opt.method == RG_METHOD_PCG_MIXED, RG_VALUE_ERROR,
              "solve_multi: method must be RG_METHOD_PCG_PLAIN or RG_METHOD_PCG_MIXED");
+    RG_CHECK(opt.preconditioner == RG_PRECONDITIONER_JACOBI && opt.sweeps <= 1, RG_VALUE_ERROR,
+             "solve_multi: one Jacobi sweep is the only preconditioner");
     RG_CHECK(opt.tolerance > 0.0 && opt.tolerance <= 1.0, RG_VALUE_ERROR, "solve_multi: tolerance must be in (0, 1]");
     const bool fp32 = opt.method == RG_METHOD_PCG_MIXED;
     hipStream_t st = m->ctx->stream;
@@ -1009,6 +1011,8 @@ int batch_solve(rg_batch* B, double* X, const double* b, const rg_options& opt, 
     const int count = (int)B->mats.size();
     RG_CHECK(opt.method == RG_METHOD_PCG_PLAIN || opt.method == RG_METHOD_PCG_MIXED, RG_VALUE_ERROR,
              "batch_solve: method must be RG_METHOD_PCG_PLAIN or RG_METHOD_PCG_MIXED");
+    RG_CHECK(opt.preconditioner == RG_PRECONDITIONER_JACOBI && opt.sweeps <= 1, RG_VALUE_ERROR,
+             "batch_solve: one Jacobi sweep is the only preconditioner");
     RG_CHECK(opt.tolerance > 0.0 && opt.tolerance <= 1.0, RG_VALUE_ERROR, "batch_solve: tolerance must be in (0, 1]");
     const bool fp32 = opt.method == RG_METHOD_PCG_MIXED;
     if (B->tile_mult <= 0) batch_autotune(B, fp32, 5, nullptr);

@@ -199,6 +199,7 @@ int rg_matrix_destroy(rg_matrix* m)
         hipSetDevice(m->ctx->device);
         hipStreamSynchronize(m->ctx->stream);
         rg::free_work(m->work);
+        rg::free_colouring(m);
         void* bufs[] = {m->val, m->val32, m->dinv, m->dinv32, m->balance};
         for (void* b : bufs)
             if (b) hipFree(b);
@@ -239,7 +240,10 @@ int rg_matrix_reset_values(rg_matrix* m, int preserve_solver_data)
         rg::launch_set_values(m, 0.0);
         m->is_balanced = false;
         m->val32_valid = false;
-        if (!preserve_solver_data) m->have_prec = false;
+        if (!preserve_solver_data) {
+            m->have_prec = false;
+            m->pval_state = 0;
+        }
         return RG_OK;
     });
 }
@@ -482,7 +486,7 @@ void rg_options_default(rg_options* opt)
     opt->accept_failed_convergence = 0;
     opt->check_interval = 0;
     opt->criterion = RG_CRITERION_PASO;
-    opt->reserved = 0;
+    opt->preconditioner = RG_PRECONDITIONER_JACOBI;
 }
 
 // Error mapping of SystemMatrix<double>::solve (paso/src/solve.cpp:119-143).
@@ -547,6 +551,37 @@ int rg_matrix_set_learned_preconditioner(rg_matrix* m, rg_lpc* net, const float*
         m->lpc = net;
         m->lpc_logk = logk;
         if (m->work) m->work->graph_method = -1;   // captured iterations refer to the old network
+        return RG_OK;
+    });
+}
+
+int rg_matrix_solve_preconditioner(rg_matrix* m, const rg_options* opt, double* x, const double* b)
+{
+    return guarded([&] {
+        check_matrix(m);
+        RG_CHECK(x && b, RG_VALUE_ERROR, "solvePreconditioner: NULL vector");
+        rg_options o;
+        rg_options_default(&o);
+        if (opt) o = *opt;
+        use_device(m->ctx);
+        rg::solve_preconditioner(m, o, x, b);
+        return RG_OK;
+    });
+}
+
+int rg_matrix_coloring(rg_matrix* m, int32_t* colours, int* num_colors)
+{
+    return guarded([&] {
+        check_matrix(m);
+        use_device(m->ctx);
+        rg::ensure_colouring(m);
+        const rg_colouring* c = m->colouring;
+        if (num_colors) *num_colors = c->ncolors;
+        if (colours) {
+            std::vector<uint8_t> c8(m->g.N);
+            RG_HIP(hipMemcpy(c8.data(), c->colour, m->g.N, hipMemcpyDeviceToHost));
+            for (int64_t i = 0; i < m->g.N; ++i) colours[i] = c8[i];
+        }
         return RG_OK;
     });
 }

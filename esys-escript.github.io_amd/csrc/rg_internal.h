@@ -95,6 +95,7 @@ struct rg_solver_work {
     double *r = nullptr, *xw = nullptr, *rs = nullptr, *p = nullptr, *v = nullptr, *x2 = nullptr;
     double *z = nullptr, *z2 = nullptr;
     double* s = nullptr;            // BiCGStab's s
+    double *pb1 = nullptr, *pb2 = nullptr;   // Gauss-Seidel sweeps > 1 (allocated on first use)
     double* bb = nullptr;           // balanced right-hand side D b
     double* norms = nullptr;        // [0] sum of squares, [1] max abs
     rg::KrylovState* st = nullptr;
@@ -102,6 +103,18 @@ struct rg_solver_work {
     hipStream_t capture = nullptr;
     hipGraphExec_t graph = nullptr;
     int graph_method = -1, graph_sweeps = -1, graph_iters = -1;
+};
+
+// PASO's colouring of the matrix graph (Pattern::borrowColoringPointer) and
+// the colour-ordered matrix copy of the Gauss-Seidel / ILU0 preconditioners.
+// Rows are stored in colour order: pos -> row is perm, row -> pos is ipos,
+// and colour c owns positions [cstart[c], cstart[c+1]).
+struct rg_colouring {
+    int ncolors = 0;
+    std::vector<int64_t> cstart;
+    int32_t* perm = nullptr;
+    int32_t* ipos = nullptr;
+    uint8_t* colour = nullptr;
 };
 
 struct rg_matrix {
@@ -121,6 +134,10 @@ struct rg_matrix {
     int multi_shape = 0;           // SpMM tile shape + 1 of the many-RHS solver (0: autotune)
     rg_lpc* lpc = nullptr;         // learned preconditioner (borrowed)
     const float* lpc_logk = nullptr;   // log10(kappa) per node (device, borrowed)
+    rg_colouring* colouring = nullptr; // built on first use (pattern only)
+    double* pval = nullptr;        // colour-ordered full-stencil values or ILU0 factors, nslot*bs*bs*ld
+    double* pdinv = nullptr;       // colour-ordered inverse main-diagonal blocks, bs*bs*ld
+    int pval_state = 0;            // 0: invalid, 1: balanced values (Gauss-Seidel), 2: ILU0 factors
 };
 
 namespace rg {
@@ -135,11 +152,23 @@ void launch_assemble_boundary(rg_matrix* m, const rg_coef* d, const rg_coef* y, 
 void launch_add_dirac(rg_matrix* m, int64_t count, const int64_t* d_nodes, const double* d_d, const double* d_y,
                       double* rhs);
 void launch_spmv(rg_matrix* m, double alpha, const double* x, double beta, double* y);
+void launch_spmv_on(rg_matrix* m, double alpha, const double* x, double beta, double* y, hipStream_t st);
 void launch_spmv32(rg_matrix* m, const double* x, double* y);   // FP32 values, y = A32 x
 void launch_nullify(rg_matrix* m, const double* row_mask, const double* col_mask, double mdv,
                     int* d_flag);
 void launch_export_csr_values(rg_matrix* m, const int32_t* d_ptr, int32_t* d_index, double* d_val);
 void launch_to_fp32(rg_matrix* m);
+
+// Multi-colour preconditioners (rg_precond.hip).  prec_setup builds what
+// preconditioner `type` (RG_PRECONDITIONER_*) needs from the balanced matrix;
+// prec_apply computes x = M^-1 b (b and x distinct device vectors of N*bs),
+// skipping all work once guard->done is set (guard may be null).
+void ensure_colouring(rg_matrix* m);
+void free_colouring(rg_matrix* m);
+void prec_setup(rg_matrix* m, int type, bool rebalanced);
+void prec_apply(rg_matrix* m, int type, int sweeps, const double* b, double* x, hipStream_t st,
+                const KrylovState* guard);
+void solve_preconditioner(rg_matrix* m, const rg_options& opt, double* x, const double* b);
 
 // Learned preconditioner (rg_learned.hip): node grid of a network and its
 // forward pass on the balanced solver vectors, z = D^-1 Net(D dinv r).

@@ -476,18 +476,23 @@ __global__ __launch_bounds__(kBlock) void k_spmv(const VT* __restrict__ val, int
 
 void launch_spmv(rg_matrix* m, double alpha, const double* x, double beta, double* y)
 {
+    launch_spmv_on(m, alpha, x, beta, y, m->ctx->stream);
+}
+
+void launch_spmv_on(rg_matrix* m, double alpha, const double* x, double beta, double* y, hipStream_t st)
+{
     const Geom& g = m->g;
     const Launch L = chunked_launch(g.N, spmv_rows_per_thread<double>());
     dispatch(g, m->symmetric, [&](auto D, auto B, auto S) {
         if (beta == 0.0)
             hipLaunchKernelGGL((k_spmv<D.value, B.value, S.value, double, 0>), dim3(L.grid), dim3(kBlock), 0,
-                               m->ctx->stream, m->val, g.ld, g.N, g.s1, g.s2, L.chunk, x, y, alpha, beta);
+                               st, m->val, g.ld, g.N, g.s1, g.s2, L.chunk, x, y, alpha, beta);
         else if (beta == 1.0)
             hipLaunchKernelGGL((k_spmv<D.value, B.value, S.value, double, 1>), dim3(L.grid), dim3(kBlock), 0,
-                               m->ctx->stream, m->val, g.ld, g.N, g.s1, g.s2, L.chunk, x, y, alpha, beta);
+                               st, m->val, g.ld, g.N, g.s1, g.s2, L.chunk, x, y, alpha, beta);
         else
             hipLaunchKernelGGL((k_spmv<D.value, B.value, S.value, double, 2>), dim3(L.grid), dim3(kBlock), 0,
-                               m->ctx->stream, m->val, g.ld, g.N, g.s1, g.s2, L.chunk, x, y, alpha, beta);
+                               st, m->val, g.ld, g.N, g.s1, g.s2, L.chunk, x, y, alpha, beta);
     });
     RG_HIP(hipGetLastError());
 }

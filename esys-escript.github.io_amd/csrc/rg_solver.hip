@@ -670,10 +670,12 @@ __global__ __launch_bounds__(kBlock) void k_bicg_p(PcgArgs a, const double* __re
             const int64_t k = i * BS + c;
             a.p[k] = first ? a.r[k] : a.r[k] + beta * (a.p[k] - omega * a.v[k]);
         }
-        double z[BS];
-        jacobi_node<BS>(dinv, a.ld, i, a.p, z);
+        if (dinv) {   // one fused Jacobi sweep; otherwise phat is computed afterwards
+            double z[BS];
+            jacobi_node<BS>(dinv, a.ld, i, a.p, z);
 #pragma unroll
-        for (int c = 0; c < BS; ++c) a.zp[i * BS + c] = z[c];
+            for (int c = 0; c < BS; ++c) a.zp[i * BS + c] = z[c];
+        }
     }
 }
 
@@ -718,10 +720,12 @@ __global__ __launch_bounds__(kBlock) void k_bicg_s(PcgArgs a, const double* __re
             a.s[k] = rv;
             s[0] += rv * rv;
         }
-        double z[BS];
-        jacobi_node<BS>(dinv, a.ld, i, a.s, z);
+        if (dinv) {
+            double z[BS];
+            jacobi_node<BS>(dinv, a.ld, i, a.s, z);
 #pragma unroll
-        for (int c = 0; c < BS; ++c) a.z2p[i * BS + c] = z[c];
+            for (int c = 0; c < BS; ++c) a.z2p[i * BS + c] = z[c];
+        }
     }
     publish_block_sums<1>(s, a.partials);
 }
@@ -812,6 +816,7 @@ struct SolverImpl {
     rg_solver_work* w;
     hipStream_t st;
     Launch L;     // node kernels without pairing (init, Jacobi sweeps)
+    int prec = RG_PRECONDITIONER_JACOBI;
 
     explicit SolverImpl(rg_matrix* mm) : m(mm), w(mm->work), st(mm->ctx->stream), L(chunked_launch(mm->g.N)) {}
 
@@ -860,17 +865,29 @@ struct SolverImpl {
         return za;
     }
 
-    void sweeps_into_z(int sweeps, const KrylovState* guard)
+    // Whether z = M^-1 r is computed by its own launches into w->z (any
+    // preconditioner but a single Jacobi sweep, which the update kernels fuse).
+    bool buffered(int sweeps) const { return prec != RG_PRECONDITIONER_JACOBI || sweeps > 1; }
+
+    // out = M^-1 in (Preconditioner_solve); in and out distinct
+    void precondition(const double* in, double* out, int sweeps, const KrylovState* guard)
     {
-        double* z = jacobi(w->r, sweeps, guard);
-        if (z != w->z) RG_HIP(hipMemcpyAsync(w->z, z, sizeof(double) * w->n, hipMemcpyDeviceToDevice, st));
+        if (prec == RG_PRECONDITIONER_JACOBI) {
+            double* z = jacobi(in, sweeps, guard);
+            if (z != out) RG_HIP(hipMemcpyAsync(out, z, sizeof(double) * w->n, hipMemcpyDeviceToDevice, st));
+        } else {
+            prec_apply(m, prec, sweeps, in, out, st, guard);
+        }
     }
+
+    void sweeps_into_z(int sweeps, const KrylovState* guard) { precondition(w->r, w->z, sweeps, guard); }
 
     // One iteration's launches of the PASO residual-smoothed PCG.
     void paso_iteration(int sweeps)
     {
         const Geom& g = m->g;
-        const double* zb = (sweeps > 1) ? w->z : nullptr;
+        const bool buf = buffered(sweeps);
+        const double* zb = buf ? w->z : nullptr;
         const Launch Lsp = spmv_launch<double>(), Ln = node_launch();
         const PcgArgs an = args(zb, Ln), asp = args(zb, Lsp);
         PcgArgs ar = args(zb, chunked_launch(g.N * g.bs, 2));
@@ -884,7 +901,7 @@ struct SolverImpl {
         stage<ST_PASO_SPMV, 1>(Lsp.grid);
         hipLaunchKernelGGL(k_paso_r, dim3(Lr.grid), dim3(kBlock), 0, st, ar);
         stage<ST_PASO_R, 2>(Lr.grid);
-        if (sweeps > 1) sweeps_into_z(sweeps, w->st);
+        if (buf) sweeps_into_z(sweeps, w->st);
         dispatch_bs(g.bs, [&](auto B) {
             hipLaunchKernelGGL((k_paso_x<B.value, double>), dim3(Ln.grid), dim3(kBlock), 0, st, an, m->dinv);
         });
@@ -899,7 +916,8 @@ struct SolverImpl {
     void cg_iteration(const VT* val, const DT* dinv, int sweeps, bool replace)
     {
         const Geom& g = m->g;
-        const double* zb = (sweeps > 1) ? w->z : nullptr;
+        const bool buf = buffered(sweeps);
+        const double* zb = buf ? w->z : nullptr;
         const Launch Lsp = spmv_launch<VT>(), Ln = node_launch(), L64 = spmv_launch<double>();
         const PcgArgs an = args(zb, Ln), asp = args(zb, Lsp), a64 = args(nullptr, L64);
         dispatch(g, m->symmetric, [&](auto D, auto B, auto S) {
@@ -907,7 +925,7 @@ struct SolverImpl {
                                st, asp, val);
             stage<ST_CG_SPMV, 1>(Lsp.grid);
             hipLaunchKernelGGL((k_cg_update<B.value, DT>), dim3(Ln.grid), dim3(kBlock), 0, st, an, dinv);
-            if (sweeps > 1) stage<ST_CG_UPDATE_NORZ, 2>(Ln.grid);
+            if (buf) stage<ST_CG_UPDATE_NORZ, 2>(Ln.grid);
             else stage<ST_CG_UPDATE, 2>(Ln.grid);
             if (replace) {
                 hipLaunchKernelGGL((k_cg_replace<D.value, B.value, S.value, DT>), dim3(L64.grid), dim3(kBlock), 0,
@@ -915,7 +933,7 @@ struct SolverImpl {
                 stage<ST_CG_REPLACE, 2>(L64.grid);
             }
         });
-        if (sweeps > 1) {
+        if (buf) {
             sweeps_into_z(sweeps, w->st);
             hipLaunchKernelGGL(k_cg_rz, dim3(L.grid), dim3(kBlock), 0, st, args(zb, L), w->n);
             stage<ST_CG_RZ, 1>(L.grid);
@@ -963,21 +981,25 @@ struct SolverImpl {
     }
 
     // One iteration of PASO's BiCGStab (FP64, one Jacobi sweep).
-    void bicg_iteration()
+    void bicg_iteration(int sweeps)
     {
         const Geom& g = m->g;
         const Launch Lsp = spmv_launch<double>(), Ln = chunked_launch(g.N);
         const PcgArgs an = args(nullptr, Ln), asp = args(nullptr, Lsp);
         const int vg = 1024;
+        const bool buf = buffered(sweeps);
+        const double* dinv = buf ? nullptr : m->dinv;
         hipLaunchKernelGGL(k_bicg_rho, dim3(vg), dim3(kBlock), 0, st, an, w->n);
         stage<ST_BICG_RHO, 1>(vg);
         dispatch(g, m->symmetric, [&](auto D, auto B, auto S) {
-            hipLaunchKernelGGL((k_bicg_p<B.value>), dim3(Ln.grid), dim3(kBlock), 0, st, an, m->dinv);
+            hipLaunchKernelGGL((k_bicg_p<B.value>), dim3(Ln.grid), dim3(kBlock), 0, st, an, dinv);
+            if (buf) precondition(w->p, w->z, sweeps, w->st);   // phat = M^-1 p
             hipLaunchKernelGGL((k_bicg_spmv<D.value, B.value, S.value, 0>), dim3(Lsp.grid), dim3(kBlock), 0, st, asp,
                                m->val, w->z, w->v);
             stage<ST_BICG_ALPHA, 2>(Lsp.grid);
-            hipLaunchKernelGGL((k_bicg_s<B.value>), dim3(Ln.grid), dim3(kBlock), 0, st, an, m->dinv);
+            hipLaunchKernelGGL((k_bicg_s<B.value>), dim3(Ln.grid), dim3(kBlock), 0, st, an, dinv);
             stage<ST_BICG_S, 1>(Ln.grid);
+            if (buf) precondition(w->s, w->z2, sweeps, w->st);  // shat = M^-1 s
             hipLaunchKernelGGL(k_bicg_xa, dim3(vg), dim3(kBlock), 0, st, an, w->n);
             stage<ST_BICG_XA, 1>(vg);
             hipLaunchKernelGGL((k_bicg_spmv<D.value, B.value, S.value, 1>), dim3(Lsp.grid), dim3(kBlock), 0, st, asp,
@@ -991,7 +1013,7 @@ struct SolverImpl {
 
     void iteration(int method, int sweeps, bool last)
     {
-        if (method == RG_METHOD_BICGSTAB) bicg_iteration();
+        if (method == RG_METHOD_BICGSTAB) bicg_iteration(sweeps);
         else if (method == RG_METHOD_FCG_LEARNED) fcg_iteration(last);
         else if (method == RG_METHOD_PCG) paso_iteration(sweeps);
         else if (method == RG_METHOD_PCG_PLAIN) cg_iteration<double, double>(m->val, m->dinv, sweeps, false);
@@ -1002,7 +1024,7 @@ struct SolverImpl {
     {
         const Geom& g = m->g;
         double* zb = nullptr;
-        if (sweeps > 1) {
+        if (buffered(sweeps) && method != RG_METHOD_BICGSTAB) {
             sweeps_into_z(sweeps, nullptr);
             zb = w->z;
         }
@@ -1037,7 +1059,8 @@ struct SolverImpl {
             for (int k = 0; k < iters; ++k) iteration(method, sweeps, k == iters - 1);
             return;
         }
-        if (!w->graph || w->graph_method != method || w->graph_sweeps != sweeps || w->graph_iters != iters) {
+        const int key = method + 8 * prec;
+        if (!w->graph || w->graph_method != key || w->graph_sweeps != sweeps || w->graph_iters != iters) {
             if (w->graph) {
                 RG_HIP(hipGraphExecDestroy(w->graph));
                 w->graph = nullptr;
@@ -1058,7 +1081,7 @@ struct SolverImpl {
             st = keep;
             RG_HIP(hipGraphInstantiate(&w->graph, graph, nullptr, nullptr, 0));
             RG_HIP(hipGraphDestroy(graph));
-            w->graph_method = method;
+            w->graph_method = key;
             w->graph_sweeps = sweeps;
             w->graph_iters = iters;
         }
@@ -1183,7 +1206,7 @@ void free_work(rg_solver_work* w)
     if (!w) return;
     if (w->graph) hipGraphExecDestroy(w->graph);
     if (w->capture) hipStreamDestroy(w->capture);
-    double* bufs[] = {w->r, w->xw, w->rs, w->p, w->v, w->x2, w->z, w->z2, w->s, w->bb, w->norms};
+    double* bufs[] = {w->r, w->xw, w->rs, w->p, w->v, w->x2, w->z, w->z2, w->s, w->bb, w->norms, w->pb1, w->pb2};
     for (double* b : bufs)
         if (b) hipFree(b);
     if (w->st) hipFree(w->st);
@@ -1194,12 +1217,13 @@ void free_work(rg_solver_work* w)
 // SystemMatrix::balance (once per set of values) and the Jacobi inverse of
 // the balanced matrix (Preconditioner_alloc -> invMain; kept until the values
 // are reset without preserving solver data).
-void prepare_solve(rg_matrix* m)
+void prepare_solve(rg_matrix* m, int prec)
 {
     const Geom& g = m->g;
     hipStream_t st = m->ctx->stream;
     const int grid = (int)((g.N + kBlock - 1) / kBlock);
     if (!m->is_balanced) {
+        if (m->pval_state == 1) m->pval_state = 0;   // the Gauss-Seidel copy holds the old values
         dispatch(g, m->symmetric, [&](auto D, auto B, auto S) {
             hipLaunchKernelGGL((k_balance_vector<D.value, B.value, S.value>), dim3(grid), dim3(kBlock), 0, st,
                                m->val, g.ld, g.N, g.s1, g.s2, m->balance);
@@ -1210,7 +1234,8 @@ void prepare_solve(rg_matrix* m)
         m->is_balanced = true;
         m->val32_valid = false;
     }
-    if (!m->have_prec) {
+    const bool needs_inverse = prec == RG_PRECONDITIONER_JACOBI || prec == RG_PRECONDITIONER_GAUSS_SEIDEL;
+    if (needs_inverse && !m->have_prec) {
         RG_CHECK(g.bs <= 3, RG_VALUE_ERROR,
                  "SparseMatrix::invMain: block size > 3 needs LAPACK (not supported)");
         RG_HIP(hipMemsetAsync(m->work->flag, 0, sizeof(int), st));
@@ -1225,6 +1250,7 @@ void prepare_solve(rg_matrix* m)
         RG_CHECK(!failed, RG_VALUE_ERROR, "SparseMatrix::invMain: non-regular main diagonal block.");
         m->have_prec = true;
     }
+    prec_setup(m, prec, false);
 }
 
 // paso::Solver (paso/src/Solver.cpp:46-331) on device vectors.
@@ -1243,8 +1269,14 @@ int solve(rg_matrix* m, double* x, const double* b, const rg_options& opt, rg_di
     RG_CHECK(tolerance <= 1., RG_VALUE_ERROR, "Solver: Tolerance must be less than one.");
     RG_CHECK(opt.method >= RG_METHOD_PCG && opt.method <= RG_METHOD_BICGSTAB, RG_VALUE_ERROR,
              "Solver: unknown solver method.");
-    RG_CHECK(opt.method != RG_METHOD_BICGSTAB || opt.sweeps <= 1, RG_VALUE_ERROR,
-             "Solver: BiCGStab runs with one Jacobi sweep.");
+    const int prec = opt.preconditioner;
+    RG_CHECK(prec >= RG_PRECONDITIONER_JACOBI && prec <= RG_PRECONDITIONER_NONE, RG_VALUE_ERROR,
+             "Solver: unknown preconditioner.");
+    RG_CHECK(prec == RG_PRECONDITIONER_JACOBI ||
+                 (opt.method != RG_METHOD_PCG_MIXED && opt.method != RG_METHOD_FCG_LEARNED),
+             RG_VALUE_ERROR, "Solver: the FP32 and learned methods use the Jacobi preconditioner.");
+    RG_CHECK(opt.method != RG_METHOD_BICGSTAB || prec != RG_PRECONDITIONER_JACOBI || opt.sweeps <= 1,
+             RG_VALUE_ERROR, "Solver: BiCGStab runs with one Jacobi sweep.");
     RG_CHECK(opt.method != RG_METHOD_FCG_LEARNED || (m->lpc && m->g.bs == 1), RG_VALUE_ERROR,
              "Solver: the flexible CG needs a learned preconditioner attached to a scalar matrix.");
     RG_CHECK(opt.criterion == RG_CRITERION_PASO || opt.criterion == RG_CRITERION_TRUE_RESIDUAL, RG_VALUE_ERROR,
@@ -1254,9 +1286,14 @@ int solve(rg_matrix* m, double* x, const double* b, const rg_options& opt, rg_di
     rg_solver_work* w = m->work;
     hipStream_t st = m->ctx->stream;
     SolverImpl S(m);
+    S.prec = prec;
     const int64_t n = w->n;
     const int vgrid = 2048;
-    prepare_solve(m);
+    prepare_solve(m, prec);
+    if (prec == RG_PRECONDITIONER_GAUSS_SEIDEL && opt.sweeps > 1) {   // before any graph capture
+        if (!w->pb1) RG_HIP(hipMalloc(&w->pb1, sizeof(double) * n));
+        if (!w->pb2) RG_HIP(hipMalloc(&w->pb2, sizeof(double) * n));
+    }
     int method = opt.method;
     if ((method == RG_METHOD_PCG_MIXED || method == RG_METHOD_FCG_LEARNED) && !m->val32_valid) {
         if (!m->val32) {
@@ -1285,10 +1322,7 @@ int solve(rg_matrix* m, double* x, const double* b, const rg_options& opt, rg_di
     } else {
         dg.set_up_time = now_s() - t0;
         // initial guess x = M^-1 D b (Solver.cpp:181)
-        {
-            double* z = S.jacobi(w->r, std::max(1, opt.sweeps), nullptr);
-            RG_HIP(hipMemcpyAsync(w->xw, z, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
-        }
+        S.precondition(w->r, w->xw, std::max(1, opt.sweeps), nullptr);
         int totIter = 1;
         bool finalize = false;
         double last2 = norm2_of_b, lastmax = norm_max_of_b;
@@ -1359,6 +1393,22 @@ int solve(rg_matrix* m, double* x, const double* b, const rg_options& opt, rg_di
     dg.status = status;
     dg.time = now_s() - t0;
     return status;
+}
+
+// SystemMatrix::setPreconditioner + solvePreconditioner
+// (paso/src/SystemMatrix.cpp:42-56) on the balanced matrix.
+void solve_preconditioner(rg_matrix* m, const rg_options& opt, double* x, const double* b)
+{
+    const int prec = opt.preconditioner;
+    RG_CHECK(prec >= RG_PRECONDITIONER_JACOBI && prec <= RG_PRECONDITIONER_NONE, RG_VALUE_ERROR,
+             "solvePreconditioner: unknown preconditioner.");
+    RG_CHECK(x != b, RG_VALUE_ERROR, "solvePreconditioner: x and b must be distinct");
+    ensure_work(m);
+    prepare_solve(m, prec);
+    SolverImpl S(m);
+    S.prec = prec;
+    S.precondition(b, x, std::max(1, opt.sweeps), nullptr);
+    RG_HIP(hipStreamSynchronize(m->ctx->stream));
 }
 
 }  // namespace rg

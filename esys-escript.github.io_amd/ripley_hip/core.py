@@ -82,9 +82,11 @@ class SolverOptions:
     TARGET_CPU, TARGET_GPU = 1, 2
     PASO = 4
     BICGSTAB = 8
-    PCG = 23
-    JACOBI = 31
-    NO_PRECONDITIONER = 32
+    PCG = 24
+    GAUSS_SEIDEL = 29
+    ILU0 = 30
+    JACOBI = 32
+    NO_PRECONDITIONER = 33
     # methods of this implementation beyond the reference's enum
     PCG_PLAIN = 1001
     PCG_MIXED = 1002
@@ -115,8 +117,9 @@ class SolverOptions:
         return self._method
 
     def setPreconditioner(self, preconditioner):
-        if preconditioner != self.JACOBI:
-            raise ValueError("only the Jacobi preconditioner is provided on this path")
+        if preconditioner not in (self.JACOBI, self.GAUSS_SEIDEL, self.ILU0, self.NO_PRECONDITIONER):
+            raise ValueError("preconditioner %r is not provided on this path "
+                             "(JACOBI, GAUSS_SEIDEL, ILU0, NO_PRECONDITIONER)" % (preconditioner,))
         self._preconditioner = preconditioner
 
     def getPreconditioner(self):
@@ -242,6 +245,10 @@ class SolverOptions:
         o.accept_failed_convergence = int(self._accept)
         o.check_interval = self._check_interval
         o.criterion = _lib.RG_CRITERION_TRUE_RESIDUAL if self._true_residual else _lib.RG_CRITERION_PASO
+        o.preconditioner = {self.JACOBI: _lib.RG_PRECONDITIONER_JACOBI,
+                            self.GAUSS_SEIDEL: _lib.RG_PRECONDITIONER_GAUSS_SEIDEL,
+                            self.ILU0: _lib.RG_PRECONDITIONER_ILU0,
+                            self.NO_PRECONDITIONER: _lib.RG_PRECONDITIONER_NONE}[self._preconditioner]
         return o
 
 
@@ -481,6 +488,30 @@ class SystemMatrix:
         options._updateDiagnostics("converged", bool(dg.converged))
         check(rc)
         return out
+
+    def solvePreconditioner(self, in_, options=None):
+        """SystemMatrix::setPreconditioner + solvePreconditioner
+        (paso/src/SystemMatrix.cpp:42-56): balances the matrix if needed,
+        builds the preconditioner the options select and returns M^-1 in_
+        (balanced space)."""
+        self._bind()
+        if options is None:
+            options = SolverOptions()
+        o = options._rg_options()
+        b = _dev_vec(in_, self.size, self.device)
+        out = torch.empty(self.size, dtype=torch.float64, device=self.device)
+        check(load().rg_matrix_solve_preconditioner(self.handle, ctypes.byref(o), _ptr(out), _ptr(b)))
+        return out
+
+    def getColoring(self):
+        """Pattern::borrowColoringPointer / getNumColors
+        (paso/src/Pattern.cpp:166-195): (colour per node row, number of colours)."""
+        self._bind()
+        import numpy as np
+        c = np.zeros(self.num_rows, dtype=np.int32)
+        nc = ctypes.c_int(0)
+        check(load().rg_matrix_coloring(self.handle, c.ctypes.data_as(ctypes.c_void_p), ctypes.byref(nc)))
+        return c, nc.value
 
     def solve(self, in_, options=None):
         """AbstractSystemMatrix::solve (escriptcore/src/AbstractSystemMatrix.cpp:89-105)."""

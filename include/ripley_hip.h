@@ -208,21 +208,32 @@ int rg_matrix_save_mm(rg_matrix* m, const char* filename);
  *                        Extension: the reference has no counterpart.
  * criterion:
  *   RG_CRITERION_PASO           balanced ||r||_2 and ||r||_inf tests (Solver.cpp:241)
- *   RG_CRITERION_TRUE_RESIDUAL  unbalanced FP64 ||b - A x||_2 <= tolerance ||b||_2 */
+ *   RG_CRITERION_TRUE_RESIDUAL  unbalanced FP64 ||b - A x||_2 <= tolerance ||b||_2
+ * preconditioner (Preconditioner_alloc / Preconditioner_solve,
+ * paso/src/Preconditioner.cpp:46-137) for RG_METHOD_PCG, RG_METHOD_PCG_PLAIN
+ * and RG_METHOD_BICGSTAB; the FP32 and learned methods use Jacobi:
+ *   RG_PRECONDITIONER_JACOBI        `sweeps` Jacobi sweeps (Smoother.cpp:100-127)
+ *   RG_PRECONDITIONER_GAUSS_SEIDEL  `sweeps` symmetric Gauss-Seidel sweeps over the
+ *                                   colouring of Pattern::borrowColoringPointer
+ *                                   (Smoother.cpp:331-507, Pattern.cpp:166-195)
+ *   RG_PRECONDITIONER_ILU0          coloured ILU(0) (ILU.cpp:43-462), block size <= 3
+ *   RG_PRECONDITIONER_NONE          identity */
 enum { RG_METHOD_PCG = 0, RG_METHOD_PCG_PLAIN = 1, RG_METHOD_PCG_MIXED = 2, RG_METHOD_FCG_LEARNED = 3,
        RG_METHOD_BICGSTAB = 4 };
 enum { RG_CRITERION_PASO = 0, RG_CRITERION_TRUE_RESIDUAL = 1 };
+enum { RG_PRECONDITIONER_JACOBI = 0, RG_PRECONDITIONER_GAUSS_SEIDEL = 1, RG_PRECONDITIONER_ILU0 = 2,
+       RG_PRECONDITIONER_NONE = 3 };
 typedef struct {
     double tolerance;            /* relative; default 1e-8 */
     double inner_tolerance;      /* reserved (default 1e-4) */
     int32_t iter_max;            /* default 100000 */
-    int32_t sweeps;              /* Jacobi sweeps, default 1 */
+    int32_t sweeps;              /* Jacobi / Gauss-Seidel sweeps, default 1 */
     int32_t method;
     int32_t verbose;
     int32_t accept_failed_convergence;
     int32_t check_interval;      /* iterations per replayed graph / host check (0: 32) */
     int32_t criterion;
-    int32_t reserved;
+    int32_t preconditioner;      /* RG_PRECONDITIONER_*, default Jacobi */
 } rg_options;
 
 /* Diagnostics keys of paso::Options::updateEscriptDiagnostics
@@ -254,6 +265,18 @@ int rg_solve(rg_matrix* m, double* x, const double* b, const rg_options* opt,
              rg_diagnostics* diag);
 int rg_solve_host(rg_matrix* m, double* x, const double* b, const rg_options* opt,
                   rg_diagnostics* diag);
+
+/* SystemMatrix::setPreconditioner + solvePreconditioner
+ * (paso/src/SystemMatrix.cpp:42-56): balances the matrix if it is not yet
+ * (as a solve would), builds opt->preconditioner with opt->sweeps if needed
+ * and computes x = M^-1 b in the balanced space.  x, b: device,
+ * num_rows*block_size doubles, distinct. */
+int rg_matrix_solve_preconditioner(rg_matrix* m, const rg_options* opt, double* x, const double* b);
+
+/* Pattern::borrowColoringPointer / getNumColors (paso/src/Pattern.cpp:
+ * 166-195, Pattern_mis.cpp:44-110) of the matrix graph, computed on the
+ * device: colours (host, num_rows entries, may be NULL) and their number. */
+int rg_matrix_coloring(rg_matrix* m, int32_t* colours, int* num_colors);
 
 /* ---------------------------------------------------------- batched solves
  * No counterpart in the reference (PASO solves one right-hand side of one

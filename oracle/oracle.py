@@ -29,7 +29,10 @@ class _Coef(ctypes.Structure):
 
 class Options(ctypes.Structure):
     _fields_ = [("tolerance", ctypes.c_double), ("iter_max", ctypes.c_int),
-                ("sweeps", ctypes.c_int), ("plain", ctypes.c_int), ("verbose", ctypes.c_int)]
+                ("sweeps", ctypes.c_int), ("plain", ctypes.c_int), ("verbose", ctypes.c_int),
+                ("preconditioner", ctypes.c_int)]
+
+PRECONDITIONERS = {"JACOBI": 0, "GAUSS_SEIDEL": 1, "ILU0": 2, "NONE": 3}
 
 
 class Diagnostics(ctypes.Structure):
@@ -68,6 +71,11 @@ def lib():
             "orc_solve": (I, [I, I64, P, P, P, P, P, ctypes.POINTER(Options),
                               ctypes.POINTER(Diagnostics), P]),
             "orc_time_pcg_iterations": (D, [I, I64, P, P, P, P, I, I, P]),
+            "orc_coloring": (I, [I64, P, P, P]),
+            "orc_gs_sweep": (None, [I, I64, P, P, P, P, P, I, P]),
+            "orc_ilu0_factor": (I, [I, I64, P, P, P, P, I, P]),
+            "orc_ilu0_solve": (None, [I, I64, P, P, P, P, I, P, P]),
+            "orc_gs_apply": (None, [I, I64, P, P, P, P, P, I, I, P, P]),
             "orc_cds_add": (None, [I, I, I, I, P, I, P, P]),
             "orc_cds_spmv": (None, [I, I, I64, I, P, P, P, P]),
             "orc_num_threads": (I, []),
@@ -263,9 +271,10 @@ def inv_main(bs, ptr, index, val):
 
 
 def solve(bs, ptr, index, val, b, tolerance=1e-8, iter_max=100000, sweeps=1, plain=False,
-          verbose=False, method=None):
+          verbose=False, method=None, preconditioner="JACOBI"):
     """PASO Solver() on a copy of val; returns (x, diagnostics dict, balanced val, d).
-    method: "PCG" (PASO's, default), "PCG_PLAIN" (plain=True) or "BICGSTAB"."""
+    method: "PCG" (PASO's, default), "PCG_PLAIN" (plain=True) or "BICGSTAB";
+    preconditioner: "JACOBI", "GAUSS_SEIDEL", "ILU0" or "NONE"."""
     if method is not None:
         plain = {"PCG": 0, "PCG_PLAIN": 1, "BICGSTAB": 2}[method]
     val = np.array(val, dtype=np.float64, copy=True)
@@ -273,13 +282,56 @@ def solve(bs, ptr, index, val, b, tolerance=1e-8, iter_max=100000, sweeps=1, pla
     N = len(ptr) - 1
     x = np.zeros(N * bs)
     d = np.zeros(N * bs)
-    opt = Options(tolerance, iter_max, sweeps, int(plain), int(verbose))
+    opt = Options(tolerance, iter_max, sweeps, int(plain), int(verbose), PRECONDITIONERS[preconditioner])
     dg = Diagnostics()
     rc = lib().orc_solve(bs, N, _p(ptr), _p(index), _p(val), _p(x), _p(b), ctypes.byref(opt),
                          ctypes.byref(dg), _p(d))
     diag = {f: getattr(dg, f) for f, _ in Diagnostics._fields_}
     diag["rc"] = rc
     return x, diag, val, d
+
+
+def coloring(ptr, index):
+    """Pattern::borrowColoringPointer: (colour per row, number of colours)."""
+    ptr = np.ascontiguousarray(ptr, dtype=np.int32)
+    index = np.ascontiguousarray(index, dtype=np.int32)
+    c = np.zeros(len(ptr) - 1, dtype=np.int32)
+    nc = lib().orc_coloring(len(ptr) - 1, _p(ptr), _p(index), _p(c))
+    return c, nc
+
+
+def gs_sweep(bs, ptr, index, val, inv, colours, ncolors, b):
+    """One coloured symmetric Gauss-Seidel sweep applied to b."""
+    x = np.array(b, dtype=np.float64, copy=True)
+    lib().orc_gs_sweep(bs, len(ptr) - 1, _p(ptr), _p(index), _p(np.ascontiguousarray(val)),
+                       _p(np.ascontiguousarray(inv)), _p(np.ascontiguousarray(colours, dtype=np.int32)),
+                       int(ncolors), _p(x))
+    return x
+
+
+def gs_apply(bs, ptr, index, val, inv, colours, ncolors, sweeps, b):
+    """Preconditioner_Smoother_solve with `sweeps` coloured Gauss-Seidel sweeps."""
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    x = np.zeros_like(b)
+    lib().orc_gs_apply(bs, len(ptr) - 1, _p(ptr), _p(index), _p(np.ascontiguousarray(val)),
+                       _p(np.ascontiguousarray(inv)), _p(np.ascontiguousarray(colours, dtype=np.int32)),
+                       int(ncolors), int(sweeps), _p(x), _p(b))
+    return x
+
+
+def ilu0_factor(bs, ptr, index, val, colours, ncolors):
+    F = np.zeros(len(index) * bs * bs)
+    rc = lib().orc_ilu0_factor(bs, len(ptr) - 1, _p(ptr), _p(index), _p(np.ascontiguousarray(val)),
+                               _p(np.ascontiguousarray(colours, dtype=np.int32)), int(ncolors), _p(F))
+    return F, rc
+
+
+def ilu0_solve(bs, ptr, index, F, colours, ncolors, b):
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    x = np.zeros_like(b)
+    lib().orc_ilu0_solve(bs, len(ptr) - 1, _p(ptr), _p(index), _p(F),
+                         _p(np.ascontiguousarray(colours, dtype=np.int32)), int(ncolors), _p(x), _p(b))
+    return x
 
 
 def time_pcg_iterations(bs, ptr, index, val, b, iters, plain=False):

@@ -606,6 +606,292 @@ static void jacobi_apply(int bs, int64_t nrows, const index_t* ptr, const index_
     }
 }
 
+// ------------------------------------------- colouring, Gauss-Seidel, ILU0
+
+// Pattern::mis (paso/src/Pattern_mis.cpp:44-110): Luby-style maximal
+// independent set with fixed priorities value_i = fmod(0.4142135623730951 *
+// (i+1), 1) for available vertices (2 for the others).  Each round puts the
+// available vertices whose value is below that of every neighbour into the
+// set and marks their neighbours connected.  On entry mis_marker is -1 for
+// available vertices; on exit it is 1 for the set, 0 otherwise.
+static void pattern_mis(int64_t n, const index_t* ptr, const index_t* index, index_t* mis_marker)
+{
+    const index_t AVAILABLE = -1, IN_MIS_NOW = -2, IN_MIS = -3, CONNECTED = -4;
+    const double seed = .4142135623730951;
+    std::vector<double> value(n);
+    for (;;) {
+        bool any = false;
+        for (int64_t i = 0; i < n; ++i) any = any || mis_marker[i] == AVAILABLE;
+        if (!any) break;
+#pragma omp parallel for schedule(static)
+        for (int64_t i = 0; i < n; ++i)
+            value[i] = (mis_marker[i] == AVAILABLE) ? std::fmod(seed * (double)(i + 1), 1.) : 2.;
+#pragma omp parallel for schedule(static)
+        for (int64_t i = 0; i < n; ++i) {
+            if (mis_marker[i] != AVAILABLE) continue;
+            index_t flag = IN_MIS_NOW;
+            for (index_t k = ptr[i]; k < ptr[i + 1]; ++k) {
+                const index_t j = index[k];
+                if (j != i && value[j] <= value[i]) { flag = AVAILABLE; break; }
+            }
+            mis_marker[i] = flag;
+        }
+        // the set is independent, so the writes below never race with a
+        // vertex of the set (as in the reference's parallel loop)
+        for (int64_t i = 0; i < n; ++i) {
+            if (mis_marker[i] != IN_MIS_NOW) continue;
+            for (index_t k = ptr[i]; k < ptr[i + 1]; ++k)
+                if (index[k] != i) mis_marker[index[k]] = CONNECTED;
+            mis_marker[i] = IN_MIS;
+        }
+    }
+    for (int64_t i = 0; i < n; ++i) mis_marker[i] = (mis_marker[i] == IN_MIS);
+}
+
+// Pattern::borrowColoringPointer (paso/src/Pattern.cpp:166-195): colour c is
+// the MIS (above) of the vertices left uncoloured by colours 0..c-1.
+// Returns the number of colours.
+ORC_API int orc_coloring(int64_t n, const index_t* ptr, const index_t* index, index_t* coloring)
+{
+    std::vector<index_t> marker(n, -1);
+    for (int64_t i = 0; i < n; ++i) coloring[i] = -1;
+    int out = 0;
+    for (;;) {
+        bool any = false;
+        for (int64_t i = 0; i < n; ++i) any = any || coloring[i] == -1;
+        if (!any) break;
+        pattern_mis(n, ptr, index, marker.data());
+        for (int64_t i = 0; i < n; ++i) {
+            if (marker[i]) coloring[i] = out;
+            marker[i] = coloring[i];
+        }
+        ++out;
+    }
+    return out;
+}
+
+// position of the main diagonal entry of row i (Pattern::borrowMainDiagonalPointer)
+static inline index_t main_ptr(const index_t* ptr, const index_t* index, int64_t i)
+{
+    for (index_t k = ptr[i]; k < ptr[i + 1]; ++k)
+        if (index[k] == i) return k;
+    return -1;
+}
+
+// R -= M V for a column-major bs x bs block (BlockOps_SMV_2/3, BlockOps.h:46-85)
+static inline void block_smv(int bs, double* R, const double* M, const double* V)
+{
+    double t[3];
+    for (int r = 0; r < bs; ++r) {
+        double s = 0.0;
+        for (int c = 0; c < bs; ++c) s += M[r + bs * c] * V[c];
+        t[r] = s;
+    }
+    for (int r = 0; r < bs; ++r) R[r] -= t[r];
+}
+
+// V <- M V in place (BlockOps_MViP_2/3, BlockOps.h:187-217)
+static inline void block_mvip(int bs, const double* M, double* V)
+{
+    double t[3];
+    for (int r = 0; r < bs; ++r) {
+        double s = 0.0;
+        for (int c = 0; c < bs; ++c) s += M[r + bs * c] * V[c];
+        t[r] = s;
+    }
+    for (int r = 0; r < bs; ++r) V[r] = t[r];
+}
+
+// Preconditioner_LocalSmoother_Sweep_colored (paso/src/Smoother.cpp:
+// 331-507), the sweep PASO runs whenever OpenMP has two or more threads:
+// forward over the colours, x_i <- inv(A_ii) (x_i - sum_{c(k)<c(i)} A_ik x_k);
+// backward over colours nc-2..0, x_i <- inv(A_ii) (A_ii x_i -
+// sum_{c(k)>c(i)} A_ik x_k).  x holds the right-hand side on entry.
+ORC_API void orc_gs_sweep(int bs, int64_t nrows, const index_t* ptr, const index_t* index, const double* val,
+                          const double* inv, const index_t* coloring, int ncolors, double* x)
+{
+    const int b2 = bs * bs;
+    for (int color = 0; color < ncolors; ++color) {
+#pragma omp parallel for schedule(static)
+        for (int64_t i = 0; i < nrows; ++i) {
+            if (coloring[i] != color) continue;
+            double* xi = x + (int64_t)bs * i;
+            if (color > 0)
+                for (index_t k = ptr[i]; k < ptr[i + 1]; ++k)
+                    if (coloring[index[k]] < color) block_smv(bs, xi, val + (int64_t)k * b2, x + (int64_t)bs * index[k]);
+            block_mvip(bs, inv + (int64_t)i * b2, xi);
+        }
+    }
+    for (int color = ncolors - 2; color > -1; --color) {
+#pragma omp parallel for schedule(static)
+        for (int64_t i = 0; i < nrows; ++i) {
+            if (coloring[i] != color) continue;
+            double* xi = x + (int64_t)bs * i;
+            block_mvip(bs, val + (int64_t)main_ptr(ptr, index, i) * b2, xi);
+            for (index_t k = ptr[i]; k < ptr[i + 1]; ++k)
+                if (coloring[index[k]] > color) block_smv(bs, xi, val + (int64_t)k * b2, x + (int64_t)bs * index[k]);
+            block_mvip(bs, inv + (int64_t)i * b2, xi);
+        }
+    }
+}
+
+// inverse of a column-major bs x bs block by the closed forms of
+// BlockOps_invM_2/3 (BlockOps.h:100-145, also used by Solver_getILU);
+// returns false for a zero determinant
+static bool block_inverse(int bs, const double* A, double* I)
+{
+    if (bs == 1) {
+        if (!(std::fabs(A[0]) > 0.)) return false;
+        I[0] = 1. / A[0];
+        return true;
+    }
+    if (bs == 2) {
+        const double A11 = A[0], A12 = A[2], A21 = A[1], A22 = A[3];
+        double D = A11 * A22 - A12 * A21;
+        if (!(std::fabs(D) > 0.)) return false;
+        D = 1. / D;
+        I[0] = A22 * D; I[1] = -A21 * D; I[2] = -A12 * D; I[3] = A11 * D;
+        return true;
+    }
+    const double A11 = A[0], A21 = A[1], A31 = A[2], A12 = A[3], A22 = A[4], A32 = A[5], A13 = A[6],
+                 A23 = A[7], A33 = A[8];
+    double D = A11 * (A22 * A33 - A23 * A32) + A12 * (A31 * A23 - A21 * A33) + A13 * (A21 * A32 - A31 * A22);
+    if (!(std::fabs(D) > 0.)) return false;
+    D = 1. / D;
+    I[0] = (A22 * A33 - A23 * A32) * D;
+    I[1] = (A31 * A23 - A21 * A33) * D;
+    I[2] = (A21 * A32 - A31 * A22) * D;
+    I[3] = (A13 * A32 - A12 * A33) * D;
+    I[4] = (A11 * A33 - A31 * A13) * D;
+    I[5] = (A12 * A31 - A11 * A32) * D;
+    I[6] = (A12 * A23 - A13 * A22) * D;
+    I[7] = (A13 * A21 - A11 * A23) * D;
+    I[8] = (A11 * A22 - A12 * A21) * D;
+    return true;
+}
+
+// Solver_getILU (paso/src/ILU.cpp:43-300): incomplete block LU in colour
+// order.  Row i of colour c: for colours c2 < c, for neighbours k of colour
+// c2 (row order), a_ij -= a_ik a_kj for every neighbour j of k with colour >
+// c2 that is in row i; then a_ii <- inv(a_ii) and a_ik <- inv(a_ii) a_ik for
+// neighbours of higher colour.  Returns 1 for a singular pivot block, 2 for a
+// block size above 3.
+ORC_API int orc_ilu0_factor(int bs, int64_t nrows, const index_t* ptr, const index_t* index, const double* val,
+                            const index_t* coloring, int ncolors, double* F)
+{
+    if (bs > 3) return 2;
+    const int b2 = bs * bs;
+    const int64_t len = (int64_t)ptr[nrows] * b2;
+    for (int64_t e = 0; e < len; ++e) F[e] = val[e];
+    int failed = 0;
+    for (int color = 0; color < ncolors; ++color) {
+#pragma omp parallel for schedule(static) reduction(| : failed)
+        for (int64_t i = 0; i < nrows; ++i) {
+            if (coloring[i] != color) continue;
+            for (int color2 = 0; color2 < color; ++color2)
+                for (index_t ik = ptr[i]; ik < ptr[i + 1]; ++ik) {
+                    const index_t k = index[ik];
+                    if (coloring[k] != color2) continue;
+                    double Aik[9];
+                    for (int e = 0; e < b2; ++e) Aik[e] = F[(int64_t)ik * b2 + e];
+                    for (index_t kj = ptr[k]; kj < ptr[k + 1]; ++kj) {
+                        const index_t j = index[kj];
+                        if (coloring[j] <= color2) continue;
+                        for (index_t ij = ptr[i]; ij < ptr[i + 1]; ++ij) {
+                            if (index[ij] != j) continue;
+                            const double* S = F + (int64_t)kj * b2;
+                            double* T = F + (int64_t)ij * b2;
+                            for (int c = 0; c < bs; ++c)
+                                for (int r = 0; r < bs; ++r) {
+                                    double s = 0.0;
+                                    for (int m = 0; m < bs; ++m) s += Aik[r + bs * m] * S[m + bs * c];
+                                    T[r + bs * c] -= s;
+                                }
+                            break;
+                        }
+                    }
+                }
+            const index_t mm = main_ptr(ptr, index, i);
+            double Di[9];
+            if (mm < 0 || !block_inverse(bs, F + (int64_t)mm * b2, Di)) {
+                failed |= 1;
+                continue;
+            }
+            for (int e = 0; e < b2; ++e) F[(int64_t)mm * b2 + e] = Di[e];
+            for (index_t ik = ptr[i]; ik < ptr[i + 1]; ++ik) {
+                if (coloring[index[ik]] <= color) continue;
+                double* T = F + (int64_t)ik * b2;
+                double t[9];
+                for (int c = 0; c < bs; ++c)
+                    for (int r = 0; r < bs; ++r) {
+                        double s = 0.0;
+                        for (int m = 0; m < bs; ++m) s += Di[r + bs * m] * T[m + bs * c];
+                        t[r + bs * c] = s;
+                    }
+                for (int e = 0; e < b2; ++e) T[e] = t[e];
+            }
+        }
+    }
+    return failed;
+}
+
+// Solver_solveILU (paso/src/ILU.cpp:316-462): x = b; forward over colours
+// x_i <- inv(a_ii) (x_i - sum_{c(k)<c(i)} a_ik x_k); backward over colours
+// nc-1..0, x_i <- x_i - sum_{c(k)>c(i)} a_ik x_k.
+ORC_API void orc_ilu0_solve(int bs, int64_t nrows, const index_t* ptr, const index_t* index, const double* F,
+                            const index_t* coloring, int ncolors, double* x, const double* b)
+{
+    const int b2 = bs * bs;
+    for (int64_t e = 0; e < nrows * bs; ++e) x[e] = b[e];
+    for (int color = 0; color < ncolors; ++color) {
+#pragma omp parallel for schedule(static)
+        for (int64_t i = 0; i < nrows; ++i) {
+            if (coloring[i] != color) continue;
+            double* xi = x + (int64_t)bs * i;
+            for (index_t k = ptr[i]; k < ptr[i + 1]; ++k)
+                if (coloring[index[k]] < color) block_smv(bs, xi, F + (int64_t)k * b2, x + (int64_t)bs * index[k]);
+            block_mvip(bs, F + (int64_t)main_ptr(ptr, index, i) * b2, xi);
+        }
+    }
+    for (int color = ncolors - 1; color > -1; --color) {
+#pragma omp parallel for schedule(static)
+        for (int64_t i = 0; i < nrows; ++i) {
+            if (coloring[i] != color) continue;
+            double* xi = x + (int64_t)bs * i;
+            for (index_t k = ptr[i]; k < ptr[i + 1]; ++k)
+                if (coloring[index[k]] > color) block_smv(bs, xi, F + (int64_t)k * b2, x + (int64_t)bs * index[k]);
+        }
+    }
+}
+
+// Preconditioner_Smoother_solve with the Gauss-Seidel sweep (x not
+// initial): x = S^-1 b, then (sweeps-1) times x += S^-1 (b - A x)
+// (Smoother.cpp:100-127, :201-216).
+static void gs_apply(int bs, int64_t nrows, const index_t* ptr, const index_t* index, const double* val,
+                     const double* inv, const index_t* coloring, int ncolors, int sweeps, double* x,
+                     const double* b, double* buf)
+{
+    const int64_t n = nrows * bs;
+    for (int64_t i = 0; i < n; ++i) x[i] = b[i];
+    orc_gs_sweep(bs, nrows, ptr, index, val, inv, coloring, ncolors, x);
+    for (int s = 1; s < sweeps; ++s) {
+        for (int64_t i = 0; i < n; ++i) buf[i] = b[i];
+        orc_spmv(bs, nrows, ptr, index, val, -1., x, 1., buf);
+        orc_gs_sweep(bs, nrows, ptr, index, val, inv, coloring, ncolors, buf);
+        for (int64_t i = 0; i < n; ++i) x[i] += buf[i];
+    }
+}
+
+// Preconditioner_Smoother_solve with the coloured Gauss-Seidel sweep, exported
+// for the parity tests of a single preconditioner application.
+ORC_API void orc_gs_apply(int bs, int64_t nrows, const index_t* ptr, const index_t* index, const double* val,
+                          const double* inv, const index_t* coloring, int ncolors, int sweeps, double* x,
+                          const double* b)
+{
+    std::vector<double> buf((size_t)nrows * bs);
+    gs_apply(bs, nrows, ptr, index, val, inv, coloring, ncolors, sweeps, x, b, buf.data());
+}
+
 // ---------------------------------------------------------- reductions
 // PASO accumulates per-thread partial sums under "omp critical", so its sums
 // depend on thread timing.  Here the partials are combined in thread order,
@@ -644,6 +930,7 @@ typedef struct {
     int sweeps;            // Jacobi sweeps, SolverBuddy default 1
     int plain;             // 0: PASO residual-smoothed PCG; 1: textbook PCG; 2: BiCGStab
     int verbose;
+    int preconditioner;    // 0: Jacobi, 1: Gauss-Seidel, 2: ILU0, 3: none (Preconditioner.cpp:46-107)
 } orc_options;
 
 typedef struct {
@@ -665,7 +952,18 @@ struct Sys {
     const double* inv;
     int sweeps;
     std::vector<double> buf;
-    void prec(double* x, const double* b) { jacobi_apply(bs, nrows, ptr, index, val, inv, sweeps, x, b, buf.data()); }
+    int precond = 0;
+    const index_t* coloring = nullptr;
+    int ncolors = 0;
+    const double* F = nullptr;
+    // Preconditioner_solve (paso/src/Preconditioner.cpp:111-137)
+    void prec(double* x, const double* b)
+    {
+        if (precond == 1) gs_apply(bs, nrows, ptr, index, val, inv, coloring, ncolors, sweeps, x, b, buf.data());
+        else if (precond == 2) orc_ilu0_solve(bs, nrows, ptr, index, F, coloring, ncolors, x, b);
+        else if (precond == 3) std::copy(b, b + n, x);
+        else jacobi_apply(bs, nrows, ptr, index, val, inv, sweeps, x, b, buf.data());
+    }
     void mv(double alpha, const double* in, double beta, double* out) { orc_spmv(bs, nrows, ptr, index, val, alpha, in, beta, out); }
 };
 
@@ -881,10 +1179,31 @@ ORC_API int orc_solve(int bs, int64_t nrows, const index_t* ptr, const index_t* 
     if (norm2_of_b <= 0.) {
         std::fill(x, x + n, 0.0);
     } else {
-        std::vector<double> inv((size_t)nrows * bs * bs);
-        if (orc_inv_main(bs, nrows, ptr, index, val, inv.data())) return 101;
+        // Preconditioner_alloc (Preconditioner.cpp:46-107): Jacobi and
+        // Gauss-Seidel invert the main diagonal blocks, ILU0 factorises
+        const int pc = opt->preconditioner;
+        std::vector<double> inv;
+        std::vector<index_t> coloring;
+        std::vector<double> F;
+        int ncolors = 0;
+        if (pc == 0 || pc == 1) {
+            inv.resize((size_t)nrows * bs * bs);
+            if (orc_inv_main(bs, nrows, ptr, index, val, inv.data())) return 101;
+        }
+        if (pc == 1 || pc == 2) {
+            coloring.resize(nrows);
+            ncolors = orc_coloring(nrows, ptr, index, coloring.data());
+        }
+        if (pc == 2) {
+            F.resize((size_t)ptr[nrows] * bs * bs);
+            if (orc_ilu0_factor(bs, nrows, ptr, index, val, coloring.data(), ncolors, F.data())) return 101;
+        }
         Sys S{bs, nrows, n, ptr, index, val, inv.data(), std::max(1, opt->sweeps),
               std::vector<double>(n)};
+        S.precond = pc;
+        S.coloring = coloring.data();
+        S.ncolors = ncolors;
+        S.F = F.data();
         diag->set_up_time = omp_get_wtime() - t0;
         S.prec(x, r.data());
         int totIter = 1;
