@@ -569,6 +569,26 @@ int rg_matrix_solve_preconditioner(rg_matrix* m, const rg_options* opt, double* 
     });
 }
 
+int rg_read_binary_grid(const char* filename, int dim, const int64_t* grid, int num_comp, int points_per_sample,
+                        const int64_t* first, const int64_t* num_values, const int32_t* multiplier,
+                        const int32_t* reverse, int byte_order, int data_type, double* out)
+{
+    return guarded([&] {
+        rg::read_binary_grid(filename, dim, grid, num_comp, points_per_sample, first, num_values, multiplier,
+                             reverse, byte_order, data_type, out);
+        return RG_OK;
+    });
+}
+
+int rg_write_binary_grid(const char* filename, int dim, const int64_t* grid, const double* in, int byte_order,
+                         int data_type)
+{
+    return guarded([&] {
+        rg::write_binary_grid(filename, dim, grid, in, byte_order, data_type);
+        return RG_OK;
+    });
+}
+
 int rg_matrix_coloring(rg_matrix* m, int32_t* colours, int* num_colors)
 {
     return guarded([&] {

@@ -135,9 +135,12 @@ struct rg_matrix {
     rg_lpc* lpc = nullptr;         // learned preconditioner (borrowed)
     const float* lpc_logk = nullptr;   // log10(kappa) per node (device, borrowed)
     rg_colouring* colouring = nullptr; // built on first use (pattern only)
-    double* pval = nullptr;        // colour-ordered full-stencil values or ILU0 factors, nslot*bs*bs*ld
-    double* pdinv = nullptr;       // colour-ordered inverse main-diagonal blocks, bs*bs*ld
-    int pval_state = 0;            // 0: invalid, 1: balanced values (Gauss-Seidel), 2: ILU0 factors
+    double* pval = nullptr;        // colour-ordered full stencil, ILU0 elimination workspace, nslot*bs*bs*ld
+    double* plow = nullptr;        // colour-ordered couplings to lower colours, nslot*bs*bs*ld
+    double* pup = nullptr;         // colour-ordered couplings to higher colours, nslot*bs*bs*ld
+    double* pdiag = nullptr;       // colour-ordered main blocks (Gauss-Seidel), bs*bs*ld
+    double* pdinv = nullptr;       // colour-ordered inverse main blocks / inverted ILU pivots, bs*bs*ld
+    int pval_state = 0;            // 0: invalid, 1: Gauss-Seidel split, 2: ILU0 split
 };
 
 namespace rg {
@@ -169,6 +172,13 @@ void prec_setup(rg_matrix* m, int type, bool rebalanced);
 void prec_apply(rg_matrix* m, int type, int sweeps, const double* b, double* x, hipStream_t st,
                 const KrylovState* guard);
 void solve_preconditioner(rg_matrix* m, const rg_options& opt, double* x, const double* b);
+
+// Binary grid I/O (rg_io.hip, host only).
+void read_binary_grid(const char* filename, int dim, const int64_t* grid, int num_comp, int dpp,
+                      const int64_t* first, const int64_t* num_values, const int32_t* multiplier,
+                      const int32_t* reverse, int byte_order, int data_type, double* out);
+void write_binary_grid(const char* filename, int dim, const int64_t* grid, const double* in, int byte_order,
+                       int data_type);
 
 // Learned preconditioner (rg_learned.hip): node grid of a network and its
 // forward pass on the balanced solver vectors, z = D^-1 Net(D dinv r).

@@ -11,12 +11,16 @@
 //     sweeps (Smoother.cpp:100-127);
 //   * ILU0: Solver_getILU / Solver_solveILU (paso/src/ILU.cpp:43-462).
 //
-// Rows are stored in colour order (pos -> row = perm[pos]) with a
-// structure-of-arrays copy of the full stencil, pval[(t*bs*bs + e)*ld + pos]:
-// the kernel of colour c streams the contiguous range of its rows' values,
-// while vectors stay in the node order of the solver.  Each kernel handles
-// one colour; rows of one colour are never neighbours, so a colour's rows
-// read only values finished by earlier launches.
+// Rows are stored in colour order (pos -> row = perm[pos]) in structure-of-
+// arrays copies of the full stencil split by colour: plow holds the
+// couplings of a row to neighbours of lower colours, pup those to higher
+// colours (each zero elsewhere, also outside the grid), pdiag / pdinv the
+// main block and its inverse, e.g. plow[(t*bs*bs + e)*ld + pos].  A
+// substitution kernel of colour c then streams the contiguous range of its
+// rows and needs no colour test: forward is b - (plow x), backward
+// pdiag x - (pup x), both branch-free stencil products with the neighbour
+// values x gathered in node order.  Rows of one colour are never neighbours,
+// so a colour's rows read only values finished by earlier launches.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -140,15 +144,77 @@ __global__ __launch_bounds__(kBlock) void k_colour_assign(int64_t n, int32_t* __
 }
 
 // ------------------------------------------------------ colour-ordered copies
-// pval(t, pos) = balanced block A(row, row + off_t) for every full slot t
-// (zero outside the grid); with symmetric storage the lower slots are the
-// transposed stored blocks of the neighbour.
+// Block A(row, row + off_t) of full slot t: from the node-ordered stencil
+// values (NODE, with the lower slots of symmetric storage as the transposed
+// stored blocks of the neighbour) or from the colour-ordered full copy.
+template <int DIM, int BS, bool SYM, bool NODE>
+__device__ __forceinline__ void fetch_block(const double* src, int64_t ld, int t, int64_t row, int64_t k,
+                                            int64_t pos, double (&v)[BS * BS])
+{
+    constexpr int C = StencilShape<DIM>::CENTER, B2 = BS * BS;
+    if (!NODE) {
+#pragma unroll
+        for (int e = 0; e < B2; ++e) v[e] = src[(int64_t)(t * B2 + e) * ld + pos];
+    } else if (!SYM || t >= C) {
+        const int ts = SYM ? t - C : t;
+#pragma unroll
+        for (int e = 0; e < B2; ++e) v[e] = src[(int64_t)(ts * B2 + e) * ld + row];
+    } else {
+        const int ts = C - t;
+#pragma unroll
+        for (int r = 0; r < BS; ++r)
+#pragma unroll
+            for (int c = 0; c < BS; ++c) v[r + BS * c] = src[(int64_t)(ts * B2 + c + BS * r) * ld + k];
+    }
+}
+
+// Colour-ordered copies of row pos: plow / pup get the blocks of the
+// neighbours of lower / higher colour (zero otherwise and outside the grid),
+// diag the main block; with NODE the source is the balanced node-ordered
+// matrix (Gauss-Seidel), else the colour-ordered ILU0 factors, whose main
+// slot holds the inverted pivot.
+template <int DIM, int BS, bool SYM, bool NODE>
+__global__ __launch_bounds__(kBlock) void k_split(const double* __restrict__ src, int64_t ld, Grid g,
+                                                  const int32_t* __restrict__ perm,
+                                                  const uint8_t* __restrict__ colour, double* __restrict__ plow,
+                                                  double* __restrict__ pup, double* __restrict__ diag)
+{
+    constexpr int NS = StencilShape<DIM>::NS, C = StencilShape<DIM>::CENTER, B2 = BS * BS;
+    const int64_t pos = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (pos >= g.N) return;
+    const int64_t row = perm[pos];
+    const int ci = colour[row];
+    int x, y, z;
+    node_coords(g, row, x, y, z);
+#pragma unroll
+    for (int t = 0; t < NS; ++t) {
+        int64_t k;
+        const bool ok = neighbour<DIM>(g, t, x, y, z, row, k);
+        double v[B2];
+#pragma unroll
+        for (int e = 0; e < B2; ++e) v[e] = 0.0;
+        if (ok) fetch_block<DIM, BS, SYM, NODE>(src, ld, t, row, k, pos, v);
+        if (t == C) {
+#pragma unroll
+            for (int e = 0; e < B2; ++e) diag[(int64_t)e * ld + pos] = v[e];
+        }
+        const int ck = ok ? (int)colour[k] : ci;
+        const bool lo = ok && t != C && ck < ci, up = ok && t != C && ck > ci;
+#pragma unroll
+        for (int e = 0; e < B2; ++e) {
+            plow[(int64_t)(t * B2 + e) * ld + pos] = lo ? v[e] : 0.0;
+            pup[(int64_t)(t * B2 + e) * ld + pos] = up ? v[e] : 0.0;
+        }
+    }
+}
+
+// full colour-ordered copy (the ILU0 elimination works in place on it)
 template <int DIM, int BS, bool SYM>
 __global__ __launch_bounds__(kBlock) void k_permute_values(const double* __restrict__ val, int64_t ld, Grid g,
                                                            const int32_t* __restrict__ perm,
                                                            double* __restrict__ pval)
 {
-    constexpr int NS = StencilShape<DIM>::NS, C = StencilShape<DIM>::CENTER, B2 = BS * BS;
+    constexpr int NS = StencilShape<DIM>::NS, B2 = BS * BS;
     const int64_t pos = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (pos >= g.N) return;
     const int64_t row = perm[pos];
@@ -161,19 +227,7 @@ __global__ __launch_bounds__(kBlock) void k_permute_values(const double* __restr
         double v[B2];
 #pragma unroll
         for (int e = 0; e < B2; ++e) v[e] = 0.0;
-        if (ok) {
-            if (!SYM || t >= C) {
-                const int ts = SYM ? t - C : t;
-#pragma unroll
-                for (int e = 0; e < B2; ++e) v[e] = val[(int64_t)(ts * B2 + e) * ld + row];
-            } else {
-                const int ts = C - t;
-#pragma unroll
-                for (int r = 0; r < BS; ++r)
-#pragma unroll
-                    for (int c = 0; c < BS; ++c) v[r + BS * c] = val[(int64_t)(ts * B2 + c + BS * r) * ld + k];
-            }
-        }
+        if (ok) fetch_block<DIM, BS, SYM, true>(val, ld, t, row, k, pos, v);
 #pragma unroll
         for (int e = 0; e < B2; ++e) pval[(int64_t)(t * B2 + e) * ld + pos] = v[e];
     }
@@ -260,22 +314,59 @@ struct ColourArgs {
     const int32_t* ipos;
     const uint8_t* colour;
     double* pval;
-    const double* pdinv;
     int64_t ld;
-    const KrylovState* guard;
 };
 
 // ------------------------------------------------------------ sweeps
-// Forward substitution for the rows of colour `color`, positions [p0, p1):
-//   x_i = Dinv_i (b_i - sum_{c(k) < color} A_ik x_k)
-// with Dinv the inverse main-diagonal block (Gauss-Seidel) or the inverted
-// ILU pivot held in the main slot of the factors (ILU).  Smoother.cpp:
-// 331-403, ILU.cpp:333-397.
-template <int DIM, int BS, bool ILU>
-__global__ __launch_bounds__(kBlock) void k_colour_forward(ColourArgs a, int color, int64_t p0, int64_t p1,
-                                                           const double* __restrict__ b, double* __restrict__ x)
+struct SweepArgs {
+    const int32_t* perm;
+    const double* plow;
+    const double* pup;
+    const double* pdiag;
+    const double* pdinv;
+    int64_t ld, N, s1, s2;
+    const KrylovState* guard;
+};
+
+// acc -= sum over the stencil slots t != C of M(t) x[row + off_t], for the
+// split copies: entries outside the grid or of the other colour side are
+// zero, so only indices outside [0, N) need care (clamped and selected away).
+template <int DIM, int BS>
+__device__ __forceinline__ void split_product(const SweepArgs& a, const double* M, int64_t pos, int64_t row,
+                                              const double* __restrict__ x, double (&acc)[BS])
 {
     constexpr int NS = StencilShape<DIM>::NS, C = StencilShape<DIM>::CENTER, B2 = BS * BS;
+#pragma unroll
+    for (int t = 0; t < NS; ++t) {
+        if (t == C) continue;
+        int dx, dy, dz;
+        slot_delta<DIM>(t, dx, dy, dz);
+        int64_t k = row + dx + dy * a.s1 + dz * a.s2;
+        const bool in = (uint64_t)k < (uint64_t)a.N;
+        k = in ? k : row;
+        double xk[BS], mv[B2];
+#pragma unroll
+        for (int c = 0; c < BS; ++c) xk[c] = x[k * BS + c];
+#pragma unroll
+        for (int e = 0; e < B2; ++e) mv[e] = M[(int64_t)(t * B2 + e) * a.ld + pos];
+#pragma unroll
+        for (int r = 0; r < BS; ++r) {
+            double s = 0.0;
+#pragma unroll
+            for (int c = 0; c < BS; ++c) s = fma(mv[r + BS * c], xk[c], s);
+            acc[r] = in ? acc[r] - s : acc[r];
+        }
+    }
+}
+
+// Forward substitution for the rows of colour positions [p0, p1):
+//   x_i = Dinv_i (b_i - sum_{c(k) < c(i)} A_ik x_k)
+// with Dinv the inverse main-diagonal block (Gauss-Seidel) or the inverted
+// ILU pivot (ILU).  Smoother.cpp:331-403, ILU.cpp:333-397.
+template <int DIM, int BS>
+__global__ __launch_bounds__(kBlock) void k_colour_forward(SweepArgs a, int64_t p0, int64_t p1,
+                                                           const double* __restrict__ b, double* __restrict__ x)
+{
     if (a.guard && a.guard->done) return;
     const int64_t pos = p0 + (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (pos >= p1) return;
@@ -283,37 +374,21 @@ __global__ __launch_bounds__(kBlock) void k_colour_forward(ColourArgs a, int col
     double acc[BS];
 #pragma unroll
     for (int q = 0; q < BS; ++q) acc[q] = b[row * BS + q];
-    if (color > 0) {
-        int x0, y0, z0;
-        node_coords(a.g, row, x0, y0, z0);
-#pragma unroll
-        for (int t = 0; t < NS; ++t) {
-            if (t == C) continue;
-            int64_t k;
-            if (!neighbour<DIM>(a.g, t, x0, y0, z0, row, k)) continue;
-            if (a.colour[k] >= color) continue;
-            double xk[BS];
-#pragma unroll
-            for (int c = 0; c < BS; ++c) xk[c] = x[k * BS + c];
-            block_smv<BS>(acc, a.pval + (int64_t)t * B2 * a.ld + pos, a.ld, xk);
-        }
-    }
-    const double* D = ILU ? a.pval + (int64_t)C * B2 * a.ld + pos : a.pdinv + pos;
+    split_product<DIM, BS>(a, a.plow, pos, row, x, acc);
     double out[BS];
-    block_mv<BS>(D, a.ld, acc, out);
+    block_mv<BS>(a.pdinv + pos, a.ld, acc, out);
 #pragma unroll
     for (int q = 0; q < BS; ++q) x[row * BS + q] = out[q];
 }
 
-// Backward substitution for colour `color` (in place):
-//   Gauss-Seidel: x_i = Dinv_i (A_ii x_i - sum_{c(k) > color} A_ik x_k)
-//   ILU:          x_i = x_i - sum_{c(k) > color} U_ik x_k
+// Backward substitution (in place):
+//   Gauss-Seidel: x_i = Dinv_i (A_ii x_i - sum_{c(k) > c(i)} A_ik x_k)
+//   ILU:          x_i = x_i - sum_{c(k) > c(i)} U_ik x_k
 // (Smoother.cpp:405-473, ILU.cpp:399-458).
 template <int DIM, int BS, bool ILU>
-__global__ __launch_bounds__(kBlock) void k_colour_backward(ColourArgs a, int color, int64_t p0, int64_t p1,
+__global__ __launch_bounds__(kBlock) void k_colour_backward(SweepArgs a, int64_t p0, int64_t p1,
                                                             double* __restrict__ x)
 {
-    constexpr int NS = StencilShape<DIM>::NS, C = StencilShape<DIM>::CENTER, B2 = BS * BS;
     if (a.guard && a.guard->done) return;
     const int64_t pos = p0 + (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (pos >= p1) return;
@@ -325,21 +400,9 @@ __global__ __launch_bounds__(kBlock) void k_colour_backward(ColourArgs a, int co
 #pragma unroll
         for (int q = 0; q < BS; ++q) acc[q] = xi[q];
     } else {
-        block_mv<BS>(a.pval + (int64_t)C * B2 * a.ld + pos, a.ld, xi, acc);
+        block_mv<BS>(a.pdiag + pos, a.ld, xi, acc);
     }
-    int x0, y0, z0;
-    node_coords(a.g, row, x0, y0, z0);
-#pragma unroll
-    for (int t = 0; t < NS; ++t) {
-        if (t == C) continue;
-        int64_t k;
-        if (!neighbour<DIM>(a.g, t, x0, y0, z0, row, k)) continue;
-        if (a.colour[k] <= color) continue;
-        double xk[BS];
-#pragma unroll
-        for (int c = 0; c < BS; ++c) xk[c] = x[k * BS + c];
-        block_smv<BS>(acc, a.pval + (int64_t)t * B2 * a.ld + pos, a.ld, xk);
-    }
+    split_product<DIM, BS>(a, a.pup, pos, row, x, acc);
     if (ILU) {
 #pragma unroll
         for (int q = 0; q < BS; ++q) x[row * BS + q] = acc[q];
@@ -457,7 +520,7 @@ int blocks_for(int64_t n)
     return (int)std::max<int64_t>(1, (n + kBlock - 1) / kBlock);
 }
 
-ColourArgs colour_args(rg_matrix* m, const KrylovState* guard)
+ColourArgs colour_args(rg_matrix* m)
 {
     const rg_colouring* c = m->colouring;
     ColourArgs a;
@@ -466,8 +529,22 @@ ColourArgs colour_args(rg_matrix* m, const KrylovState* guard)
     a.ipos = c->ipos;
     a.colour = c->colour;
     a.pval = m->pval;
+    a.ld = m->g.ld;
+    return a;
+}
+
+SweepArgs sweep_args(rg_matrix* m, const KrylovState* guard)
+{
+    SweepArgs a;
+    a.perm = m->colouring->perm;
+    a.plow = m->plow;
+    a.pup = m->pup;
+    a.pdiag = m->pdiag;
     a.pdinv = m->pdinv;
     a.ld = m->g.ld;
+    a.N = m->g.N;
+    a.s1 = m->g.s1;
+    a.s2 = m->g.s2;
     a.guard = guard;
     return a;
 }
@@ -479,20 +556,20 @@ template <bool ILU>
 void colour_sweep(rg_matrix* m, const double* b, double* x, hipStream_t st, const KrylovState* guard)
 {
     const rg_colouring* c = m->colouring;
-    const ColourArgs a = colour_args(m, guard);
+    const SweepArgs a = sweep_args(m, guard);
     dispatch(m->g, false, [&](auto D, auto B, auto) {
         if constexpr (!ILU || B.value <= 3) {
             for (int col = 0; col < c->ncolors; ++col) {
                 const int64_t p0 = c->cstart[col], p1 = c->cstart[col + 1];
                 if (p1 > p0)
-                    hipLaunchKernelGGL((k_colour_forward<D.value, B.value, ILU>), dim3(blocks_for(p1 - p0)),
-                                       dim3(kBlock), 0, st, a, col, p0, p1, b, x);
+                    hipLaunchKernelGGL((k_colour_forward<D.value, B.value>), dim3(blocks_for(p1 - p0)),
+                                       dim3(kBlock), 0, st, a, p0, p1, b, x);
             }
             for (int col = c->ncolors - 2; col >= 0; --col) {
                 const int64_t p0 = c->cstart[col], p1 = c->cstart[col + 1];
                 if (p1 > p0)
                     hipLaunchKernelGGL((k_colour_backward<D.value, B.value, ILU>), dim3(blocks_for(p1 - p0)),
-                                       dim3(kBlock), 0, st, a, col, p0, p1, x);
+                                       dim3(kBlock), 0, st, a, p0, p1, x);
             }
         }
     });
@@ -603,10 +680,10 @@ void free_colouring(rg_matrix* m)
         delete m->colouring;
         m->colouring = nullptr;
     }
-    if (m->pval) hipFree(m->pval);
-    if (m->pdinv) hipFree(m->pdinv);
-    m->pval = nullptr;
-    m->pdinv = nullptr;
+    double* bufs[] = {m->pval, m->plow, m->pup, m->pdiag, m->pdinv};
+    for (double* b : bufs)
+        if (b) hipFree(b);
+    m->pval = m->plow = m->pup = m->pdiag = m->pdinv = nullptr;
     m->pval_state = 0;
 }
 
@@ -626,27 +703,38 @@ void prec_setup(rg_matrix* m, int type, bool rebalanced)
     ensure_colouring(m);
     hipStream_t st = m->ctx->stream;
     const int B2 = g.bs * g.bs;
-    if (!m->pval) RG_HIP(hipMalloc(&m->pval, sizeof(double) * (size_t)g.nslot * B2 * g.ld));
+    const size_t full = sizeof(double) * (size_t)g.nslot * B2 * g.ld, blk = sizeof(double) * (size_t)B2 * g.ld;
+    if (!m->plow) RG_HIP(hipMalloc(&m->plow, full));
+    if (!m->pup) RG_HIP(hipMalloc(&m->pup, full));
+    if (!m->pdiag) RG_HIP(hipMalloc(&m->pdiag, blk));
+    if (!m->pdinv) RG_HIP(hipMalloc(&m->pdinv, blk));
     m->pval_state = 0;
     const Grid gr = grid_of(g);
     const int nb = blocks_for(g.N);
-    dispatch(g, m->symmetric, [&](auto D, auto B, auto S) {
-        hipLaunchKernelGGL((k_permute_values<D.value, B.value, S.value>), dim3(nb), dim3(kBlock), 0, st, m->val,
-                           g.ld, gr, m->colouring->perm, m->pval);
-    });
-    RG_HIP(hipGetLastError());
+    const rg_colouring* c = m->colouring;
     if (want == 1) {
-        if (!m->pdinv) RG_HIP(hipMalloc(&m->pdinv, sizeof(double) * (size_t)B2 * g.ld));
+        // Gauss-Seidel: split the balanced values, inverse diagonal from invMain
+        dispatch(g, m->symmetric, [&](auto D, auto B, auto S) {
+            hipLaunchKernelGGL((k_split<D.value, B.value, S.value, true>), dim3(nb), dim3(kBlock), 0, st, m->val,
+                               g.ld, gr, c->perm, c->colour, m->plow, m->pup, m->pdiag);
+        });
         dispatch_bs(g.bs, [&](auto B) {
-            hipLaunchKernelGGL((k_permute_diag<B.value>), dim3(nb), dim3(kBlock), 0, st, m->dinv, g.ld, g.N,
-                               m->colouring->perm, m->pdinv);
+            hipLaunchKernelGGL((k_permute_diag<B.value>), dim3(nb), dim3(kBlock), 0, st, m->dinv, g.ld, g.N, c->perm,
+                               m->pdinv);
         });
         RG_HIP(hipGetLastError());
     } else {
+        // ILU0: eliminate in a full colour-ordered copy, then split the
+        // factors (the main slot holds the inverted pivot)
+        if (!m->pval) RG_HIP(hipMalloc(&m->pval, full));
+        dispatch(g, m->symmetric, [&](auto D, auto B, auto S) {
+            hipLaunchKernelGGL((k_permute_values<D.value, B.value, S.value>), dim3(nb), dim3(kBlock), 0, st, m->val,
+                               g.ld, gr, c->perm, m->pval);
+        });
+        RG_HIP(hipGetLastError());
         int* flag = m->work->flag;
         RG_HIP(hipMemsetAsync(flag, 0, sizeof(int), st));
-        const ColourArgs a = colour_args(m, nullptr);
-        const rg_colouring* c = m->colouring;
+        const ColourArgs a = colour_args(m);
         dispatch(g, false, [&](auto D, auto B, auto) {
             if constexpr (B.value <= 3) {
                 for (int col = 0; col < c->ncolors; ++col) {
@@ -655,6 +743,8 @@ void prec_setup(rg_matrix* m, int type, bool rebalanced)
                         hipLaunchKernelGGL((k_ilu0_colour<D.value, B.value>), dim3(blocks_for(p1 - p0)),
                                            dim3(kBlock), 0, st, a, col, p0, p1, flag);
                 }
+                hipLaunchKernelGGL((k_split<D.value, B.value, false, false>), dim3(nb), dim3(kBlock), 0, st, m->pval,
+                                   g.ld, gr, c->perm, c->colour, m->plow, m->pup, m->pdinv);
             }
         });
         RG_HIP(hipGetLastError());

@@ -11,7 +11,13 @@ from ._lib import PasoException, RipleyHipError, LIB_PATH, load as load_library
 from .core import Coefficient, Context, SolverOptions, SystemMatrix, get_context
 from .domain import Brick, Rectangle
 from .linearpde import IsotropicCoefficient, LameEquation, LinearPDE, Poisson, kronecker_scaled
+from .gridio import (BYTEORDER_BIG_ENDIAN, BYTEORDER_LITTLE_ENDIAN, BYTEORDER_NATIVE, DATATYPE_FLOAT32,
+                     DATATYPE_FLOAT64, DATATYPE_INT32, ContinuousFunction, Function, FunctionSpace,
+                     ReducedFunction, Solution, readBinaryGrid, readBinaryGridFromZipped)
 
 __all__ = ["Brick", "Rectangle", "LinearPDE", "Poisson", "LameEquation", "SolverOptions", "SystemMatrix",
            "Coefficient", "Context", "get_context", "kronecker_scaled", "IsotropicCoefficient",
-           "PasoException", "RipleyHipError", "LIB_PATH", "load_library"]
+           "PasoException", "RipleyHipError", "LIB_PATH", "load_library", "FunctionSpace", "Solution",
+           "ContinuousFunction", "Function", "ReducedFunction", "readBinaryGrid", "readBinaryGridFromZipped",
+           "BYTEORDER_NATIVE", "BYTEORDER_LITTLE_ENDIAN", "BYTEORDER_BIG_ENDIAN", "DATATYPE_INT32",
+           "DATATYPE_FLOAT32", "DATATYPE_FLOAT64"]

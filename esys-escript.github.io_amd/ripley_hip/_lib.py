@@ -25,6 +25,9 @@ RG_COEF_EMPTY, RG_COEF_ELEMENT, RG_COEF_EXPANDED = 0, 1, 2
 RG_METHOD_PCG, RG_METHOD_PCG_PLAIN, RG_METHOD_PCG_MIXED, RG_METHOD_FCG_LEARNED, RG_METHOD_BICGSTAB = 0, 1, 2, 3, 4
 RG_CRITERION_PASO, RG_CRITERION_TRUE_RESIDUAL = 0, 1
 RG_PRECONDITIONER_JACOBI, RG_PRECONDITIONER_GAUSS_SEIDEL, RG_PRECONDITIONER_ILU0, RG_PRECONDITIONER_NONE = 0, 1, 2, 3
+RG_BYTEORDER_LITTLE_ENDIAN, RG_BYTEORDER_BIG_ENDIAN = 1234, 4321
+RG_BYTEORDER_NATIVE = RG_BYTEORDER_LITTLE_ENDIAN
+RG_DATATYPE_INT32, RG_DATATYPE_FLOAT32, RG_DATATYPE_FLOAT64 = 1, 2, 3
 
 # every function the header declares (checked by the CPU test suite)
 EXPORTS = [
@@ -40,7 +43,7 @@ EXPORTS = [
     "rg_solve_multi", "rg_matrix_vector_multi", "rg_multi_autotune", "rg_multi_set_shape",
     "rg_batch_create", "rg_batch_destroy", "rg_matrix_assemble_boundary", "rg_matrix_add_dirac",
     "rg_batch_offsets", "rg_batch_vector", "rg_batch_set_tile", "rg_batch_autotune", "rg_batch_solve",
-    "rg_matrix_solve_preconditioner", "rg_matrix_coloring",
+    "rg_matrix_solve_preconditioner", "rg_matrix_coloring", "rg_read_binary_grid", "rg_write_binary_grid",
 ]
 
 
@@ -118,6 +121,8 @@ def load():
         "rg_solve_host": (I, [P, P, P, ctypes.POINTER(RgOptions), ctypes.POINTER(RgDiagnostics)]),
         "rg_matrix_solve_preconditioner": (I, [P, ctypes.POINTER(RgOptions), P, P]),
         "rg_matrix_coloring": (I, [P, P, ctypes.POINTER(ctypes.c_int)]),
+        "rg_read_binary_grid": (I, [ctypes.c_char_p, I, P, I, I, P, P, P, P, I, I, P]),
+        "rg_write_binary_grid": (I, [ctypes.c_char_p, I, P, P, I, I]),
         "rg_lpc_num_params": (I64, []),
         "rg_lpc_create": (I, [P, P, I, PP]),
         "rg_lpc_destroy": (I, [P]),

@@ -149,6 +149,11 @@ class _Grid:
         x = self.getX()[:, axis]
         return (torch.abs(x - value) <= tol * max(self._len)).to(torch.float64)
 
+    # --- binary grid I/O (RipleyDomain::writeBinaryGrid) ------------------
+    def writeBinaryGrid(self, data, filename, byteOrder, dataType):
+        from .gridio import writeBinaryGrid
+        writeBinaryGrid(self, data, filename, byteOrder, dataType)
+
     # --- operator factory (RipleyDomain::newSystemMatrix) ---------------
     def newOperator(self, block_size=1, symmetric=False):
         return SystemMatrix(self, block_size=block_size, symmetric=symmetric, device=self._dev_index)

@@ -266,6 +266,33 @@ int rg_solve(rg_matrix* m, double* x, const double* b, const rg_options* opt,
 int rg_solve_host(rg_matrix* m, double* x, const double* b, const rg_options* opt,
                   rg_diagnostics* diag);
 
+/* ------------------------------------------------------- binary grid I/O
+ * ripley's readBinaryGrid / writeBinaryGrid at one rank (host memory, no GPU
+ * needed).  Byte orders and data types as ripley/src/system_dep.h:44-70
+ * (the native order of the hosts this library runs on is little endian). */
+enum { RG_BYTEORDER_LITTLE_ENDIAN = 1234, RG_BYTEORDER_BIG_ENDIAN = 4321,
+       RG_BYTEORDER_NATIVE = RG_BYTEORDER_LITTLE_ENDIAN };
+enum { RG_DATATYPE_INT32 = 1, RG_DATATYPE_FLOAT32 = 2, RG_DATATYPE_FLOAT64 = 3 };
+
+/* readBinaryGrid (ripley/src/ripleycpp.cpp:63-86, Rectangle.cpp:572-690,
+ * Brick.cpp:669-818): grid = samples per axis of the target function space
+ * (nodes, or elements for Function / ReducedFunction); out = host array of
+ * grid[0]*grid[1](*grid[2]) samples x points_per_sample x num_comp, already
+ * holding the fill value.  num_values[d] values per axis are read from the
+ * file (x fastest, num_comp components per value); value v lands on
+ * multiplier[d] consecutive samples per axis from first[d] on; reverse is
+ * supported for the z axis of a Brick only (as in the reference).  NaN values
+ * leave the sample unchanged. */
+int rg_read_binary_grid(const char* filename, int dim, const int64_t* grid, int num_comp, int points_per_sample,
+                        const int64_t* first, const int64_t* num_values, const int32_t* multiplier,
+                        const int32_t* reverse, int byte_order, int data_type, double* out);
+
+/* writeBinaryGrid (Brick.cpp:964-1050 and Rectangle's): one scalar value per
+ * sample, x fastest, converted to data_type (C conversion; truncation for
+ * INT32) in byte_order. */
+int rg_write_binary_grid(const char* filename, int dim, const int64_t* grid, const double* in, int byte_order,
+                         int data_type);
+
 /* SystemMatrix::setPreconditioner + solvePreconditioner
  * (paso/src/SystemMatrix.cpp:42-56): balances the matrix if it is not yet
  * (as a solve would), builds opt->preconditioner with opt->sweeps if needed
