@@ -96,7 +96,7 @@ class Problem:
         o.setTrueResidualCriterion(True)
         o.setSolverMethod({"paso": R.SolverOptions.PCG, "plain": R.SolverOptions.PCG_PLAIN,
                            "mixed": R.SolverOptions.PCG_MIXED, "learned": R.SolverOptions.FCG_LEARNED,
-                           "plain_sym": R.SolverOptions.PCG_PLAIN,
+                           "plain_sym": R.SolverOptions.PCG_PLAIN, "mixed_sym": R.SolverOptions.PCG_MIXED,
                            "paso_sym": R.SolverOptions.PCG, "paso_gs": R.SolverOptions.PCG,
                            "paso_ilu0": R.SolverOptions.PCG}[method])
         if method == "paso_gs":
@@ -160,11 +160,12 @@ def breakdown(P, method, torch):
             "num_iter": o.getDiagnostics("num_iter"), "outer_steps": o.getDiagnostics("outer_steps")}
 
 
-def spmv_roofline(P, reps, torch):
+def spmv_roofline(P, reps, torch, sym=False):
     """Average duration of the FP64 stencil SpMV (MatrixVector, one kernel per
     call) over `reps` back-to-back launches, timed with HIP events on the
-    stream the library launches on."""
-    P.op = P.ops["full"]
+    stream the library launches on.  With sym the operator keeps the main and
+    upper slots only, and the algorithmic bytes count those."""
+    P.op = P.ops["sym" if sym else "full"]
     P.assemble()
     P.constrain()
     N = P.op.num_rows
@@ -181,7 +182,8 @@ def spmv_roofline(P, reps, torch):
     torch.cuda.synchronize()
     dt = s.elapsed_time(e) / 1000.0 / reps
     nnz = P.op.nnz
-    alg_bytes = 8 * nnz + 8 * N + 8 * N          # values + x + y (SpMV algorithmic bytes)
+    stored = (nnz + N) // 2 if sym else nnz     # main + upper entries with symmetric storage
+    alg_bytes = 8 * stored + 8 * N + 8 * N      # values + x + y (SpMV algorithmic bytes)
     return dt, alg_bytes
 
 
@@ -362,8 +364,8 @@ def bench_c3(R, torch, np, args):
     cY = R.Coefficient(torch.tensor([0.0, 0.0, -2700.0 * 9.81], dtype=torch.float64), (3,), E, Q, dom.device)
     q = dom.whereZero(2).repeat_interleave(3)
 
-    def system():
-        op = dom.newOperator(3)
+    def system(sym=False):
+        op = dom.newOperator(3, symmetric=sym)
         rhs = torch.zeros(op.size, dtype=torch.float64, device="cuda")
         op.assemble({"A": cA, "Y": cY}, rhs)
         op.nullifyRowsAndCols(q, q, 1.0)
@@ -390,13 +392,26 @@ def bench_c3(R, torch, np, args):
     out["spmv_fp64"] = {"avg_launch_s": dt, "alg_bytes": alg, "GBps": alg / dt / 1e9,
                         "frac_of_8TBps": alg / dt / 1e9 / HBM_PEAK_GBPS}
     del op
-    for method in ("PCG", "PCG_MIXED"):
-        op, rhs = system()
+    # symmetric storage (main + upper 13 blocks): the transposed lower blocks
+    # are re-read from the neighbour rows
+    op, _ = system(True)
+    for _ in range(3):
+        op.MatrixVector(1.0, x, 0.0, y)
+    s.record()
+    for _ in range(20):
+        op.MatrixVector(1.0, x, 0.0, y)
+    e.record()
+    torch.cuda.synchronize()
+    out["spmv_fp64_sym"] = {"avg_launch_s": s.elapsed_time(e) / 1e3 / 20}
+    del op
+    for method, sym in (("PCG", False), ("PCG_MIXED", False), ("PCG", True), ("PCG_MIXED", True)):
+        op, rhs = system(sym)
         o = _opts(R, method)
         t, x = _timed(torch, lambda: op.solve(rhs, o))
         op2, _ = system()
-        out[method] = {"solve_s": t, "num_iter": o.getDiagnostics("num_iter"),
-                       "converged": o.hasConverged(), "rel_residual_fp64": _rel_res(torch, op2, x, rhs)}
+        out[method + ("_sym" if sym else "")] = {"solve_s": t, "num_iter": o.getDiagnostics("num_iter"),
+                                                 "converged": o.hasConverged(),
+                                                 "rel_residual_fp64": _rel_res(torch, op2, x, rhs)}
         del op, op2
     if not args.no_cpu:
         # the oracle's PASO path on the host (FP64 CSR, 3x3 blocks): assembly
@@ -541,8 +556,8 @@ def bench_c5(R, torch, np, args):
     cY = R.Coefficient(1.0, (), E, Q, dom.device)
     q = dom.whereZero(2)
 
-    def system():
-        op = dom.newOperator(1)
+    def system(sym=False):
+        op = dom.newOperator(1, symmetric=sym)
         rhs = torch.zeros(op.size, dtype=torch.float64, device="cuda")
         op.assemble({"A": cA, "Y": cY}, rhs)
         op.nullifyRowsAndCols(q, q, 1.0)
@@ -552,10 +567,11 @@ def bench_c5(R, torch, np, args):
     out = {"workload": "ripley Brick 192^3 Poisson, ln kappa GRF (sigma 2, corr. 12 cells), Y=1, u=0 on z=0, "
                        "rtol 1e-8", "unknowns": dom.getNumNodes()}
     weights = os.path.join(ROOT, "esys-escript.github.io_amd", "ripley_hip", "weights", "lpc_lognormal_s2_l12.npy")
-    methods = ["PCG_PLAIN", "PCG_MIXED"] + (["FCG_LEARNED"] if os.path.exists(weights) else [])
+    methods = ["PCG_PLAIN", "PCG_MIXED", "PCG_PLAIN_sym"] + (["FCG_LEARNED"] if os.path.exists(weights) else [])
     net = None
-    for method in methods:
-        op, rhs = system()
+    for name in methods:
+        method = name.replace("_sym", "")
+        op, rhs = system(name.endswith("_sym"))
         o = _opts(R, method)
         if method == "FCG_LEARNED":
             net = learned.LearnedPreconditioner(tuple(n + 1 for n in ne), params=np.load(weights, allow_pickle=False))
@@ -563,8 +579,8 @@ def bench_c5(R, torch, np, args):
             o.setCheckInterval(8)
         t, x = _timed(torch, lambda: op.solve(rhs, o))
         op2, _ = system()
-        out[method] = {"solve_s": t, "num_iter": o.getDiagnostics("num_iter"), "converged": o.hasConverged(),
-                       "fallbacks": o.getDiagnostics("fallbacks"), "rel_residual_fp64": _rel_res(torch, op2, x, rhs)}
+        out[name] = {"solve_s": t, "num_iter": o.getDiagnostics("num_iter"), "converged": o.hasConverged(),
+                     "fallbacks": o.getDiagnostics("fallbacks"), "rel_residual_fp64": _rel_res(torch, op2, x, rhs)}
         del op, op2
     if "FCG_LEARNED" in out:
         out["learned_speedup_vs_plain"] = out["PCG_PLAIN"]["solve_s"] / out["FCG_LEARNED"]["solve_s"]
@@ -587,7 +603,8 @@ def main():
         return
     log(args, "building C2 problem ne=%d" % args.ne)
     P = Problem(args.ne, R, torch, np, fields)
-    methods = (["plain", "paso", "mixed", "plain_sym", "paso_sym", "paso_gs", "paso_ilu0"] if args.method == "all"
+    methods = (["plain", "paso", "mixed", "plain_sym", "paso_sym", "mixed_sym", "paso_gs", "paso_ilu0"]
+               if args.method == "all"
                else [args.method])
     weights = os.path.join(ROOT, "esys-escript.github.io_amd", "ripley_hip", "weights", "lpc_log10grf_s1_l8.npy")
     if args.method == "all" and os.path.exists(weights):
@@ -607,6 +624,7 @@ def main():
     ok = {m: r for m, r in results.items() if r["converged"] and r["rel_residual_fp64"] <= 1e-8 * 1.01}
     head = min(ok, key=lambda m: ok[m]["step_s"]) if ok else methods[0]
     dt, alg = spmv_roofline(P, args.spmv_reps, torch)
+    dts, algs = spmv_roofline(P, args.spmv_reps, torch, sym=True)
     achieved = alg / dt / 1e9
     traffic = None
     tfile = os.path.join(ROOT, "profiles", "spmv_fp64_brick128_traffic.json")
@@ -628,7 +646,7 @@ def main():
         "metric": "time to rel. residual 1e-8 (s), Brick 128^3: plain PCG vs learned-PC FCG",
         "value": round(hr["step_s"], 6), "unit": "s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(hr["step_s"] * 1000, 3), "higher_is_better": False,
-        "scaling": "strong", "vs_baseline": None, "dtype": "f64" if head != "mixed" else "f32-values/f64",
+        "scaling": "strong", "vs_baseline": None, "dtype": "f64" if not head.startswith("mixed") else "f32-values/f64",
         "data": "synthetic (seeded Gaussian random field, seed 20261202)",
         "config": {"workload": "ripley Brick %d^3 Poisson, log10 kappa GRF (sigma 1, corr. 8 cells), Y=1, "
                                "u=0 on z=0, rtol 1e-8" % args.ne,
@@ -639,6 +657,10 @@ def main():
                                          if "plain" in results and "learned" in results else None),
         "dof_iter_per_s": round(N * hr["num_iter"] / hr["solve_s"], 1),
         "roofline": roof,
+        "roofline_symmetric_storage": {
+            "kernel": "k_spmv<3,1,true,double,0> (main + 13 upper slots; the lower ones re-read transposed)",
+            "achieved": round(algs / dts / 1e9, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+            "frac": round(algs / dts / 1e9 / HBM_PEAK_GBPS, 4), "alg_bytes_per_launch": algs, "avg_launch_s": dts},
         "device_copy_GBps": round(copy_bw, 1),
         "cpu_baseline": cpu,
     }
