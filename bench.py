@@ -622,7 +622,12 @@ def main():
                           fallbacks=opts.getDiagnostics("fallbacks"))
         log(args, m, json.dumps(results[m]))
     ok = {m: r for m, r in results.items() if r["converged"] and r["rel_residual_fp64"] <= 1e-8 * 1.01}
-    head = min(ok, key=lambda m: ok[m]["step_s"]) if ok else methods[0]
+    # the headline value is the fastest method computing in FP64 throughout
+    # (the reference's arithmetic); the FP32-value and learned methods are
+    # reported beside it
+    fp64 = {m: r for m, r in ok.items() if not m.startswith(("mixed", "learned"))}
+    head = min(fp64, key=lambda m: fp64[m]["step_s"]) if fp64 else methods[0]
+    fastest = min(ok, key=lambda m: ok[m]["step_s"]) if ok else head
     dt, alg = spmv_roofline(P, args.spmv_reps, torch)
     dts, algs = spmv_roofline(P, args.spmv_reps, torch, sym=True)
     achieved = alg / dt / 1e9
@@ -646,13 +651,14 @@ def main():
         "metric": "time to rel. residual 1e-8 (s), Brick 128^3: plain PCG vs learned-PC FCG",
         "value": round(hr["step_s"], 6), "unit": "s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(hr["step_s"] * 1000, 3), "higher_is_better": False,
-        "scaling": "strong", "vs_baseline": None, "dtype": "f64" if not head.startswith("mixed") else "f32-values/f64",
+        "scaling": "strong", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic (seeded Gaussian random field, seed 20261202)",
         "config": {"workload": "ripley Brick %d^3 Poisson, log10 kappa GRF (sigma 1, corr. 8 cells), Y=1, "
                                "u=0 on z=0, rtol 1e-8" % args.ne,
                    "unknowns": N, "nnz": P.op.nnz, "method": head},
         "methods": results,
         "speedup_vs_plain": round(results["plain"]["step_s"] / hr["step_s"], 3) if "plain" in results else None,
+        "fastest_any_precision": {"method": fastest, "step_s": round(results[fastest]["step_s"], 6)},
         "learned_fcg_speedup_vs_plain": (round(results["plain"]["step_s"] / results["learned"]["step_s"], 3)
                                          if "plain" in results and "learned" in results else None),
         "dof_iter_per_s": round(N * hr["num_iter"] / hr["solve_s"], 1),

@@ -41,8 +41,9 @@ __device__ __forceinline__ bool slot_in_grid(int s, int64_t x, int64_t y, int64_
 // Between the z-groups of a row group's loads: with ON the loads of one
 // z-plane group are not hoisted above the multiplies of the previous one
 // (fewer live registers, higher occupancy).  Measured on the 128^3 Brick:
-// 2% faster for the full-storage vectorised path, 9% slower for the
-// symmetric one, so only the former sets it.
+// 2% faster for the full-storage vectorised path; for the symmetric path
+// neutral in FP64 and, with FP32 values, 256 -> ~195 VGPRs (1 -> 2 waves per
+// SIMD: the FP32-SpMV PCG 0.240 -> 0.170 s), so both paths set it.
 template <bool ON = false>
 __device__ __forceinline__ void zgroup_fence()
 {
@@ -83,7 +84,12 @@ __device__ __forceinline__ void stencil_row(const VT* __restrict__ val, int64_t 
 {
     constexpr int NS = StencilShape<DIM>::NS;
     constexpr int C = StencilShape<DIM>::CENTER;
-    constexpr int G = 9;                       // slots per z-plane group
+    // slots per group of loads: a z-plane, or one (dy, dz) line of three
+    // blocks for symmetric storage with block size > 1 (its transposed
+    // loads otherwise need 256 VGPRs, one wave per SIMD; 111 and four waves
+    // this way, while full storage is best left with z-plane groups)
+    constexpr bool LINE = SYM && BS > 1;
+    constexpr int G = LINE ? 3 : 9;
 #pragma unroll
     for (int g0 = 0; g0 < NS; g0 += G) {
         double xv[G][BS];
@@ -124,7 +130,7 @@ __device__ __forceinline__ void stencil_row(const VT* __restrict__ val, int64_t 
                     const double t = fma(av[q][r + BS * c], xv[q][c], acc[r]);
                     acc[r] = (!CHECK || ok[q]) ? t : acc[r];
                 }
-        zgroup_fence();
+        zgroup_fence<LINE>();
     }
 }
 
@@ -285,7 +291,7 @@ __device__ __forceinline__ void stencil_rows_s1_sym(const VT* __restrict__ val, 
         for (int dx = 0; dx < 3; ++dx)
 #pragma unroll
             for (int r = 0; r < R; ++r) acc[r] = fma(av[dx][r], w[r + dx], acc[r]);
-        if (pl % 3 == 2) zgroup_fence();
+        if (pl % 3 == 2) zgroup_fence<true>();
     }
 }
 
