@@ -168,7 +168,7 @@ void launch_to_fp32(rg_matrix* m);
 // skipping all work once guard->done is set (guard may be null).
 void ensure_colouring(rg_matrix* m);
 void free_colouring(rg_matrix* m);
-void prec_setup(rg_matrix* m, int type, bool rebalanced);
+void prec_setup(rg_matrix* m, int type);
 void prec_apply(rg_matrix* m, int type, int sweeps, const double* b, double* x, hipStream_t st,
                 const KrylovState* guard);
 void solve_preconditioner(rg_matrix* m, const rg_options& opt, double* x, const double* b);

@@ -692,14 +692,14 @@ void free_colouring(rg_matrix* m)
 // values and the inverse diagonal kept with the solver data, so its copy is
 // refreshed whenever the matrix was re-balanced; ILU0 factors are solver
 // data and are kept until the values are reset without preserving it.
-void prec_setup(rg_matrix* m, int type, bool rebalanced)
+void prec_setup(rg_matrix* m, int type)
 {
     if (type != RG_PRECONDITIONER_GAUSS_SEIDEL && type != RG_PRECONDITIONER_ILU0) return;
     const Geom& g = m->g;
     RG_CHECK(type != RG_PRECONDITIONER_ILU0 || g.bs <= 3, RG_VALUE_ERROR,
              "Solver_getILU: block size greater than 3 is not supported.");
     const int want = (type == RG_PRECONDITIONER_GAUSS_SEIDEL) ? 1 : 2;
-    if (m->pval_state == want && !(want == 1 && rebalanced)) return;
+    if (m->pval_state == want) return;   // reset when the values are re-balanced or the solver data dropped
     ensure_colouring(m);
     hipStream_t st = m->ctx->stream;
     const int B2 = g.bs * g.bs;

@@ -1250,7 +1250,7 @@ void prepare_solve(rg_matrix* m, int prec)
         RG_CHECK(!failed, RG_VALUE_ERROR, "SparseMatrix::invMain: non-regular main diagonal block.");
         m->have_prec = true;
     }
-    prec_setup(m, prec, false);
+    prec_setup(m, prec);
 }
 
 // paso::Solver (paso/src/Solver.cpp:46-331) on device vectors.
