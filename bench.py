@@ -633,8 +633,11 @@ def main():
     achieved = alg / dt / 1e9
     traffic = None
     tfile = os.path.join(ROOT, "profiles", "spmv_fp64_brick128_traffic.json")
+    traffic_sym = None
     if os.path.exists(tfile):
-        traffic = json.load(open(tfile)).get("hbm_bytes_per_launch")
+        tj = json.load(open(tfile))
+        traffic = tj.get("hbm_bytes_per_launch")
+        traffic_sym = tj.get("symmetric", {}).get("hbm_bytes_per_launch")
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": traffic,
             "kernel": "k_spmv<3,1,false,double,0> (FP64 27-point stencil SpMV, y = A x)",
@@ -666,7 +669,8 @@ def main():
         "roofline_symmetric_storage": {
             "kernel": "k_spmv<3,1,true,double,0> (main + 13 upper slots; the lower ones re-read transposed)",
             "achieved": round(algs / dts / 1e9, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-            "frac": round(algs / dts / 1e9 / HBM_PEAK_GBPS, 4), "alg_bytes_per_launch": algs, "avg_launch_s": dts},
+            "frac": round(algs / dts / 1e9 / HBM_PEAK_GBPS, 4), "traffic": traffic_sym,
+            "alg_bytes_per_launch": algs, "avg_launch_s": dts},
         "device_copy_GBps": round(copy_bw, 1),
         "cpu_baseline": cpu,
     }
