@@ -273,10 +273,10 @@ def inv_main(bs, ptr, index, val):
 def solve(bs, ptr, index, val, b, tolerance=1e-8, iter_max=100000, sweeps=1, plain=False,
           verbose=False, method=None, preconditioner="JACOBI"):
     """PASO Solver() on a copy of val; returns (x, diagnostics dict, balanced val, d).
-    method: "PCG" (PASO's, default), "PCG_PLAIN" (plain=True) or "BICGSTAB";
+    method: "PCG" (PASO's, default), "PCG_PLAIN" (plain=True), "BICGSTAB" or "MINRES";
     preconditioner: "JACOBI", "GAUSS_SEIDEL", "ILU0" or "NONE"."""
     if method is not None:
-        plain = {"PCG": 0, "PCG_PLAIN": 1, "BICGSTAB": 2}[method]
+        plain = {"PCG": 0, "PCG_PLAIN": 1, "BICGSTAB": 2, "MINRES": 3}[method]
     val = np.array(val, dtype=np.float64, copy=True)
     b = np.ascontiguousarray(b, dtype=np.float64)
     N = len(ptr) - 1

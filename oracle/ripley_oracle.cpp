@@ -928,7 +928,7 @@ typedef struct {
     double tolerance;      // SolverBuddy default 1e-8 (SolverOptions.cpp:40-72)
     int iter_max;          // default 100000
     int sweeps;            // Jacobi sweeps, SolverBuddy default 1
-    int plain;             // 0: PASO residual-smoothed PCG; 1: textbook PCG; 2: BiCGStab
+    int plain;             // 0: PASO residual-smoothed PCG; 1: textbook PCG; 2: BiCGStab; 3: MINRES
     int verbose;
     int preconditioner;    // 0: Jacobi, 1: Gauss-Seidel, 2: ILU0, 3: none (Preconditioner.cpp:46-107)
 } orc_options;
@@ -1107,6 +1107,95 @@ static int solver_bicgstab(Sys& S, double* r, double* x, int* iter, double* tole
     return NoError;
 }
 
+// Solver_MINRES (paso/src/MINRES.cpp:59-191): preconditioned MINRES by the
+// Lanczos recurrence r_new = A z - (delta/gamma) r - (gamma/gamma_old) r_old
+// with z = M^-1 r / gamma, Givens rotations of the tridiagonal matrix, and
+// x += c eta w.  Stops when the preconditioned residual estimate
+// rnorm_prec <= tol * ||r||_M / ||r||_2, returns NegativeNormError for
+// r.M^-1 r < 0, converges on r.M^-1 r == 0 (the reference's test
+// |dp| > 0.e-13 |dp0| is |dp| > 0).  On exit *tolerance holds
+// rnorm_prec ||r||_2 / ||r||_M.
+static int solver_minres(Sys& S, double* r, double* x, int* iter, double* tolerance)
+{
+    const int64_t n = S.n;
+    const int maxit = *iter;
+    if (maxit <= 0) return InputError;
+    std::vector<double> znew(n), z(n), az(n), w(n, 0.0), r_old(n, 0.0), w_old(n, 0.0), r_anc(n, 0.0),
+        w_anc(n, 0.0);
+    double delta, gamma = 0., gamma_old = 0., eta = 0., dp0 = 0., c = 1.0, c_old = 1.0;
+    double c_anc = 1., s = 0.0, s_old = 0.0, s_anc, rnorm_prec = 1;
+    double tol = 1., norm_scal = 1.;
+    int num_iter = 0;
+    bool converge = false;
+    int status = NoError;
+    S.prec(z.data(), r);                                                       // :77-79
+    double dp = par_sum(n, [&](int64_t i) { return r[i] * z[i]; });
+    dp0 = dp;
+    if (dp < 0) {
+        status = NegativeNormError;
+    } else if (std::fabs(dp) <= 0) {
+        converge = true;
+    } else {
+        gamma = std::sqrt(dp);
+        eta = gamma;
+        rnorm_prec = gamma;
+        const double norm_r = std::sqrt(par_sum(n, [&](int64_t i) { return r[i] * r[i]; }));
+        norm_scal = rnorm_prec / norm_r;
+        tol = (*tolerance) * norm_scal;
+    }
+    (void)dp0;
+    while (!converge && status == NoError) {
+        for (int64_t i = 0; i < n; ++i) z[i] *= 1. / gamma;                   // :103
+        S.mv(1.0, z.data(), 0.0, az.data());                                   // :106
+        delta = par_sum(n, [&](int64_t i) { return az[i] * z[i]; });
+        if (num_iter > 0) std::copy(r_old.begin(), r_old.end(), r_anc.begin());
+        std::copy(r, r + n, r_old.begin());
+        for (int64_t i = 0; i < n; ++i) {
+            double v = az[i] - delta / gamma * r_old[i];
+            if (num_iter > 0) v -= gamma / gamma_old * r_anc[i];
+            r[i] = v;
+        }
+        S.prec(znew.data(), r);                                                // :122
+        dp = par_sum(n, [&](int64_t i) { return r[i] * znew[i]; });
+        if (dp < 0.) {
+            status = NegativeNormError;
+        } else if (std::fabs(dp) == 0.) {
+            converge = true;
+        } else {
+            gamma_old = gamma;                                                 // :131-145
+            gamma = std::sqrt(dp);
+            c_anc = c_old;
+            c_old = c;
+            s_anc = s_old;
+            s_old = s;
+            const double a0 = c_old * delta - c_anc * s_old * gamma_old;
+            const double a1 = std::sqrt(a0 * a0 + gamma * gamma);
+            const double a2 = s_old * delta + c_anc * c_old * gamma_old;
+            const double a3 = s_anc * gamma_old;
+            c = a0 / a1;
+            s = gamma / a1;
+            rnorm_prec = rnorm_prec * s;
+            if (num_iter > 1) std::copy(w_old.begin(), w_old.end(), w_anc.begin());   // :149-160
+            if (num_iter > 0) std::copy(w.begin(), w.end(), w_old.begin());
+            for (int64_t i = 0; i < n; ++i) {
+                double v = z[i];
+                if (num_iter > 1) v -= a3 * w_anc[i];
+                if (num_iter > 0) v -= a2 * w_old[i];
+                w[i] = v * (1.0 / a1);
+            }
+            for (int64_t i = 0; i < n; ++i) x[i] += c * eta * w[i];
+            eta = -s * eta;
+            converge = rnorm_prec <= tol;
+        }
+        std::copy(znew.begin(), znew.end(), z.begin());
+        ++num_iter;
+        if (!converge && num_iter >= maxit) status = MaxIterReached;
+    }
+    *iter = num_iter;
+    *tolerance = rnorm_prec / norm_scal;
+    return status;
+}
+
 // Textbook Jacobi-PCG with the same calling convention as solver_pcg_paso:
 // the "plain PCG" comparator.  Stops when ||r||_2 <= tol.
 static int solver_pcg_plain(Sys& S, double* r, double* x, int* iter, double* tolerance)
@@ -1231,9 +1320,10 @@ ORC_API int orc_solve(int bs, int64_t nrows, const index_t* ptr, const index_t* 
                 double tol = tolerance * std::min(norm2_of_b, 0.1 * norm2 / normmax * norm_max_of_b);
                 last2 = norm2;
                 lastmax = normmax;
-                status = opt->plain == 2 ? solver_bicgstab(S, r.data(), x, &cntIter, &tol)
-                         : opt->plain ? solver_pcg_plain(S, r.data(), x, &cntIter, &tol)
-                                      : solver_pcg_paso(S, r.data(), x, &cntIter, &tol);
+                status = opt->plain == 3   ? solver_minres(S, r.data(), x, &cntIter, &tol)
+                         : opt->plain == 2 ? solver_bicgstab(S, r.data(), x, &cntIter, &tol)
+                         : opt->plain      ? solver_pcg_plain(S, r.data(), x, &cntIter, &tol)
+                                           : solver_pcg_paso(S, r.data(), x, &cntIter, &tol);
                 totIter += cntIter;
                 if (status == NoError) {
                     finalize = false;

@@ -113,3 +113,56 @@ def test_full_size_other_methods_reach_the_fp64_residual(c2, method, prec, sym):
     r = frhs.clone()
     fresh.MatrixVector(-1.0, x, 1.0, r)
     assert (torch.linalg.norm(r) / torch.linalg.norm(frhs)).item() <= 1e-8 * 1.001
+
+
+def _paso_parity(dom, n, coefs_gpu, ptr, idx, oval, orhs, q):
+    """PASO PCG on the GPU (full and symmetric storage) against the oracle."""
+    E, Q = dom.getNumElements(), dom.getNumQuadraturePoints()
+    qn = np.repeat(q, n)
+    oval = oval.copy()
+    O.nullify(n, ptr, idx, oval, qn, qn, 1.0)
+    orhs = np.where(qn > 0, 0.0, orhs)
+    xo, dg, _, _ = O.solve(n, ptr, idx, oval, orhs, tolerance=1e-8)
+    assert dg["rc"] == 0 and dg["converged"] == 1
+    for sym in (False, True):
+        op = dom.newOperator(n, symmetric=sym)
+        rhs = torch.zeros(op.size, dtype=torch.float64, device="cuda")
+        op.assemble({k: R.Coefficient(v, s, E, Q, dom.device, isotropic=iso) for k, (v, s, iso) in coefs_gpu.items()},
+                    rhs)
+        qd = to_dev(qn)
+        op.nullifyRowsAndCols(qd, qd, 1.0)
+        rhs.masked_fill_(qd > 0, 0.0)
+        opts = R.SolverOptions()
+        opts.setSymmetryOn()
+        opts.setSolverMethod(R.SolverOptions.PCG)
+        x = op.solve(rhs, opts).cpu().numpy()
+        print("PASO PCG", dom, "symmetric" if sym else "full", "iterations gpu", opts.getDiagnostics("num_iter"),
+              "oracle", dg["num_iter"])
+        assert opts.hasConverged()
+        assert abs(opts.getDiagnostics("num_iter") - dg["num_iter"]) <= max(2, 0.02 * dg["num_iter"])
+        assert np.linalg.norm(x - xo) <= 1e-6 * np.linalg.norm(xo)
+
+
+def test_full_size_rectangle_256():
+    # Rectangle 256^2, A = I, Y = 1, u = 0 on x0 = 0 (the reference's CPU-runnable case)
+    dom = R.Rectangle(256, 256)
+    ptr, idx, oval, orhs = O.assemble(2, (256, 256), (1.0, 1.0), A=np.eye(2), Y=1.0)
+    q = dom.whereZero(0).cpu().numpy()
+    _paso_parity(dom, 1, {"A": (to_dev(np.eye(2)), (2, 2), False), "Y": (1.0, (), False)}, ptr, idx, oval, orhs, q)
+
+
+def test_full_size_brick_96_elasticity():
+    # Brick 96^3 Lame elasticity, 3x3 blocks, 8 layers of lambda/mu U[1e9, 5e10] Pa,
+    # gravity, u = 0 on z = 0 (2,738,019 unknowns, 217 M scalar entries)
+    ne = (96, 96, 96)
+    dom = R.Brick(*ne)
+    lam, mu = fields.layered_lame(ne)
+    eye = np.eye(3)
+    A = (lam[:, None, None, None, None] * np.einsum("ij,kl->ijkl", eye, eye)
+         + mu[:, None, None, None, None] * (np.einsum("ik,jl->ijkl", eye, eye) + np.einsum("il,jk->ijkl", eye, eye)))
+    Y = np.array([0.0, 0.0, -2700.0 * 9.81])
+    ptr, idx, oval, orhs = O.assemble(3, ne, (1.0, 1.0, 1.0), n=3, A=A, Y=Y)
+    q = dom.whereZero(2).cpu().numpy()
+    coefs = {"A": (to_dev(A), (3, 3, 3, 3), False), "Y": (to_dev(Y), (3,), False)}
+    del A
+    _paso_parity(dom, 3, coefs, ptr, idx, oval, orhs, q)

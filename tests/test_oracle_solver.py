@@ -199,3 +199,60 @@ def test_bicgstab_on_spd_matches_pcg_solution():
     xb, db, _, _ = O.solve(1, ptr, idx, val, rhs, tolerance=1e-10, method="BICGSTAB")
     xp, dp, _, _ = O.solve(1, ptr, idx, val, rhs, tolerance=1e-10)
     assert np.linalg.norm(xb - xp) <= 1e-8 * np.linalg.norm(xp)
+
+
+def _reaction_system(D, ne=(10, 9)):
+    """-lap u + D u = 1 on Rectangle(ne), u = 0 on the x1 = 0 edge."""
+    ptr, idx, val, rhs = O.assemble(2, ne, (1.0, 1.0), A=np.eye(2), D=D, Y=1.0)
+    N = len(ptr) - 1
+    q = np.zeros(N)
+    q[: ne[0] + 1] = 1.0
+    val, rhs = constrain(ptr, idx, val, rhs, q, np.zeros(N))
+    return ptr, idx, val, rhs
+
+
+@pytest.mark.parametrize("prec", ["JACOBI", "NONE"])
+def test_minres_restatement_spd_matches_direct(prec):
+    # Solver_MINRES (paso/src/MINRES.cpp:59-191) on an SPD system
+    ptr, idx, val, rhs = _reaction_system(1.0)
+    N = len(ptr) - 1
+    M = sp.csr_matrix((val, idx, ptr), shape=(N, N))
+    x, dg, _, _ = O.solve(1, ptr, idx, val, rhs, tolerance=1e-10, method="MINRES", preconditioner=prec)
+    assert dg["rc"] == 0 and dg["converged"] == 1
+    xs = spla.spsolve(M.tocsc(), rhs)
+    assert np.linalg.norm(x - xs) <= 1e-8 * np.linalg.norm(xs)
+
+
+def test_minres_restatement_symmetric_indefinite():
+    # a negative reaction term makes the matrix indefinite while its diagonal
+    # (and so the Jacobi preconditioner) stays positive: MINRES converges
+    # where CG has no guarantee
+    ptr, idx, val, rhs = _reaction_system(-30.0)
+    N = len(ptr) - 1
+    M = sp.csr_matrix((val, idx, ptr), shape=(N, N)).toarray()
+    ev = np.linalg.eigvalsh(M)
+    assert ev.min() < 0 < ev.max()
+    assert np.all(np.diag(M) > 0)
+    x, dg, _, _ = O.solve(1, ptr, idx, val, rhs, tolerance=1e-10, method="MINRES")
+    assert dg["rc"] == 0 and dg["converged"] == 1
+    xs = np.linalg.solve(M, rhs)
+    assert np.linalg.norm(x - xs) <= 1e-8 * np.linalg.norm(xs)
+
+
+def test_minres_restatement_negative_norm():
+    # MINRES.cpp:88-90: r.M^-1 r < 0 (negative definite system, Jacobi)
+    # -> NegativeNormError
+    ptr, idx, val, rhs = _reaction_system(1.0)
+    x, dg, _, _ = O.solve(1, ptr, idx, -val, rhs, tolerance=1e-8, method="MINRES")
+    assert dg["rc"] == 5
+
+
+def test_minres_restatement_iteration_count_is_minimal_residual():
+    # MINRES minimises ||r||_{M^-1} over the Krylov space: without a
+    # preconditioner and with the outer loop's first step only it needs no
+    # more iterations than CG on an SPD system (same Krylov space)
+    ptr, idx, val, rhs = _reaction_system(1.0, ne=(20, 17))
+    _, dm, _, _ = O.solve(1, ptr, idx, val, rhs, tolerance=1e-8, method="MINRES", preconditioner="NONE")
+    _, dc, _, _ = O.solve(1, ptr, idx, val, rhs, tolerance=1e-8, method="PCG_PLAIN", preconditioner="NONE")
+    assert dm["rc"] == 0 and dc["rc"] == 0
+    assert dm["num_iter"] <= dc["num_iter"] + 2
