@@ -114,24 +114,36 @@ __device__ __forceinline__ void reduce_partials(const double* partials, int G, d
 // Sum of the G partials of each of NV rows by one workgroup of NT threads
 // (NT a multiple of 64): thread t adds partials t, t+NT, ... in order, then
 // the waves' sums are combined in a fixed order; valid in all threads.
-// With NT = 1024 a stage reads 4096 partials in a single round trip.
-template <int NV, int NT>
-__device__ __forceinline__ void reduce_partials_wide(const double* partials, int G, double (&tot)[NV])
+// With NT = 1024 a stage reads 8192 partials of every row in a single round
+// trip: all those loads are issued before the first add, with indices past G
+// clamped (G >= 1) and their values selected away -- a load behind a branch
+// would be waited for before the next one is issued.
+// after_issue() runs once those loads are in flight (more loads of the
+// caller, served in the same round trip).
+struct NoOp {
+    __device__ void operator()() const {}
+};
+template <int NV, int NT, class F = NoOp>
+__device__ __forceinline__ void reduce_partials_wide(const double* partials, int G, double (&tot)[NV],
+                                                     F&& after_issue = F())
 {
     __shared__ double red[NV][NT / 64];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    double v[NV][8];
 #pragma unroll
-    for (int k = 0; k < NV; ++k) {
-        const double* p = partials + (int64_t)k * G;
-        double v[8];
+    for (int k = 0; k < NV; ++k)
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
             const int b = threadIdx.x + u * NT;
-            v[u] = (b < G) ? p[b] : 0.0;
+            v[k][u] = partials[(int64_t)k * G + (b < G ? b : G - 1)];
         }
+    after_issue();
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+        const double* p = partials + (int64_t)k * G;
         double s = 0.0;
 #pragma unroll
-        for (int u = 0; u < 8; ++u) s += v[u];
+        for (int u = 0; u < 8; ++u) s += ((int)threadIdx.x + u * NT < G) ? v[k][u] : 0.0;
         for (int b = threadIdx.x + 8 * NT; b < G; b += NT) s += p[b];
         s = wave_sum(s);
         if (lane == 0) red[k][w] = s;

@@ -72,10 +72,15 @@ struct KrylovState {
     double rho, rnorm0, rho_prev, pad1;
     int32_t num_iter, maxit, done, status;
     int32_t replace_mode;   // 1: convergence is only declared on a replaced (true) residual
-    int32_t early;          // BiCGStab: converged on ||s|| this iteration (x += alpha phat pending)
+    int32_t early;          // BiCGStab: converged on ||s|| this iteration (x += alpha phat pending);
+                            // MINRES: the w / x update of this iteration is pending
     int32_t pad[2];
+    // MINRES (paso/src/MINRES.cpp:67-69): the last two Givens rotations, the
+    // coefficients of the w recurrence, c * eta for the x update and the
+    // scaling of the preconditioned residual norm
+    double c, c_old, s, s_old, a1, a2, a3, ceta, norm_scal, pad2[3];
 };
-enum { KS_RUNNING = 0, KS_CONVERGED = 1, KS_MAXITER = 2, KS_BREAKDOWN = 3, KS_STAGNATED = 4 };
+enum { KS_RUNNING = 0, KS_CONVERGED = 1, KS_MAXITER = 2, KS_BREAKDOWN = 3, KS_STAGNATED = 4, KS_NEGATIVE = 5 };
 
 }  // namespace rg
 

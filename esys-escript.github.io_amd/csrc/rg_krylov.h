@@ -30,7 +30,8 @@ enum Stage {
     ST_CG_INIT, ST_CG_SPMV, ST_CG_UPDATE, ST_CG_UPDATE_NORZ, ST_CG_RZ, ST_CG_REPLACE,
     ST_FCG_INIT, ST_FCG_BETA, ST_FCG_ALPHA, ST_FCG_UPDATE, ST_FCG_REPLACE,
     ST_BATCH_INIT,
-    ST_BICG_RHO, ST_BICG_ALPHA, ST_BICG_S, ST_BICG_XA, ST_BICG_OMEGA, ST_BICG_END
+    ST_BICG_RHO, ST_BICG_ALPHA, ST_BICG_S, ST_BICG_XA, ST_BICG_OMEGA, ST_BICG_END,
+    ST_MR_INIT, ST_MR_DELTA, ST_MR_GAMMA
 };
 
 // Stages that start a solve run even when the state says done.
@@ -38,7 +39,7 @@ template <int STAGE>
 __device__ __forceinline__ bool stage_starts()
 {
     return STAGE == ST_PASO_INIT || STAGE == ST_CG_INIT || STAGE == ST_FCG_INIT || STAGE == ST_BATCH_INIT ||
-           STAGE == ST_BICG_XA;
+           STAGE == ST_BICG_XA || STAGE == ST_MR_INIT || STAGE == ST_MR_DELTA;
 }
 
 // The scalar recurrence of one stage, given the reduced totals (one thread).
@@ -142,6 +143,60 @@ __device__ __forceinline__ void stage_apply(KrylovState* st, const double (&tot)
         } else {
             st->tau_new = st->tau;
         }
+    } else if constexpr (STAGE == ST_MR_INIT) {
+        // Solver_MINRES (paso/src/MINRES.cpp:77-101): tot = (r.z, r.r), z = M^-1 r
+        const double dp = tot[0];
+        st->num_iter = 0;
+        st->done = 0;
+        st->status = KS_RUNNING;
+        st->early = 0;
+        st->c = st->c_old = 1.0;
+        st->s = st->s_old = 0.0;
+        st->norm = 1.0;
+        st->norm_scal = 1.0;
+        if (dp < 0.0) { st->done = 1; st->status = KS_NEGATIVE; }
+        else if (fabs(dp) <= 0.0) { st->done = 1; st->status = KS_CONVERGED; }
+        else if (st->maxit <= 0) { st->done = 1; st->status = KS_MAXITER; }
+        else {
+            st->rho = sqrt(dp);            // gamma
+            st->rho_prev = 0.0;            // gamma_old
+            st->tau = st->rho;             // eta
+            st->norm = st->rho;            // rnorm_prec
+            st->norm_scal = st->rho / sqrt(tot[1]);
+            st->tol = st->tol * st->norm_scal;
+        }
+    } else if constexpr (STAGE == ST_MR_DELTA) {
+        // delta = Az.z (:106-109); also retires the previous iteration's
+        // pending w / x update
+        st->early = 0;
+        if (!st->done) st->delta = tot[0];
+    } else if constexpr (STAGE == ST_MR_GAMMA) {
+        // :124-165 after z = M^-1 r_new: tot[0] = r_new.z
+        const double dp = tot[0];
+        if (dp < 0.0) { st->done = 1; st->status = KS_NEGATIVE; }
+        else if (fabs(dp) <= 0.0) { st->done = 1; st->status = KS_CONVERGED; }
+        else {
+            const double g_old = st->rho, g = sqrt(dp), delta = st->delta;
+            const double c_anc = st->c_old, c_old = st->c, s_anc = st->s_old, s_old = st->s;
+            const double a0 = c_old * delta - c_anc * s_old * g_old;
+            const double a1 = sqrt(a0 * a0 + g * g);
+            st->a1 = a1;
+            st->a2 = s_old * delta + c_anc * c_old * g_old;
+            st->a3 = s_anc * g_old;
+            st->c_old = c_old;
+            st->s_old = s_old;
+            st->c = a0 / a1;
+            st->s = g / a1;
+            st->rho_prev = g_old;
+            st->rho = g;
+            st->norm = st->norm * st->s;
+            st->ceta = st->c * st->tau;
+            st->tau = -st->s * st->tau;
+            st->early = 1;
+            if (st->norm <= st->tol) { st->done = 1; st->status = KS_CONVERGED; }
+        }
+        st->num_iter += 1;
+        if (!st->done && st->num_iter >= st->maxit) { st->done = 1; st->status = KS_MAXITER; }
     } else if constexpr (STAGE == ST_FCG_INIT) {
         st->norm = sqrt(tot[1]);
         st->num_iter = 0;

@@ -753,17 +753,144 @@ __global__ __launch_bounds__(kBlock) void k_bicg_update(PcgArgs a, int64_t n)
     publish_block_sums<1>(s, a.partials);
 }
 
+// ------------------------------------------------------------ MINRES
+// Solver_MINRES (paso/src/MINRES.cpp:59-191); vectors: r, r_old = rs,
+// Az = v, w = p, w_old = x2, z = M^-1 r (unscaled) and its scaled copy
+// z / gamma = s.
+//   z = M^-1 r, r.z, r.r                                  k_mr_init  -> ST_MR_INIT
+//   s = z / gamma, Az = A s, Az.s                          k_mr_spmv  -> ST_MR_DELTA
+//   r_old, r = r, Az - delta/gamma r - gamma/gamma_old r_old,
+//   z = M^-1 r (fused Jacobi), r.z                         k_mr_r     -> ST_MR_GAMMA
+//   w_old, w = w, (s - a3 w_old - a2 w) / a1, x += c eta w  k_mr_w
+// Terms of the first two iterations that the reference skips are skipped
+// (not multiplied by zero): the buffers hold stale values then.
+template <int BS, typename DT>
+__device__ __forceinline__ void jacobi_regs(const DT* __restrict__ dinv, int64_t ld, int64_t i,
+                                            const double (&r)[BS], double (&z)[BS])
+{
+#pragma unroll
+    for (int a = 0; a < BS; ++a) {
+        double s = 0.0;
+#pragma unroll
+        for (int c = 0; c < BS; ++c) s = fma((double)dinv[(int64_t)(a + BS * c) * ld + i], r[c], s);
+        z[a] = s;
+    }
+}
+
+template <int BS>
+__global__ __launch_bounds__(kBlock) void k_mr_init(PcgArgs a, const double* __restrict__ dinv)
+{
+    const RowRange rr = block_rows(a.N, a.chunk);
+    const double* __restrict__ r = a.r;
+    double s[2] = {0.0, 0.0};
+    for (int64_t i = rr.r0 + threadIdx.x; i < rr.r1; i += kBlock) {
+        double z[BS];
+        prec_node<BS>(dinv, a.ld, i, r, a.z, z);
+#pragma unroll
+        for (int c = 0; c < BS; ++c) {
+            const int64_t k = i * BS + c;
+            if (!a.z) a.zp[k] = z[c];
+            s[0] += r[k] * z[c];
+            s[1] += r[k] * r[k];
+        }
+    }
+    publish_block_sums<2>(s, a.partials);
+}
+
+template <int DIM, int BS, bool SYM>
+__global__ __launch_bounds__(kBlock) void k_mr_spmv(PcgArgs a, const double* __restrict__ val)
+{
+    if (a.st->done) return;
+    const double ig = 1.0 / a.st->rho;
+    const RowRange rr = block_rows(a.N, a.chunk);
+    const double* __restrict__ z = a.zp;
+    double* __restrict__ az = a.v;
+    double* __restrict__ zs = a.s;
+    double d[1] = {0.0};
+    stencil_block<DIM, BS, SYM, double>(val, a.ld, a.N, a.s1, a.s2, rr.r0, rr.r1, z,
+                                        [&](int64_t i, const double (&acc)[BS]) {
+#pragma unroll
+                                            for (int c = 0; c < BS; ++c) {
+                                                const int64_t k = i * BS + c;
+                                                const double zk = z[k] * ig, azk = acc[c] * ig;
+                                                zs[k] = zk;
+                                                az[k] = azk;
+                                                d[0] = fma(azk, zk, d[0]);
+                                            }
+                                        });
+    publish_block_sums<1>(d, a.partials);
+}
+
+template <int BS>
+__global__ __launch_bounds__(kBlock) void k_mr_r(PcgArgs a, const double* __restrict__ dinv)
+{
+    if (a.st->done) return;
+    const double g = a.st->rho, f1 = a.st->delta / g;
+    const bool second = a.st->num_iter > 0;
+    const double f2 = second ? g / a.st->rho_prev : 0.0;
+    const RowRange rr = block_rows(a.N, a.chunk);
+    double s[1] = {0.0};
+    for (int64_t i = rr.r0 + threadIdx.x; i < rr.r1; i += kBlock) {
+        double rn[BS];
+#pragma unroll
+        for (int c = 0; c < BS; ++c) {
+            const int64_t k = i * BS + c;
+            const double rk = a.r[k];
+            double v = fma(-f1, rk, a.v[k]);
+            if (second) v = fma(-f2, a.rs[k], v);
+            a.rs[k] = rk;
+            a.r[k] = v;
+            rn[c] = v;
+        }
+        if (dinv) {   // one fused Jacobi sweep; otherwise z is computed afterwards
+            double z[BS];
+            jacobi_regs<BS>(dinv, a.ld, i, rn, z);
+#pragma unroll
+            for (int c = 0; c < BS; ++c) {
+                a.zp[i * BS + c] = z[c];
+                s[0] += rn[c] * z[c];
+            }
+        }
+    }
+    publish_block_sums<1>(s, a.partials);
+}
+
+__global__ __launch_bounds__(kBlock) void k_mr_w(PcgArgs a, int64_t n)
+{
+    if (!a.st->early) return;
+    const double ia1 = 1.0 / a.st->a1, a2 = a.st->a2, a3 = a.st->a3, ceta = a.st->ceta;
+    const int it = a.st->num_iter - 1;   // the iteration whose update this is
+    const bool k1 = it > 0, k2 = it > 1;
+    for (int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x; q < n; q += (int64_t)gridDim.x * kBlock) {
+        double wv = a.s[q];
+        if (k2) wv = fma(-a3, a.x2[q], wv);   // w_ancient = w_old
+        if (k1) {
+            const double w = a.p[q];          // w_old = w
+            wv = fma(-a2, w, wv);
+            a.x2[q] = w;
+        }
+        wv *= ia1;
+        a.p[q] = wv;
+        a.x[q] = fma(ceta, wv, a.x[q]);
+    }
+}
+
 // ------------------------------------------------------------ stage kernels
 // One workgroup after each reducing kernel (recurrences in rg_krylov.h).
 constexpr int kStageThreads = 1024;
 
+// The done flag is loaded while the partials are in flight (one memory
+// round trip instead of two dependent ones; the partials of a finished solve
+// are reduced and ignored), which also brings the state's first cache line,
+// holding the common scalars, into L2 for thread 0's recurrence.
 template <int STAGE, int NV>
 __global__ __launch_bounds__(kStageThreads) void k_stage(KrylovState* st, const double* __restrict__ partials, int G)
 {
-    if (!stage_starts<STAGE>() && st->done) return;
+    int done = 0;
     double tot[NV];
-    reduce_partials_wide<NV, kStageThreads>(partials, G, tot);
+    reduce_partials_wide<NV, kStageThreads>(partials, G, tot, [&] { done = st->done; });
     if (threadIdx.x != 0) return;
+    if (!stage_starts<STAGE>() && done) return;
     stage_apply<STAGE, NV>(st, tot);
 }
 
@@ -1011,9 +1138,36 @@ struct SolverImpl {
         RG_HIP(hipGetLastError());
     }
 
+    // One iteration of PASO's MINRES (FP64, any preconditioner).
+    void mr_iteration(int sweeps)
+    {
+        const Geom& g = m->g;
+        const bool buf = buffered(sweeps);
+        const Launch Lsp = spmv_launch<double>(), Ln = chunked_launch(g.N);
+        const PcgArgs an = args(nullptr, Ln), asp = args(nullptr, Lsp);
+        const int vg = 1024;
+        dispatch(g, m->symmetric, [&](auto D, auto B, auto S) {
+            hipLaunchKernelGGL((k_mr_spmv<D.value, B.value, S.value>), dim3(Lsp.grid), dim3(kBlock), 0, st, asp,
+                               m->val);
+            stage<ST_MR_DELTA, 1>(Lsp.grid);
+            hipLaunchKernelGGL((k_mr_r<B.value>), dim3(Ln.grid), dim3(kBlock), 0, st, an,
+                               buf ? nullptr : m->dinv);
+        });
+        if (buf) {
+            sweeps_into_z(sweeps, w->st);
+            hipLaunchKernelGGL(k_cg_rz, dim3(L.grid), dim3(kBlock), 0, st, args(w->z, L), w->n);
+            stage<ST_MR_GAMMA, 1>(L.grid);
+        } else {
+            stage<ST_MR_GAMMA, 1>(Ln.grid);
+        }
+        hipLaunchKernelGGL(k_mr_w, dim3(vg), dim3(kBlock), 0, st, an, w->n);
+        RG_HIP(hipGetLastError());
+    }
+
     void iteration(int method, int sweeps, bool last)
     {
         if (method == RG_METHOD_BICGSTAB) bicg_iteration(sweeps);
+        else if (method == RG_METHOD_MINRES) mr_iteration(sweeps);
         else if (method == RG_METHOD_FCG_LEARNED) fcg_iteration(last);
         else if (method == RG_METHOD_PCG) paso_iteration(sweeps);
         else if (method == RG_METHOD_PCG_PLAIN) cg_iteration<double, double>(m->val, m->dinv, sweeps, false);
@@ -1041,6 +1195,9 @@ struct SolverImpl {
                 const Launch Lv = chunked_launch(g.N * B.value);
                 ai.chunk = Lv.chunk;
                 hipLaunchKernelGGL((k_bicg_init<B.value>), dim3(Lv.grid), dim3(kBlock), 0, st, ai);
+            } else if (method == RG_METHOD_MINRES) {
+                hipLaunchKernelGGL((k_mr_init<B.value>), dim3(L.grid), dim3(kBlock), 0, st, a, m->dinv);
+                stage<ST_MR_INIT, 2>(L.grid);
             } else if (method == RG_METHOD_FCG_LEARNED) {
                 hipLaunchKernelGGL((k_cg_init<B.value, double>), dim3(L.grid), dim3(kBlock), 0, st, a, m->dinv);
                 stage<ST_FCG_INIT, 2>(L.grid);
@@ -1093,6 +1250,8 @@ struct SolverImpl {
     void krylov(int method, int sweeps, int maxit, double tol, int check, bool use_graph, int* status,
                 int* iters, double* norm)
     {
+        // the host copy and two polling slots sit below the flag words at pinned + 128
+        static_assert(3 * sizeof(KrylovState) <= 128 * sizeof(double), "KrylovState outgrew the pinned buffer");
         KrylovState h;
         std::memset(&h, 0, sizeof(h));
         h.maxit = maxit;
@@ -1267,7 +1426,7 @@ int solve(rg_matrix* m, double* x, const double* b, const rg_options& opt, rg_di
     const double tolerance = opt.tolerance;
     RG_CHECK(tolerance >= 100. * EPS, RG_VALUE_ERROR, "Solver: Tolerance is too small.");
     RG_CHECK(tolerance <= 1., RG_VALUE_ERROR, "Solver: Tolerance must be less than one.");
-    RG_CHECK(opt.method >= RG_METHOD_PCG && opt.method <= RG_METHOD_BICGSTAB, RG_VALUE_ERROR,
+    RG_CHECK(opt.method >= RG_METHOD_PCG && opt.method <= RG_METHOD_MINRES, RG_VALUE_ERROR,
              "Solver: unknown solver method.");
     const int prec = opt.preconditioner;
     RG_CHECK(prec >= RG_PRECONDITIONER_JACOBI && prec <= RG_PRECONDITIONER_NONE, RG_VALUE_ERROR,
@@ -1371,6 +1530,9 @@ int solve(rg_matrix* m, double* x, const double* b, const rg_options& opt, rg_di
                     last2 = lastmax = std::numeric_limits<double>::infinity();
                 } else if (ks == KS_MAXITER) {
                     status = RG_MAX_ITER_REACHED;
+                    break;
+                } else if (ks == KS_NEGATIVE) {
+                    status = RG_NEGATIVE_NORM;
                     break;
                 } else if (ks == KS_BREAKDOWN) {
                     if (it > 1) {

@@ -22,7 +22,8 @@ RG_DEVICE_ERROR = 22
 
 RG_MATRIX_SYMMETRIC = 1
 RG_COEF_EMPTY, RG_COEF_ELEMENT, RG_COEF_EXPANDED = 0, 1, 2
-RG_METHOD_PCG, RG_METHOD_PCG_PLAIN, RG_METHOD_PCG_MIXED, RG_METHOD_FCG_LEARNED, RG_METHOD_BICGSTAB = 0, 1, 2, 3, 4
+(RG_METHOD_PCG, RG_METHOD_PCG_PLAIN, RG_METHOD_PCG_MIXED, RG_METHOD_FCG_LEARNED, RG_METHOD_BICGSTAB,
+ RG_METHOD_MINRES) = 0, 1, 2, 3, 4, 5
 RG_CRITERION_PASO, RG_CRITERION_TRUE_RESIDUAL = 0, 1
 RG_PRECONDITIONER_JACOBI, RG_PRECONDITIONER_GAUSS_SEIDEL, RG_PRECONDITIONER_ILU0, RG_PRECONDITIONER_NONE = 0, 1, 2, 3
 RG_BYTEORDER_LITTLE_ENDIAN, RG_BYTEORDER_BIG_ENDIAN = 1234, 4321

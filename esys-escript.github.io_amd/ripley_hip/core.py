@@ -82,6 +82,7 @@ class SolverOptions:
     TARGET_CPU, TARGET_GPU = 1, 2
     PASO = 4
     BICGSTAB = 8
+    MINRES = 22
     PCG = 24
     GAUSS_SEIDEL = 29
     ILU0 = 30
@@ -109,7 +110,8 @@ class SolverOptions:
 
     # setters / getters named as SolverBuddy's python binding
     def setSolverMethod(self, method):
-        if method not in (self.DEFAULT, self.PCG, self.PCG_PLAIN, self.PCG_MIXED, self.FCG_LEARNED, self.BICGSTAB):
+        if method not in (self.DEFAULT, self.PCG, self.PCG_PLAIN, self.PCG_MIXED, self.FCG_LEARNED, self.BICGSTAB,
+                          self.MINRES):
             raise ValueError("unknown solver method %r" % (method,))
         self._method = method
 
@@ -240,7 +242,7 @@ class SolverOptions:
             m = self.PCG if self._symmetric else self.BICGSTAB
         o.method = {self.PCG: _lib.RG_METHOD_PCG, self.PCG_PLAIN: _lib.RG_METHOD_PCG_PLAIN,
                     self.PCG_MIXED: _lib.RG_METHOD_PCG_MIXED, self.FCG_LEARNED: _lib.RG_METHOD_FCG_LEARNED,
-                    self.BICGSTAB: _lib.RG_METHOD_BICGSTAB}[m]
+                    self.BICGSTAB: _lib.RG_METHOD_BICGSTAB, self.MINRES: _lib.RG_METHOD_MINRES}[m]
         o.verbose = int(self._verbose)
         o.accept_failed_convergence = int(self._accept)
         o.check_interval = self._check_interval

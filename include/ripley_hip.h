@@ -198,6 +198,9 @@ int rg_matrix_save_mm(rg_matrix* m, const char* filename);
  *   RG_METHOD_BICGSTAB    PASO's Solver_BiCGStab (paso/src/BiCGStab.cpp:77-257)
  *                         in FP64 with the Jacobi preconditioner (one sweep),
  *                         the method PASO uses for non-symmetric systems.
+ *   RG_METHOD_MINRES      PASO's Solver_MINRES (paso/src/MINRES.cpp:59-191) in
+ *                         FP64 for symmetric (possibly indefinite) systems;
+ *                         r.M^-1 r < 0 ends the solve with RG_NEGATIVE_NORM.
  *   RG_METHOD_FCG_LEARNED flexible CG (truncated to one direction) whose
  *                        preconditioner is the learned network attached with
  *                        rg_matrix_set_learned_preconditioner; SpMV on FP32
@@ -210,8 +213,9 @@ int rg_matrix_save_mm(rg_matrix* m, const char* filename);
  *   RG_CRITERION_PASO           balanced ||r||_2 and ||r||_inf tests (Solver.cpp:241)
  *   RG_CRITERION_TRUE_RESIDUAL  unbalanced FP64 ||b - A x||_2 <= tolerance ||b||_2
  * preconditioner (Preconditioner_alloc / Preconditioner_solve,
- * paso/src/Preconditioner.cpp:46-137) for RG_METHOD_PCG, RG_METHOD_PCG_PLAIN
- * and RG_METHOD_BICGSTAB; the FP32 and learned methods use Jacobi:
+ * paso/src/Preconditioner.cpp:46-137) for RG_METHOD_PCG, RG_METHOD_PCG_PLAIN,
+ * RG_METHOD_BICGSTAB and RG_METHOD_MINRES; the FP32 and learned methods use
+ * Jacobi:
  *   RG_PRECONDITIONER_JACOBI        `sweeps` Jacobi sweeps (Smoother.cpp:100-127)
  *   RG_PRECONDITIONER_GAUSS_SEIDEL  `sweeps` symmetric Gauss-Seidel sweeps over the
  *                                   colouring of Pattern::borrowColoringPointer
@@ -219,7 +223,7 @@ int rg_matrix_save_mm(rg_matrix* m, const char* filename);
  *   RG_PRECONDITIONER_ILU0          coloured ILU(0) (ILU.cpp:43-462), block size <= 3
  *   RG_PRECONDITIONER_NONE          identity */
 enum { RG_METHOD_PCG = 0, RG_METHOD_PCG_PLAIN = 1, RG_METHOD_PCG_MIXED = 2, RG_METHOD_FCG_LEARNED = 3,
-       RG_METHOD_BICGSTAB = 4 };
+       RG_METHOD_BICGSTAB = 4, RG_METHOD_MINRES = 5 };
 enum { RG_CRITERION_PASO = 0, RG_CRITERION_TRUE_RESIDUAL = 1 };
 enum { RG_PRECONDITIONER_JACOBI = 0, RG_PRECONDITIONER_GAUSS_SEIDEL = 1, RG_PRECONDITIONER_ILU0 = 2,
        RG_PRECONDITIONER_NONE = 3 };

@@ -1,8 +1,8 @@
 """The reference's SimpleSolveTestCase (escriptcore/test/python/
 test_simplesolve.py:42-160, 267-292) as run on ripley by
 ripley/test/python/run_pasoSolversOnRipley.py:40-116 (Rectangle(11, 11),
-Brick(11, 11, 7), PASO with PCG or BiCGStab and Jacobi), here also with
-Gauss-Seidel and ILU0: a linear exact solution u_ex, Dirichlet data on x0 = 0,
+Brick(11, 11, 7), PASO with PCG, BiCGStab or MINRES and Jacobi), here also
+with Gauss-Seidel and ILU0: a linear exact solution u_ex, Dirichlet data on x0 = 0,
 Neumann data y = g_ex . n on the rest of the boundary; the system variant
 couples the components through D = (1 - f) I + f 1 1^T (f = -0.4) with
 Y = D u_ex.  Passes when Lsup(u - u_ex) < 1e-6 Lsup(u_ex) at solver tolerance
@@ -45,8 +45,8 @@ def face_normals(dom):
 
 @pytest.mark.parametrize("dim", [2, 3])
 @pytest.mark.parametrize("system", [False, True])
-@pytest.mark.parametrize("method,prec", [("PCG", "JACOBI"), ("BICGSTAB", "JACOBI"), ("PCG", "GAUSS_SEIDEL"),
-                                         ("PCG", "ILU0")])
+@pytest.mark.parametrize("method,prec", [("PCG", "JACOBI"), ("BICGSTAB", "JACOBI"), ("MINRES", "JACOBI"),
+                                         ("PCG", "GAUSS_SEIDEL"), ("PCG", "ILU0")])
 def test_simple_solve(dim, system, method, prec):
     dom = R.Rectangle(11, 11) if dim == 2 else R.Brick(11, 11, 7)
     n = dim if system else 1
