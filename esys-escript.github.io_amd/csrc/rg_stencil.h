@@ -2,7 +2,7 @@
 //
 // Rows are processed R at a time per thread (R = 16 bytes / value size, so
 // one dwordx4 load fetches a slot's values for all R rows), and x is read
-// through aligned 16-byte windows: for each (dy, dz) pair the R+2 values
+// through 16-byte windows: for each (dy, dz) pair the R+2 values
 // x[b-1 .. b+R] serve all 3*R products of the three dx slots.  Every load of
 // a row group is issued before the first multiply; there is no per-slot
 // branch.  Blocks whose rows lie within one stencil reach of either end of
@@ -134,24 +134,27 @@ __device__ __forceinline__ void stencil_row(const VT* __restrict__ val, int64_t 
     }
 }
 
-// x[b-1+k], k = 0..R+1, from aligned 16-byte loads starting at the even
-// address at or below b-1 (R/2 + 2 loads; no branch: the parity of b is the
-// same for the whole wave and only selects which loaded values are used).
+typedef double rg_dbl2_t __attribute__((ext_vector_type(2)));
+
+// x[b-1+k], k = 0..R+1, from (R+2)/2 16-byte loads at b-1, b+1, ... .  The
+// loads are 8-byte aligned: the target reads unaligned vectors (the
+// compiler emits one dwordx4 for the 16-byte copy), and this halves the
+// cache lines each wave requests against aligned loads with a select
+// (x: 3 loads of ~9 lines each per plane -> 2; 128^3 SpMV, symmetric storage
+// 53.1 -> 47.0 us, full storage 88.3 -> 85.8 us).  Loading only each lane's
+// own aligned pair and taking x[b-1], x[b+R] from the neighbour lanes by
+// DPP wave shifts (one more load for the wave edges) requested fewer lines
+// still but ran slower (47.4 vs 46.5 us).
 template <int R>
 __device__ __forceinline__ void x_window(const double* __restrict__ x, int64_t b, double (&w)[R + 2])
 {
-    const int64_t base = (b - 1) & ~(int64_t)1;
-    const bool odd = (b - 1) != base;
-    const double2* p = reinterpret_cast<const double2*>(x + base);
-    double t[R + 4];
 #pragma unroll
-    for (int k = 0; k < (R + 4) / 2; ++k) {
-        const double2 v = p[k];
-        t[2 * k] = v.x;
-        t[2 * k + 1] = v.y;
+    for (int k = 0; k < (R + 2) / 2; ++k) {
+        rg_dbl2_t v;
+        __builtin_memcpy(&v, x + b - 1 + 2 * k, sizeof(v));
+        w[2 * k] = v[0];
+        w[2 * k + 1] = v[1];
     }
-#pragma unroll
-    for (int k = 0; k < R + 2; ++k) w[k] = odd ? t[k + 1] : t[k];
 }
 
 typedef double rg_dbl2 __attribute__((ext_vector_type(2)));
@@ -196,7 +199,7 @@ __device__ __forceinline__ double vget(const V& v, int k)
 }
 
 // Scalar (bs = 1) full-storage stencil for R consecutive rows i0..i0+R-1 of
-// an interior row group (all columns in range, x 16-byte aligned, i0 % R == 0).
+// an interior row group (all columns in range, i0 % R == 0).
 template <int DIM, typename VT, int R>
 __device__ __forceinline__ void stencil_rows_s1(const VT* __restrict__ val, int64_t ld, int64_t s1, int64_t s2,
                                                 int64_t i0, const double* __restrict__ x, double (&acc)[R])
@@ -231,30 +234,16 @@ __device__ __forceinline__ void stencil_rows_s1(const VT* __restrict__ val, int6
     }
 }
 
-// R consecutive values v[j0 .. j0+R-1] of one value stream from two aligned
-// 16-byte loads at the aligned address at or below j0 (the misalignment is
-// the same for the whole wave).
+// R consecutive values v[j0 .. j0+R-1] of one value stream by one
+// (unaligned, see x_window) 16-byte load.
 template <typename VT, int R>
 __device__ __forceinline__ void value_window(const VT* __restrict__ v, int64_t j0, double (&w)[R])
 {
     using V = typename VecOf<VT, R>::T;
-    const int64_t base = j0 & ~(int64_t)(R - 1);
-    const int o = (int)(j0 - base);
-    const V a = *reinterpret_cast<const V*>(v + base);
-    const V b = *reinterpret_cast<const V*>(v + base + R);
-    double t[2 * R];
+    V a;
+    __builtin_memcpy(&a, v + j0, sizeof(V));
 #pragma unroll
-    for (int k = 0; k < R; ++k) {
-        t[k] = (double)a[k];
-        t[k + R] = (double)b[k];
-    }
-#pragma unroll
-    for (int k = 0; k < R; ++k) {
-        double r = t[k];
-#pragma unroll
-        for (int q = 1; q < R; ++q) r = (o == q) ? t[k + q] : r;
-        w[k] = r;
-    }
+    for (int k = 0; k < R; ++k) w[k] = (double)a[k];
 }
 
 // Scalar symmetric-storage stencil for R consecutive rows of an interior row
@@ -296,8 +285,8 @@ __device__ __forceinline__ void stencil_rows_s1_sym(const VT* __restrict__ val, 
 }
 
 // Rows of one block of a chunked SpMV-shaped kernel.  Calls epi(i, acc) for
-// every row i of [r0, r1) (acc: BS values of row i).  For bs = 1 with full
-// storage, FP64/FP32 values and an aligned x, interior row groups take the
+// every row i of [r0, r1) (acc: BS values of row i).  For bs = 1 (full or
+// symmetric storage, FP64 or FP32 values) interior row groups take the
 // vectorised path.
 template <int DIM, int BS, bool SYM, typename VT, class Epi>
 __device__ __forceinline__ void stencil_block(const VT* __restrict__ val, int64_t ld, int64_t N, int64_t s1,
@@ -306,7 +295,7 @@ __device__ __forceinline__ void stencil_block(const VT* __restrict__ val, int64_
 {
     constexpr int R = spmv_rows_per_thread<VT>();
     const int64_t reach = s2 * (DIM == 3) + s1 + R + 2;
-    const bool fast = (BS == 1) && ((reinterpret_cast<uintptr_t>(x) & 15) == 0) &&
+    const bool fast = (BS == 1) && ((reinterpret_cast<uintptr_t>(x) & 7) == 0) &&
                       (r0 % R == 0) && ((r1 - r0) % R == 0) && (r0 >= reach) && (r1 + reach <= N);
     if (fast) {
         for (int64_t i0 = r0 + (int64_t)threadIdx.x * R; i0 < r1; i0 += (int64_t)kBlock * R) {
