@@ -31,7 +31,8 @@ enum Stage {
     ST_FCG_INIT, ST_FCG_BETA, ST_FCG_ALPHA, ST_FCG_UPDATE, ST_FCG_REPLACE,
     ST_BATCH_INIT,
     ST_BICG_RHO, ST_BICG_ALPHA, ST_BICG_S, ST_BICG_XA, ST_BICG_OMEGA, ST_BICG_END,
-    ST_MR_INIT, ST_MR_DELTA, ST_MR_GAMMA
+    ST_MR_INIT, ST_MR_DELTA, ST_MR_GAMMA,
+    ST_CGX_SPMV, ST_CGX_UPDATE
 };
 
 // Stages that start a solve run even when the state says done.
@@ -39,7 +40,7 @@ template <int STAGE>
 __device__ __forceinline__ bool stage_starts()
 {
     return STAGE == ST_PASO_INIT || STAGE == ST_CG_INIT || STAGE == ST_FCG_INIT || STAGE == ST_BATCH_INIT ||
-           STAGE == ST_BICG_XA || STAGE == ST_MR_INIT || STAGE == ST_MR_DELTA;
+           STAGE == ST_BICG_XA || STAGE == ST_MR_INIT || STAGE == ST_MR_DELTA || STAGE == ST_CGX_SPMV;
 }
 
 // The scalar recurrence of one stage, given the reduced totals (one thread).
@@ -98,13 +99,19 @@ __device__ __forceinline__ void stage_apply(KrylovState* st, const double (&tot)
         if (st->norm <= st->tol) { st->done = 1; st->status = KS_CONVERGED; }
         else if (st->maxit <= 0) { st->done = 1; st->status = KS_MAXITER; }
         else if (fabs(tot[0]) <= 0.0) { st->done = 1; st->status = KS_BREAKDOWN; }
-    } else if constexpr (STAGE == ST_CG_SPMV) {
-        st->delta = tot[0];
-        if (fabs(tot[0]) <= 0.0) { st->done = 1; st->status = KS_BREAKDOWN; }
-        else st->alpha = st->rho / tot[0];
-    } else if constexpr (STAGE == ST_CG_UPDATE || STAGE == ST_CG_UPDATE_NORZ) {
+    } else if constexpr (STAGE == ST_CG_SPMV || STAGE == ST_CGX_SPMV) {
+        // the deferred-x variant also retires the previous iteration's
+        // pending x update, and so runs after the solve has ended too
+        if (STAGE == ST_CGX_SPMV) st->early = 0;
+        if (!st->done) {
+            st->delta = tot[0];
+            if (fabs(tot[0]) <= 0.0) { st->done = 1; st->status = KS_BREAKDOWN; }
+            else st->alpha = st->rho / tot[0];
+        }
+    } else if constexpr (STAGE == ST_CG_UPDATE || STAGE == ST_CG_UPDATE_NORZ || STAGE == ST_CGX_UPDATE) {
         st->num_iter += 1;
-        cg_end(st, tot[0], tot[1], STAGE == ST_CG_UPDATE);
+        if (STAGE == ST_CGX_UPDATE) st->early = 1;   // x += alpha p pending in k_p_update
+        cg_end(st, tot[0], tot[1], STAGE != ST_CG_UPDATE_NORZ);
     } else if constexpr (STAGE == ST_CG_RZ) {
         if (fabs(tot[0]) <= 0.0) { st->done = 1; st->status = KS_BREAKDOWN; }
         else { st->beta = tot[0] / st->rho; st->rho_prev = st->rho; st->rho = tot[0]; }

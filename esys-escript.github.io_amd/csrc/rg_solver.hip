@@ -340,31 +340,49 @@ __global__ __launch_bounds__(kBlock) void k_paso_init(PcgArgs a, const DT* __res
     publish_block_sums<1>(t, a.partials);
 }
 
-// p = M^-1 r + beta p (paso k_paso_p and textbook k_cg_p).
-template <int BS, typename DT>
+// p = M^-1 r + beta p (paso k_paso_p and textbook k_cg_p).  With XD the
+// textbook PCG's x += alpha p of the same iteration happens here instead of
+// in k_cg_update (the p read is shared: one vector pass less per
+// iteration); it is pending (st->early) after every update stage, also the
+// one that ends the solve, when p itself is left alone.
+template <int BS, typename DT, bool XD = false>
 __global__ __launch_bounds__(kBlock) void k_p_update(PcgArgs a, const DT* __restrict__ dinv)
 {
-    if (a.st->done) return;
-    const double beta = a.st->beta;
+    if (XD ? !a.st->early : a.st->done) return;
+    const bool upd = !XD || !a.st->done;
+    const double beta = a.st->beta, alpha = a.st->alpha;
     const RowRange rr = block_rows(a.N, a.chunk);
     const double* __restrict__ r = a.r;
     const double* __restrict__ zb = a.z;
     double* __restrict__ p = a.p;
+    double* __restrict__ x = a.x;
     node_loop<BS>(rr, [&](int64_t i, bool two) {
         if (BS == 1) {
-            Two<double> z;
-            if (zb) z = ld2(zb + i, two);
-            else {
-                const Two<double> rv = ld2(r + i, two), dv = ld2(dinv + i, two);
-                z = {dv.a * rv.a, dv.b * rv.b};
-            }
             const Two<double> pv = ld2(p + i, two);
-            st2(p + i, {fma(beta, pv.a, z.a), fma(beta, pv.b, z.b)}, two);
+            if (XD) {
+                const Two<double> xv = ld2(x + i, two);
+                st2(x + i, {fma(alpha, pv.a, xv.a), fma(alpha, pv.b, xv.b)}, two);
+            }
+            if (upd) {
+                Two<double> z;
+                if (zb) z = ld2(zb + i, two);
+                else {
+                    const Two<double> rv = ld2(r + i, two), dv = ld2(dinv + i, two);
+                    z = {dv.a * rv.a, dv.b * rv.b};
+                }
+                st2(p + i, {fma(beta, pv.a, z.a), fma(beta, pv.b, z.b)}, two);
+            }
         } else {
-            double z[BS];
-            prec_node<BS>(dinv, a.ld, i, r, zb, z);
+            if (XD) {
 #pragma unroll
-            for (int c = 0; c < BS; ++c) p[i * BS + c] = fma(beta, p[i * BS + c], z[c]);
+                for (int c = 0; c < BS; ++c) x[i * BS + c] = fma(alpha, p[i * BS + c], x[i * BS + c]);
+            }
+            if (upd) {
+                double z[BS];
+                prec_node<BS>(dinv, a.ld, i, r, zb, z);
+#pragma unroll
+                for (int c = 0; c < BS; ++c) p[i * BS + c] = fma(beta, p[i * BS + c], z[c]);
+            }
         }
     });
 }
@@ -486,7 +504,8 @@ __global__ __launch_bounds__(kBlock) void k_cg_init(PcgArgs a, const DT* __restr
     publish_block_sums<2>(s, a.partials);
 }
 
-template <int BS, typename DT>
+// With XD the x update is left to k_p_update<.., true>.
+template <int BS, typename DT, bool XD = false>
 __global__ __launch_bounds__(kBlock) void k_cg_update(PcgArgs a, const DT* __restrict__ dinv)
 {
     if (a.st->done) return;
@@ -500,9 +519,11 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(PcgArgs a, const DT* __res
     double s[2] = {0.0, 0.0};
     node_loop<BS>(rr, [&](int64_t i, bool two) {
         if (BS == 1) {
-            const Two<double> xv = ld2(x + i, two), pv = ld2(p + i, two), rv = ld2(r + i, two),
-                              vv = ld2(v + i, two), dv = ld2(dinv + i, two);
-            st2(x + i, {fma(alpha, pv.a, xv.a), fma(alpha, pv.b, xv.b)}, two);
+            const Two<double> rv = ld2(r + i, two), vv = ld2(v + i, two), dv = ld2(dinv + i, two);
+            if (!XD) {
+                const Two<double> xv = ld2(x + i, two), pv = ld2(p + i, two);
+                st2(x + i, {fma(alpha, pv.a, xv.a), fma(alpha, pv.b, xv.b)}, two);
+            }
             const Two<double> rn = {fma(-alpha, vv.a, rv.a), fma(-alpha, vv.b, rv.b)};
             st2(r + i, rn, two);
             s[0] += dv.a * rn.a * rn.a + (two ? dv.b * rn.b * rn.b : 0.0);
@@ -511,7 +532,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(PcgArgs a, const DT* __res
 #pragma unroll
             for (int c = 0; c < BS; ++c) {
                 const int64_t k = i * BS + c;
-                x[k] = fma(alpha, p[k], x[k]);
+                if (!XD) x[k] = fma(alpha, p[k], x[k]);
                 r[k] = fma(-alpha, v[k], r[k]);
             }
             double z[BS];
@@ -1047,13 +1068,23 @@ struct SolverImpl {
         const double* zb = buf ? w->z : nullptr;
         const Launch Lsp = spmv_launch<VT>(), Ln = node_launch(), L64 = spmv_launch<double>();
         const PcgArgs an = args(zb, Ln), asp = args(zb, Lsp), a64 = args(nullptr, L64);
+        // x += alpha p moves into the direction update unless a residual
+        // replacement needs x first or the preconditioner has its own
+        // launches between the two
+        const bool xd = !replace && !buf;
         dispatch(g, m->symmetric, [&](auto D, auto B, auto S) {
             hipLaunchKernelGGL((k_spmv_dot<D.value, B.value, S.value, VT, false>), dim3(Lsp.grid), dim3(kBlock), 0,
                                st, asp, val);
-            stage<ST_CG_SPMV, 1>(Lsp.grid);
-            hipLaunchKernelGGL((k_cg_update<B.value, DT>), dim3(Ln.grid), dim3(kBlock), 0, st, an, dinv);
-            if (buf) stage<ST_CG_UPDATE_NORZ, 2>(Ln.grid);
-            else stage<ST_CG_UPDATE, 2>(Ln.grid);
+            if (xd) {
+                stage<ST_CGX_SPMV, 1>(Lsp.grid);
+                hipLaunchKernelGGL((k_cg_update<B.value, DT, true>), dim3(Ln.grid), dim3(kBlock), 0, st, an, dinv);
+                stage<ST_CGX_UPDATE, 2>(Ln.grid);
+            } else {
+                stage<ST_CG_SPMV, 1>(Lsp.grid);
+                hipLaunchKernelGGL((k_cg_update<B.value, DT>), dim3(Ln.grid), dim3(kBlock), 0, st, an, dinv);
+                if (buf) stage<ST_CG_UPDATE_NORZ, 2>(Ln.grid);
+                else stage<ST_CG_UPDATE, 2>(Ln.grid);
+            }
             if (replace) {
                 hipLaunchKernelGGL((k_cg_replace<D.value, B.value, S.value, DT>), dim3(L64.grid), dim3(kBlock), 0,
                                    st, a64, m->val, dinv);
@@ -1066,7 +1097,8 @@ struct SolverImpl {
             stage<ST_CG_RZ, 1>(L.grid);
         }
         dispatch_bs(g.bs, [&](auto B) {
-            hipLaunchKernelGGL((k_p_update<B.value, DT>), dim3(Ln.grid), dim3(kBlock), 0, st, an, dinv);
+            if (xd) hipLaunchKernelGGL((k_p_update<B.value, DT, true>), dim3(Ln.grid), dim3(kBlock), 0, st, an, dinv);
+            else hipLaunchKernelGGL((k_p_update<B.value, DT>), dim3(Ln.grid), dim3(kBlock), 0, st, an, dinv);
         });
         RG_HIP(hipGetLastError());
     }
