@@ -406,7 +406,9 @@ __global__ __launch_bounds__(kBlock) void k_p_update(PcgArgs a, const DT* __rest
     });
 }
 
-// v = A p and p.v; the last block forms alpha = (tau or rho) / delta.
+// v = A p and the block partials of p.v (alpha = (tau or rho) / p.v follows
+// in the stage).  A non-temporal store of v under the stream policy measured
+// the same as this cached one.
 template <int DIM, int BS, bool SYM, typename VT, bool PASO>
 __global__ __launch_bounds__(kBlock) void k_spmv_dot(PcgArgs a, const VT* __restrict__ val)
 {
