@@ -74,11 +74,14 @@ struct KrylovState {
     int32_t replace_mode;   // 1: convergence is only declared on a replaced (true) residual
     int32_t early;          // BiCGStab: converged on ||s|| this iteration (x += alpha phat pending);
                             // MINRES: the w / x update of this iteration is pending
-    int32_t pad[2];
+    int32_t pend;           // textbook PCG with the folded update stage: the update kernel's
+                            // totals (pend_rz, pend_rr) await their commit
+    int32_t pad;
     // MINRES (paso/src/MINRES.cpp:67-69): the last two Givens rotations, the
     // coefficients of the w recurrence, c * eta for the x update and the
     // scaling of the preconditioned residual norm
-    double c, c_old, s, s_old, a1, a2, a3, ceta, norm_scal, pad2[3];
+    double c, c_old, s, s_old, a1, a2, a3, ceta, norm_scal;
+    double pend_rz, pend_rr, pad2;
 };
 enum { KS_RUNNING = 0, KS_CONVERGED = 1, KS_MAXITER = 2, KS_BREAKDOWN = 3, KS_STAGNATED = 4, KS_NEGATIVE = 5 };
 

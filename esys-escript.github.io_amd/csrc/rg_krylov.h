@@ -32,7 +32,7 @@ enum Stage {
     ST_BATCH_INIT,
     ST_BICG_RHO, ST_BICG_ALPHA, ST_BICG_S, ST_BICG_XA, ST_BICG_OMEGA, ST_BICG_END,
     ST_MR_INIT, ST_MR_DELTA, ST_MR_GAMMA,
-    ST_CGX_SPMV, ST_CGX_UPDATE
+    ST_CGX_SPMV, ST_CGX_UPDATE, ST_CG_COMMIT
 };
 
 // Stages that start a solve run even when the state says done.
@@ -40,7 +40,21 @@ template <int STAGE>
 __device__ __forceinline__ bool stage_starts()
 {
     return STAGE == ST_PASO_INIT || STAGE == ST_CG_INIT || STAGE == ST_FCG_INIT || STAGE == ST_BATCH_INIT ||
-           STAGE == ST_BICG_XA || STAGE == ST_MR_INIT || STAGE == ST_MR_DELTA || STAGE == ST_CGX_SPMV;
+           STAGE == ST_BICG_XA || STAGE == ST_MR_INIT || STAGE == ST_MR_DELTA || STAGE == ST_CGX_SPMV ||
+           STAGE == ST_CG_COMMIT;
+}
+
+// The update stage of the textbook PCG when k_p_update<.., FOLD> has
+// reduced the update kernel's partials itself: the iteration count, the
+// stopping tests and rho advance here, from the same totals, at the next
+// stage (so the state the update kernel's blocks read stays unchanged while
+// they run).
+__device__ __forceinline__ void cg_commit(KrylovState* st)
+{
+    if (!st->pend) return;
+    st->pend = 0;
+    st->num_iter += 1;
+    cg_end(st, st->pend_rz, st->pend_rr, true);
 }
 
 // The scalar recurrence of one stage, given the reduced totals (one thread).
@@ -102,7 +116,10 @@ __device__ __forceinline__ void stage_apply(KrylovState* st, const double (&tot)
     } else if constexpr (STAGE == ST_CG_SPMV || STAGE == ST_CGX_SPMV) {
         // the deferred-x variant also retires the previous iteration's
         // pending x update, and so runs after the solve has ended too
-        if (STAGE == ST_CGX_SPMV) st->early = 0;
+        if (STAGE == ST_CGX_SPMV) {
+            st->early = 0;
+            cg_commit(st);
+        }
         if (!st->done) {
             st->delta = tot[0];
             if (fabs(tot[0]) <= 0.0) { st->done = 1; st->status = KS_BREAKDOWN; }
@@ -122,6 +139,8 @@ __device__ __forceinline__ void stage_apply(KrylovState* st, const double (&tot)
         if (st->norm <= st->tol) { st->done = 1; st->status = KS_CONVERGED; }
         else if (fabs(tot[0]) <= 0.0) { st->done = 1; st->status = KS_BREAKDOWN; }
         else { st->beta = tot[0] / st->rho_prev; st->rho = tot[0]; }
+    } else if constexpr (STAGE == ST_CG_COMMIT) {
+        cg_commit(st);
     } else if constexpr (STAGE == ST_BICG_RHO) {
         // Solver_BiCGStab (paso/src/BiCGStab.cpp:125-150)
         st->num_iter += 1;
