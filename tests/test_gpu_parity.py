@@ -408,3 +408,42 @@ def test_symmetric_storage_solves(method, dim, ne, n):
     assert res <= 1e-9 * 1.001
     xo, dg, _, _ = O.solve(n, ptr, idx, oval, orhs, tolerance=1e-11, plain=(method == "PCG_PLAIN"))
     assert np.linalg.norm(x - xo) <= 1e-6 * np.linalg.norm(xo)
+
+
+@pytest.mark.parametrize("method", ["PCG_PLAIN", "PCG"])
+def test_chunk_boundaries_and_iteration_limits(method):
+    """The textbook PCG folds its update reduction into the direction update
+    and commits the scalars at the next stage or at the end of a chunk of
+    iterations: the iteration count, the stopping test and the solution must
+    not depend on where chunks end (check intervals 1, 7, 32), and an
+    iteration limit must stop at the same count as the oracle's."""
+    op, rhs, ptr, idx, oval, orhs = _poisson_problem(3, (12, 10, 14))
+    xs, its = [], []
+    for check in (1, 7, 32):
+        opts = R.SolverOptions()
+        opts.setSymmetryOn()
+        opts.setSolverMethod(getattr(R.SolverOptions, method))
+        opts.setTolerance(1e-8)
+        opts.setCheckInterval(check)
+        xs.append(op.solve(rhs, opts).cpu().numpy())
+        its.append(opts.getDiagnostics("num_iter"))
+    assert its[0] == its[1] == its[2]
+    assert np.array_equal(xs[0], xs[1]) and np.array_equal(xs[0], xs[2])
+    for limit in (3, 17):
+        for check in (1, 5, 32):
+            opts = R.SolverOptions()
+            opts.setSymmetryOn()
+            opts.setSolverMethod(getattr(R.SolverOptions, method))
+            opts.setTolerance(1e-8)
+            opts.setCheckInterval(check)
+            opts.setIterMax(limit)
+            with pytest.raises(R.PasoException, match="maximum number of iteration"):
+                op.solve(rhs, opts)
+            opts.setAcceptanceConvergenceFailureOn()
+            op.solve(rhs, opts)
+            _, dg, _, _ = O.solve(1, ptr, idx, oval, orhs, tolerance=1e-8, iter_max=limit,
+                                  plain=(method == "PCG_PLAIN"))
+            print(method, "limit", limit, "check", check, "iterations gpu", opts.getDiagnostics("num_iter"),
+                  "oracle", dg["num_iter"])
+            assert dg["rc"] == 1 and not opts.hasConverged()
+            assert opts.getDiagnostics("num_iter") == dg["num_iter"]
