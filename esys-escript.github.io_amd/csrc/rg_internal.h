@@ -73,7 +73,8 @@ struct KrylovState {
     int32_t num_iter, maxit, done, status;
     int32_t replace_mode;   // 1: convergence is only declared on a replaced (true) residual
     int32_t early;          // BiCGStab: converged on ||s|| this iteration (x += alpha phat pending);
-                            // MINRES: the w / x update of this iteration is pending
+                            // MINRES: the w / x update of this iteration is pending; textbook
+                            // PCG with the deferred x update: x += alpha p pending
     int32_t pend;           // textbook PCG with the folded update stage: the update kernel's
                             // totals (pend_rz, pend_rr) await their commit
     int32_t pad;

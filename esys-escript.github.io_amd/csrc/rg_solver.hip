@@ -479,11 +479,20 @@ __global__ __launch_bounds__(kBlock) void k_paso_r(PcgArgs a)
     publish_block_sums<2>(s, a.partials);
 }
 
-template <int BS, typename DT, bool NT = false>
+// With FOLD there is no stage between k_paso_r and this kernel: every block
+// reduces k_paso_r's partials itself (fixed order, the same totals in every
+// block) and forms gamma as ST_PASO_R would.
+template <int BS, typename DT, bool NT = false, bool FOLD = false>
 __global__ __launch_bounds__(kBlock) void k_paso_x(PcgArgs a, const DT* __restrict__ dinv)
 {
     if (a.st->done) return;
-    const double g1 = a.st->gamma1, g2 = 1.0 - g1;
+    double g1 = a.st->gamma1;
+    if constexpr (FOLD) {
+        double tot[2];
+        reduce_partials_wide<2, kBlock>(a.fold_partials, a.fold_G, tot);
+        g1 = (fabs(tot[0]) <= 0.0) ? 0.0 : -tot[1] / tot[0];
+    }
+    const double g2 = 1.0 - g1;
     const RowRange rr = block_rows(a.N, a.chunk);
     const double* __restrict__ r = a.r;
     const double* __restrict__ x2 = a.x2;
@@ -1107,13 +1116,33 @@ struct SolverImpl {
                                st, asp, m->val);
         });
         stage<ST_PASO_SPMV, 1>(Lsp.grid);
-        hipLaunchKernelGGL(k_paso_r<NT>, dim3(Lr.grid), dim3(kBlock), 0, st, ar);
-        stage<ST_PASO_R, 2>(Lr.grid);
-        if (buf) sweeps_into_z(sweeps, w->st);
-        dispatch_bs(g.bs, [&](auto B) {
-            hipLaunchKernelGGL((k_paso_x<B.value, double, NT>), dim3(Ln.grid), dim3(kBlock), 0, st, an, m->dinv);
-        });
-        stage<ST_PASO_X, 2>(Ln.grid);
+        // gamma's reduction folded into k_paso_x (both on the fold shape):
+        // 128^3 Brick, full storage 0.2676 -> 0.2632 s per PASO solve; with
+        // the non-temporal streams of symmetric storage 0.2004 -> 0.2012 s,
+        // so only with cached streams
+        if (!buf && !NT) {
+            const Launch Lrf = chunked_launch(g.N * g.bs, 2, kFoldGrid), Lf = fold_launch();
+            PcgArgs arf = args(zb, Lrf);
+            arf.N = g.N * g.bs;
+            arf.partials = m->ctx->partials + 8 * kMaxGrid;   // rows 8.. of the partials: untouched by the SpMV
+            PcgArgs axf = args(zb, Lf);
+            axf.fold_partials = arf.partials;
+            axf.fold_G = Lrf.grid;
+            hipLaunchKernelGGL(k_paso_r<NT>, dim3(Lrf.grid), dim3(kBlock), 0, st, arf);
+            dispatch_bs(g.bs, [&](auto B) {
+                hipLaunchKernelGGL((k_paso_x<B.value, double, NT, true>), dim3(Lf.grid), dim3(kBlock), 0, st, axf,
+                                   m->dinv);
+            });
+            stage<ST_PASO_X, 2>(Lf.grid);
+        } else {
+            hipLaunchKernelGGL(k_paso_r<NT>, dim3(Lr.grid), dim3(kBlock), 0, st, ar);
+            stage<ST_PASO_R, 2>(Lr.grid);
+            if (buf) sweeps_into_z(sweeps, w->st);
+            dispatch_bs(g.bs, [&](auto B) {
+                hipLaunchKernelGGL((k_paso_x<B.value, double, NT>), dim3(Ln.grid), dim3(kBlock), 0, st, an, m->dinv);
+            });
+            stage<ST_PASO_X, 2>(Ln.grid);
+        }
         RG_HIP(hipGetLastError());
     }
 
