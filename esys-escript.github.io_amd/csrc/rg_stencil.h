@@ -87,7 +87,9 @@ __device__ __forceinline__ void stencil_row(const VT* __restrict__ val, int64_t 
     // slots per group of loads: a z-plane, or one (dy, dz) line of three
     // blocks for symmetric storage with block size > 1 (its transposed
     // loads otherwise need 256 VGPRs, one wave per SIMD; 111 and four waves
-    // this way, while full storage is best left with z-plane groups)
+    // this way, while full storage is best left with z-plane groups).  x is
+    // loaded value by value: 16-byte windows over a line's 3 BS values ran
+    // the 96^3 3x3 SpMV at 338 instead of 332 us.
     constexpr bool LINE = SYM && BS > 1;
     constexpr int G = LINE ? 3 : 9;
 #pragma unroll
