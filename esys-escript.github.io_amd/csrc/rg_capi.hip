@@ -87,12 +87,9 @@ int rg_context_create(int device, void* stream, rg_context** out)
         c->num_cu = prop.multiProcessorCount;
         try {
             RG_HIP(hipMalloc(&c->partials, sizeof(double) * 16 * rg::kMaxGrid));
-            RG_HIP(hipMalloc(&c->counters, sizeof(unsigned) * rg::kNumCounters));
-            RG_HIP(hipMemset(c->counters, 0, sizeof(unsigned) * rg::kNumCounters));
             RG_HIP(hipHostMalloc(&c->pinned, sizeof(double) * 256, hipHostMallocDefault));
         } catch (...) {
             if (c->partials) hipFree(c->partials);
-            if (c->counters) hipFree(c->counters);
             delete c;
             throw;
         }
@@ -109,7 +106,6 @@ int rg_context_destroy(rg_context* ctx)
         if (ctx->stream) hipStreamSynchronize(ctx->stream);
         else hipDeviceSynchronize();
         if (ctx->partials) hipFree(ctx->partials);
-        if (ctx->counters) hipFree(ctx->counters);
         if (ctx->pinned) hipHostFree(ctx->pinned);
         delete ctx;
         return RG_OK;

@@ -45,7 +45,6 @@ struct Error : std::runtime_error {
 constexpr int kBlock = 256;          // threads per workgroup of every streaming kernel
 constexpr int kNumXcd = 8;           // MI355X: 8 XCDs, blocks dispatched round-robin
 constexpr int kMaxGrid = 8192;       // grid cap of chunked streaming kernels (partials per reduction)
-constexpr int kNumCounters = 64;     // last-block ticket counters per context
 
 // Geometry of one matrix (one rank's share).
 struct Geom {
@@ -93,7 +92,6 @@ struct rg_context {
     hipStream_t stream = nullptr;
     int num_cu = 256;
     double* partials = nullptr;     // 16 * kMaxGrid doubles of per-block partial sums
-    unsigned* counters = nullptr;   // kNumCounters last-block tickets (kept at zero between uses)
     double* pinned = nullptr;       // host-pinned staging for scalars (256 doubles)
     hipEvent_t ev = nullptr;
 };

@@ -270,7 +270,6 @@ struct PcgArgs {
     const double* b;        // right-hand side of a residual replacement
     KrylovState* st;
     double* partials;
-    unsigned* counter;
     int64_t N, ld, s1, s2, chunk;
     const double* fold_partials;   // k_p_update<.., FOLD>: the update kernel's partials (2 x fold_G)
     int fold_G;
@@ -384,14 +383,14 @@ __global__ __launch_bounds__(kBlock) void k_p_update(PcgArgs a, const DT* __rest
         double tot[2];
         reduce_partials_wide<2, kBlock>(a.fold_partials, a.fold_G, tot);
         const KrylovState* st = a.st;
-        const double rz = tot[0], rr = tot[1];
-        const bool fin = (sqrt(rr) <= st->tol && !st->replace_mode) || st->num_iter + 1 >= st->maxit ||
+        const double rz = tot[0], rnorm2 = tot[1];
+        const bool fin = (sqrt(rnorm2) <= st->tol && !st->replace_mode) || st->num_iter + 1 >= st->maxit ||
                          fabs(rz) <= 0.0;
         upd = !fin;
         beta = rz / st->rho;
         if (blockIdx.x == 0 && threadIdx.x == 0) {
             a.st->pend_rz = rz;
-            a.st->pend_rr = rr;
+            a.st->pend_rr = rnorm2;
             a.st->pend = 1;
         }
     }
@@ -1036,7 +1035,6 @@ struct SolverImpl {
         a.b = w->bb;
         a.st = w->st;
         a.partials = m->ctx->partials;
-        a.counter = m->ctx->counters;
         a.fold_partials = nullptr;
         a.fold_G = 0;
         a.N = m->g.N; a.ld = m->g.ld; a.s1 = m->g.s1; a.s2 = m->g.s2; a.chunk = l.chunk;
