@@ -255,9 +255,9 @@ __global__ __launch_bounds__(kBlock) void k_norms_stage(const double* __restrict
 // Solver_PCG (paso/src/PCG.cpp:62-368).  Iteration k:
 //   (1)-(3) v = M^-1 r, tau = v.r, p = v + (tau/tau_old) p      k_paso_p
 //   (4)-(5) v = A p, delta = p.v, alpha = tau/delta             k_paso_spmv
-//   (6)     r -= alpha v, d = r - rs, s3 = d.d, s4 = d.rs,
-//           x2 += alpha p                                       k_paso_r
-//   (7)     gamma = -s4/s3, rs = (1-gamma) rs + gamma r,
+//   (6)     r -= alpha v, d = r - rs, s3 = d.d, s4 = d.rs     k_paso_r
+//   (6)-(7) x2 += alpha p, gamma = -s4/s3,
+//           rs = (1-gamma) rs + gamma r,
 //           x = (1-gamma) x + gamma x2, s5 = rs.rs              k_paso_x
 //   (8)     stop if sqrt(s5) <= tol, iter > maxit or tau == 0
 // tau of iteration k+1 is reduced in k_paso_x of iteration k (r is final
@@ -460,17 +460,13 @@ __global__ __launch_bounds__(kBlock) void k_paso_r(PcgArgs a)
     const double alpha = a.st->alpha;
     const RowRange rr = block_rows(a.N, a.chunk);
     double* __restrict__ r = a.r;
-    double* __restrict__ x2 = a.x2;
     const double* __restrict__ rs = a.rs;
     const double* __restrict__ v = a.v;
-    const double* __restrict__ p = a.p;
     double s[2] = {0.0, 0.0};
     node_loop<1>(rr, [&](int64_t i, bool two) {
         const Two<double> rv = ld2<NT>(r + i, two), vv = ld2<NT>(v + i, two), rsv = ld2<NT>(rs + i, two);
-        const Two<double> pv = ld2<NT>(p + i, two), xv = ld2<NT>(x2 + i, two);
         const Two<double> rn = {fma(-alpha, vv.a, rv.a), fma(-alpha, vv.b, rv.b)};
         st2<NT>(r + i, rn, two);
-        st2<NT>(x2 + i, {fma(alpha, pv.a, xv.a), fma(alpha, pv.b, xv.b)}, two);
         const double d0 = rn.a - rsv.a, d1 = two ? rn.b - rsv.b : 0.0;
         s[0] += d0 * d0 + d1 * d1;
         s[1] += d0 * rsv.a + (two ? d1 * rsv.b : 0.0);
@@ -478,13 +474,16 @@ __global__ __launch_bounds__(kBlock) void k_paso_r(PcgArgs a)
     publish_block_sums<2>(s, a.partials);
 }
 
-// With FOLD there is no stage between k_paso_r and this kernel: every block
-// reduces k_paso_r's partials itself (fixed order, the same totals in every
-// block) and forms gamma as ST_PASO_R would.
+// PCG.cpp's x2 += alpha p of the same iteration happens here (not in
+// k_paso_r, which then streams three vectors instead of five; p and x2 are
+// read here once).  With FOLD there is no stage between k_paso_r and this
+// kernel: every block reduces k_paso_r's partials itself (fixed order, the
+// same totals in every block) and forms gamma as ST_PASO_R would.
 template <int BS, typename DT, bool NT = false, bool FOLD = false>
 __global__ __launch_bounds__(kBlock) void k_paso_x(PcgArgs a, const DT* __restrict__ dinv)
 {
     if (a.st->done) return;
+    const double alpha = a.st->alpha;
     double g1 = a.st->gamma1;
     if constexpr (FOLD) {
         double tot[2];
@@ -494,15 +493,18 @@ __global__ __launch_bounds__(kBlock) void k_paso_x(PcgArgs a, const DT* __restri
     const double g2 = 1.0 - g1;
     const RowRange rr = block_rows(a.N, a.chunk);
     const double* __restrict__ r = a.r;
-    const double* __restrict__ x2 = a.x2;
+    const double* __restrict__ p = a.p;
+    double* __restrict__ x2 = a.x2;
     const double* __restrict__ zb = a.z;
     double* __restrict__ rs = a.rs;
     double* __restrict__ x = a.x;
     double s[2] = {0.0, 0.0};
     node_loop<BS>(rr, [&](int64_t i, bool two) {
         if (BS == 1) {
-            const Two<double> rv = ld2<NT>(r + i, two), rsv = ld2<NT>(rs + i, two), xv = ld2<NT>(x + i, two),
-                              x2v = ld2<NT>(x2 + i, two);
+            const Two<double> rv = ld2<NT>(r + i, two), rsv = ld2<NT>(rs + i, two), xv = ld2<NT>(x + i, two);
+            const Two<double> pv = ld2<NT>(p + i, two), x2o = ld2<NT>(x2 + i, two);
+            const Two<double> x2v = {fma(alpha, pv.a, x2o.a), fma(alpha, pv.b, x2o.b)};
+            st2<NT>(x2 + i, x2v, two);
             Two<double> z;
             if (zb) z = ld2<NT>(zb + i, two);
             else {
@@ -522,8 +524,10 @@ __global__ __launch_bounds__(kBlock) void k_paso_x(PcgArgs a, const DT* __restri
                 const int64_t k = i * BS + c;
                 const double rk = r[k];
                 const double rsk = g2 * rs[k] + g1 * rk;
+                const double x2k = fma(alpha, p[k], x2[k]);
+                x2[k] = x2k;
                 rs[k] = rsk;
-                x[k] = g2 * x[k] + g1 * x2[k];
+                x[k] = g2 * x[k] + g1 * x2k;
                 s[0] += rsk * rsk;
                 s[1] += z[c] * rk;
             }
