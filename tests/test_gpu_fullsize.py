@@ -115,6 +115,29 @@ def test_full_size_other_methods_reach_the_fp64_residual(c2, method, prec, sym):
     assert (torch.linalg.norm(r) / torch.linalg.norm(frhs)).item() <= 1e-8 * 1.001
 
 
+@pytest.mark.parametrize("method", ["MINRES", "PCG_PLAIN"])
+def test_full_size_minres_and_plain_pcg_against_oracle(c2, method):
+    """PASO's MINRES and the textbook PCG at 128^3 (symmetric storage, the
+    folded and non-temporal paths) against the oracle's restatements."""
+    qn = c2["q"].cpu().numpy()
+    oval = c2["oval"].copy()
+    O.nullify(1, c2["ptr"], c2["idx"], oval, qn, qn, 1.0)
+    orhs = np.where(qn > 0, 0.0, c2["orhs"])
+    xo, dg, _, _ = O.solve(1, c2["ptr"], c2["idx"], oval, orhs, tolerance=1e-8, method=method)
+    assert dg["rc"] == 0 and dg["converged"] == 1
+    op, rhs = _constrained(c2, True)
+    opts = R.SolverOptions()
+    opts.setSymmetryOn()
+    opts.setSolverMethod(getattr(R.SolverOptions, method))
+    x = op.solve(rhs, opts).cpu().numpy()
+    print(method, "128^3 symmetric iterations gpu", opts.getDiagnostics("num_iter"), "oracle", dg["num_iter"])
+    assert opts.hasConverged()
+    assert abs(opts.getDiagnostics("num_iter") - dg["num_iter"]) <= 0.02 * dg["num_iter"]
+    assert np.linalg.norm(x - xo) <= 1e-6 * np.linalg.norm(xo)
+    res = np.linalg.norm(orhs - O.spmv(1, c2["ptr"], c2["idx"], oval, x)) / np.linalg.norm(orhs)
+    assert res <= 1e-7
+
+
 def _paso_parity(dom, n, coefs_gpu, ptr, idx, oval, orhs, q):
     """PASO PCG on the GPU (full and symmetric storage) against the oracle."""
     E, Q = dom.getNumElements(), dom.getNumQuadraturePoints()
