@@ -115,10 +115,13 @@ def test_full_size_other_methods_reach_the_fp64_residual(c2, method, prec, sym):
     assert (torch.linalg.norm(r) / torch.linalg.norm(frhs)).item() <= 1e-8 * 1.001
 
 
-@pytest.mark.parametrize("method", ["MINRES", "PCG_PLAIN"])
-def test_full_size_minres_and_plain_pcg_against_oracle(c2, method):
-    """PASO's MINRES and the textbook PCG at 128^3 (symmetric storage, the
-    folded and non-temporal paths) against the oracle's restatements."""
+@pytest.mark.parametrize("method", ["MINRES", "PCG_PLAIN", "BICGSTAB"])
+def test_full_size_other_solvers_against_oracle(c2, method):
+    """PASO's MINRES and BiCGStab and the textbook PCG at 128^3 (symmetric
+    storage, the folded and non-temporal paths) against the oracle's
+    restatements.  BiCGStab's residual is not monotone and over ~1200
+    iterations its count moves with the reduction order (measured: GPU 1210,
+    oracle 1260), so its bar is 10 %; the others match to 2 %."""
     qn = c2["q"].cpu().numpy()
     oval = c2["oval"].copy()
     O.nullify(1, c2["ptr"], c2["idx"], oval, qn, qn, 1.0)
@@ -132,7 +135,8 @@ def test_full_size_minres_and_plain_pcg_against_oracle(c2, method):
     x = op.solve(rhs, opts).cpu().numpy()
     print(method, "128^3 symmetric iterations gpu", opts.getDiagnostics("num_iter"), "oracle", dg["num_iter"])
     assert opts.hasConverged()
-    assert abs(opts.getDiagnostics("num_iter") - dg["num_iter"]) <= 0.02 * dg["num_iter"]
+    bar = 0.10 if method == "BICGSTAB" else 0.02
+    assert abs(opts.getDiagnostics("num_iter") - dg["num_iter"]) <= bar * dg["num_iter"]
     assert np.linalg.norm(x - xo) <= 1e-6 * np.linalg.norm(xo)
     res = np.linalg.norm(orhs - O.spmv(1, c2["ptr"], c2["idx"], oval, x)) / np.linalg.norm(orhs)
     assert res <= 1e-7
