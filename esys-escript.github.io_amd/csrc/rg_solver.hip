@@ -334,6 +334,19 @@ __device__ __forceinline__ void st2(double* p, Two<double> v, bool two)
     }
 }
 
+template <bool NT>
+__device__ __forceinline__ double ld1(const double* p)
+{
+    if constexpr (NT) return __builtin_nontemporal_load(p);
+    else return *p;
+}
+template <bool NT>
+__device__ __forceinline__ void st1(double* p, double v)
+{
+    if constexpr (NT) __builtin_nontemporal_store(v, p);
+    else *p = v;
+}
+
 template <int BS, typename DT>
 __global__ __launch_bounds__(kBlock) void k_paso_init(PcgArgs a, const DT* __restrict__ dinv)
 {
@@ -902,7 +915,7 @@ __global__ __launch_bounds__(kBlock) void k_mr_spmv(PcgArgs a, const double* __r
     publish_block_sums<1>(d, a.partials);
 }
 
-template <int BS>
+template <int BS, bool NT = false>
 __global__ __launch_bounds__(kBlock) void k_mr_r(PcgArgs a, const double* __restrict__ dinv)
 {
     if (a.st->done) return;
@@ -916,11 +929,11 @@ __global__ __launch_bounds__(kBlock) void k_mr_r(PcgArgs a, const double* __rest
 #pragma unroll
         for (int c = 0; c < BS; ++c) {
             const int64_t k = i * BS + c;
-            const double rk = a.r[k];
-            double v = fma(-f1, rk, a.v[k]);
-            if (second) v = fma(-f2, a.rs[k], v);
-            a.rs[k] = rk;
-            a.r[k] = v;
+            const double rk = ld1<NT>(a.r + k);
+            double v = fma(-f1, rk, ld1<NT>(a.v + k));
+            if (second) v = fma(-f2, ld1<NT>(a.rs + k), v);
+            st1<NT>(a.rs + k, rk);
+            st1<NT>(a.r + k, v);
             rn[c] = v;
         }
         if (dinv) {   // one fused Jacobi sweep; otherwise z is computed afterwards
@@ -936,6 +949,7 @@ __global__ __launch_bounds__(kBlock) void k_mr_r(PcgArgs a, const double* __rest
     publish_block_sums<1>(s, a.partials);
 }
 
+template <bool NT = false>
 __global__ __launch_bounds__(kBlock) void k_mr_w(PcgArgs a, int64_t n)
 {
     if (!a.st->early) return;
@@ -943,16 +957,16 @@ __global__ __launch_bounds__(kBlock) void k_mr_w(PcgArgs a, int64_t n)
     const int it = a.st->num_iter - 1;   // the iteration whose update this is
     const bool k1 = it > 0, k2 = it > 1;
     for (int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x; q < n; q += (int64_t)gridDim.x * kBlock) {
-        double wv = a.s[q];
-        if (k2) wv = fma(-a3, a.x2[q], wv);   // w_ancient = w_old
+        double wv = ld1<NT>(a.s + q);
+        if (k2) wv = fma(-a3, ld1<NT>(a.x2 + q), wv);   // w_ancient = w_old
         if (k1) {
-            const double w = a.p[q];          // w_old = w
+            const double w = ld1<NT>(a.p + q);          // w_old = w
             wv = fma(-a2, w, wv);
-            a.x2[q] = w;
+            st1<NT>(a.x2 + q, w);
         }
         wv *= ia1;
-        a.p[q] = wv;
-        a.x[q] = fma(ceta, wv, a.x[q]);
+        st1<NT>(a.p + q, wv);
+        st1<NT>(a.x + q, fma(ceta, wv, ld1<NT>(a.x + q)));
     }
 }
 
@@ -1297,6 +1311,13 @@ struct SolverImpl {
     // One iteration of PASO's MINRES (FP64, any preconditioner).
     void mr_iteration(int sweeps)
     {
+        if (nt) mr_iteration_nt<true>(sweeps);
+        else mr_iteration_nt<false>(sweeps);
+    }
+
+    template <bool NT>
+    void mr_iteration_nt(int sweeps)
+    {
         const Geom& g = m->g;
         const bool buf = buffered(sweeps);
         const Launch Lsp = spmv_launch<double>(), Ln = chunked_launch(g.N);
@@ -1306,7 +1327,7 @@ struct SolverImpl {
             hipLaunchKernelGGL((k_mr_spmv<D.value, B.value, S.value>), dim3(Lsp.grid), dim3(kBlock), 0, st, asp,
                                m->val);
             stage<ST_MR_DELTA, 1>(Lsp.grid);
-            hipLaunchKernelGGL((k_mr_r<B.value>), dim3(Ln.grid), dim3(kBlock), 0, st, an,
+            hipLaunchKernelGGL((k_mr_r<B.value, NT>), dim3(Ln.grid), dim3(kBlock), 0, st, an,
                                buf ? nullptr : m->dinv);
         });
         if (buf) {
@@ -1316,7 +1337,7 @@ struct SolverImpl {
         } else {
             stage<ST_MR_GAMMA, 1>(Ln.grid);
         }
-        hipLaunchKernelGGL(k_mr_w, dim3(vg), dim3(kBlock), 0, st, an, w->n);
+        hipLaunchKernelGGL(k_mr_w<NT>, dim3(vg), dim3(kBlock), 0, st, an, w->n);
         RG_HIP(hipGetLastError());
     }
 
@@ -1498,14 +1519,17 @@ constexpr int64_t kMallBytes = 256ll << 20;
 
 // Whether the update kernels stream their vectors non-temporally: when the
 // SpMV's values fit in the MALL but values plus the vectors an iteration
-// touches (textbook CG: p, q, r, x, M^-1; PASO adds rs, x2) do not, the
-// vectors would otherwise evict the values between SpMVs.
+// touches (textbook CG: p, q, r, x, M^-1; PASO adds rs, x2; MINRES: r,
+// r_old, Az, z, z / gamma, w, w_old, x, M^-1) do not, the vectors would
+// otherwise evict the values between SpMVs.
 bool stream_policy(const rg_matrix* m, int method)
 {
-    if (method != RG_METHOD_PCG && method != RG_METHOD_PCG_PLAIN && method != RG_METHOD_PCG_MIXED) return false;
+    if (method != RG_METHOD_PCG && method != RG_METHOD_PCG_PLAIN && method != RG_METHOD_PCG_MIXED &&
+        method != RG_METHOD_MINRES)
+        return false;
     const int64_t vsize = method == RG_METHOD_PCG_MIXED ? (int64_t)sizeof(float) : (int64_t)sizeof(double);
     const int64_t val_bytes = (int64_t)m->g.nstore * m->g.bs * m->g.bs * m->g.ld * vsize;
-    const int64_t nvec = method == RG_METHOD_PCG ? 7 : 5;
+    const int64_t nvec = method == RG_METHOD_PCG ? 7 : (method == RG_METHOD_MINRES ? 9 : 5);
     const int64_t vec_bytes = nvec * (int64_t)m->g.N * m->g.bs * (int64_t)sizeof(double);
     return val_bytes <= kMallBytes && val_bytes + vec_bytes > kMallBytes;
 }
