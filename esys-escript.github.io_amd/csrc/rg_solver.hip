@@ -55,6 +55,20 @@ __device__ __forceinline__ void jacobi_node(const DT* __restrict__ dinv, int64_t
     }
 }
 
+// z = inv(D_i) r for a node whose r values are in registers.
+template <int BS, typename DT>
+__device__ __forceinline__ void jacobi_regs(const DT* __restrict__ dinv, int64_t ld, int64_t i,
+                                            const double (&r)[BS], double (&z)[BS])
+{
+#pragma unroll
+    for (int a = 0; a < BS; ++a) {
+        double s = 0.0;
+#pragma unroll
+        for (int c = 0; c < BS; ++c) s = fma((double)dinv[(int64_t)(a + BS * c) * ld + i], r[c], s);
+        z[a] = s;
+    }
+}
+
 // z = M^-1 r for node i: the stored sweep result if zbuf is given, else one
 // Jacobi sweep computed on the fly.
 template <int BS, typename DT>
@@ -742,16 +756,17 @@ __global__ __launch_bounds__(kBlock) void k_bicg_init(PcgArgs a)
     }
 }
 
+template <bool NT = false>
 __global__ __launch_bounds__(kBlock) void k_bicg_rho(PcgArgs a, int64_t n)
 {
     if (a.st->done) return;
     double s[1] = {0.0};
     for (int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x; k < n; k += (int64_t)gridDim.x * kBlock)
-        s[0] += a.rs[k] * a.r[k];
+        s[0] += ld1<NT>(a.rs + k) * ld1<NT>(a.r + k);
     publish_block_sums<1>(s, a.partials);
 }
 
-template <int BS>
+template <int BS, bool NT = false>
 __global__ __launch_bounds__(kBlock) void k_bicg_p(PcgArgs a, const double* __restrict__ dinv)
 {
     if (a.st->done) return;
@@ -759,16 +774,19 @@ __global__ __launch_bounds__(kBlock) void k_bicg_p(PcgArgs a, const double* __re
     const bool first = a.st->num_iter <= 1;
     const RowRange rr = block_rows(a.N, a.chunk);
     for (int64_t i = rr.r0 + threadIdx.x; i < rr.r1; i += kBlock) {
+        double pv[BS];
 #pragma unroll
         for (int c = 0; c < BS; ++c) {
             const int64_t k = i * BS + c;
-            a.p[k] = first ? a.r[k] : a.r[k] + beta * (a.p[k] - omega * a.v[k]);
+            const double rk = ld1<NT>(a.r + k);
+            pv[c] = first ? rk : rk + beta * (ld1<NT>(a.p + k) - omega * ld1<NT>(a.v + k));
+            st1<NT>(a.p + k, pv[c]);
         }
         if (dinv) {   // one fused Jacobi sweep; otherwise phat is computed afterwards
             double z[BS];
-            jacobi_node<BS>(dinv, a.ld, i, a.p, z);
+            jacobi_regs<BS>(dinv, a.ld, i, pv, z);
 #pragma unroll
-            for (int c = 0; c < BS; ++c) a.zp[i * BS + c] = z[c];
+            for (int c = 0; c < BS; ++c) st1<NT>(a.zp + i * BS + c, z[c]);
         }
     }
 }
@@ -798,7 +816,7 @@ __global__ __launch_bounds__(kBlock) void k_bicg_spmv(PcgArgs a, const double* _
     publish_block_sums<2>(s, a.partials);
 }
 
-template <int BS>
+template <int BS, bool NT = false>
 __global__ __launch_bounds__(kBlock) void k_bicg_s(PcgArgs a, const double* __restrict__ dinv)
 {
     if (a.st->done) return;
@@ -806,42 +824,46 @@ __global__ __launch_bounds__(kBlock) void k_bicg_s(PcgArgs a, const double* __re
     const RowRange rr = block_rows(a.N, a.chunk);
     double s[1] = {0.0};
     for (int64_t i = rr.r0 + threadIdx.x; i < rr.r1; i += kBlock) {
+        double sv[BS];
 #pragma unroll
         for (int c = 0; c < BS; ++c) {
             const int64_t k = i * BS + c;
-            const double rv = a.r[k] - alpha * a.v[k];
-            a.r[k] = rv;
-            a.s[k] = rv;
+            const double rv = ld1<NT>(a.r + k) - alpha * ld1<NT>(a.v + k);
+            st1<NT>(a.r + k, rv);
+            st1<NT>(a.s + k, rv);
+            sv[c] = rv;
             s[0] += rv * rv;
         }
         if (dinv) {
             double z[BS];
-            jacobi_node<BS>(dinv, a.ld, i, a.s, z);
+            jacobi_regs<BS>(dinv, a.ld, i, sv, z);
 #pragma unroll
-            for (int c = 0; c < BS; ++c) a.z2p[i * BS + c] = z[c];
+            for (int c = 0; c < BS; ++c) st1<NT>(a.z2p + i * BS + c, z[c]);
         }
     }
     publish_block_sums<1>(s, a.partials);
 }
 
 // x += alpha phat after convergence on ||s|| (BiCGStab.cpp:181-186)
+template <bool NT = false>
 __global__ __launch_bounds__(kBlock) void k_bicg_xa(PcgArgs a, int64_t n)
 {
     if (!a.st->early) return;
     const double alpha = a.st->alpha;
     for (int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x; k < n; k += (int64_t)gridDim.x * kBlock)
-        a.x[k] += alpha * a.zp[k];
+        st1<NT>(a.x + k, ld1<NT>(a.x + k) + alpha * ld1<NT>(a.zp + k));
 }
 
+template <bool NT = false>
 __global__ __launch_bounds__(kBlock) void k_bicg_update(PcgArgs a, int64_t n)
 {
     if (a.st->done) return;
     const double alpha = a.st->alpha, omega = a.st->gamma1;
     double s[1] = {0.0};
     for (int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x; k < n; k += (int64_t)gridDim.x * kBlock) {
-        a.x[k] += alpha * a.zp[k] + omega * a.z2p[k];
-        const double rv = a.s[k] - omega * a.x2[k];
-        a.r[k] = rv;
+        st1<NT>(a.x + k, ld1<NT>(a.x + k) + (alpha * ld1<NT>(a.zp + k) + omega * ld1<NT>(a.z2p + k)));
+        const double rv = ld1<NT>(a.s + k) - omega * ld1<NT>(a.x2 + k);
+        st1<NT>(a.r + k, rv);
         s[0] += rv * rv;
     }
     publish_block_sums<1>(s, a.partials);
@@ -858,19 +880,6 @@ __global__ __launch_bounds__(kBlock) void k_bicg_update(PcgArgs a, int64_t n)
 //   w_old, w = w, (s - a3 w_old - a2 w) / a1, x += c eta w  k_mr_w
 // Terms of the first two iterations that the reference skips are skipped
 // (not multiplied by zero): the buffers hold stale values then.
-template <int BS, typename DT>
-__device__ __forceinline__ void jacobi_regs(const DT* __restrict__ dinv, int64_t ld, int64_t i,
-                                            const double (&r)[BS], double (&z)[BS])
-{
-#pragma unroll
-    for (int a = 0; a < BS; ++a) {
-        double s = 0.0;
-#pragma unroll
-        for (int c = 0; c < BS; ++c) s = fma((double)dinv[(int64_t)(a + BS * c) * ld + i], r[c], s);
-        z[a] = s;
-    }
-}
-
 template <int BS>
 __global__ __launch_bounds__(kBlock) void k_mr_init(PcgArgs a, const double* __restrict__ dinv)
 {
@@ -1280,30 +1289,37 @@ struct SolverImpl {
     // One iteration of PASO's BiCGStab (FP64, one Jacobi sweep).
     void bicg_iteration(int sweeps)
     {
+        if (nt) bicg_iteration_nt<true>(sweeps);
+        else bicg_iteration_nt<false>(sweeps);
+    }
+
+    template <bool NT>
+    void bicg_iteration_nt(int sweeps)
+    {
         const Geom& g = m->g;
         const Launch Lsp = spmv_launch<double>(), Ln = chunked_launch(g.N);
         const PcgArgs an = args(nullptr, Ln), asp = args(nullptr, Lsp);
         const int vg = 1024;
         const bool buf = buffered(sweeps);
         const double* dinv = buf ? nullptr : m->dinv;
-        hipLaunchKernelGGL(k_bicg_rho, dim3(vg), dim3(kBlock), 0, st, an, w->n);
+        hipLaunchKernelGGL(k_bicg_rho<NT>, dim3(vg), dim3(kBlock), 0, st, an, w->n);
         stage<ST_BICG_RHO, 1>(vg);
         dispatch(g, m->symmetric, [&](auto D, auto B, auto S) {
-            hipLaunchKernelGGL((k_bicg_p<B.value>), dim3(Ln.grid), dim3(kBlock), 0, st, an, dinv);
+            hipLaunchKernelGGL((k_bicg_p<B.value, NT>), dim3(Ln.grid), dim3(kBlock), 0, st, an, dinv);
             if (buf) precondition(w->p, w->z, sweeps, w->st);   // phat = M^-1 p
             hipLaunchKernelGGL((k_bicg_spmv<D.value, B.value, S.value, 0>), dim3(Lsp.grid), dim3(kBlock), 0, st, asp,
                                m->val, w->z, w->v);
             stage<ST_BICG_ALPHA, 2>(Lsp.grid);
-            hipLaunchKernelGGL((k_bicg_s<B.value>), dim3(Ln.grid), dim3(kBlock), 0, st, an, dinv);
+            hipLaunchKernelGGL((k_bicg_s<B.value, NT>), dim3(Ln.grid), dim3(kBlock), 0, st, an, dinv);
             stage<ST_BICG_S, 1>(Ln.grid);
             if (buf) precondition(w->s, w->z2, sweeps, w->st);  // shat = M^-1 s
-            hipLaunchKernelGGL(k_bicg_xa, dim3(vg), dim3(kBlock), 0, st, an, w->n);
+            hipLaunchKernelGGL(k_bicg_xa<NT>, dim3(vg), dim3(kBlock), 0, st, an, w->n);
             stage<ST_BICG_XA, 1>(vg);
             hipLaunchKernelGGL((k_bicg_spmv<D.value, B.value, S.value, 1>), dim3(Lsp.grid), dim3(kBlock), 0, st, asp,
                                m->val, w->z2, w->x2);
             stage<ST_BICG_OMEGA, 2>(Lsp.grid);
         });
-        hipLaunchKernelGGL(k_bicg_update, dim3(vg), dim3(kBlock), 0, st, an, w->n);
+        hipLaunchKernelGGL(k_bicg_update<NT>, dim3(vg), dim3(kBlock), 0, st, an, w->n);
         stage<ST_BICG_END, 1>(vg);
         RG_HIP(hipGetLastError());
     }
@@ -1520,16 +1536,20 @@ constexpr int64_t kMallBytes = 256ll << 20;
 // Whether the update kernels stream their vectors non-temporally: when the
 // SpMV's values fit in the MALL but values plus the vectors an iteration
 // touches (textbook CG: p, q, r, x, M^-1; PASO adds rs, x2; MINRES: r,
-// r_old, Az, z, z / gamma, w, w_old, x, M^-1) do not, the vectors would
-// otherwise evict the values between SpMVs.
+// r_old, Az, z, z / gamma, w, w_old, x, M^-1; BiCGStab: r, r~, p, v, phat,
+// s, shat, t, x, M^-1) do not, the vectors would otherwise evict the values
+// between SpMVs.
 bool stream_policy(const rg_matrix* m, int method)
 {
     if (method != RG_METHOD_PCG && method != RG_METHOD_PCG_PLAIN && method != RG_METHOD_PCG_MIXED &&
-        method != RG_METHOD_MINRES)
+        method != RG_METHOD_MINRES && method != RG_METHOD_BICGSTAB)
         return false;
     const int64_t vsize = method == RG_METHOD_PCG_MIXED ? (int64_t)sizeof(float) : (int64_t)sizeof(double);
     const int64_t val_bytes = (int64_t)m->g.nstore * m->g.bs * m->g.bs * m->g.ld * vsize;
-    const int64_t nvec = method == RG_METHOD_PCG ? 7 : (method == RG_METHOD_MINRES ? 9 : 5);
+    const int64_t nvec = method == RG_METHOD_PCG        ? 7
+                         : method == RG_METHOD_MINRES   ? 9
+                         : method == RG_METHOD_BICGSTAB ? 10
+                                                        : 5;
     const int64_t vec_bytes = nvec * (int64_t)m->g.N * m->g.bs * (int64_t)sizeof(double);
     return val_bytes <= kMallBytes && val_bytes + vec_bytes > kMallBytes;
 }
