@@ -256,3 +256,23 @@ def test_minres_restatement_iteration_count_is_minimal_residual():
     _, dc, _, _ = O.solve(1, ptr, idx, val, rhs, tolerance=1e-8, method="PCG_PLAIN", preconditioner="NONE")
     assert dm["rc"] == 0 and dc["rc"] == 0
     assert dm["num_iter"] <= dc["num_iter"] + 2
+
+
+def test_minres_restatement_block_system_matches_direct():
+    # a coupled 2-component SPD system (block size 2): D = M M^T + 2 I per
+    # element, A = I per component
+    rng = np.random.default_rng(12)
+    ne = (9, 8)
+    E = int(np.prod(ne))
+    n = 2
+    A = np.zeros((E, n, 2, n, 2))
+    for k in range(n):
+        A[:, k, :, k, :] = np.eye(2) * (1.0 + k)
+    M = rng.standard_normal((n, n))
+    D = np.tile(M @ M.T + n * np.eye(n), (E, 1, 1))
+    ptr, idx, val, rhs = O.assemble(2, ne, (1.0, 1.0), n=n, A=A, D=D, Y=np.ones((E, n)))
+    dense = O.csr_to_dense(n, ptr, idx, val)
+    x, dg, _, _ = O.solve(n, ptr, idx, val, rhs, tolerance=1e-10, method="MINRES")
+    assert dg["rc"] == 0 and dg["converged"] == 1
+    xs = np.linalg.solve(dense, rhs)
+    assert np.linalg.norm(x - xs) <= 1e-8 * np.linalg.norm(xs)
