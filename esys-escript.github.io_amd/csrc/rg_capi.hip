@@ -253,6 +253,8 @@ int rg_matrix_add_element_blocks(rg_matrix* m, int64_t num_elements, int nodes_p
                  "addToSystemMatrix: number of equations does not match the block size");
         RG_CHECK(num_elements >= 0 && (num_elements == 0 || (nodes && em)), RG_VALUE_ERROR,
                  "addToSystemMatrix: NULL element data");
+        RG_CHECK(nodes_per_element == (1 << m->g.dim), RG_VALUE_ERROR,
+                 "add_element_blocks: nodes_per_element must be 4 (2-D) or 8 (3-D)");
         if (num_elements == 0) return RG_OK;
         use_device(m->ctx);
         for (int64_t k = 0; k < num_elements * nodes_per_element; ++k)

@@ -30,18 +30,61 @@ bool is_nan_value(T v)
     return false;
 }
 
-// One rank's share is the whole grid here (offset 0), so the reference's
-// per-rank offsets and "rest" counts are zero.
+// The file holds nv[0] x nv[1] (x nv[2]) values, x fastest, each value
+// num_comp components of type T.  Along axis d the values are placed from
+// sample max(first[d], 0) on, starting with value max(-first[d], 0), and each
+// value covers multiplier[d] consecutive samples; along z a reversed file
+// runs backwards from its last plane.  So every output sample s along an axis
+// has at most one source value, and the placement is a per-axis map
+//   src_d(s) = base_d + (s - start_d) / mul_d  (or none)
+// with start_d = max(first_d, 0), base_d = max(-first_d, 0), valid while
+// s >= start_d and (s - start_d) / mul_d < min(nv_d - base_d, n_d - start_d)
+// -- the count of values the reference's reader takes along each axis
+// (ripley/src/Brick.cpp:733-748 at one rank, offset 0).  The reader below
+// walks the output samples, maps each to its source value, and reads each
+// file line (one y row of one z plane) once, when the first sample that needs
+// it comes up.
+struct AxisMap {
+    std::vector<int64_t> src;   // file index of the value feeding sample s, or -1
+    int64_t lo = -1, hi = -1;   // smallest / largest file index used
+};
+
+AxisMap axis_map(int64_t n, int64_t first, int64_t nv, int64_t mul, bool reversed)
+{
+    AxisMap a;
+    a.src.assign((size_t)n, -1);
+    const int64_t start = std::max<int64_t>(first, 0);
+    const int64_t base = std::max<int64_t>(-first, 0);
+    const int64_t count = std::min(nv - base, n - start);
+    for (int64_t s = start; s < n; ++s) {
+        const int64_t j = (s - start) / mul;
+        if (j >= count) break;
+        const int64_t v = base + j;
+        const int64_t f = reversed ? nv - 1 - v : v;
+        a.src[(size_t)s] = f;
+        a.lo = a.lo < 0 ? f : std::min(a.lo, f);
+        a.hi = std::max(a.hi, f);
+    }
+    return a;
+}
+
+template <typename T>
+T decode(const unsigned char* p, bool swap)
+{
+    unsigned char b[sizeof(T)];
+    for (size_t k = 0; k < sizeof(T); ++k) b[k] = swap ? p[sizeof(T) - 1 - k] : p[k];
+    T v;
+    std::memcpy(&v, b, sizeof(T));
+    return v;
+}
+
 template <typename T>
 void read_grid(const std::string& filename, int dim, const int64_t* grid, int num_comp, int dpp,
                const int64_t* first, const int64_t* num_values, const int32_t* multiplier, const int32_t* reverse,
                bool swap, double* out)
 {
-    int64_t n[3] = {grid[0], grid[1], dim == 3 ? grid[2] : 1};
-    int64_t fst[3] = {first[0], first[1], dim == 3 ? first[2] : 0};
-    int64_t nv[3] = {num_values[0], num_values[1], dim == 3 ? num_values[2] : 1};
-    int64_t mul[3] = {multiplier[0], multiplier[1], dim == 3 ? multiplier[2] : 1};
-    const bool rev_z = dim == 3 && reverse[2] != 0;
+    const int64_t n[3] = {grid[0], grid[1], dim == 3 ? grid[2] : 1};
+    const int64_t nv[3] = {num_values[0], num_values[1], dim == 3 ? num_values[2] : 1};
     for (int d = 0; d < dim; ++d)
         RG_CHECK(multiplier[d] >= 1, RG_VALUE_ERROR, "readBinaryGrid(): all multipliers must be positive");
     if (dim == 2)
@@ -53,54 +96,50 @@ void read_grid(const std::string& filename, int dim, const int64_t* grid, int nu
     std::ifstream f(filename, std::ifstream::binary);
     RG_CHECK(!f.fail(), RG_VALUE_ERROR, "readBinaryGrid(): cannot open file " + filename);
     f.seekg(0, std::ios::end);
-    const int64_t filesize = f.tellg();
-    const int64_t reqsize = nv[0] * nv[1] * nv[2] * num_comp * (int64_t)sizeof(T);
-    RG_CHECK(filesize >= reqsize, RG_VALUE_ERROR, "readBinaryGrid(): not enough data in file");
+    const int64_t value_bytes = num_comp * (int64_t)sizeof(T);
+    RG_CHECK((int64_t)f.tellg() >= nv[0] * nv[1] * nv[2] * value_bytes, RG_VALUE_ERROR,
+             "readBinaryGrid(): not enough data in file");
 
-    for (int d = 0; d < dim; ++d)
-        if (fst[d] >= n[d] || fst[d] + nv[d] * mul[d] <= 0) return;   // nothing lands in the grid
-
-    int64_t first_w[3], idx[3], num[3];
+    AxisMap ax[3];
     for (int d = 0; d < 3; ++d) {
-        first_w[d] = std::max<int64_t>(0, fst[d]);       // first sample written
-        idx[d] = std::max<int64_t>(0, -fst[d]);          // first value read
-        num[d] = std::min(nv[d] - idx[d], n[d] - first_w[d]);
+        const bool active = d < dim;
+        ax[d] = axis_map(n[d], active ? first[d] : 0, nv[d], active ? multiplier[d] : 1,
+                         active && d == 2 && reverse[2] != 0);
     }
-    if (rev_z) idx[2] = nv[2] - idx[2] - 1;
-    const int z_step = rev_z ? -1 : 1;
+    if (ax[0].lo < 0 || ax[1].lo < 0 || ax[2].lo < 0) return;   // no sample receives a value
 
-    std::vector<T> values((size_t)std::max<int64_t>(0, num[0]) * num_comp);
-    for (int64_t z = 0; z < num[2]; ++z) {
-        for (int64_t y = 0; y < num[1]; ++y) {
-            const int64_t fileofs = num_comp * (idx[0] + (idx[1] + y) * nv[0] + (idx[2] + z_step * z) * nv[0] * nv[1]);
-            f.seekg(fileofs * (int64_t)sizeof(T));
-            f.read(reinterpret_cast<char*>(values.data()), num[0] * num_comp * (int64_t)sizeof(T));
-            RG_CHECK(!f.fail(), RG_VALUE_ERROR, "readBinaryGrid(): read error in " + filename);
-            for (int64_t x = 0; x < num[0]; ++x) {
-                // a block of mul0 x mul1 x mul2 samples receives this value
-                for (int64_t m2 = 0; m2 < mul[2]; ++m2) {
-                    const int64_t dz = first_w[2] + z * mul[2] + m2;
-                    if (dz >= n[2]) break;
-                    for (int64_t m1 = 0; m1 < mul[1]; ++m1) {
-                        const int64_t dy = first_w[1] + y * mul[1] + m1;
-                        if (dy >= n[1]) break;
-                        for (int64_t m0 = 0; m0 < mul[0]; ++m0) {
-                            const int64_t dx = first_w[0] + x * mul[0] + m0;
-                            if (dx >= n[0]) break;
-                            double* dest = out + (dx + dy * n[0] + dz * n[0] * n[1]) * (int64_t)dpp * num_comp;
-                            // as the reference: the components are written one
-                            // after the other, each to all data points of the
-                            // sample; a NaN component is skipped (the sample
-                            // keeps its fill) and the next one moves up
-                            for (int c = 0; c < num_comp; ++c) {
-                                T raw = values[x * num_comp + c];
-                                if (swap) swap_bytes(reinterpret_cast<char*>(&raw), sizeof(T));
-                                if (is_nan_value(raw)) continue;
-                                const double v = static_cast<double>(raw);
-                                for (int q = 0; q < dpp; ++q) *dest++ = v;
-                            }
-                        }
-                    }
+    // one file line: the x range [lo, hi] of row (fy, fz)
+    const int64_t xlo = ax[0].lo, width = ax[0].hi - ax[0].lo + 1;
+    std::vector<unsigned char> line((size_t)(width * value_bytes));
+    int64_t line_y = -1, line_z = -1;
+    const int64_t sample_size = (int64_t)dpp * num_comp;
+    for (int64_t z = 0; z < n[2]; ++z) {
+        const int64_t fz = ax[2].src[(size_t)z];
+        if (fz < 0) continue;
+        for (int64_t y = 0; y < n[1]; ++y) {
+            const int64_t fy = ax[1].src[(size_t)y];
+            if (fy < 0) continue;
+            if (fy != line_y || fz != line_z) {
+                f.seekg(((fz * nv[1] + fy) * nv[0] + xlo) * value_bytes);
+                f.read(reinterpret_cast<char*>(line.data()), width * value_bytes);
+                RG_CHECK(!f.fail(), RG_VALUE_ERROR, "readBinaryGrid(): read error in " + filename);
+                line_y = fy;
+                line_z = fz;
+            }
+            double* row = out + (z * n[1] + y) * n[0] * sample_size;
+            for (int64_t x = 0; x < n[0]; ++x) {
+                const int64_t fx = ax[0].src[(size_t)x];
+                if (fx < 0) continue;
+                // the sample's components in order, each copied to all of
+                // its data points; a NaN component is dropped, so the
+                // following components move up and the sample's last slots
+                // keep what they held (the reference reader's behaviour)
+                double* dest = row + x * sample_size;
+                const unsigned char* src = line.data() + (fx - xlo) * value_bytes;
+                for (int c = 0; c < num_comp; ++c) {
+                    const T v = decode<T>(src + c * sizeof(T), swap);
+                    if (is_nan_value(v)) continue;
+                    for (int q = 0; q < dpp; ++q) *dest++ = static_cast<double>(v);
                 }
             }
         }

@@ -164,8 +164,10 @@ __device__ __forceinline__ void stage_apply(KrylovState* st, const double (&tot)
         const bool maxi = st->num_iter > st->maxit;
         const bool brk = fabs(st->gamma1) <= 0.0;
         if (conv || maxi || brk) {
+            // PASO ranks the flags as BiCGStab.cpp:236-240 does: a limit or a
+            // breakdown reached in the same iteration as convergence wins
             st->done = 1;
-            st->status = conv ? KS_CONVERGED : (maxi ? KS_MAXITER : KS_BREAKDOWN);
+            st->status = maxi ? KS_MAXITER : (brk ? KS_BREAKDOWN : KS_CONVERGED);
         } else {
             st->tau_new = st->tau;
         }

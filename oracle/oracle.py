@@ -78,6 +78,9 @@ def lib():
             "orc_gs_apply": (None, [I, I64, P, P, P, P, P, I, I, P, P]),
             "orc_cds_add": (None, [I, I, I, I, P, I, P, P]),
             "orc_cds_spmv": (None, [I, I, I64, I, P, P, P, P]),
+            "orc_trace_begin": (None, [P, I, I64]),
+            "orc_trace_count": (I, []),
+            "orc_set_plant": (None, [I]),
             "orc_num_threads": (I, []),
             "orc_set_num_threads": (None, [I]),
         }
@@ -289,6 +292,32 @@ def solve(bs, ptr, index, val, b, tolerance=1e-8, iter_max=100000, sweeps=1, pla
     diag = {f: getattr(dg, f) for f, _ in Diagnostics._fields_}
     diag["rc"] = rc
     return x, diag, val, d
+
+
+PLANTS = {"none": 0, "pcg_no_smoothing": 1, "pcg_gamma_sign": 2, "bicgstab_beta": 3,
+          "bicgstab_omega_sign": 4, "minres_w_a3": 5}
+
+
+def traced_solve(bs, ptr, index, val, b, method="PCG", cap=200, plant="none", **kw):
+    """solve() with the first inner Krylov solve traced.  Returns
+    (x, diag, balanced val, d, vecs, scal): vecs[k] holds the four vectors
+    recorded after iteration k+1 and scal[k] its four scalars --
+    PCG: (x, rs, x2, r), (alpha, tau, gamma_1, ||rs||); BiCGStab: (x, r, s, t),
+    (rho, alpha, omega, ||r||); MINRES: (x, r, w, z), (delta, gamma, c,
+    rnorm_prec).  plant switches on a known mistake (PLANTS) for this call."""
+    n = (len(ptr) - 1) * bs
+    buf = np.zeros(cap * (4 * n + 4))
+    L = lib()
+    L.orc_set_plant(PLANTS[plant])
+    L.orc_trace_begin(_p(buf), int(cap), n)
+    try:
+        x, diag, v, d = solve(bs, ptr, index, val, b, method=method, **kw)
+        k = L.orc_trace_count()
+    finally:
+        L.orc_set_plant(0)
+        L.orc_trace_begin(None, 0, 0)
+    rec = buf[: k * (4 * n + 4)].reshape(k, 4 * n + 4)
+    return x, diag, v, d, rec[:, : 4 * n].reshape(k, 4, n), rec[:, 4 * n:]
 
 
 def coloring(ptr, index):

@@ -922,6 +922,54 @@ static double par_max_abs(int64_t n, const double* v)
     return m;
 }
 
+// ---------------------------------------------- test hooks: trace, plant
+// orc_trace_begin(buf, cap, n) makes the next Krylov solve (the first inner
+// call of the next orc_solve) record, after each of its iterations, four
+// vectors of n doubles and four scalars into buf (stride 4n + 4, at most cap
+// iterations); recording stops when that solve returns.  The tests check the
+// recorded iterates against invariants of the recurrences and against
+// library implementations of the same methods.  orc_set_plant(p) switches on
+// a known mistake in one recurrence so that those tests can show they catch
+// it (0: none; 1: PCG without residual smoothing, gamma = 1; 2: PCG with the
+// smoothing coefficient's sign flipped; 3: BiCGStab's beta without the
+// alpha/omega factor; 4: BiCGStab's omega with its sign flipped; 5: MINRES's
+// w recurrence without the a3 term).
+static double* g_trace = nullptr;
+static int g_trace_cap = 0, g_trace_count = 0, g_trace_on = 0;
+static int64_t g_trace_n = 0;
+static int g_plant = 0;
+
+ORC_API void orc_trace_begin(double* buf, int cap, int64_t n)
+{
+    g_trace = buf;
+    g_trace_cap = cap;
+    g_trace_n = n;
+    g_trace_count = 0;
+    g_trace_on = buf != nullptr && cap > 0;
+}
+
+ORC_API int orc_trace_count(void) { return g_trace_count; }
+
+ORC_API void orc_set_plant(int p) { g_plant = p; }
+
+static void trace_record(const double* v0, const double* v1, const double* v2, const double* v3, double s0,
+                         double s1, double s2, double s3)
+{
+    if (!g_trace_on || g_trace_count >= g_trace_cap) return;
+    const int64_t n = g_trace_n;
+    double* rec = g_trace + (size_t)g_trace_count * (4 * n + 4);
+    const double* v[4] = {v0, v1, v2, v3};
+    for (int k = 0; k < 4; ++k)
+        for (int64_t i = 0; i < n; ++i) rec[k * n + i] = v[k] ? v[k][i] : 0.0;
+    rec[4 * n] = s0;
+    rec[4 * n + 1] = s1;
+    rec[4 * n + 2] = s2;
+    rec[4 * n + 3] = s3;
+    ++g_trace_count;
+}
+
+static void trace_end() { g_trace_on = 0; }
+
 // ------------------------------------------------------------- solvers
 
 typedef struct {
@@ -1018,7 +1066,9 @@ static int solver_pcg_paso(Sys& S, double* r, double* x, int* iter, double* tole
                 sum_3 = 0; sum_4 = 0;
                 for (int t = 0; t < nt; ++t) { sum_3 += p3[t]; sum_4 += p4[t]; }
             }
-            const double gamma_1 = (std::fabs(sum_3) <= 0.0) ? 0 : -sum_4 / sum_3; // :306-343
+            double gamma_1 = (std::fabs(sum_3) <= 0.0) ? 0 : -sum_4 / sum_3;       // :306-343
+            if (g_plant == 1) gamma_1 = 1.0;
+            if (g_plant == 2) gamma_1 = -gamma_1;
             const double gamma_2 = 1.0 - gamma_1;
             const double sum_5 = par_sum(n, [&](int64_t i) {
                 rs[i] = gamma_2 * rs[i] + gamma_1 * r[i];
@@ -1030,8 +1080,10 @@ static int solver_pcg_paso(Sys& S, double* r, double* x, int* iter, double* tole
             convergeFlag = norm_of_residual <= tol;
             maxIterFlag = num_iter > maxit;
             breakFlag = (std::fabs(tau) <= 0.0);
+            trace_record(x, rs.data(), x2.data(), r, alpha, tau, gamma_1, norm_of_residual);
         }
     }
+    trace_end();
     *iter = num_iter;
     *tolerance = norm_of_residual;
     if (maxIterFlag) return MaxIterReached;
@@ -1057,7 +1109,7 @@ static int solver_bicgstab(Sys& S, double* r, double* x, int* iter, double* tole
         rho = par_sum(n, [&](int64_t i) { return rtld[i] * r[i]; });                 // :133
         if ((breakFlag = (std::fabs(rho) <= 0.0))) break;                            // :139
         if (num_iter > 1) {                                                          // :142-150
-            const double beta = rho / rho1 * (alpha / omega);
+            const double beta = g_plant == 3 ? rho / rho1 : rho / rho1 * (alpha / omega);
 #pragma omp parallel for schedule(static)
             for (int64_t i = 0; i < n; ++i) p[i] = r[i] + beta * (p[i] - omega * v[i]);
         } else {
@@ -1080,6 +1132,7 @@ static int solver_bicgstab(Sys& S, double* r, double* x, int* iter, double* tole
             for (int64_t i = 0; i < n; ++i) x[i] += alpha * phat[i];
             maxIterFlag = false;
             breakFlag = false;
+            trace_record(x, r, s.data(), nullptr, rho, alpha, 0.0, norm_of_residual);
             break;
         }
         S.prec(shat.data(), s.data());                                               // :189
@@ -1088,6 +1141,7 @@ static int solver_bicgstab(Sys& S, double* r, double* x, int* iter, double* tole
         const double omegaDenumtr = par_sum(n, [&](int64_t i) { return t[i] * t[i]; });
         if ((breakFlag = (std::fabs(omegaDenumtr) <= 0.0))) break;                   // :206
         omega = omegaNumtr / omegaDenumtr;
+        if (g_plant == 4) omega = -omega;
 #pragma omp parallel for schedule(static)
         for (int64_t i = 0; i < n; ++i) {                                            // :209-214
             x[i] += alpha * phat[i] + omega * shat[i];
@@ -1097,9 +1151,11 @@ static int solver_bicgstab(Sys& S, double* r, double* x, int* iter, double* tole
         convergeFlag = norm_of_residual <= tol;                                      // :221-223
         maxIterFlag = num_iter > maxit;
         breakFlag = std::fabs(omega) <= 0.0;
+        trace_record(x, r, s.data(), t.data(), rho, alpha, omega, norm_of_residual);
         if (convergeFlag || maxIterFlag || breakFlag) break;
         rho1 = rho;                                                                  // :228
     }
+    trace_end();
     *iter = num_iter;
     *tolerance = norm_of_residual;
     if (maxIterFlag) return MaxIterReached;
@@ -1119,10 +1175,13 @@ static int solver_minres(Sys& S, double* r, double* x, int* iter, double* tolera
 {
     const int64_t n = S.n;
     const int maxit = *iter;
-    if (maxit <= 0) return InputError;
+    if (maxit <= 0) {
+        trace_end();
+        return InputError;
+    }
     std::vector<double> znew(n), z(n), az(n), w(n, 0.0), r_old(n, 0.0), w_old(n, 0.0), r_anc(n, 0.0),
         w_anc(n, 0.0);
-    double delta, gamma = 0., gamma_old = 0., eta = 0., dp0 = 0., c = 1.0, c_old = 1.0;
+    double delta = 0., gamma = 0., gamma_old = 0., eta = 0., dp0 = 0., c = 1.0, c_old = 1.0;
     double c_anc = 1., s = 0.0, s_old = 0.0, s_anc, rnorm_prec = 1;
     double tol = 1., norm_scal = 1.;
     int num_iter = 0;
@@ -1179,7 +1238,7 @@ static int solver_minres(Sys& S, double* r, double* x, int* iter, double* tolera
             if (num_iter > 0) std::copy(w.begin(), w.end(), w_old.begin());
             for (int64_t i = 0; i < n; ++i) {
                 double v = z[i];
-                if (num_iter > 1) v -= a3 * w_anc[i];
+                if (num_iter > 1 && g_plant != 5) v -= a3 * w_anc[i];
                 if (num_iter > 0) v -= a2 * w_old[i];
                 w[i] = v * (1.0 / a1);
             }
@@ -1190,7 +1249,9 @@ static int solver_minres(Sys& S, double* r, double* x, int* iter, double* tolera
         std::copy(znew.begin(), znew.end(), z.begin());
         ++num_iter;
         if (!converge && num_iter >= maxit) status = MaxIterReached;
+        trace_record(x, r, w.data(), z.data(), delta, gamma, c, rnorm_prec);
     }
+    trace_end();
     *iter = num_iter;
     *tolerance = rnorm_prec / norm_scal;
     return status;

@@ -447,3 +447,28 @@ def test_chunk_boundaries_and_iteration_limits(method):
                   "oracle", dg["num_iter"])
             assert dg["rc"] == 1 and not opts.hasConverged()
             assert opts.getDiagnostics("num_iter") == dg["num_iter"]
+
+
+def test_bicgstab_iteration_limit_ranks_like_paso():
+    """Solver_BiCGStab reports a reached iteration limit (and a breakdown)
+    ahead of convergence found in the same iteration (BiCGStab.cpp:236-240):
+    for every limit up to past the converging count the GPU solve ends with
+    the oracle's status and iteration count (a small system, where both take
+    the same iterations)."""
+    op, rhs, ptr, idx, oval, orhs = _advection_problem(2, (7, 6), 9, 1)
+    _, dg0, _, _ = O.solve(1, ptr, idx, oval, orhs, tolerance=1e-9, method="BICGSTAB")
+    assert dg0["rc"] == 0
+    for limit in range(1, dg0["num_iter"] + 3):
+        _, dg, _, _ = O.solve(1, ptr, idx, oval, orhs, tolerance=1e-9, iter_max=limit, method="BICGSTAB")
+        opts = R.SolverOptions()
+        opts.setTolerance(1e-9)
+        opts.setIterMax(limit)
+        print("limit", limit, "oracle rc", dg["rc"], "iterations", dg["num_iter"], flush=True)
+        if dg["rc"] == 1:
+            with pytest.raises(R.PasoException, match="maximum number of iteration"):
+                op.solve(rhs, opts)
+        else:
+            assert dg["rc"] == 0
+            op.solve(rhs, opts)
+            assert opts.hasConverged()
+        assert opts.getDiagnostics("num_iter") == dg["num_iter"]
