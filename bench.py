@@ -17,6 +17,12 @@ import os
 import sys
 import time
 
+# Thread placement of the host OpenMP runtime used by the CPU baseline (the
+# oracle's PASO path): one thread per core, packed.  Set before numpy, torch
+# or the oracle load an OpenMP runtime, which reads these once.
+os.environ.setdefault("OMP_PROC_BIND", "close")
+os.environ.setdefault("OMP_PLACES", "cores")
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "esys-escript.github.io_amd"))
 
@@ -31,7 +37,7 @@ def parse():
     ap.add_argument("--ne", type=int, default=128)
     ap.add_argument("--method", default="all")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-iters", type=int, default=40)
+    ap.add_argument("--cpu-samples", type=int, default=3)
     ap.add_argument("--spmv-reps", type=int, default=50)
     ap.add_argument("--quiet", action="store_true")
     ap.add_argument("--extras", default="c1,c3,c4,c5",
@@ -79,23 +85,28 @@ class Problem:
         self.op.nullifyRowsAndCols(self.q, self.q, 1.0)
         self.rhs.masked_fill_(self.q > 0, 0.0)
 
-    def attach_network(self, weights):
-        """The learned preconditioner for this grid (weights trained at 64^3
-        on the same field family by ripley_hip.learned)."""
+    def attach_network(self, weights, halvings=2):
+        """The learned coarse-level solver for this grid: the network on the
+        level `halvings` times coarser (33^3 nodes for 128^3), weights trained
+        by ripley_hip.learned on Galerkin operators of the same field family."""
         from ripley_hip import learned
-        torch = self.torch
-        nodes = tuple(n + 1 for n in self.ne)
+        nodes = tuple(n // 2 ** halvings + 1 for n in self.ne)
         self.net = learned.LearnedPreconditioner(nodes, params=self.np.load(weights, allow_pickle=False))
-        self.logk = learned.nodal_log10(self.kappa.cpu(), self.ne).to("cuda")
 
-    def solve(self, method):
+    def solve(self, method, tolerance=1e-8, accept_failure=False):
         R = self.R
         o = R.SolverOptions()
         o.setSymmetryOn()
-        o.setTolerance(1e-8)
-        o.setTrueResidualCriterion(True)
+        o.setTolerance(tolerance)
+        if accept_failure:
+            o.setAcceptanceConvergenceFailureOn()
+        # "_crit": PASO's own stopping test (balanced ||r||_2 and ||r||_inf,
+        # paso/src/Solver.cpp:241); otherwise the FP64 true residual
+        o.setTrueResidualCriterion(not method.endswith("_crit"))
+        method = method[:-len("_crit")] if method.endswith("_crit") else method
         o.setSolverMethod({"paso": R.SolverOptions.PCG, "plain": R.SolverOptions.PCG_PLAIN,
-                           "mixed": R.SolverOptions.PCG_MIXED, "learned": R.SolverOptions.FCG_LEARNED,
+                           "mixed": R.SolverOptions.PCG_MIXED, "learned_sym": R.SolverOptions.FCG_LEARNED,
+                           "mg_sym": R.SolverOptions.FCG_LEARNED,
                            "plain_sym": R.SolverOptions.PCG_PLAIN, "mixed_sym": R.SolverOptions.PCG_MIXED,
                            "paso_sym": R.SolverOptions.PCG, "paso_gs": R.SolverOptions.PCG,
                            "paso_ilu0": R.SolverOptions.PCG, "bicgstab_sym": R.SolverOptions.BICGSTAB,
@@ -104,15 +115,23 @@ class Problem:
             o.setPreconditioner(R.SolverOptions.GAUSS_SEIDEL)
         elif method == "paso_ilu0":
             o.setPreconditioner(R.SolverOptions.ILU0)
-        if method == "learned":
-            self.op.setLearnedPreconditioner(self.net, self.logk)
-            o.setCheckInterval(8)
+        if method == "learned_sym":
+            # the multilevel flexible CG with the trained network on its
+            # coarsest level
+            self.op.setLearnedPreconditioner(self.net)
+            o.setCheckInterval(4)
+        elif method == "mg_sym":
+            # the same flexible CG without the network: the hierarchy goes
+            # down to 9^3 nodes, its coarsest level smoothed
+            self.op.setLearnedPreconditioner(None)
+            self.op.setMultilevel(0, 2, 8)
+            o.setCheckInterval(4)
         self.op.setToSolution(self.x, self.rhs, o)
         return o
 
     def select(self, method):
         """The operator storage a method runs on."""
-        self.op = self.ops["sym" if method.endswith("_sym") else "full"]
+        self.op = self.ops["sym" if "_sym" in method else "full"]
 
     def step(self, method):
         self.select(method)
@@ -129,6 +148,23 @@ def true_residual(P):
     r = P.rhs.clone()
     P.op.MatrixVector(-1.0, P.x, 1.0, r)
     return (torch.linalg.norm(r) / torch.linalg.norm(P.rhs)).item()
+
+
+def reference_solution(P, torch):
+    """The FP64 PASO solve of the same inputs (PASO PCG with Jacobi on full
+    storage, the drop-in, PASO's own criterion) to rtol 1e-11, where FP64
+    leaves PASO's outer loop little room (a stop on "no improvement" is
+    accepted): what every method's solution is compared with."""
+    P.select("paso")
+    P.assemble()
+    P.constrain()
+    P.solve("paso_crit", tolerance=1e-11, accept_failure=True)
+    torch.cuda.synchronize()
+    return P.x.clone()
+
+
+def rel_error(torch, x, xref):
+    return (torch.linalg.norm(x - xref) / torch.linalg.norm(xref)).item()
 
 
 def time_steps(P, method, steps, warmup, torch):
@@ -236,37 +272,80 @@ def copy_bandwidth(torch):
     return 2 * 8 * n / dt / 1e9
 
 
-def cpu_baseline(P, gpu_paso_iters, iters, np):
-    """The oracle's PASO restatement (OpenMP) on the host: assembly timed in
-    full, PASO PCG timed over a bounded number of iterations and scaled to
-    the iteration count the same algorithm needs on this problem."""
+def physical_cores():
+    """Physical cores of the host (lscpu: sockets x cores per socket), or None."""
+    import subprocess
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+    except Exception:
+        return None
+    f = {}
+    for line in out.splitlines():
+        if ":" in line:
+            k, v = line.split(":", 1)
+            f[k.strip()] = v.strip()
+    try:
+        return int(f["Socket(s)"]) * int(f["Core(s) per socket"])
+    except (KeyError, ValueError):
+        return None
+
+
+def cpu_threads():
+    """Threads of the CPU baseline: the host's physical cores, capped by the
+    CPU share this process is given (OMP_NUM_THREADS where set)."""
+    cores = physical_cores() or os.cpu_count() or 1
+    share = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    return (min(cores, share) if share > 0 else cores), cores
+
+
+def cpu_baseline(P, samples, np):
+    """The oracle's PASO restatement (C++/OpenMP) on the host, complete solves
+    of the same problem: pattern, assembly, constraints, then Solver() with
+    PASO PCG (balance + Jacobi set-up, iterations) to PASO's own rtol 1e-8
+    criterion.  Median over `samples` runs; per-phase times of the median
+    run.  Returns (baseline dict, oracle solution of the median run)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
-    nthreads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
+    nthreads, cores = cpu_threads()
     O.set_num_threads(nthreads)
     ne = P.ne
     kap = P.kappa_np
-    t0 = time.perf_counter()
-    ptr, idx = O.pattern(3, ne)
-    t1 = time.perf_counter()
-    # isotropic kappa I per element, passed as a full tensor (the oracle has no compact form)
-    Afull = kap[:, None, None] * np.eye(3)[None]
-    _, _, val, rhs = O.assemble(3, ne, (1.0, 1.0, 1.0), ptr=ptr, index=idx, A=Afull, Y=1.0)
-    del Afull
-    t2 = time.perf_counter()
     q = P.q.cpu().numpy()
-    O.nullify(1, ptr, idx, val, q, q, 1.0)
-    rhs = np.where(q > 0, 0.0, rhs)
-    t3 = time.perf_counter()
-    tit = O.time_pcg_iterations(1, ptr, idx, val, rhs, iters, plain=False)
-    per_iter = tit / iters
-    est = (t1 - t0) + (t2 - t1) + (t3 - t2) + per_iter * gpu_paso_iters
-    return {"value": est, "unit": "s", "cores": nthreads, "kind": "port",
-            "sample": ("oracle PASO path on %d host threads: pattern %.2fs + assembly %.2fs + constraints "
-                       "%.2fs timed in full; %d PASO PCG iterations timed (%.4f s/iter) and scaled to the "
-                       "%d iterations PASO PCG needs on this problem" %
-                       (nthreads, t1 - t0, t2 - t1, t3 - t2, iters, per_iter, gpu_paso_iters)),
-            "per_iter_s": per_iter, "assemble_s": t2 - t1}
+    runs = []
+    for _ in range(samples):
+        t0 = time.perf_counter()
+        ptr, idx = O.pattern(3, ne)
+        t1 = time.perf_counter()
+        # isotropic kappa I per element, passed as a full tensor (the oracle has no compact form)
+        Afull = kap[:, None, None] * np.eye(3)[None]
+        _, _, val, rhs = O.assemble(3, ne, (1.0, 1.0, 1.0), ptr=ptr, index=idx, A=Afull, Y=1.0)
+        del Afull
+        t2 = time.perf_counter()
+        O.nullify(1, ptr, idx, val, q, q, 1.0)
+        rhs = np.where(q > 0, 0.0, rhs)
+        t3 = time.perf_counter()
+        x, dg, _, _ = O.solve(1, ptr, idx, val, rhs, tolerance=1e-8)
+        t4 = time.perf_counter()
+        runs.append({"total_s": t4 - t0, "pattern_s": t1 - t0, "assemble_s": t2 - t1, "constrain_s": t3 - t2,
+                     "solve_s": t4 - t3, "set_up_s": dg["set_up_time"], "iterations_s": dg["net_time"],
+                     "num_iter": dg["num_iter"], "converged": bool(dg["converged"]), "x": x})
+        del val, ptr, idx
+    order = sorted(range(len(runs)), key=lambda k: runs[k]["total_s"])
+    med = runs[order[len(runs) // 2]]
+    x = med.pop("x")
+    for r in runs:
+        r.pop("x", None)
+    base = {"value": round(med["total_s"], 4), "unit": "s", "cores": nthreads, "kind": "port",
+            "physical_cores": cores, "samples": len(runs),
+            "sample": ("oracle PASO path (pattern, assembly, constraints, balance + Jacobi set-up, PASO PCG to "
+                       "PASO's rtol 1e-8 criterion) solved %d times in full on %d host threads "
+                       "(OMP_PROC_BIND=%s, OMP_PLACES=%s; host has %s physical cores, the GPU job's CPU share "
+                       "is OMP_NUM_THREADS=%s); median run" %
+                       (len(runs), nthreads, os.environ.get("OMP_PROC_BIND"), os.environ.get("OMP_PLACES"), cores,
+                        os.environ.get("OMP_NUM_THREADS", "unset"))),
+            "phases_s": {k: round(v, 4) for k, v in med.items() if k.endswith("_s")},
+            "num_iter": med["num_iter"], "all_runs_s": [round(r["total_s"], 4) for r in runs]}
+    return base, x
 
 
 # ------------------------------------------------------------------ C1, C3, C4
@@ -328,7 +407,7 @@ def bench_c1(R, torch, np, args):
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle as O
-        nth = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
+        nth, _ = cpu_threads()
         O.set_num_threads(nth)
         t0 = time.perf_counter()
         ptr, idx = O.pattern(2, (256, 256))
@@ -420,7 +499,7 @@ def bench_c3(R, torch, np, args):
         # to the iteration count PASO PCG needs here
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle as O
-        nth = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
+        nth, _ = cpu_threads()
         O.set_num_threads(nth)
         t0 = time.perf_counter()
         ptr, idx = O.pattern(3, ne)
@@ -545,8 +624,9 @@ def bench_c4(R, torch, np, args):
 def bench_c5(R, torch, np, args):
     """Brick 192^3 Poisson, kappa = exp(GRF) with sigma_log 2 and correlation
     length 12 cells (seed 20261206), Y = 1, u = 0 on z = 0: plain FP64
-    Jacobi-PCG vs FP32-SpMV PCG vs the flexible CG with the learned
-    preconditioner trained on 64^3 fields of the same family."""
+    Jacobi-PCG vs FP32-SpMV PCG vs the multilevel flexible CG with the
+    learned coarse-level solver (trained on Galerkin operators of 64^3 fields
+    of the same family) and without it."""
     from ripley_hip import fields, learned
     ne = (192, 192, 192)
     dom = R.Brick(*ne)
@@ -567,24 +647,48 @@ def bench_c5(R, torch, np, args):
 
     out = {"workload": "ripley Brick 192^3 Poisson, ln kappa GRF (sigma 2, corr. 12 cells), Y=1, u=0 on z=0, "
                        "rtol 1e-8", "unknowns": dom.getNumNodes()}
-    weights = os.path.join(ROOT, "esys-escript.github.io_amd", "ripley_hip", "weights", "lpc_lognormal_s2_l12.npy")
-    methods = ["PCG_PLAIN", "PCG_MIXED", "PCG_PLAIN_sym"] + (["FCG_LEARNED"] if os.path.exists(weights) else [])
+    weights = os.path.join(ROOT, "esys-escript.github.io_amd", "ripley_hip", "weights",
+                           "lpc_coarse_lognormal_s2_l12.npy")
+    methods = ["PCG_PLAIN", "PCG_MIXED", "PCG_PLAIN_sym", "MULTILEVEL_sym"]
+    if os.path.exists(weights):
+        methods.append("FCG_LEARNED_sym")
     net = None
+    xref = None
     for name in methods:
+        sym = name.endswith("_sym")
         method = name.replace("_sym", "")
-        op, rhs = system(name.endswith("_sym"))
-        o = _opts(R, method)
+        op, rhs = system(sym)
+        o = _opts(R, "FCG_LEARNED" if method == "MULTILEVEL" else method)
         if method == "FCG_LEARNED":
-            net = learned.LearnedPreconditioner(tuple(n + 1 for n in ne), params=np.load(weights, allow_pickle=False))
-            op.setLearnedPreconditioner(net, learned.nodal_log10(kap.cpu(), ne).to("cuda"))
-            o.setCheckInterval(8)
+            # the network on the level two halvings down (49^3 nodes)
+            net = learned.LearnedPreconditioner(tuple(n // 4 + 1 for n in ne),
+                                                params=np.load(weights, allow_pickle=False))
+            op.setLearnedPreconditioner(net)
+            o.setCheckInterval(4)
+        elif method == "MULTILEVEL":
+            op.setMultilevel(0, 2, 8)
+            o.setCheckInterval(4)
+        op.solve(rhs, o)   # warm-up: set-up and graph capture
+        op, rhs = system(sym)
+        if method == "FCG_LEARNED":
+            op.setLearnedPreconditioner(net)
+        elif method == "MULTILEVEL":
+            op.setMultilevel(0, 2, 8)
         t, x = _timed(torch, lambda: op.solve(rhs, o))
         op2, _ = system()
         out[name] = {"solve_s": t, "num_iter": o.getDiagnostics("num_iter"), "converged": o.hasConverged(),
                      "fallbacks": o.getDiagnostics("fallbacks"), "rel_residual_fp64": _rel_res(torch, op2, x, rhs)}
+        if xref is None:
+            # FP64 PASO PCG to rtol 1e-12 on the same inputs: the reference solution
+            op3, rhs3 = system()
+            xref = op3.solve(rhs3, _opts(R, "PCG", tol=1e-12, true_res=False))
+            del op3
+        out[name]["rel_error_vs_fp64_paso"] = (torch.linalg.norm(x - xref) / torch.linalg.norm(xref)).item()
         del op, op2
-    if "FCG_LEARNED" in out:
-        out["learned_speedup_vs_plain"] = out["PCG_PLAIN"]["solve_s"] / out["FCG_LEARNED"]["solve_s"]
+    best_plain = min(out[m]["solve_s"] for m in ("PCG_PLAIN", "PCG_PLAIN_sym"))
+    for m in ("MULTILEVEL_sym", "FCG_LEARNED_sym"):
+        if m in out:
+            out[m]["speedup_vs_plain_pcg"] = best_plain / out[m]["solve_s"]
     return out
 
 
@@ -605,25 +709,33 @@ def main():
     log(args, "building C2 problem ne=%d" % args.ne)
     P = Problem(args.ne, R, torch, np, fields)
     methods = (["plain", "paso", "mixed", "plain_sym", "paso_sym", "mixed_sym", "paso_gs", "paso_ilu0",
-                "bicgstab_sym", "minres_sym"]
+                "bicgstab_sym", "minres_sym", "paso_crit", "paso_sym_crit"]
                if args.method == "all"
                else [args.method])
-    weights = os.path.join(ROOT, "esys-escript.github.io_amd", "ripley_hip", "weights", "lpc_log10grf_s1_l8.npy")
+    if args.method == "all":
+        methods.append("mg_sym")
+    weights = os.path.join(ROOT, "esys-escript.github.io_amd", "ripley_hip", "weights", "lpc_coarse_log10grf_s1_l8.npy")
     if args.method == "all" and os.path.exists(weights):
-        methods.append("learned")
-    if "learned" in methods:
+        methods.append("learned_sym")
+    if "learned_sym" in methods:
         P.attach_network(weights)
     results = {}
+    log(args, "reference solution (FP64 PASO PCG, rtol 1e-12)")
+    xref = reference_solution(P, torch)
     for m in methods:
         log(args, "method", m)
         bd = breakdown(P, m, torch)
         rel = true_residual(P)
-        t, opts = time_steps(P, m, args.steps if m != "learned" else 1, args.warmup if m != "learned" else 0,
-                             torch)
-        results[m] = dict(bd, step_s=t, rel_residual_fp64=rel, converged=bool(opts.hasConverged()),
-                          fallbacks=opts.getDiagnostics("fallbacks"))
+        err = rel_error(torch, P.x, xref)
+        t, opts = time_steps(P, m, args.steps, args.warmup, torch)
+        results[m] = dict(bd, step_s=t, rel_residual_fp64=rel, rel_error_vs_fp64_paso=err,
+                          converged=bool(opts.hasConverged()), fallbacks=opts.getDiagnostics("fallbacks"),
+                          criterion="paso" if m.endswith("_crit") else "true_residual")
         log(args, m, json.dumps(results[m]))
-    ok = {m: r for m, r in results.items() if r["converged"] and r["rel_residual_fp64"] <= 1e-8 * 1.01}
+    # the methods on the true-residual criterion must meet the metric's 1e-8
+    # on the FP64 residual; PASO's own criterion is reported beside them
+    ok = {m: r for m, r in results.items()
+          if r["converged"] and r["rel_residual_fp64"] <= 1e-8 * 1.01 and not m.endswith("_crit")}
     # the headline value is the fastest PCG computing in FP64 throughout
     # (the reference's arithmetic and the metric's method); the FP32-value,
     # learned, BiCGStab and MINRES runs are reported beside it
@@ -640,12 +752,14 @@ def main():
         tj = json.load(open(tfile))
         traffic = tj.get("hbm_bytes_per_launch")
         traffic_sym = tj.get("symmetric", {}).get("hbm_bytes_per_launch")
+    tsrc = ("rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of the same kernel, stored in "
+            "profiles/spmv_fp64_brick128_traffic.json (not measured in this run)")
     roof_full = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                 "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": traffic,
+                 "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": traffic, "traffic_source": tsrc,
                  "kernel": "k_spmv<3,1,false,double,0> (FP64 27-point stencil SpMV, y = A x, all 27 slots)",
                  "alg_bytes_per_launch": alg, "avg_launch_s": dt}
     roof_sym = {"bound": "hbm", "achieved": round(algs / dts / 1e9, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                "frac": round(algs / dts / 1e9 / HBM_PEAK_GBPS, 4), "traffic": traffic_sym,
+                "frac": round(algs / dts / 1e9 / HBM_PEAK_GBPS, 4), "traffic": traffic_sym, "traffic_source": tsrc,
                 "kernel": "k_spmv<3,1,true,double,0> (FP64 27-point stencil SpMV, y = A x, main + 13 upper "
                           "slots; the lower ones re-read transposed)",
                 "alg_bytes_per_launch": algs, "avg_launch_s": dts}
@@ -655,9 +769,9 @@ def main():
     copy_bw = copy_bandwidth(torch)
     cpu = None
     if not args.no_cpu:
-        paso_iters = results.get("paso", {}).get("num_iter") or results[head]["num_iter"]
         log(args, "cpu baseline")
-        cpu = cpu_baseline(P, paso_iters, args.cpu_iters, np)
+        cpu, xo = cpu_baseline(P, args.cpu_samples, np)
+        cpu["rel_error_vs_fp64_paso"] = rel_error(torch, torch.as_tensor(xo, device="cuda"), xref)
     N = P.op.num_rows
     hr = results[head]
     line = {
@@ -672,8 +786,10 @@ def main():
         "methods": results,
         "speedup_vs_plain": round(results["plain"]["step_s"] / hr["step_s"], 3) if "plain" in results else None,
         "fastest_any_precision": {"method": fastest, "step_s": round(results[fastest]["step_s"], 6)},
-        "learned_fcg_speedup_vs_plain": (round(results["plain"]["step_s"] / results["learned"]["step_s"], 3)
-                                         if "plain" in results and "learned" in results else None),
+        "learned_fcg_speedup_vs_headline": (round(hr["step_s"] / results["learned_sym"]["step_s"], 3)
+                                            if "learned_sym" in results else None),
+        "multilevel_fcg_speedup_vs_headline": (round(hr["step_s"] / results["mg_sym"]["step_s"], 3)
+                                               if "mg_sym" in results else None),
         "dof_iter_per_s": round(N * hr["num_iter"] / hr["solve_s"], 1),
         "roofline": roof,
         ("roofline_full_storage" if head_sym else "roofline_symmetric_storage"): roof_other,

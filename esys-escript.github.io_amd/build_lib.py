@@ -17,7 +17,7 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libripley_hip.so")
-SOURCES = ["rg_matrix.hip", "rg_solver.hip", "rg_precond.hip", "rg_capi.hip", "rg_batch.hip", "rg_learned.hip", "rg_io.hip"]
+SOURCES = ["rg_matrix.hip", "rg_solver.hip", "rg_precond.hip", "rg_capi.hip", "rg_batch.hip", "rg_learned.hip", "rg_io.hip", "rg_mg.hip"]
 
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
 # Packed FP32 instructions are disabled: hipcc -O3 would otherwise form

@@ -13,6 +13,7 @@
 #include <vector>
 
 #include "rg_internal.h"
+#include "rg_mg.h"
 
 namespace rg {
 int solve(rg_matrix* m, double* x, const double* b, const rg_options& opt, rg_diagnostics& dg);
@@ -57,6 +58,44 @@ struct DevBuf {
 void check_matrix(const rg_matrix* m)
 {
     RG_CHECK(m != nullptr, RG_VALUE_ERROR, "matrix handle is NULL");
+}
+
+size_t align256(size_t n) { return (n + 255) & ~(size_t)255; }
+
+// Argument checks of addToSystemMatrix (RipleyDomain.cpp:1433-1459): block
+// size, element size and node ids, before any array is touched.  Returns
+// false when there is nothing to add; sets the staging sizes of the node ids
+// and element matrices.
+bool check_element_blocks(const rg_matrix* m, int64_t num_elements, int nodes_per_element, const int32_t* nodes,
+                          int num_eq, const double* em, size_t* nsz, size_t* esz)
+{
+    check_matrix(m);
+    RG_CHECK(num_eq == m->g.bs, RG_VALUE_ERROR, "addToSystemMatrix: number of equations does not match the block size");
+    RG_CHECK(num_elements >= 0 && (num_elements == 0 || (nodes && em)), RG_VALUE_ERROR,
+             "addToSystemMatrix: NULL element data");
+    RG_CHECK(nodes_per_element == (1 << m->g.dim), RG_VALUE_ERROR,
+             "add_element_blocks: nodes_per_element must be 4 (2-D) or 8 (3-D)");
+    if (num_elements == 0) return false;
+    for (int64_t k = 0; k < num_elements * nodes_per_element; ++k)
+        RG_CHECK(nodes[k] >= 0 && nodes[k] < m->g.N, RG_VALUE_ERROR, "addToSystemMatrix: node index out of range");
+    *nsz = sizeof(int32_t) * (size_t)num_elements * nodes_per_element;
+    *esz = sizeof(double) * (size_t)num_elements * nodes_per_element * nodes_per_element * num_eq * num_eq;
+    return true;
+}
+
+// Stages the node ids at buf and the element matrices at buf + off (device)
+// and launches the colour passes.  The copies read pageable host memory, so
+// they return once the host arrays are staged by the runtime; no stream
+// synchronisation follows, and the next call's copies are ordered after
+// these kernels on the same stream.
+void add_element_blocks_via(rg_matrix* m, unsigned char* buf, size_t off, int64_t num_elements, int npe,
+                            const int32_t* nodes, size_t nsz, const double* em, size_t esz)
+{
+    RG_HIP(hipMemcpyAsync(buf, nodes, nsz, hipMemcpyHostToDevice, m->ctx->stream));
+    RG_HIP(hipMemcpyAsync(buf + off, em, esz, hipMemcpyHostToDevice, m->ctx->stream));
+    rg::launch_add_element_blocks(m, num_elements, npe, reinterpret_cast<const int32_t*>(buf),
+                                  reinterpret_cast<const double*>(buf + off));
+    m->val32_valid = false;
 }
 
 }  // namespace
@@ -194,9 +233,10 @@ int rg_matrix_destroy(rg_matrix* m)
         if (!m) return RG_OK;
         hipSetDevice(m->ctx->device);
         hipStreamSynchronize(m->ctx->stream);
+        rg::mg_free(m);
         rg::free_work(m->work);
         rg::free_colouring(m);
-        void* bufs[] = {m->val, m->val32, m->dinv, m->dinv32, m->balance};
+        void* bufs[] = {m->val, m->val32, m->dinv, m->dinv32, m->balance, m->stage};
         for (void* b : bufs)
             if (b) hipFree(b);
         delete m;
@@ -248,37 +288,24 @@ int rg_matrix_add_element_blocks(rg_matrix* m, int64_t num_elements, int nodes_p
                                  const int32_t* nodes, int num_eq, const double* em)
 {
     return guarded([&] {
-        check_matrix(m);
-        RG_CHECK(num_eq == m->g.bs, RG_VALUE_ERROR,
-                 "addToSystemMatrix: number of equations does not match the block size");
-        RG_CHECK(num_elements >= 0 && (num_elements == 0 || (nodes && em)), RG_VALUE_ERROR,
-                 "addToSystemMatrix: NULL element data");
-        RG_CHECK(nodes_per_element == (1 << m->g.dim), RG_VALUE_ERROR,
-                 "add_element_blocks: nodes_per_element must be 4 (2-D) or 8 (3-D)");
-        if (num_elements == 0) return RG_OK;
+        size_t nsz = 0, esz = 0;
+        if (!check_element_blocks(m, num_elements, nodes_per_element, nodes, num_eq, em, &nsz, &esz)) return RG_OK;
         use_device(m->ctx);
-        for (int64_t k = 0; k < num_elements * nodes_per_element; ++k)
-            RG_CHECK(nodes[k] >= 0 && nodes[k] < m->g.N, RG_VALUE_ERROR,
-                     "addToSystemMatrix: node index out of range");
-        const size_t nsz = sizeof(int32_t) * num_elements * nodes_per_element;
-        const size_t esz = sizeof(double) * num_elements * nodes_per_element * nodes_per_element * num_eq * num_eq;
-        void* dn = nullptr;
-        void* de = nullptr;
-        try {
-            RG_HIP(hipMalloc(&dn, nsz));
-            RG_HIP(hipMalloc(&de, esz));
-            RG_HIP(hipMemcpyAsync(dn, nodes, nsz, hipMemcpyHostToDevice, m->ctx->stream));
-            RG_HIP(hipMemcpyAsync(de, em, esz, hipMemcpyHostToDevice, m->ctx->stream));
-            rg::launch_add_element_blocks(m, num_elements, nodes_per_element, (const int32_t*)dn, (const double*)de);
-            RG_HIP(hipStreamSynchronize(m->ctx->stream));
-        } catch (...) {
-            if (dn) hipFree(dn);
-            if (de) hipFree(de);
-            throw;
+        // persistent staging: grown (never per call) when a larger batch arrives
+        const size_t off = align256(nsz);
+        if (off + esz > m->stage_bytes) {
+            if (m->stage) {
+                RG_HIP(hipStreamSynchronize(m->ctx->stream));   // the old buffer may still be read
+                RG_HIP(hipFree(m->stage));
+                m->stage = nullptr;
+                m->stage_bytes = 0;
+            }
+            const size_t want = std::max<size_t>(off + esz, (size_t)1 << 20);
+            RG_HIP(hipMalloc(&m->stage, want));
+            m->stage_bytes = want;
         }
-        hipFree(dn);
-        hipFree(de);
-        m->val32_valid = false;
+        add_element_blocks_via(m, static_cast<unsigned char*>(m->stage), off, num_elements, nodes_per_element,
+                               nodes, nsz, em, esz);
         return RG_OK;
     });
 }
@@ -533,22 +560,55 @@ int rg_matrix_set_learned_preconditioner(rg_matrix* m, rg_lpc* net, const float*
 {
     return guarded([&] {
         check_matrix(m);
+        (void)logk;   // reserved: the network's coefficient input comes from its level's operator
+        if (m->work) m->work->graph_method = -1;   // captured iterations refer to the old network
         if (!net) {
             m->lpc = nullptr;
             m->lpc_logk = nullptr;
-            if (m->work) m->work->graph_method = -1;
             return RG_OK;
         }
         RG_CHECK(m->g.dim == 3 && m->g.bs == 1, RG_VALUE_ERROR,
                  "set_learned_preconditioner: scalar problems on a Brick only");
         int32_t nodes[3];
         rg::lpc_nodes(net, nodes);
-        for (int d = 0; d < 3; ++d)
-            RG_CHECK(nodes[d] == m->g.nn[d], RG_VALUE_ERROR,
-                     "set_learned_preconditioner: network node grid does not match the matrix");
+        bool level = false;
+        int ne[3] = {m->g.ne[0], m->g.ne[1], m->g.ne[2]};
+        while (ne[0] % 2 == 0 && ne[1] % 2 == 0 && ne[2] % 2 == 0 && ne[0] >= 2 && ne[1] >= 2 && ne[2] >= 2) {
+            for (int d = 0; d < 3; ++d) ne[d] /= 2;
+            level = level || (nodes[0] == ne[0] + 1 && nodes[1] == ne[1] + 1 && nodes[2] == ne[2] + 1);
+        }
+        RG_CHECK(level, RG_VALUE_ERROR,
+                 "set_learned_preconditioner: the network's node grid is not a coarse level of the matrix's grid "
+                 "(every other node per level)");
         m->lpc = net;
-        m->lpc_logk = logk;
-        if (m->work) m->work->graph_method = -1;   // captured iterations refer to the old network
+        m->lpc_logk = nullptr;
+        return RG_OK;
+    });
+}
+
+int rg_matrix_coarse_operator(rg_matrix* m, int levels, rg_matrix** out)
+{
+    return guarded([&] {
+        check_matrix(m);
+        RG_CHECK(out != nullptr, RG_VALUE_ERROR, "coarse_operator: out is NULL");
+        use_device(m->ctx);
+        *out = rg::coarse_operator(m, levels);
+        return RG_OK;
+    });
+}
+
+int rg_matrix_set_multilevel(rg_matrix* m, int levels, int degree, int coarse_steps)
+{
+    return guarded([&] {
+        check_matrix(m);
+        RG_CHECK(levels >= 0 && levels <= rg::kMaxLevels, RG_VALUE_ERROR, "set_multilevel: levels must be 0..8");
+        RG_CHECK(degree >= 1 && degree <= rg::kChebSteps, RG_VALUE_ERROR, "set_multilevel: degree must be 1..16");
+        RG_CHECK(coarse_steps >= 1 && coarse_steps <= rg::kChebSteps, RG_VALUE_ERROR,
+                 "set_multilevel: coarse_steps must be 1..16");
+        m->mg_levels = levels;
+        m->mg_degree = degree;
+        m->mg_coarse_steps = coarse_steps;
+        if (m->work) m->work->graph_method = -1;   // captured iterations hold the old cycle
         return RG_OK;
     });
 }

@@ -96,6 +96,10 @@ struct rg_context {
     hipEvent_t ev = nullptr;
 };
 
+namespace rg {
+struct rg_mg_state;
+}
+
 // Work vectors and cached launch graph of the Krylov solvers of one matrix.
 struct rg_solver_work {
     int64_t n = 0;   // N*bs
@@ -137,6 +141,7 @@ struct rg_matrix {
     double* balance = nullptr;     // N*bs balance vector
     bool is_balanced = false;
     bool have_prec = false;
+    bool val_zero = true;          // every stored value is known to be 0 (assembly may write instead of add)
     rg_solver_work* work = nullptr;
     int multi_shape = 0;           // SpMM tile shape + 1 of the many-RHS solver (0: autotune)
     rg_lpc* lpc = nullptr;         // learned preconditioner (borrowed)
@@ -148,6 +153,13 @@ struct rg_matrix {
     double* pdiag = nullptr;       // colour-ordered main blocks (Gauss-Seidel), bs*bs*ld
     double* pdinv = nullptr;       // colour-ordered inverse main blocks / inverted ILU pivots, bs*bs*ld
     int pval_state = 0;            // 0: invalid, 1: Gauss-Seidel split, 2: ILU0 split
+    void* stage = nullptr;         // device staging of host element blocks (grown on demand)
+    size_t stage_bytes = 0;
+    // multilevel preconditioner of RG_METHOD_FCG_LEARNED (rg_mg.hip)
+    rg::rg_mg_state* mg = nullptr;
+    int mg_levels = 0;             // levels without a network (0: down to at most 9^3 nodes)
+    int mg_degree = 2;             // Chebyshev steps before and after each coarse correction
+    int mg_coarse_steps = 8;       // Chebyshev steps on the coarsest level without a network
 };
 
 namespace rg {

@@ -43,6 +43,7 @@ void launch_set_values(rg_matrix* m, double value)
                            m->ctx->stream, m->val, g.ld, g.N, g.nn[0], g.nn[1], g.nn[2], value);
     });
     RG_HIP(hipGetLastError());
+    m->val_zero = (value == 0.0);
 }
 
 // -------------------------------------------------- element block insertion
@@ -50,29 +51,42 @@ void launch_set_values(rg_matrix* m, double value)
 // stencil-matrix counterpart of addToPasoMatrix
 // (ripley/src/RipleyDomain.cpp:1488-1517): the pair (a, b) of local element
 // nodes lands in the slot given by their local positions; with symmetric
-// storage only pairs with b >= a are added.  One workgroup walks the elements
-// in order (deterministic); its threads cover the distinct (a, b, k, m)
-// targets of one element.
+// storage only pairs with b >= a are added.
+//
+// One launch per element colour: the colour of an element is the parity of
+// the grid coordinates of its local node 0 (2^DIM colours), so two distinct
+// grid cells of one colour share no node -- the slab colouring that makes
+// the reference's assembler loops race-free (DefaultAssembler3D.cpp:243-245),
+// taken in all axes.  Inside a launch every thread owns one (element, a, b,
+// k, m) target.  For distinct grid cells each matrix entry then receives at
+// most one contribution per launch and the sums are formed in colour order,
+// the same on every run.  The adds are atomic, so inputs whose "elements"
+// share nodes within a colour (repeated elements, node lists that are not
+// grid cells) stay race-free; only their summation order is then open.
 template <int DIM, int BS, bool SYM>
-__global__ __launch_bounds__(kBlock) void k_add_elements(double* __restrict__ val, int64_t ld,
-                                                         int64_t num_el, const int32_t* __restrict__ nodes,
-                                                         const double* __restrict__ em)
+__global__ __launch_bounds__(kBlock) void k_add_elements(double* __restrict__ val, int64_t ld, int64_t n0,
+                                                         int64_t n1, int64_t num_el,
+                                                         const int32_t* __restrict__ nodes,
+                                                         const double* __restrict__ em, int colour)
 {
     constexpr int NN = 1 << DIM;
     constexpr int C = StencilShape<DIM>::CENTER;
     constexpr int NT = NN * NN * BS * BS;
-    for (int64_t e = 0; e < num_el; ++e) {
-        for (int t = threadIdx.x; t < NT; t += kBlock) {
-            const int k = t % BS, mm = (t / BS) % BS, a = (t / (BS * BS)) % NN, b = t / (BS * BS * NN);
-            if (SYM && b < a) continue;
-            const int s = ((b & 1) - (a & 1) + 1) + 3 * (((b >> 1) & 1) - ((a >> 1) & 1) + 1) +
-                          (DIM == 3 ? 9 * (((b >> 2) & 1) - ((a >> 2) & 1) + 1) : 0);
-            const int st = SYM ? s - C : s;
-            const int64_t row = nodes[e * NN + a];
-            val[(int64_t)(st * BS * BS + k + BS * mm) * ld + row] += em[e * NT + t];
-        }
-        __syncthreads();
-    }
+    const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (t >= num_el * NT) return;
+    const int64_t e = t / NT;
+    const int j = (int)(t - e * NT);
+    const int64_t base = nodes[e * NN];
+    const int64_t bx = base % n0, by = (base / n0) % n1, bz = base / (n0 * n1);
+    const int c = (int)((bx & 1) | ((by & 1) << 1) | (DIM == 3 ? (bz & 1) << 2 : 0));
+    if (c != colour) return;
+    const int k = j % BS, mm = (j / BS) % BS, a = (j / (BS * BS)) % NN, b = j / (BS * BS * NN);
+    if (SYM && b < a) return;
+    const int s = ((b & 1) - (a & 1) + 1) + 3 * (((b >> 1) & 1) - ((a >> 1) & 1) + 1) +
+                  (DIM == 3 ? 9 * (((b >> 2) & 1) - ((a >> 2) & 1) + 1) : 0);
+    const int st = SYM ? s - C : s;
+    const int64_t row = nodes[e * NN + a];
+    atomicAdd(&val[(int64_t)(st * BS * BS + k + BS * mm) * ld + row], em[t]);
 }
 
 void launch_add_element_blocks(rg_matrix* m, int64_t ne, int npe, const int32_t* d_nodes,
@@ -81,11 +95,17 @@ void launch_add_element_blocks(rg_matrix* m, int64_t ne, int npe, const int32_t*
     const Geom& g = m->g;
     RG_CHECK(npe == (1 << g.dim), RG_VALUE_ERROR,
              "add_element_blocks: nodes_per_element must be 4 (2-D) or 8 (3-D)");
+    if (ne == 0) return;
+    const int64_t targets = ne * npe * npe * g.bs * g.bs;
+    const int64_t grid = (targets + kBlock - 1) / kBlock;
+    RG_CHECK(grid < ((int64_t)1 << 31), RG_VALUE_ERROR, "add_element_blocks: too many elements in one call");
     dispatch(g, m->symmetric, [&](auto D, auto B, auto S) {
-        hipLaunchKernelGGL((k_add_elements<D.value, B.value, S.value>), dim3(1), dim3(kBlock), 0,
-                           m->ctx->stream, m->val, g.ld, ne, d_nodes, d_em);
+        for (int c = 0; c < npe; ++c)
+            hipLaunchKernelGGL((k_add_elements<D.value, B.value, S.value>), dim3((unsigned)grid), dim3(kBlock), 0,
+                               m->ctx->stream, m->val, g.ld, g.nn[0], g.nn[1], ne, d_nodes, d_em, c);
     });
     RG_HIP(hipGetLastError());
+    m->val_zero = false;
 }
 
 // ---------------------------------------------------------------- assembly
@@ -254,6 +274,85 @@ __global__ __launch_bounds__(kBlock) void k_assemble(AsmArgs a, double* __restri
     if (doF) rhs[i * NB + k] += f;
 }
 
+// Element-constant isotropic A = kappa_e I (single PDE, full quadrature),
+// optionally with element-constant Y and no other coefficient: the
+// reference's constant-coefficient branch (DefaultAssembler3D.cpp:906-993,
+// load :2205-2214; DefaultAssembler2D.cpp:301-326, :578-583).  The element
+// matrix is then kappa_e K with K(a, b) = sum_d s_d(a, b) / h_d
+// prod_{o != d} h_o m_o(a, b), s = +1 / -1 and m = 1/3 / 1/6 for equal /
+// different local coordinates along an axis (the 2-point Gauss rule
+// integrates these products exactly), so K depends only on which axes a and
+// b agree in: kt[bit d set = agree along d] (2^DIM values, from the host).
+// One thread per node gathers the 2^DIM elements around it; each value slot
+// is written once (plainly when the matrix is known to hold zeros, ZERO),
+// and Y contributes Y_e vol / 2^DIM to the node's rhs.
+struct IsoArgs {
+    const double* kappa;
+    int64_t kstride;       // doubles between elements' kappa (0: one value for all)
+    const double* Y;
+    int64_t ystride;
+    int64_t n0, n1, n2;
+    int ne0, ne1, ne2;
+    double kt[8];
+    double fy;             // vol / 2^DIM
+};
+
+template <int DIM, bool SYM, bool ZERO>
+__global__ __launch_bounds__(kBlock) void k_assemble_iso(IsoArgs a, double* __restrict__ val, int64_t ld, int64_t N,
+                                                         double* __restrict__ rhs)
+{
+    constexpr int NS = StencilShape<DIM>::NS;
+    constexpr int CEN = StencilShape<DIM>::CENTER;
+    constexpr int NST = stored_slots<DIM, SYM>();
+    constexpr int FIRST = SYM ? CEN : 0;
+    constexpr int NNODE = 1 << DIM;
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= N) return;
+    const int64_t x = i % a.n0, y = (i / a.n0) % a.n1, z = (DIM == 3) ? i / (a.n0 * a.n1) : 0;
+    double kap[NNODE];
+#pragma unroll
+    for (int e = 0; e < NNODE; ++e) {
+        const int ex = e & 1, ey = (e >> 1) & 1, ez = (e >> 2) & 1;
+        const int64_t k0 = x - 1 + ex, k1 = y - 1 + ey, k2 = (DIM == 3) ? z - 1 + ez : 0;
+        const bool in = k0 >= 0 && k0 < a.ne0 && k1 >= 0 && k1 < a.ne1 && (DIM == 2 || (k2 >= 0 && k2 < a.ne2));
+        const int64_t el = in ? k0 + (int64_t)a.ne0 * (k1 + (int64_t)a.ne1 * k2) : 0;
+        kap[e] = in ? a.kappa[a.kstride * el] : 0.0;
+    }
+    double acc[NS];
+#pragma unroll
+    for (int s = 0; s < NS; ++s) acc[s] = 0.0;
+    // element e = (ex, ey, ez) holds node i at local bits (1-ex, 1-ey, 1-ez);
+    // local node b of e is the neighbour at offset (ex+b0-1, ey+b1-1, ez+b2-1)
+#pragma unroll
+    for (int e = 0; e < NNODE; ++e) {
+#pragma unroll
+        for (int b = 0; b < NNODE; ++b) {
+            const int agree = ((b & 1) == 1 - (e & 1) ? 1 : 0) | (((b >> 1) & 1) == 1 - ((e >> 1) & 1) ? 2 : 0) |
+                              (DIM == 3 && ((b >> 2) & 1) == 1 - ((e >> 2) & 1) ? 4 : 0);
+            const int s = ((e & 1) + (b & 1)) + 3 * (((e >> 1) & 1) + ((b >> 1) & 1)) +
+                          (DIM == 3 ? 9 * (((e >> 2) & 1) + ((b >> 2) & 1)) : 0);
+            acc[s] = fma(kap[e], a.kt[agree], acc[s]);
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < NST; ++t) {
+        double* p = &val[(int64_t)t * ld + i];
+        *p = ZERO ? acc[FIRST + t] : *p + acc[FIRST + t];
+    }
+    if (rhs && a.Y) {
+        double f = 0.0;
+#pragma unroll
+        for (int e = 0; e < NNODE; ++e) {
+            const int ex = e & 1, ey = (e >> 1) & 1, ez = (e >> 2) & 1;
+            const int64_t k0 = x - 1 + ex, k1 = y - 1 + ey, k2 = (DIM == 3) ? z - 1 + ez : 0;
+            const bool in = k0 >= 0 && k0 < a.ne0 && k1 >= 0 && k1 < a.ne1 && (DIM == 2 || (k2 >= 0 && k2 < a.ne2));
+            const int64_t el = in ? k0 + (int64_t)a.ne0 * (k1 + (int64_t)a.ne1 * k2) : 0;
+            f += in ? a.Y[a.ystride * el] * a.fy : 0.0;
+        }
+        rhs[i] += f;
+    }
+}
+
 // Boundary terms, colour free: one thread per (node i, equation k,
 // component m) visits the face elements around node i on every face of the
 // domain the node lies on and adds row a (node i's local face index) of
@@ -405,6 +504,7 @@ void launch_assemble_boundary(rg_matrix* m, const rg_coef* d, const rg_coef* y, 
     });
     RG_HIP(hipGetLastError());
     m->val32_valid = false;
+    if (a.d.mode) m->val_zero = false;
 }
 
 void launch_add_dirac(rg_matrix* m, int64_t count, const int64_t* d_nodes, const double* d_d, const double* d_y,
@@ -418,6 +518,7 @@ void launch_add_dirac(rg_matrix* m, int64_t count, const int64_t* d_nodes, const
     });
     RG_HIP(hipGetLastError());
     m->val32_valid = false;
+    if (d_d) m->val_zero = false;
 }
 
 void launch_assemble(rg_matrix* m, const rg_pde_coefs& c, int reduced, double* rhs)
@@ -442,13 +543,58 @@ void launch_assemble(rg_matrix* m, const rg_pde_coefs& c, int reduced, double* r
     a.ne0 = g.ne[0]; a.ne1 = g.ne[1]; a.ne2 = g.ne[2];
     a.h0 = g.h[0]; a.h1 = g.h[1]; a.h2 = g.h[2];
     a.reduced = reduced;
-    const dim3 grid((unsigned)((g.N + kBlock - 1) / kBlock), (unsigned)(g.bs * g.bs));
-    dispatch(g, m->symmetric, [&](auto D, auto B, auto S) {
-        hipLaunchKernelGGL((k_assemble<D.value, B.value, S.value>), grid, dim3(kBlock), 0,
-                           m->ctx->stream, a, m->val, g.ld, g.N, rhs);
-    });
+    const bool iso_const = g.bs == 1 && !reduced && c.A.mode == RG_COEF_ELEMENT && c.A.isotropic && c.A.data &&
+                           !c.B.mode && !c.C.mode && !c.D.mode && !c.X.mode &&
+                           (c.Y.mode == RG_COEF_EMPTY || c.Y.mode == RG_COEF_ELEMENT);
+    if (iso_const) {
+        IsoArgs ia;
+        ia.kappa = c.A.data;
+        ia.kstride = c.A.elem_stride;
+        ia.Y = (c.Y.mode == RG_COEF_ELEMENT) ? c.Y.data : nullptr;
+        ia.ystride = c.Y.elem_stride;
+        ia.n0 = g.nn[0]; ia.n1 = g.nn[1]; ia.n2 = g.nn[2];
+        ia.ne0 = g.ne[0]; ia.ne1 = g.ne[1]; ia.ne2 = g.ne[2];
+        double vol = 1.0;
+        for (int d = 0; d < g.dim; ++d) vol *= g.h[d];
+        ia.fy = vol / (1 << g.dim);
+        // K(agree) = sum_d s_d / h_d prod_{o != d} h_o m_o
+        for (int ag = 0; ag < (1 << g.dim); ++ag) {
+            double kt = 0.0;
+            for (int d = 0; d < g.dim; ++d) {
+                double t = (((ag >> d) & 1) ? 1.0 : -1.0) / g.h[d];
+                for (int o = 0; o < g.dim; ++o)
+                    if (o != d) t *= g.h[o] * (((ag >> o) & 1) ? 1.0 / 3.0 : 1.0 / 6.0);
+                kt += t;
+            }
+            ia.kt[ag] = kt;
+        }
+        for (int ag = 1 << g.dim; ag < 8; ++ag) ia.kt[ag] = 0.0;
+        const dim3 grid((unsigned)((g.N + kBlock - 1) / kBlock));
+        auto go = [&](auto D, auto S, auto Z) {
+            hipLaunchKernelGGL((k_assemble_iso<D.value, S.value, Z.value>), grid, dim3(kBlock), 0, m->ctx->stream, ia,
+                               m->val, g.ld, g.N, rhs);
+        };
+        auto byz = [&](auto D, auto S) {
+            if (m->val_zero) go(D, S, BC<true>{});
+            else go(D, S, BC<false>{});
+        };
+        if (g.dim == 3) {
+            if (m->symmetric) byz(IC<3>{}, BC<true>{});
+            else byz(IC<3>{}, BC<false>{});
+        } else {
+            if (m->symmetric) byz(IC<2>{}, BC<true>{});
+            else byz(IC<2>{}, BC<false>{});
+        }
+    } else {
+        const dim3 grid((unsigned)((g.N + kBlock - 1) / kBlock), (unsigned)(g.bs * g.bs));
+        dispatch(g, m->symmetric, [&](auto D, auto B, auto S) {
+            hipLaunchKernelGGL((k_assemble<D.value, B.value, S.value>), grid, dim3(kBlock), 0,
+                               m->ctx->stream, a, m->val, g.ld, g.N, rhs);
+        });
+    }
     RG_HIP(hipGetLastError());
     m->val32_valid = false;
+    if (c.A.mode || c.B.mode || c.C.mode || c.D.mode) m->val_zero = false;
 }
 
 // -------------------------------------------------------------------- SpMV
@@ -582,6 +728,7 @@ void launch_nullify(rg_matrix* m, const double* row_mask, const double* col_mask
     });
     RG_HIP(hipGetLastError());
     m->val32_valid = false;
+    m->val_zero = false;
 }
 
 // ------------------------------------------------------------- CSR export

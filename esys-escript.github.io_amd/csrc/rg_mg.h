@@ -1,0 +1,27 @@
+// Multilevel preconditioner and truncated flexible CG of
+// RG_METHOD_FCG_LEARNED (rg_mg.hip).
+#pragma once
+
+#include "rg_internal.h"
+
+namespace rg {
+
+constexpr int kMaxLevels = 8;     // grid levels of a hierarchy
+constexpr int kChebSteps = 16;    // Chebyshev steps a level's table holds
+constexpr double kChebRatio = 10.0;   // smoothing interval [lmax / ratio, lmax]
+constexpr int kDirs = 4;          // directions the flexible CG keeps
+
+// (Re)builds the hierarchy of the balanced matrix m (after prepare_solve and
+// the FP32 copy of its values).
+void mg_setup(rg_matrix* m);
+// z = M(r), one V-cycle on the balanced level-0 vectors; kernels skip their
+// work once guard->done is set (guard may be null).
+void mg_vcycle(rg_matrix* m, const double* r, double* z, hipStream_t st, const KrylovState* guard);
+// One iteration of the flexible CG on x, r with the V-cycle (z: work vector).
+void mg_fcg_iteration(rg_matrix* m, double* x, double* r, double* z, KrylovState* st_dev, hipStream_t st);
+void mg_free(rg_matrix* m);
+// The Galerkin operator `levels` halvings below the unbalanced matrix m
+// (a new matrix owned by the caller).
+rg_matrix* coarse_operator(rg_matrix* m, int levels);
+
+}  // namespace rg

@@ -1,0 +1,931 @@
+// Multilevel preconditioner and flexible CG of RG_METHOD_FCG_LEARNED.
+//
+// No counterpart in the reference: PASO's preconditioners are Jacobi,
+// Gauss-Seidel, ILU0, RILU and an algebraic multigrid
+// (paso/src/Preconditioner.cpp:46-137).  This is geometric multigrid on
+// ripley's vertex-centred Brick grid, with the learned network as the solver
+// of its coarsest level:
+//
+//   * level l+1 has every other node of level l (ne/2 elements per axis);
+//     prolongation P is trilinear interpolation (per axis P[2I, I] = 1,
+//     P[2I+1, I] = P[2I+1, I+1] = 1/2), restriction is P^T;
+//   * A_{l+1} = P^T A_l P (Galerkin), each coarse operator an ordinary
+//     27-point stencil matrix of the coarse Brick; A_0 is the unbalanced
+//     operator, D^-1 A_bal D^-1 of the balanced matrix the solver holds
+//     (D = diag(balance), paso/src/SystemMatrix.cpp:262-312);
+//   * smoothing: `degree` Chebyshev steps on D_l^-1 A_l (D_l = diag A_l)
+//     over [lmax / 10, lmax], lmax the Gershgorin bound
+//     max_i sum_j |a_ij| / a_ii, before (from zero) and after the coarse
+//     correction, values in FP32, vectors in FP64;
+//   * coarsest level: the network y = Net(r_c) (its inputs the Jacobi-scaled
+//     residual and log10 of the level's diagonal relative to that of a unit
+//     coefficient), scaled by alpha = (r_c.y)/(y.A_c y), the step that
+//     minimises the A_c-norm error along y; without a network, `coarse_steps`
+//     Chebyshev steps from zero.
+//
+// The V-cycle runs in the physical (unbalanced) variables below level 0:
+// r_1 = P^T D^-1 r_bal going down, y_bal += D^-1 P x_1 coming up, so that
+// the interpolated corrections are smooth in the coefficient-weighted sense
+// and not scaled by sqrt(kappa).
+//
+// The outer solver is Notay's flexible CG truncated to the last kDirs
+// directions (FCG(kDirs)): each new direction A-orthogonalised against the
+// kept ones, FP64 operator, FP64 vectors and scalars:
+//   beta_j = (z.q_j) / (p_j.q_j)                    k_fcgm_dots    -> k_fcgm_beta
+//   p = z - sum_j beta_j p_j                        k_fcgm_dir
+//   q = A p, alpha = (p.r) / (p.q)                  k_fcgm_spmv    -> k_fcgm_alpha
+//   x += alpha p, r -= alpha q, ||r||               k_fcgm_update  -> ST_FCG_UPDATE
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+
+#include "rg_device.h"
+#include "rg_dispatch.h"
+#include "rg_krylov.h"
+#include "rg_mg.h"
+#include "rg_stencil.h"
+
+namespace rg {
+namespace {
+
+constexpr int C3 = 13;   // main slot of the 27-point stencil
+
+struct MgGeom {
+    int64_t n0, n1, n2, N, ld;
+};
+
+MgGeom geom_of(const rg_matrix* a)
+{
+    return {a->g.nn[0], a->g.nn[1], a->g.nn[2], a->g.N, a->g.ld};
+}
+
+__device__ __forceinline__ bool guard_done(const KrylovState* g) { return g && g->done; }
+
+// Block maximum of v into part[blockIdx.x] (plain store; a one-workgroup
+// kernel launched next reads it).
+__device__ __forceinline__ void publish_block_max(double v, double* part)
+{
+    __shared__ double red[kBlock / 64];
+    const double w = wave_max(v);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = w;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = red[0];
+#pragma unroll
+        for (int k = 1; k < kBlock / 64; ++k) m = fmax(m, red[k]);
+        part[blockIdx.x] = m;
+    }
+}
+
+// Entry a(i, j) of a 27-point operator, j = i + offset of full slot s (the
+// caller passes ok = both nodes in the grid; otherwise 0).  With symmetric
+// storage a slot below the main one is the transposed upper slot of row j.
+template <bool SYM>
+__device__ __forceinline__ double entry27(const double* __restrict__ val, int64_t ld, int64_t i, int64_t j, int s,
+                                          bool ok)
+{
+    int64_t k;
+    if (!SYM) k = (int64_t)s * ld + i;
+    else if (s >= C3) k = (int64_t)(s - C3) * ld + i;
+    else k = (int64_t)(C3 - s) * ld + j;
+    return ok ? val[ok ? k : 0] : 0.0;
+}
+
+// ------------------------------------------------------------ set-up kernels
+// Galerkin product A_c = P^T A P for one coarse node per thread: fine node
+// i = 2I + di (weight prod (1 - |di|/2)), its stencil neighbour j = i + s,
+// and the (at most 8) coarse nodes J whose interpolation reaches j, per axis
+// J = I + e/2 for even e = di + s (weight 1), I + (e-1)/2 and I + (e+1)/2 for
+// odd e (weight 1/2 each).  With BAL the fine matrix is balanced and the
+// entries are divided by bal_i bal_j.  Writes the coarse values (main and
+// upper slots with SYMC), 1 / a_II and the block maximum of the Gershgorin
+// ratio sum_J |a_IJ| / a_II.
+template <bool SYMF, bool SYMC, bool BAL>
+__global__ __launch_bounds__(kBlock) void k_galerkin(const double* __restrict__ fv, MgGeom f,
+                                                     const double* __restrict__ bal, double* __restrict__ cv,
+                                                     MgGeom c, double* __restrict__ cdinv, double* __restrict__ gpart)
+{
+    const int64_t I = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    double g = 0.0;
+    if (I < c.N) {
+        const int64_t X = I % c.n0, Y = (I / c.n0) % c.n1, Z = I / (c.n0 * c.n1);
+        double acc[27];
+#pragma unroll
+        for (int t = 0; t < 27; ++t) acc[t] = 0.0;
+#pragma unroll
+        for (int di = 0; di < 27; ++di) {
+            const int dx = di % 3 - 1, dy = (di / 3) % 3 - 1, dz = di / 9 - 1;
+            const int64_t fx = 2 * X + dx, fy = 2 * Y + dy, fz = 2 * Z + dz;
+            const bool iok = fx >= 0 && fx < f.n0 && fy >= 0 && fy < f.n1 && fz >= 0 && fz < f.n2;
+            const int64_t i = iok ? fx + f.n0 * (fy + f.n1 * fz) : 0;
+            const double wi = (dx ? 0.5 : 1.0) * (dy ? 0.5 : 1.0) * (dz ? 0.5 : 1.0);
+            const double ibi = BAL ? 1.0 / bal[i] : 1.0;
+#pragma unroll
+            for (int s = 0; s < 27; ++s) {
+                const int sx = s % 3 - 1, sy = (s / 3) % 3 - 1, sz = s / 9 - 1;
+                const int64_t jx = fx + sx, jy = fy + sy, jz = fz + sz;
+                const bool ok = iok && jx >= 0 && jx < f.n0 && jy >= 0 && jy < f.n1 && jz >= 0 && jz < f.n2;
+                const int64_t j = ok ? jx + f.n0 * (jy + f.n1 * jz) : 0;
+                double a = entry27<SYMF>(fv, f.ld, i, j, s, ok);
+                if (BAL) a *= ibi / bal[j];
+                const double t = wi * a;
+                const int ex = dx + sx, ey = dy + sy, ez = dz + sz;
+#pragma unroll
+                for (int cz = 0; cz < 2; ++cz) {
+                    if (cz == 1 && !(ez & 1)) continue;
+                    const int Dz = (ez & 1) ? ((ez - 1) >> 1) + cz : (ez >> 1);
+                    const double wz = (ez & 1) ? 0.5 : 1.0;
+#pragma unroll
+                    for (int cy = 0; cy < 2; ++cy) {
+                        if (cy == 1 && !(ey & 1)) continue;
+                        const int Dy = (ey & 1) ? ((ey - 1) >> 1) + cy : (ey >> 1);
+                        const double wy = (ey & 1) ? 0.5 : 1.0;
+#pragma unroll
+                        for (int cx = 0; cx < 2; ++cx) {
+                            if (cx == 1 && !(ex & 1)) continue;
+                            const int Dx = (ex & 1) ? ((ex - 1) >> 1) + cx : (ex >> 1);
+                            const double wx = (ex & 1) ? 0.5 : 1.0;
+                            const int D = (Dx + 1) + 3 * (Dy + 1) + 9 * (Dz + 1);
+                            acc[D] = fma(t, wx * wy * wz, acc[D]);
+                        }
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < 27; ++t) {
+            if (SYMC && t < C3) continue;
+            cv[(int64_t)(SYMC ? t - C3 : t) * c.ld + I] = acc[t];
+        }
+        cdinv[I] = 1.0 / acc[C3];
+        double s = 0.0;
+#pragma unroll
+        for (int t = 0; t < 27; ++t) s += fabs(acc[t]);
+        g = s / acc[C3];
+    }
+    publish_block_max(g, gpart);
+}
+
+// Gershgorin ratio max_i sum_j |a_ij| / a_ii of a 27-point operator (block
+// maxima).
+template <bool SYM>
+__global__ __launch_bounds__(kBlock) void k_gershgorin(const double* __restrict__ val, MgGeom a,
+                                                       double* __restrict__ gpart)
+{
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    double g = 0.0;
+    if (i < a.N) {
+        const int64_t x = i % a.n0, y = (i / a.n0) % a.n1, z = i / (a.n0 * a.n1);
+        double s = 0.0;
+#pragma unroll
+        for (int t = 0; t < 27; ++t) {
+            const int64_t jx = x + t % 3 - 1, jy = y + (t / 3) % 3 - 1, jz = z + t / 9 - 1;
+            const bool ok = jx >= 0 && jx < a.n0 && jy >= 0 && jy < a.n1 && jz >= 0 && jz < a.n2;
+            const int64_t j = ok ? jx + a.n0 * (jy + a.n1 * jz) : 0;
+            s += fabs(entry27<SYM>(val, a.ld, i, j, t, ok));
+        }
+        g = s / entry27<SYM>(val, a.ld, i, i, C3, true);
+    }
+    publish_block_max(g, gpart);
+}
+
+// Chebyshev coefficients of one level from its Gershgorin bound (one
+// workgroup): coef[2k] multiplies the previous update d, coef[2k+1] the
+// Jacobi-scaled residual, for steps k = 0 .. kChebSteps-1; coef[2 kChebSteps]
+// keeps lmax.
+__global__ void k_cheb_table(const double* __restrict__ gpart, int G, double ratio, double* __restrict__ coef)
+{
+    __shared__ double red[kBlock / 64];
+    double m = 0.0;
+    for (int b = threadIdx.x; b < G; b += blockDim.x) m = fmax(m, gpart[b]);
+    m = wave_max(m);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    double lmax = red[0];
+    for (int k = 1; k < (int)(blockDim.x / 64); ++k) lmax = fmax(lmax, red[k]);
+    const double lmin = lmax / ratio;
+    const double theta = 0.5 * (lmax + lmin), delta = 0.5 * (lmax - lmin), sigma = theta / delta;
+    double rho_old = 1.0 / sigma;
+    coef[0] = 0.0;
+    coef[1] = 1.0 / theta;
+    for (int k = 1; k < kChebSteps; ++k) {
+        const double rho = 1.0 / (2.0 * sigma - rho_old);
+        coef[2 * k] = rho * rho_old;
+        coef[2 * k + 1] = 2.0 * rho / delta;
+        rho_old = rho;
+    }
+    coef[2 * kChebSteps] = lmax;
+}
+
+// Network input of the coarsest level: log10 of a_ii relative to the main
+// entry aref of a unit coefficient on that grid (float per node).
+__global__ void k_level_logk(const double* __restrict__ main_slot, int64_t N, double aref, float* __restrict__ logk)
+{
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i < N) logk[i] = (float)log10(fmax(main_slot[i], 1e-300) / aref);
+}
+
+// ------------------------------------------------------------ V-cycle kernels
+// First Chebyshev step from x = 0: d = c D^-1 b, x = d.
+__global__ __launch_bounds__(kBlock) void k_cheb_first(int64_t N, const double* __restrict__ b,
+                                                       const double* __restrict__ dinv, const double* __restrict__ coef,
+                                                       double* __restrict__ x, double* __restrict__ d,
+                                                       const KrylovState* guard)
+{
+    if (guard_done(guard)) return;
+    const double c = coef[1];
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < N; i += (int64_t)gridDim.x * kBlock) {
+        const double v = c * dinv[i] * b[i];
+        d[i] = v;
+        x[i] = v;
+    }
+}
+
+// One Chebyshev step k from x_in: r = b - A x_in, d = c1 d + c2 D^-1 r,
+// x_out = x_in + d (coef points at the step's pair (c1, c2); c1 = 0 for the
+// first step from a non-zero x).
+template <bool SYM, typename VT>
+__global__ __launch_bounds__(kBlock) void k_cheb_step(const VT* __restrict__ val, int64_t ld, int64_t N, int64_t s1,
+                                                      int64_t s2, int64_t chunk, const double* __restrict__ dinv,
+                                                      const double* __restrict__ b, const double* __restrict__ xin,
+                                                      double* __restrict__ xout, double* __restrict__ d,
+                                                      const double* __restrict__ coef, const KrylovState* guard)
+{
+    if (guard_done(guard)) return;
+    const double c1 = coef[0], c2 = coef[1];
+    const RowRange rr = block_rows(N, chunk);
+    stencil_block<3, 1, SYM, VT>(val, ld, N, s1, s2, rr.r0, rr.r1, xin, [&](int64_t i, const double (&acc)[1]) {
+        const double r = b[i] - acc[0];
+        // c1 = 0 (a first step): the previous update is not read
+        const double dn = c2 * dinv[i] * r + (c1 != 0.0 ? c1 * d[i] : 0.0);
+        d[i] = dn;
+        xout[i] = xin[i] + dn;
+    });
+}
+
+// r = b - A x
+template <bool SYM, typename VT>
+__global__ __launch_bounds__(kBlock) void k_residual(const VT* __restrict__ val, int64_t ld, int64_t N, int64_t s1,
+                                                     int64_t s2, int64_t chunk, const double* __restrict__ b,
+                                                     const double* __restrict__ x, double* __restrict__ r,
+                                                     const KrylovState* guard)
+{
+    if (guard_done(guard)) return;
+    const RowRange rr = block_rows(N, chunk);
+    stencil_block<3, 1, SYM, VT>(val, ld, N, s1, s2, rr.r0, rr.r1, x,
+                                 [&](int64_t i, const double (&acc)[1]) { r[i] = b[i] - acc[0]; });
+}
+
+// b_c = P^T (r / bal): one coarse node per thread over its 27 fine nodes.
+template <bool BAL>
+__global__ __launch_bounds__(kBlock) void k_restrict(MgGeom f, MgGeom c, const double* __restrict__ r,
+                                                     const double* __restrict__ bal, double* __restrict__ bc,
+                                                     const KrylovState* guard)
+{
+    if (guard_done(guard)) return;
+    const int64_t I = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (I >= c.N) return;
+    const int64_t X = I % c.n0, Y = (I / c.n0) % c.n1, Z = I / (c.n0 * c.n1);
+    double s = 0.0;
+#pragma unroll
+    for (int di = 0; di < 27; ++di) {
+        const int dx = di % 3 - 1, dy = (di / 3) % 3 - 1, dz = di / 9 - 1;
+        const int64_t fx = 2 * X + dx, fy = 2 * Y + dy, fz = 2 * Z + dz;
+        const bool ok = fx >= 0 && fx < f.n0 && fy >= 0 && fy < f.n1 && fz >= 0 && fz < f.n2;
+        const int64_t i = ok ? fx + f.n0 * (fy + f.n1 * fz) : 0;
+        const double w = (dx ? 0.5 : 1.0) * (dy ? 0.5 : 1.0) * (dz ? 0.5 : 1.0);
+        const double v = BAL ? r[i] / bal[i] : r[i];
+        s += ok ? w * v : 0.0;
+    }
+    bc[I] = s;
+}
+
+// x_f += scale * (P x_c) / bal: one fine node per thread over the (up to 8)
+// coarse nodes that interpolate it; scale (device, may be null) is the
+// coarse step length.
+template <bool BAL>
+__global__ __launch_bounds__(kBlock) void k_prolong(MgGeom f, MgGeom c, const double* __restrict__ xc,
+                                                    const double* __restrict__ bal, const double* __restrict__ scale,
+                                                    double* __restrict__ xf, const KrylovState* guard)
+{
+    if (guard_done(guard)) return;
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= f.N) return;
+    const int64_t x = i % f.n0, y = (i / f.n0) % f.n1, z = i / (f.n0 * f.n1);
+    const int64_t X0 = x >> 1, Y0 = y >> 1, Z0 = z >> 1;
+    const int ox = (int)(x & 1), oy = (int)(y & 1), oz = (int)(z & 1);
+    double s = 0.0;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const int kx = k & 1, ky = (k >> 1) & 1, kz = (k >> 2) & 1;
+        const bool use = (kx <= ox) && (ky <= oy) && (kz <= oz);
+        const int64_t J = use ? (X0 + kx) + c.n0 * ((Y0 + ky) + c.n1 * (Z0 + kz)) : 0;
+        const double w = (ox ? 0.5 : 1.0) * (oy ? 0.5 : 1.0) * (oz ? 0.5 : 1.0);
+        s += use ? w * xc[J] : 0.0;
+    }
+    if (scale) s *= scale[0];
+    if (BAL) s /= bal[i];
+    xf[i] += s;
+}
+
+// Ay for the coarse step length: partials (r.y, y.Ay) of A y.
+template <bool SYM>
+__global__ __launch_bounds__(kBlock) void k_coarse_energy(const double* __restrict__ val, int64_t ld, int64_t N,
+                                                          int64_t s1, int64_t s2, int64_t chunk,
+                                                          const double* __restrict__ y, const double* __restrict__ r,
+                                                          double* __restrict__ partials, const KrylovState* guard)
+{
+    if (guard_done(guard)) return;
+    const RowRange rr = block_rows(N, chunk);
+    double s[2] = {0.0, 0.0};
+    stencil_block<3, 1, SYM, double>(val, ld, N, s1, s2, rr.r0, rr.r1, y, [&](int64_t i, const double (&acc)[1]) {
+        s[0] = fma(r[i], y[i], s[0]);
+        s[1] = fma(y[i], acc[0], s[1]);
+    });
+    publish_block_sums<2>(s, partials);
+}
+
+// alpha = (r.y) / (y.A y), 0 when y.A y is not positive.
+__global__ __launch_bounds__(kBlock) void k_coarse_alpha(const double* __restrict__ partials, int G,
+                                                         double* __restrict__ alpha, const KrylovState* guard)
+{
+    if (guard_done(guard)) return;
+    double tot[2];
+    reduce_partials<2>(partials, G, tot);
+    if (threadIdx.x == 0) alpha[0] = tot[1] > 0.0 ? tot[0] / tot[1] : 0.0;
+}
+
+// ------------------------------------------------------------ FCG(kDirs)
+// Direction t of the ring lives in slot t % kDirs; the iteration with
+// st->num_iter = k forms direction k from the kept directions k-1 .. k-kDirs.
+struct FcgArgs {
+    double* x;
+    double* r;
+    const double* z;
+    double* P;          // kDirs vectors of n, ld apart
+    double* Q;
+    double* coef;       // [0, kDirs): p_j.q_j per slot; [kDirs, 2 kDirs): beta_j
+    KrylovState* st;
+    double* partials;
+    int64_t n, vld, chunk;
+};
+
+__global__ __launch_bounds__(kBlock) void k_fcgm_dots(FcgArgs a)
+{
+    if (a.st->done) return;
+    const int k = a.st->num_iter;
+    const double* q[kDirs];
+    bool use[kDirs];
+#pragma unroll
+    for (int j = 0; j < kDirs; ++j) {
+        const int t = k - 1 - j;
+        use[j] = t >= 0;
+        q[j] = a.Q + (int64_t)((t >= 0 ? t : 0) % kDirs) * a.vld;
+    }
+    double s[kDirs];
+#pragma unroll
+    for (int j = 0; j < kDirs; ++j) s[j] = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < a.n; i += (int64_t)gridDim.x * kBlock) {
+        const double zi = a.z[i];
+#pragma unroll
+        for (int j = 0; j < kDirs; ++j)
+            if (use[j]) s[j] = fma(zi, q[j][i], s[j]);
+    }
+    publish_block_sums<kDirs>(s, a.partials);
+}
+
+__global__ __launch_bounds__(kBlock) void k_fcgm_beta(FcgArgs a, int G)
+{
+    if (a.st->done) return;
+    double tot[kDirs];
+    reduce_partials<kDirs>(a.partials, G, tot);
+    if (threadIdx.x == 0) {
+        const int k = a.st->num_iter;
+#pragma unroll
+        for (int j = 0; j < kDirs; ++j) {
+            const int t = k - 1 - j;
+            a.coef[kDirs + j] = (t >= 0) ? tot[j] / a.coef[t % kDirs] : 0.0;
+        }
+    }
+}
+
+// p_k = z - sum_j beta_j p_{k-1-j}; the slot of p_k is that of p_{k-kDirs},
+// read by the same thread before it is overwritten.
+__global__ __launch_bounds__(kBlock) void k_fcgm_dir(FcgArgs a)
+{
+    if (a.st->done) return;
+    const int k = a.st->num_iter;
+    const double* p[kDirs];
+    double beta[kDirs];
+#pragma unroll
+    for (int j = 0; j < kDirs; ++j) {
+        const int t = k - 1 - j;
+        beta[j] = a.coef[kDirs + j];
+        p[j] = a.P + (int64_t)((t >= 0 ? t : 0) % kDirs) * a.vld;
+    }
+    double* pk = a.P + (int64_t)(k % kDirs) * a.vld;
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < a.n; i += (int64_t)gridDim.x * kBlock) {
+        double v = a.z[i];
+#pragma unroll
+        for (int j = 0; j < kDirs; ++j) v = fma(-beta[j], beta[j] != 0.0 ? p[j][i] : 0.0, v);
+        pk[i] = v;
+    }
+}
+
+// q_k = A p_k (FP64 values) with the partials (p.q, p.r).
+template <bool SYM>
+__global__ __launch_bounds__(kBlock) void k_fcgm_spmv(FcgArgs a, const double* __restrict__ val, int64_t ld,
+                                                      int64_t s1, int64_t s2)
+{
+    if (a.st->done) return;
+    const int k = a.st->num_iter;
+    const double* __restrict__ p = a.P + (int64_t)(k % kDirs) * a.vld;
+    double* __restrict__ q = a.Q + (int64_t)(k % kDirs) * a.vld;
+    const RowRange rr = block_rows(a.n, a.chunk);
+    double s[2] = {0.0, 0.0};
+    stencil_block<3, 1, SYM, double>(val, ld, a.n, s1, s2, rr.r0, rr.r1, p, [&](int64_t i, const double (&acc)[1]) {
+        q[i] = acc[0];
+        s[0] = fma(acc[0], p[i], s[0]);
+        s[1] = fma(a.r[i], p[i], s[1]);
+    });
+    publish_block_sums<2>(s, a.partials);
+}
+
+__global__ __launch_bounds__(kBlock) void k_fcgm_alpha(FcgArgs a, int G)
+{
+    if (a.st->done) return;
+    double tot[2];
+    reduce_partials<2>(a.partials, G, tot);
+    if (threadIdx.x == 0) {
+        KrylovState* st = a.st;
+        if (!(tot[0] > 0.0)) {
+            st->done = 1;
+            st->status = KS_BREAKDOWN;
+        } else {
+            st->alpha = tot[1] / tot[0];
+            a.coef[st->num_iter % kDirs] = tot[0];
+        }
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_fcgm_update(FcgArgs a)
+{
+    if (a.st->done) return;
+    const double alpha = a.st->alpha;
+    const int k = a.st->num_iter;
+    const double* __restrict__ p = a.P + (int64_t)(k % kDirs) * a.vld;
+    const double* __restrict__ q = a.Q + (int64_t)(k % kDirs) * a.vld;
+    double s[1] = {0.0};
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < a.n; i += (int64_t)gridDim.x * kBlock) {
+        a.x[i] = fma(alpha, p[i], a.x[i]);
+        const double rn = fma(-alpha, q[i], a.r[i]);
+        a.r[i] = rn;
+        s[0] = fma(rn, rn, s[0]);
+    }
+    publish_block_sums<1>(s, a.partials);
+}
+
+__global__ __launch_bounds__(kBlock) void k_fcgm_end(KrylovState* st, const double* __restrict__ partials, int G)
+{
+    if (st->done) return;
+    double tot[1];
+    reduce_partials<1>(partials, G, tot);
+    if (threadIdx.x == 0) stage_apply<ST_FCG_UPDATE, 1>(st, tot);
+}
+
+int grid_for(int64_t n) { return (int)std::min<int64_t>((n + kBlock - 1) / kBlock, 2048); }
+
+// the grid of a one-node-per-thread kernel
+unsigned grid1(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+
+}  // namespace
+
+// ================================================================== host side
+struct MgLevel {
+    rg_matrix* A = nullptr;        // level >= 1: owned coarse operator (FP64 values + FP32 copy)
+    double* dinv = nullptr;        // level >= 1: owned 1/a_ii; level 0: the solver's Jacobi inverse
+    double *xa = nullptr, *xb = nullptr, *b = nullptr, *r = nullptr, *d = nullptr;
+    double* coef = nullptr;        // Chebyshev table (device), 2 kChebSteps + 1 doubles
+    MgGeom g{};
+};
+
+struct rg_mg_state {
+    int nlev = 0;
+    MgLevel lev[kMaxLevels];
+    double* coefs = nullptr;       // kMaxLevels Chebyshev tables
+    double* part = nullptr;        // block partials of the coarse-step reduction
+    double* gpart = nullptr;       // block maxima of the set-up's Gershgorin bounds (one per 256 fine rows)
+    double* alpha = nullptr;       // coarse step length
+    float* logk = nullptr;         // network input of the coarsest level
+    double* y = nullptr;           // network output
+    double* P = nullptr;           // FCG ring of directions and their products
+    double* Q = nullptr;
+    double* fcg = nullptr;
+    int64_t vld = 0;
+    bool with_net = false;
+};
+
+namespace {
+
+void free_level_vectors(MgLevel& L)
+{
+    for (double* p : {L.xa, L.xb, L.b, L.r, L.d})
+        if (p) hipFree(p);
+    L.xa = L.xb = L.b = L.r = L.d = nullptr;
+}
+
+double* dalloc(int64_t n)
+{
+    double* p = nullptr;
+    RG_HIP(hipMalloc(&p, sizeof(double) * std::max<int64_t>(n, 1)));
+    return p;
+}
+
+// Number of levels the hierarchy of m gets: down to the network's grid when
+// one is attached, else m->mg_levels (0: until a level has at most 9^3
+// nodes or cannot be halved).
+int plan_levels(const rg_matrix* m, bool* with_net)
+{
+    const Geom& g = m->g;
+    int ne[3] = {g.ne[0], g.ne[1], g.ne[2]};
+    int32_t target[3] = {0, 0, 0};
+    *with_net = m->lpc != nullptr;
+    if (*with_net) lpc_nodes(m->lpc, target);
+    int levels = 1;
+    for (;;) {
+        const int64_t nodes = (int64_t)(ne[0] + 1) * (ne[1] + 1) * (ne[2] + 1);
+        if (*with_net) {
+            if (ne[0] + 1 == target[0] && ne[1] + 1 == target[1] && ne[2] + 1 == target[2]) {
+                RG_CHECK(levels >= 2, RG_VALUE_ERROR,
+                         "FCG_LEARNED: the network's node grid must be a coarse level (at least one halving) of the "
+                         "matrix's grid");
+                return levels;
+            }
+        } else if (m->mg_levels > 0) {
+            if (levels == m->mg_levels) return levels;
+        } else if (nodes <= 9 * 9 * 9) {
+            return levels;
+        }
+        const bool halvable = ne[0] % 2 == 0 && ne[1] % 2 == 0 && ne[2] % 2 == 0 && ne[0] >= 2 && ne[1] >= 2 &&
+                              ne[2] >= 2;
+        if (!halvable || levels == kMaxLevels) {
+            RG_CHECK(!*with_net, RG_VALUE_ERROR,
+                     "FCG_LEARNED: the network's node grid is not a level of this matrix's grid hierarchy "
+                     "(every other node per level)");
+            RG_CHECK(m->mg_levels <= 0, RG_VALUE_ERROR,
+                     "FCG_LEARNED: the grid cannot be halved that many times");
+            return levels;
+        }
+        for (int d = 0; d < 3; ++d) ne[d] /= 2;
+        ++levels;
+    }
+}
+
+}  // namespace
+
+void mg_free(rg_matrix* m)
+{
+    rg_mg_state* s = m->mg;
+    if (!s) return;
+    hipStreamSynchronize(m->ctx->stream);
+    for (int l = 0; l < s->nlev; ++l) {
+        MgLevel& L = s->lev[l];
+        free_level_vectors(L);
+        if (l > 0) {
+            if (L.dinv) hipFree(L.dinv);
+            if (L.A) rg_matrix_destroy(L.A);
+        }
+    }
+    for (void* p : {(void*)s->coefs, (void*)s->part, (void*)s->gpart, (void*)s->alpha, (void*)s->logk, (void*)s->y, (void*)s->P,
+                    (void*)s->Q, (void*)s->fcg})
+        if (p) hipFree(p);
+    delete s;
+    m->mg = nullptr;
+}
+
+// Builds (on first use or when the plan changes) and refreshes the
+// hierarchy of the balanced matrix m: Galerkin operators, their FP32 copies,
+// Jacobi inverses, Chebyshev tables and the network's coefficient input.
+void mg_setup(rg_matrix* m)
+{
+    const Geom& g = m->g;
+    RG_CHECK(g.dim == 3 && g.bs == 1, RG_VALUE_ERROR, "FCG_LEARNED: scalar Brick matrices only");
+    RG_CHECK(m->is_balanced && m->val32_valid, RG_STATE_ERROR, "FCG_LEARNED: matrix not prepared");
+    bool with_net = false;
+    const int nlev = plan_levels(m, &with_net);
+    if (m->mg && (m->mg->nlev != nlev || m->mg->with_net != with_net)) mg_free(m);
+    hipStream_t st = m->ctx->stream;
+    if (!m->mg) {
+        if (m->work) m->work->graph_method = -1;   // captured iterations hold the old buffers
+        rg_mg_state* s = new rg_mg_state();
+        m->mg = s;
+        s->nlev = nlev;
+        s->with_net = with_net;
+        s->coefs = dalloc((int64_t)kMaxLevels * (2 * kChebSteps + 1));
+        s->part = dalloc(2 * kMaxGrid);
+        s->gpart = dalloc(grid1(g.N));
+        s->alpha = dalloc(1);
+        s->vld = ((g.N + 63) / 64) * 64;
+        s->P = dalloc(kDirs * s->vld);
+        s->Q = dalloc(kDirs * s->vld);
+        s->fcg = dalloc(2 * kDirs);
+        for (int l = 0; l < nlev; ++l) {
+            MgLevel& L = s->lev[l];
+            L.coef = s->coefs + (int64_t)l * (2 * kChebSteps + 1);
+            if (l == 0) {
+                L.g = geom_of(m);
+            } else {
+                const rg_matrix* f = s->lev[l - 1].A ? s->lev[l - 1].A : m;
+                int32_t ne[3];
+                double len[3];
+                for (int d = 0; d < 3; ++d) {
+                    ne[d] = f->g.ne[d] / 2;
+                    len[d] = f->g.len[d];
+                }
+                const int rc = rg_matrix_create(m->ctx, 3, ne, len, 1, m->symmetric ? RG_MATRIX_SYMMETRIC : 0, &L.A);
+                RG_CHECK(rc == RG_OK, rc, "FCG_LEARNED: cannot create a coarse level");
+                const size_t cnt = (size_t)L.A->g.nstore * L.A->g.ld;
+                RG_HIP(hipMalloc(&L.A->val32, cnt * sizeof(float)));
+                L.dinv = dalloc(L.A->g.N);
+                L.g = geom_of(L.A);
+                L.b = dalloc(L.g.N);
+            }
+            L.xa = dalloc(L.g.N);
+            L.xb = dalloc(L.g.N);
+            L.r = dalloc(L.g.N);
+            L.d = dalloc(L.g.N);
+        }
+        const MgGeom& gc = s->lev[nlev - 1].g;
+        if (with_net) {
+            RG_HIP(hipMalloc(&s->logk, sizeof(float) * gc.N));
+            s->y = dalloc(gc.N);
+        }
+    }
+    rg_mg_state* s = m->mg;
+    s->lev[0].A = nullptr;
+    s->lev[0].dinv = m->dinv;
+    // level 0: Gershgorin bound of the balanced operator (the spectrum of
+    // D^-1 A does not change under the balancing)
+    {
+        const MgGeom a = s->lev[0].g;
+        const unsigned G = grid1(a.N);
+        if (m->symmetric) hipLaunchKernelGGL(k_gershgorin<true>, dim3(G), dim3(kBlock), 0, st, m->val, a, s->gpart);
+        else hipLaunchKernelGGL(k_gershgorin<false>, dim3(G), dim3(kBlock), 0, st, m->val, a, s->gpart);
+        hipLaunchKernelGGL(k_cheb_table, dim3(1), dim3(kBlock), 0, st, s->gpart, (int)G, kChebRatio, s->lev[0].coef);
+        RG_HIP(hipGetLastError());
+    }
+    for (int l = 1; l < s->nlev; ++l) {
+        MgLevel& L = s->lev[l];
+        const rg_matrix* F = (l == 1) ? m : s->lev[l - 1].A;
+        const MgGeom f = s->lev[l - 1].g, c = L.g;
+        const unsigned G = grid1(c.N);
+        const double* bal = (l == 1) ? m->balance : nullptr;
+        auto launch = [&](auto SF, auto SC, auto B) {
+            hipLaunchKernelGGL((k_galerkin<SF.value, SC.value, B.value>), dim3(G), dim3(kBlock), 0, st, F->val, f, bal,
+                               L.A->val, c, L.dinv, s->gpart);
+        };
+        auto by_b = [&](auto SF, auto SC) {
+            if (bal) launch(SF, SC, BC<true>{});
+            else launch(SF, SC, BC<false>{});
+        };
+        const bool sf = F->symmetric, sc = L.A->symmetric;
+        if (sf && sc) by_b(BC<true>{}, BC<true>{});
+        else if (sf) by_b(BC<true>{}, BC<false>{});
+        else if (sc) by_b(BC<false>{}, BC<true>{});
+        else by_b(BC<false>{}, BC<false>{});
+        hipLaunchKernelGGL(k_cheb_table, dim3(1), dim3(kBlock), 0, st, s->gpart, (int)G, kChebRatio, L.coef);
+        RG_HIP(hipGetLastError());
+        L.A->val_zero = false;
+        L.A->is_balanced = false;
+        launch_to_fp32(L.A);
+        L.A->val32_valid = true;
+    }
+    if (s->with_net) {
+        const MgLevel& L = s->lev[s->nlev - 1];
+        const rg_matrix* A = L.A ? L.A : m;
+        // main entry of a unit coefficient: 8 elements x sum_d vol / (9 h_d^2)
+        double aref = 0.0;
+        const double vol = A->g.h[0] * A->g.h[1] * A->g.h[2];
+        for (int d = 0; d < 3; ++d) aref += 8.0 * vol / (9.0 * A->g.h[d] * A->g.h[d]);
+        const double* main_slot = A->val + (int64_t)(A->symmetric ? 0 : C3) * A->g.ld;
+        hipLaunchKernelGGL(k_level_logk, dim3(grid1(L.g.N)), dim3(kBlock), 0, st, main_slot, L.g.N, aref, s->logk);
+        RG_HIP(hipGetLastError());
+    }
+}
+
+namespace {
+
+template <class F>
+void by_sym_vt(bool sym, F&& f)
+{
+    if (sym) f(BC<true>{});
+    else f(BC<false>{});
+}
+
+// Chebyshev smoothing of level l: `steps` steps from x (x_from_zero: from 0,
+// b only).  Returns the buffer holding the result (xa or xb of the level, or
+// `out` for the last step when out is not null).
+double* smooth(rg_mg_state* s, const rg_matrix* Am, int l, const double* b, double* x, bool from_zero, int steps,
+               double* out, hipStream_t st, const KrylovState* guard)
+{
+    MgLevel& L = s->lev[l];
+    const rg_matrix* A = L.A ? L.A : Am;
+    const Geom& g = A->g;
+    const Launch Ls = chunked_launch(g.N, spmv_rows_per_thread<float>());
+    double* cur = x;
+    int k = 0;
+    if (from_zero) {
+        double* dst = (steps == 1 && out) ? out : L.xa;
+        hipLaunchKernelGGL(k_cheb_first, dim3(grid_for(g.N)), dim3(kBlock), 0, st, g.N, b, L.dinv, L.coef, dst, L.d,
+                           guard);
+        cur = dst;
+        k = 1;
+    }
+    for (; k < steps; ++k) {
+        double* dst = (k == steps - 1 && out) ? out : (cur == L.xa ? L.xb : L.xa);
+        const double* cf = L.coef + 2 * k;
+        by_sym_vt(A->symmetric, [&](auto S) {
+            hipLaunchKernelGGL((k_cheb_step<S.value, float>), dim3(Ls.grid), dim3(kBlock), 0, st, A->val32, g.ld, g.N,
+                               g.s1, g.s2, Ls.chunk, L.dinv, b, cur, dst, L.d, cf, guard);
+        });
+        cur = dst;
+    }
+    RG_HIP(hipGetLastError());
+    return cur;
+}
+
+}  // namespace
+
+// Launches one V-cycle: z = M(r) on the balanced level-0 vectors (r, z
+// distinct, z written in full).
+void mg_vcycle(rg_matrix* m, const double* r, double* z, hipStream_t st, const KrylovState* guard)
+{
+    rg_mg_state* s = m->mg;
+    RG_CHECK(s != nullptr, RG_STATE_ERROR, "FCG_LEARNED: no multilevel hierarchy");
+    const int deg = std::max(1, std::min(m->mg_degree, kChebSteps));
+    const int nl = s->nlev;
+    const double* bl[kMaxLevels];
+    double* xl[kMaxLevels];
+    bl[0] = r;
+    // down
+    for (int l = 0; l + 1 < nl; ++l) {
+        MgLevel& L = s->lev[l];
+        const rg_matrix* A = L.A ? L.A : m;
+        xl[l] = smooth(s, m, l, bl[l], nullptr, true, deg, nullptr, st, guard);
+        const Launch Ls = chunked_launch(A->g.N, spmv_rows_per_thread<float>());
+        by_sym_vt(A->symmetric, [&](auto S) {
+            hipLaunchKernelGGL((k_residual<S.value, float>), dim3(Ls.grid), dim3(kBlock), 0, st, A->val32, A->g.ld,
+                               A->g.N, A->g.s1, A->g.s2, Ls.chunk, bl[l], xl[l], L.r, guard);
+        });
+        MgLevel& C = s->lev[l + 1];
+        if (l == 0)
+            hipLaunchKernelGGL(k_restrict<true>, dim3(grid1(C.g.N)), dim3(kBlock), 0, st, L.g, C.g, L.r, m->balance,
+                               C.b, guard);
+        else
+            hipLaunchKernelGGL(k_restrict<false>, dim3(grid1(C.g.N)), dim3(kBlock), 0, st, L.g, C.g, L.r, nullptr,
+                               C.b, guard);
+        bl[l + 1] = C.b;
+        RG_HIP(hipGetLastError());
+    }
+    // coarsest level
+    MgLevel& Lc = s->lev[nl - 1];
+    const double* coarse_scale = nullptr;
+    if (nl == 1) {
+        xl[0] = smooth(s, m, 0, r, nullptr, true, std::max(deg, m->mg_coarse_steps), z, st, guard);
+    } else if (s->with_net) {
+        const rg_matrix* A = Lc.A;
+        lpc_forward_balanced(m->lpc, bl[nl - 1], Lc.dinv, nullptr, s->logk, s->y, st);
+        const Launch Le = chunked_launch(A->g.N, spmv_rows_per_thread<double>(), 1024);
+        by_sym_vt(A->symmetric, [&](auto S) {
+            hipLaunchKernelGGL((k_coarse_energy<S.value>), dim3(Le.grid), dim3(kBlock), 0, st, A->val, A->g.ld,
+                               A->g.N, A->g.s1, A->g.s2, Le.chunk, s->y, bl[nl - 1], s->part, guard);
+        });
+        hipLaunchKernelGGL(k_coarse_alpha, dim3(1), dim3(kBlock), 0, st, s->part, Le.grid, s->alpha, guard);
+        xl[nl - 1] = s->y;
+        coarse_scale = s->alpha;
+    } else {
+        xl[nl - 1] = smooth(s, m, nl - 1, bl[nl - 1], nullptr, true, std::min(m->mg_coarse_steps, kChebSteps),
+                            nullptr, st, guard);
+    }
+    RG_HIP(hipGetLastError());
+    // up
+    for (int l = nl - 2; l >= 0; --l) {
+        MgLevel& L = s->lev[l];
+        const MgLevel& C = s->lev[l + 1];
+        if (l == 0)
+            hipLaunchKernelGGL(k_prolong<true>, dim3(grid1(L.g.N)), dim3(kBlock), 0, st, L.g, C.g, xl[l + 1],
+                               m->balance, coarse_scale, xl[l], guard);
+        else
+            hipLaunchKernelGGL(k_prolong<false>, dim3(grid1(L.g.N)), dim3(kBlock), 0, st, L.g, C.g, xl[l + 1],
+                               nullptr, coarse_scale, xl[l], guard);
+        coarse_scale = nullptr;
+        RG_HIP(hipGetLastError());
+        // post-smoothing from the corrected iterate: the Chebyshev recursion
+        // restarts (its first step has no previous update)
+        MgLevel& Lm = s->lev[l];
+        const rg_matrix* A = Lm.A ? Lm.A : m;
+        const Geom& g = A->g;
+        const Launch Ls = chunked_launch(g.N, spmv_rows_per_thread<float>());
+        double* cur = xl[l];
+        for (int k = 0; k < deg; ++k) {
+            double* dst = (l == 0 && k == deg - 1) ? z : (cur == Lm.xa ? Lm.xb : Lm.xa);
+            by_sym_vt(A->symmetric, [&](auto S) {
+                hipLaunchKernelGGL((k_cheb_step<S.value, float>), dim3(Ls.grid), dim3(kBlock), 0, st, A->val32, g.ld,
+                                   g.N, g.s1, g.s2, Ls.chunk, Lm.dinv, bl[l], cur, dst, Lm.d, Lm.coef + 2 * k, guard);
+            });
+            cur = dst;
+        }
+        xl[l] = cur;
+        RG_HIP(hipGetLastError());
+    }
+}
+
+// The Galerkin operator `levels` halvings below the (unbalanced) matrix m, as
+// a new matrix of the coarse Brick with m's storage.
+rg_matrix* coarse_operator(rg_matrix* m, int levels)
+{
+    RG_CHECK(m->g.dim == 3 && m->g.bs == 1, RG_VALUE_ERROR, "coarse_operator: scalar Brick matrices only");
+    RG_CHECK(!m->is_balanced, RG_STATE_ERROR, "coarse_operator: balanced matrix is not supported.");
+    RG_CHECK(levels >= 1, RG_VALUE_ERROR, "coarse_operator: levels must be at least 1");
+    hipStream_t st = m->ctx->stream;
+    rg_matrix* F = m;
+    for (int l = 0; l < levels; ++l) {
+        int32_t ne[3];
+        for (int d = 0; d < 3; ++d) {
+            RG_CHECK(F->g.ne[d] % 2 == 0 && F->g.ne[d] >= 2, RG_VALUE_ERROR,
+                     "coarse_operator: the grid cannot be halved that many times");
+            ne[d] = F->g.ne[d] / 2;
+        }
+        rg_matrix* C = nullptr;
+        const int rc = rg_matrix_create(m->ctx, 3, ne, F->g.len, 1, m->symmetric ? RG_MATRIX_SYMMETRIC : 0, &C);
+        if (rc != RG_OK) {
+            if (F != m) rg_matrix_destroy(F);
+            throw Error(rc, "coarse_operator: cannot create the coarse matrix");
+        }
+        const MgGeom f = geom_of(F), c = geom_of(C);
+        const unsigned G = grid1(c.N);
+        double* tmp = nullptr;
+        try {
+            RG_HIP(hipMalloc(&tmp, sizeof(double) * (c.N + G)));
+            auto launch = [&](auto SF, auto SC) {
+                hipLaunchKernelGGL((k_galerkin<SF.value, SC.value, false>), dim3(G), dim3(kBlock), 0, st, F->val, f,
+                                   nullptr, C->val, c, tmp, tmp + c.N);
+            };
+            if (F->symmetric && C->symmetric) launch(BC<true>{}, BC<true>{});
+            else if (F->symmetric) launch(BC<true>{}, BC<false>{});
+            else if (C->symmetric) launch(BC<false>{}, BC<true>{});
+            else launch(BC<false>{}, BC<false>{});
+            RG_HIP(hipGetLastError());
+            RG_HIP(hipStreamSynchronize(st));
+        } catch (...) {
+            if (tmp) hipFree(tmp);
+            rg_matrix_destroy(C);
+            if (F != m) rg_matrix_destroy(F);
+            throw;
+        }
+        hipFree(tmp);
+        C->val_zero = false;
+        if (F != m) rg_matrix_destroy(F);
+        F = C;
+    }
+    return F;
+}
+
+// One FCG(kDirs) iteration: z = M(r), the direction, q = A p, the updates.
+void mg_fcg_iteration(rg_matrix* m, double* x, double* r, double* z, KrylovState* st_dev, hipStream_t st)
+{
+    rg_mg_state* s = m->mg;
+    mg_vcycle(m, r, z, st, st_dev);
+    const Geom& g = m->g;
+    FcgArgs a;
+    a.x = x;
+    a.r = r;
+    a.z = z;
+    a.P = s->P;
+    a.Q = s->Q;
+    a.coef = s->fcg;
+    a.st = st_dev;
+    a.partials = m->ctx->partials;
+    a.n = g.N;
+    a.vld = s->vld;
+    const int vg = 1024;
+    a.chunk = 0;
+    hipLaunchKernelGGL(k_fcgm_dots, dim3(vg), dim3(kBlock), 0, st, a);
+    hipLaunchKernelGGL(k_fcgm_beta, dim3(1), dim3(kBlock), 0, st, a, vg);
+    hipLaunchKernelGGL(k_fcgm_dir, dim3(vg), dim3(kBlock), 0, st, a);
+    const Launch Lsp = chunked_launch(g.N, spmv_rows_per_thread<double>());
+    FcgArgs asp = a;
+    asp.chunk = Lsp.chunk;
+    by_sym_vt(m->symmetric, [&](auto S) {
+        hipLaunchKernelGGL((k_fcgm_spmv<S.value>), dim3(Lsp.grid), dim3(kBlock), 0, st, asp, m->val, g.ld, g.s1,
+                           g.s2);
+    });
+    hipLaunchKernelGGL(k_fcgm_alpha, dim3(1), dim3(kBlock), 0, st, a, Lsp.grid);
+    hipLaunchKernelGGL(k_fcgm_update, dim3(vg), dim3(kBlock), 0, st, a);
+    hipLaunchKernelGGL(k_fcgm_end, dim3(1), dim3(kBlock), 0, st, st_dev, m->ctx->partials, vg);
+    RG_HIP(hipGetLastError());
+}
+
+}  // namespace rg

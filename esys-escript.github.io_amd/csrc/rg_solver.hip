@@ -21,6 +21,7 @@
 #include "rg_device.h"
 #include "rg_dispatch.h"
 #include "rg_krylov.h"
+#include "rg_mg.h"
 #include "rg_stencil.h"
 
 namespace rg {
@@ -674,67 +675,6 @@ __global__ __launch_bounds__(kBlock) void k_cg_replace(PcgArgs a, const double* 
     publish_block_sums<2>(s, a.partials);
 }
 
-// ------------------------------------------------------------ flexible CG
-// Truncated flexible CG (one previous direction) with a variable
-// preconditioner z = M(r) (the learned network):
-//   beta = -(z.q_prev)/(p_prev.q_prev)                 k_fcg_zq      -> ST_FCG_BETA
-//   p = z + beta p                                     k_p_update
-//   q = A p, alpha = (p.r)/(p.q)                       k_fcg_spmv    -> ST_FCG_ALPHA
-//   x += alpha p, r -= alpha q, ||r||                  k_fcg_update  -> ST_FCG_UPDATE
-__global__ __launch_bounds__(kBlock) void k_fcg_zq(PcgArgs a, int64_t n)
-{
-    if (a.st->done) return;
-    const double* __restrict__ z = a.z;
-    const double* __restrict__ q = a.v;
-    double s[1] = {0.0};
-    for (int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x; k < n; k += (int64_t)gridDim.x * kBlock)
-        s[0] += z[k] * q[k];
-    publish_block_sums<1>(s, a.partials);
-}
-
-template <int DIM, int BS, bool SYM, typename VT>
-__global__ __launch_bounds__(kBlock) void k_fcg_spmv(PcgArgs a, const VT* __restrict__ val)
-{
-    if (a.st->done) return;
-    const RowRange rr = block_rows(a.N, a.chunk);
-    const double* __restrict__ p = a.p;
-    const double* __restrict__ r = a.r;
-    double* __restrict__ q = a.v;
-    double s[2] = {0.0, 0.0};
-    stencil_block<DIM, BS, SYM, VT>(val, a.ld, a.N, a.s1, a.s2, rr.r0, rr.r1, p,
-                                    [&](int64_t i, const double (&acc)[BS]) {
-#pragma unroll
-                                        for (int c = 0; c < BS; ++c) {
-                                            q[i * BS + c] = acc[c];
-                                            s[0] = fma(acc[c], p[i * BS + c], s[0]);
-                                            s[1] = fma(r[i * BS + c], p[i * BS + c], s[1]);
-                                        }
-                                    });
-    publish_block_sums<2>(s, a.partials);
-}
-
-// a.N counts scalar entries here
-__global__ __launch_bounds__(kBlock) void k_fcg_update(PcgArgs a)
-{
-    if (a.st->done) return;
-    const double alpha = a.st->alpha;
-    const RowRange rr = block_rows(a.N, a.chunk);
-    double* __restrict__ x = a.x;
-    double* __restrict__ r = a.r;
-    const double* __restrict__ p = a.p;
-    const double* __restrict__ q = a.v;
-    double s[1] = {0.0};
-    node_loop<1>(rr, [&](int64_t i, bool two) {
-        const Two<double> xv = ld2<false>(x + i, two), pv = ld2<false>(p + i, two);
-        const Two<double> rv = ld2<false>(r + i, two), qv = ld2<false>(q + i, two);
-        st2<false>(x + i, {fma(alpha, pv.a, xv.a), fma(alpha, pv.b, xv.b)}, two);
-        const Two<double> rn = {fma(-alpha, qv.a, rv.a), fma(-alpha, qv.b, rv.b)};
-        st2<false>(r + i, rn, two);
-        s[0] += rn.a * rn.a + (two ? rn.b * rn.b : 0.0);
-    });
-    publish_block_sums<1>(s, a.partials);
-}
-
 // ------------------------------------------------------------ BiCGStab
 // Solver_BiCGStab (paso/src/BiCGStab.cpp:77-257), Jacobi preconditioner,
 // vectors: rtld = rs, p, v, t = x2, phat = zp, shat = z2p, s.
@@ -1250,42 +1190,6 @@ struct SolverImpl {
         RG_HIP(hipGetLastError());
     }
 
-    // One iteration of the learned-preconditioner flexible CG (FP32 SpMV,
-    // FP64 vectors and scalars; `replace` ends it with an FP64 residual
-    // replacement).
-    void fcg_iteration(bool replace)
-    {
-        const Geom& g = m->g;
-        RG_CHECK(m->lpc != nullptr, RG_STATE_ERROR, "FCG: no learned preconditioner attached");
-        lpc_forward_balanced(m->lpc, w->r, m->dinv, m->balance, m->lpc_logk, w->z, st);
-        const Launch Lsp = spmv_launch<float>(), L64 = spmv_launch<double>();
-        const Launch Ln = node_launch();
-        PcgArgs an = args(w->z, Ln);
-        const PcgArgs asp = args(nullptr, Lsp), a64 = args(nullptr, L64);
-        PcgArgs ar = args(nullptr, chunked_launch(g.N * g.bs, 2));
-        ar.N = g.N * g.bs;
-        const Launch Lr = chunked_launch(ar.N, 2);
-        const int zg = 1024;
-        hipLaunchKernelGGL(k_fcg_zq, dim3(zg), dim3(kBlock), 0, st, an, w->n);
-        stage<ST_FCG_BETA, 1>(zg);
-        hipLaunchKernelGGL((k_p_update<1, double>), dim3(Ln.grid), dim3(kBlock), 0, st, an, m->dinv);
-        dispatch(g, m->symmetric, [&](auto D, auto B, auto S) {
-            if constexpr (B.value == 1) {
-                hipLaunchKernelGGL((k_fcg_spmv<D.value, 1, S.value, float>), dim3(Lsp.grid), dim3(kBlock), 0, st, asp,
-                                   m->val32);
-                stage<ST_FCG_ALPHA, 2>(Lsp.grid);
-                hipLaunchKernelGGL(k_fcg_update, dim3(Lr.grid), dim3(kBlock), 0, st, ar);
-                stage<ST_FCG_UPDATE, 1>(Lr.grid);
-                if (replace) {
-                    hipLaunchKernelGGL((k_cg_replace<D.value, 1, S.value, float>), dim3(L64.grid), dim3(kBlock), 0,
-                                       st, a64, m->val, m->dinv32);
-                    stage<ST_FCG_REPLACE, 2>(L64.grid);
-                }
-            }
-        });
-        RG_HIP(hipGetLastError());
-    }
-
     // One iteration of PASO's BiCGStab (FP64, one Jacobi sweep).
     void bicg_iteration(int sweeps)
     {
@@ -1361,7 +1265,7 @@ struct SolverImpl {
     {
         if (method == RG_METHOD_BICGSTAB) bicg_iteration(sweeps);
         else if (method == RG_METHOD_MINRES) mr_iteration(sweeps);
-        else if (method == RG_METHOD_FCG_LEARNED) fcg_iteration(last);
+        else if (method == RG_METHOD_FCG_LEARNED) mg_fcg_iteration(m, w->xw, w->r, w->z, w->st, st);
         else if (method == RG_METHOD_PCG) paso_iteration(sweeps);
         else if (method == RG_METHOD_PCG_PLAIN)
             cg_iteration<double, double>(m->val, m->dinv, sweeps, false, folds(method, sweeps));
@@ -1453,7 +1357,7 @@ struct SolverImpl {
         std::memset(&h, 0, sizeof(h));
         h.maxit = maxit;
         h.tol = tol;
-        h.replace_mode = (method == RG_METHOD_PCG_MIXED || method == RG_METHOD_FCG_LEARNED) ? 1 : 0;
+        h.replace_mode = (method == RG_METHOD_PCG_MIXED) ? 1 : 0;
         KrylovState* pin = reinterpret_cast<KrylovState*>(m->ctx->pinned);
         *pin = h;
         RG_HIP(hipMemcpyAsync(w->st, pin, sizeof(KrylovState), hipMemcpyHostToDevice, st));
@@ -1613,6 +1517,7 @@ void prepare_solve(rg_matrix* m, int prec)
         RG_HIP(hipGetLastError());
         m->is_balanced = true;
         m->val32_valid = false;
+        m->val_zero = false;
     }
     const bool needs_inverse = prec == RG_PRECONDITIONER_JACOBI || prec == RG_PRECONDITIONER_GAUSS_SEIDEL;
     if (needs_inverse && !m->have_prec) {
@@ -1657,8 +1562,8 @@ int solve(rg_matrix* m, double* x, const double* b, const rg_options& opt, rg_di
              RG_VALUE_ERROR, "Solver: the FP32 and learned methods use the Jacobi preconditioner.");
     RG_CHECK(opt.method != RG_METHOD_BICGSTAB || prec != RG_PRECONDITIONER_JACOBI || opt.sweeps <= 1,
              RG_VALUE_ERROR, "Solver: BiCGStab runs with one Jacobi sweep.");
-    RG_CHECK(opt.method != RG_METHOD_FCG_LEARNED || (m->lpc && m->g.bs == 1), RG_VALUE_ERROR,
-             "Solver: the flexible CG needs a learned preconditioner attached to a scalar matrix.");
+    RG_CHECK(opt.method != RG_METHOD_FCG_LEARNED || (m->g.bs == 1 && m->g.dim == 3), RG_VALUE_ERROR,
+             "Solver: the flexible CG with the multilevel preconditioner needs a scalar Brick matrix.");
     RG_CHECK(opt.criterion == RG_CRITERION_PASO || opt.criterion == RG_CRITERION_TRUE_RESIDUAL, RG_VALUE_ERROR,
              "Solver: unknown stopping criterion.");
     const bool true_res = opt.criterion == RG_CRITERION_TRUE_RESIDUAL;
@@ -1684,6 +1589,7 @@ int solve(rg_matrix* m, double* x, const double* b, const rg_options& opt, rg_di
         launch_to_fp32(m);
         m->val32_valid = true;
     }
+    if (method == RG_METHOD_FCG_LEARNED) mg_setup(m);
     // r = D b, norms of D b (Solver.cpp:104-130)
     hipLaunchKernelGGL(k_mul, dim3(vgrid), dim3(kBlock), 0, st, w->bb, b, m->balance, n);
     RG_HIP(hipMemcpyAsync(w->r, w->bb, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
@@ -1789,6 +1695,20 @@ void solve_preconditioner(rg_matrix* m, const rg_options& opt, double* x, const 
     RG_CHECK(x != b, RG_VALUE_ERROR, "solvePreconditioner: x and b must be distinct");
     ensure_work(m);
     prepare_solve(m, prec);
+    if (opt.method == RG_METHOD_FCG_LEARNED) {
+        // one V-cycle of the multilevel preconditioner
+        RG_CHECK(prec == RG_PRECONDITIONER_JACOBI && m->g.bs == 1 && m->g.dim == 3, RG_VALUE_ERROR,
+                 "solvePreconditioner: the multilevel preconditioner needs a scalar Brick matrix and Jacobi");
+        if (!m->val32_valid) {
+            if (!m->val32) RG_HIP(hipMalloc(&m->val32, (size_t)m->g.nstore * m->g.ld * sizeof(float)));
+            launch_to_fp32(m);
+            m->val32_valid = true;
+        }
+        mg_setup(m);
+        mg_vcycle(m, b, x, m->ctx->stream, nullptr);
+        RG_HIP(hipStreamSynchronize(m->ctx->stream));
+        return;
+    }
     SolverImpl S(m);
     S.prec = prec;
     S.precondition(b, x, std::max(1, opt.sweeps), nullptr);

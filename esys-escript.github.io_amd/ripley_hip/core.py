@@ -414,21 +414,56 @@ class SystemMatrix:
         check(load().rg_matrix_assemble(self.handle, ctypes.byref(pc), int(bool(reduced)), _ptr(rhs)))
 
     def setLearnedPreconditioner(self, net, logk=None):
-        """Attach a LearnedPreconditioner (built for this domain's node grid)
-        and log10(kappa) per node for SolverOptions.FCG_LEARNED; net=None
-        detaches.  Both are borrowed: keep them alive while solving."""
+        """Attach a LearnedPreconditioner as the coarsest-level solver of
+        SolverOptions.FCG_LEARNED's multilevel preconditioner: its node grid
+        must be a coarse level of this Brick (every other node per level),
+        which fixes the number of levels.  net=None detaches.  The network is
+        borrowed: keep it alive while solving.  logk is accepted for
+        compatibility and ignored (the network reads its level's operator)."""
         self._bind()
         if net is None:
             check(load().rg_matrix_set_learned_preconditioner(self.handle, None, None))
             self._lpc = None
             return
-        lk = None
-        if logk is not None:
-            lk = torch.as_tensor(logk).to(device=self.device, dtype=torch.float32).contiguous().reshape(-1)
-            if lk.numel() != self.num_rows:
-                raise ValueError("logk needs one value per node")
-        check(load().rg_matrix_set_learned_preconditioner(self.handle, net.handle, _ptr(lk)))
-        self._lpc = (net, lk)
+        check(load().rg_matrix_set_learned_preconditioner(self.handle, net.handle, None))
+        self._lpc = net
+
+    def setMultilevel(self, levels=0, degree=2, coarse_steps=8):
+        """Hierarchy of FCG_LEARNED without a network: grid levels (0:
+        automatic), Chebyshev smoothing steps per side of each coarse
+        correction, Chebyshev steps on the coarsest level."""
+        self._bind()
+        check(load().rg_matrix_set_multilevel(self.handle, int(levels), int(degree), int(coarse_steps)))
+
+    def coarseOperator(self, levels=1):
+        """The Galerkin operator P^T A P `levels` halvings down (P trilinear
+        interpolation), as a SystemMatrix of the coarse Brick."""
+        from .domain import Brick
+        self._bind()
+        h = ctypes.c_void_p()
+        check(load().rg_matrix_coarse_operator(self.handle, int(levels), ctypes.byref(h)))
+        ne = [n // (2 ** levels) for n in self.domain.getNumElementsPerAxis()]
+        dom = Brick(*ne, *self.domain.getLength(), device=self.device.index or 0)
+        return SystemMatrix._wrap(dom, h, self.symmetric, self.device.index or 0)
+
+    @classmethod
+    def _wrap(cls, domain, handle, symmetric, device=0):
+        """A SystemMatrix owning an existing library handle."""
+        self = cls.__new__(cls)
+        self.ctx = get_context(device)
+        self.domain = domain
+        self.device = torch.device("cuda", device)
+        self.block_size = 1
+        self.symmetric = bool(symmetric)
+        self.handle = handle
+        nr, nnz, ns, bs = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int(), ctypes.c_int()
+        check(load().rg_matrix_get_info(handle, ctypes.byref(nr), ctypes.byref(nnz), ctypes.byref(ns),
+                                        ctypes.byref(bs)))
+        self.num_rows = nr.value
+        self.nnz = nnz.value
+        self.num_slots = ns.value
+        self.size = self.num_rows
+        return self
 
     def assembleBoundary(self, coefs, rhs=None, reduced=False):
         """Face terms d (matrix) and y (rhs) of the Coefficient objects under

@@ -1,20 +1,23 @@
-"""Learned preconditioner for the flexible CG (SolverOptions.FCG_LEARNED).
+"""Learned coarse-level solver of the multilevel flexible CG
+(SolverOptions.FCG_LEARNED).
 
 The network (forward, backward and Adam kernels are the library's own BF16
-MFMA kernels, FP32 accumulate) is a 3-level 3-D U-Net on the node grid of a
-Brick:
+MFMA kernels, FP32 accumulate) is a 3-level 3-D U-Net on a node grid:
 
     level 1 (node grid):      2 -> 16 -> 16          skip s1
     level 2 (1/2, avg pool):  16 -> 32 -> 32         skip s2
     level 3 (1/4, avg pool):  32 -> 64 -> 64
     decoder:  up(64) ++ s2 -> 32,  up(32) ++ s1 -> 16,  16 -> 1 (linear)
 
-Input channels: u / rms(u) with u = r / diag(A) (the Jacobi-scaled
-residual) and log10(kappa); output: a correction proportional to the error
-e = A^-1 r.  It is trained with the cosine loss 1 - <y, e>/(|y||e|) (the
-flexible CG is invariant to the scale of its preconditioner) on
-residual/error pairs that the plain Jacobi-PCG produces: e_k = x* - x_k and
-r_k = b - A x_k for iterates x_k of the plain solver.
+It solves the coarsest level of the geometric-multigrid V-cycle that
+preconditions the flexible CG: the level's Galerkin operator A_c = P^T A P
+(trilinear P, the Brick halved per level), input channels u / rms(u) with
+u = r / diag(A_c) and log10(diag(A_c) / diag of a unit coefficient), output
+a correction proportional to the level's error e = A_c^-1 r (the solver
+scales it to minimise the energy-norm error).  It is trained with the
+cosine loss 1 - <y, e>/(|y||e|) on residual/error pairs that the plain
+Jacobi-PCG produces on A_c: r_k = b - A_c x_k and e_k = x* - x_k for
+iterates x_k of the plain solver, with b = A_c x* for smooth random x*.
 
 The reference has no learned preconditioner (PASO offers Jacobi,
 Gauss-Seidel, ILU0, RILU and AMG, paso/src/Preconditioner.cpp:46-107); the
@@ -160,53 +163,46 @@ def operator_diagonal(op):
     return d
 
 
-def plain_pcg_pairs(make_operator, b, iterations, tol=1e-9):
-    """Residual/error pairs from the plain FP64 Jacobi-PCG.
+def level_inputs(op):
+    """(dinv, logk) of the network on a level's operator: 1 / diag(A) and
+    log10(diag(A) / diag of a unit coefficient on that grid), the inputs the
+    library computes for its coarsest level."""
+    d = operator_diagonal(op)
+    h = op.domain.getLength()
+    ne = op.domain.getNumElementsPerAxis()
+    hs = [h[k] / ne[k] for k in range(3)]
+    vol = hs[0] * hs[1] * hs[2]
+    aref = sum(8.0 * vol / (9.0 * hk * hk) for hk in hs)
+    dinv = torch.as_tensor(1.0 / d, device=op.device)
+    logk = torch.as_tensor(np.log10(d / aref).astype(np.float32), device=op.device)
+    return dinv, logk
 
-    make_operator() returns a freshly assembled and constrained SystemMatrix
-    (it is called once for the reference solve, once per sampled iterate and
-    once for an unbalanced copy used for products).  For each k in
-    `iterations`, x_k is the plain solver's iterate after k iterations, and
-    the pair is (r_k = b - A x_k, e_k = x* - x_k), with x* solved to `tol`.
-    Returns (dinv, [(r_k, e_k), ...]) as device tensors."""
+
+def plain_pcg_pairs(make_operator, xstar, iterations, check_interval=32):
+    """Residual/error pairs from the plain FP64 Jacobi-PCG on A x = b with
+    b = A xstar: for each k in `iterations`, x_k is the plain solver's
+    iterate after k iterations and the pair is (r_k = b - A x_k,
+    e_k = xstar - x_k).  make_operator() returns a fresh, unbalanced
+    operator (called once for products and once per iterate: a solve
+    balances its matrix).  Returns [(r_k, e_k), ...] as device tensors."""
     A = make_operator()
-    dinv = torch.as_tensor(1.0 / operator_diagonal(A), device=A.device)
-    S = make_operator()
-    opt = SolverOptions()
-    opt.setSolverMethod(SolverOptions.PCG_PLAIN)
-    opt.setTolerance(tol)
-    opt.setTrueResidualCriterion(True)
-    # near FP64's limit the outer loop may stop on "no improvement": the
-    # iterate is still far more accurate than the sampled ones
-    opt.setAcceptanceConvergenceFailureOn()
-    xs = S.solve(b, opt)
-    del S
+    b = torch.zeros(A.size, dtype=torch.float64, device=A.device)
+    A.MatrixVector(1.0, xstar, 0.0, b)
     pairs = []
     for k in iterations:
         S = make_operator()
         o = SolverOptions()
         o.setSolverMethod(SolverOptions.PCG_PLAIN)
-        o.setTolerance(tol)
+        o.setTolerance(1e-12)
         o.setIterMax(int(k))
         o.setAcceptanceConvergenceFailureOn()
-        o.setCheckInterval(max(1, min(32, int(k))))
+        o.setCheckInterval(max(1, min(check_interval, int(k))))
         xk = S.solve(b, o)
         del S
         r = b.clone()
         A.MatrixVector(-1.0, xk, 1.0, r)
-        pairs.append((r, xs - xk))
-    return dinv, pairs
-
-
-def nodal_log10(kappa_elem, ne):
-    """log10(kappa) per node: the mean over the (up to 8) adjacent elements
-    (element-constant kappa, x fastest), float32 device tensor."""
-    n0, n1, n2 = ne
-    k = torch.as_tensor(kappa_elem, dtype=torch.float64).reshape(1, 1, n2, n1, n0)
-    lk = torch.log10(k)
-    lk = torch.nn.functional.pad(lk, (1, 1, 1, 1, 1, 1), mode="replicate")
-    nod = torch.nn.functional.avg_pool3d(lk, 2, stride=1)
-    return nod.reshape(-1).to(torch.float32)
+        pairs.append((r, xstar - xk))
+    return pairs
 
 
 class BrickPoisson:
@@ -227,7 +223,6 @@ class BrickPoisson:
         self.cY = Coefficient(float(f), (), E, Q, self.device)
         self.q = self.dom.whereZero(2)
         self.nodes = tuple(n + 1 for n in self.ne)
-        self.logk = nodal_log10(self.kappa.cpu(), self.ne).to(self.device)
         self.rhs = None
 
     def operator(self):
@@ -241,26 +236,43 @@ class BrickPoisson:
         return op
 
 
-DEFAULT_ITERATES = (1, 2, 4, 8, 16, 32, 64, 128, 256, 512)
+DEFAULT_ITERATES = (1, 2, 4, 8, 16, 32, 64, 128, 256)
+
+
+def smooth_target(nodes, seed, device):
+    """A smooth random x* on a node grid (Gaussian random field, correlation
+    2..8 nodes by the seed), zero on the Dirichlet plane z = 0."""
+    from . import fields
+    rng = np.random.default_rng(seed)
+    corr = float(rng.uniform(2.0, 8.0))
+    g = fields.gaussian_random_field(nodes, corr, int(rng.integers(1 << 30))).reshape(-1)
+    g[: nodes[0] * nodes[1]] = 0.0
+    return torch.as_tensor(g, device=device)
 
 
 def generate_training_set(ne, num_fields, kappa_fn, iterations=DEFAULT_ITERATES, first_seed=1, log=None,
-                          per_field=None):
-    """[(r, e, dinv, logk), ...] on the device: for each seed s in
-    first_seed .. first_seed+num_fields-1, kappa = kappa_fn(ne, s) and the
-    plain Jacobi-PCG's residual/error pairs at the given iteration counts
-    (with per_field = m: m of them per field, chosen by the seed)."""
+                          per_field=None, levels=1):
+    """[(r, e, dinv, logk), ...] on the device for the network that solves
+    the level `levels` halvings below Brick(ne): for each seed s in
+    first_seed .. first_seed+num_fields-1, kappa = kappa_fn(ne, s), the
+    Galerkin operator A_c of the constrained fine operator, and the plain
+    Jacobi-PCG's residual/error pairs on A_c x = A_c x* (x* smooth, random)
+    at the given iteration counts (with per_field = m: m of them per field,
+    chosen by the seed)."""
     data = []
     for s in range(first_seed, first_seed + num_fields):
         prob = BrickPoisson(ne, kappa_fn(ne, s))
-        prob.operator()
-        b = prob.rhs
+        fine = prob.operator()
+        make = lambda: fine.coarseOperator(levels)
+        C = make()
+        dinv, logk = level_inputs(C)
+        nodes = tuple(n + 1 for n in C.domain.getNumElementsPerAxis())
+        xstar = smooth_target(nodes, 7919 * s + 1, C.device)
         its = iterations
         if per_field:
             its = [iterations[(7 * s + j) % len(iterations)] for j in range(per_field)]
-        dinv, pairs = plain_pcg_pairs(prob.operator, b, its)
-        for r, e in pairs:
-            data.append((r, e, dinv, prob.logk))
+        for r, e in plain_pcg_pairs(make, xstar, its):
+            data.append((r, e, dinv, logk))
         n = s - first_seed + 1
         if log and (n % max(1, num_fields // 20) == 0 or n == num_fields):
             log("training set: field %d/%d (%d pairs)" % (n, num_fields, len(data)))
@@ -288,10 +300,11 @@ def _main():
     import time
     from . import fields
     ap = argparse.ArgumentParser(description="train the learned preconditioner on the GPU")
-    ap.add_argument("--ne", type=int, default=64)
+    ap.add_argument("--ne", type=int, default=64, help="elements per axis of the fine training grids")
+    ap.add_argument("--levels", type=int, default=1, help="halvings from the fine grid to the network's level")
     ap.add_argument("--fields", type=int, default=32)
     ap.add_argument("--per-field", type=int, default=0,
-                    help="residual/error pairs per field (0: one per iteration count 1, 2, 4, ..., 512)")
+                    help="residual/error pairs per field (0: one per iteration count 1, 2, 4, ..., 256)")
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--family", default="log10grf", choices=["log10grf", "lognormal"])
@@ -329,10 +342,10 @@ def _main():
 
     ne = (a.ne,) * 3
     t0 = time.perf_counter()
-    data = generate_training_set(ne, a.fields, kfn, log=log, per_field=a.per_field or None)
+    data = generate_training_set(ne, a.fields, kfn, log=log, per_field=a.per_field or None, levels=a.levels)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
-    net = LearnedPreconditioner(tuple(n + 1 for n in ne), training=True)
+    net = LearnedPreconditioner(tuple(n // 2 ** a.levels + 1 for n in ne), training=True)
     if a.init:
         net.load(a.init)
     hist = train(net, data, a.steps, lr=a.lr, log=log)

@@ -122,11 +122,45 @@ class LinearPDE:
             t = t.reshape(1, -1).expand(N, n)
         return t.reshape(-1).contiguous()
 
+    def _symmetric_coefficient(self, name):
+        """Whether coefficient `name` leaves the operator symmetric: A_ij = A_ji
+        (A_ijkl = A_klij for systems), D_km = D_mk, d_km = d_mk."""
+        v = self._coef.get(name)
+        if v is None or isinstance(v, IsotropicCoefficient):
+            return True
+        t = torch.as_tensor(v, dtype=torch.float64) if not isinstance(v, torch.Tensor) else v
+        shape = self._point_shape(name)
+        if len(shape) < 2 or t.dim() < len(shape):
+            return True
+        t = t.reshape(-1, *shape)
+        if len(shape) == 2:
+            tt = t.transpose(-1, -2)
+        else:
+            tt = t.permute(0, 3, 4, 1, 2)
+        return bool(torch.equal(t, tt))
+
+    def _use_symmetric_storage(self):
+        """Symmetric storage (main + upper stencil diagonals) for a PDE declared
+        symmetric -- the flag PASO reads through SolverBuddy::isSymmetric
+        (ripley/src/RipleyDomain.cpp:925, paso/src/Options.cpp:39) -- whose
+        coefficients give a symmetric matrix: no B or C, symmetric A, D, d."""
+        if not self.isSymmetric():
+            return False
+        if any(self._coef.get(k) is not None for k in ("B", "C", "B_reduced", "C_reduced")):
+            return False
+        return all(self._symmetric_coefficient(k) for k in ("A", "A_reduced", "D", "D_reduced", "d", "d_reduced",
+                                                            "d_dirac"))
+
     def getSystem(self):
         """(operator, rhs): create, assemble both orders, insert constraints
-        (linearPDEs.py:1881-2082)."""
+        (linearPDEs.py:1881-2082).  The operator uses symmetric storage when
+        the PDE is declared symmetric and its coefficients keep it so."""
+        sym = self._use_symmetric_storage()
+        if self._operator is not None and self._operator.symmetric != sym:
+            self._operator = None
+            self._operator_valid = False
         if self._operator is None:
-            self._operator = self.domain.newOperator(self.n, symmetric=False)
+            self._operator = self.domain.newOperator(self.n, symmetric=sym)
         if not self._operator_valid or not self._rhs_valid:
             op = self._operator
             op.resetValues(preserveSolverData=False)

@@ -98,7 +98,13 @@ int rg_matrix_reset_values(rg_matrix* m, int preserve_solver_data);
  * element's local order; em (host) holds per element
  * EM[k + n*(m + n*(a + nN*b))] with n = num_eq = block size.  The stencil slot
  * of a node pair follows from the local positions a, b, as in the reference
- * stencil matrix.  Elements are applied in order (deterministic). */
+ * stencil matrix.  Elements are added in 2^dim launches, one per element
+ * colour (the parity of local node 0's grid coordinates), so grid cells of
+ * one colour share no node and the result is the same on every run; the
+ * node ids and element matrices go through a device staging buffer that the
+ * matrix keeps (grown when a larger batch arrives) and the call returns
+ * without waiting for the kernels.  Adapters pass many elements per call
+ * (all of them, or one colour's), not one. */
 int rg_matrix_add_element_blocks(rg_matrix* m, int64_t num_elements, int nodes_per_element,
                                  const int32_t* nodes, int num_eq, const double* em);
 
@@ -201,11 +207,19 @@ int rg_matrix_save_mm(rg_matrix* m, const char* filename);
  *   RG_METHOD_MINRES      PASO's Solver_MINRES (paso/src/MINRES.cpp:59-191) in
  *                         FP64 for symmetric (possibly indefinite) systems;
  *                         r.M^-1 r < 0 ends the solve with RG_NEGATIVE_NORM.
- *   RG_METHOD_FCG_LEARNED flexible CG (truncated to one direction) whose
- *                        preconditioner is the learned network attached with
- *                        rg_matrix_set_learned_preconditioner; SpMV on FP32
- *                        values with FP64 residual replacement as above.  If
- *                        the residual stagnates (no 1% decrease over two
+ *   RG_METHOD_FCG_LEARNED flexible CG keeping the last 4 directions (each
+ *                        new one A-orthogonalised against them; FP64
+ *                        operator, vectors and scalars) preconditioned by one
+ *                        geometric-multigrid V-cycle per iteration: Galerkin
+ *                        coarse operators P^T A P of the Brick halved per
+ *                        level (P trilinear), Chebyshev smoothing on FP32
+ *                        values before and after each coarse correction, and
+ *                        the coarsest level solved by the network attached
+ *                        with rg_matrix_set_learned_preconditioner (scaled to
+ *                        minimise the level's energy-norm error) or, with
+ *                        none attached, by Chebyshev steps
+ *                        (rg_matrix_set_multilevel).  Scalar Brick matrices.
+ *                        If the residual stagnates (no 1% decrease over two
  *                        check intervals) the solve falls back to
  *                        RG_METHOD_PCG_MIXED (Jacobi) from the current iterate.
  *                        Extension: the reference has no counterpart.
@@ -300,7 +314,9 @@ int rg_write_binary_grid(const char* filename, int dim, const int64_t* grid, con
 /* SystemMatrix::setPreconditioner + solvePreconditioner
  * (paso/src/SystemMatrix.cpp:42-56): balances the matrix if it is not yet
  * (as a solve would), builds opt->preconditioner with opt->sweeps if needed
- * and computes x = M^-1 b in the balanced space.  x, b: device,
+ * and computes x = M^-1 b in the balanced space.  With opt->method =
+ * RG_METHOD_FCG_LEARNED, x is one V-cycle of that method's multilevel
+ * preconditioner.  x, b: device,
  * num_rows*block_size doubles, distinct. */
 int rg_matrix_solve_preconditioner(rg_matrix* m, const rg_options* opt, double* x, const double* b);
 
@@ -381,10 +397,29 @@ int rg_lpc_gradients(rg_lpc* net, const double* r, const double* dinv, const flo
 /* One Adam step (lr) on that loss. */
 int rg_lpc_train_step(rg_lpc* net, const double* r, const double* dinv, const float* logk, const double* e,
                       float lr, double* loss);
-/* Attach a network (created for this matrix's node grid, scalar 3-D
- * problems) and log10(kappa) per node (device, kept borrowed) for
- * RG_METHOD_FCG_LEARNED.  net = NULL detaches. */
+/* Attach a network as the coarsest-level solver of RG_METHOD_FCG_LEARNED's
+ * multilevel preconditioner: its node grid must be a coarse level of the
+ * matrix's Brick (ne / 2^k elements per axis, k >= 1), which sets the
+ * number of levels.  On that level the network's inputs are the
+ * Jacobi-scaled residual and log10(a_ii / a_ii of a unit coefficient) of the
+ * level's Galerkin operator.  logk is reserved (pass NULL).  net = NULL
+ * detaches. */
 int rg_matrix_set_learned_preconditioner(rg_matrix* m, rg_lpc* net, const float* logk);
+
+/* Hierarchy of RG_METHOD_FCG_LEARNED without a network: `levels` grid levels
+ * (0: until a level has at most 9^3 nodes or cannot be halved), `degree`
+ * Chebyshev steps before and after each coarse correction (1..16, default
+ * 2) and `coarse_steps` Chebyshev steps from zero on the coarsest level
+ * (1..16, default 8).  degree applies with a network too.  No counterpart in
+ * the reference. */
+int rg_matrix_set_multilevel(rg_matrix* m, int levels, int degree, int coarse_steps);
+
+/* The Galerkin operator P^T A P of the matrix `levels` halvings down (P the
+ * trilinear interpolation of the multilevel preconditioner): a new matrix of
+ * the coarse Brick with the same storage, owned by the caller
+ * (rg_matrix_destroy).  The matrix must not be balanced.  Scalar Brick
+ * matrices.  No counterpart in the reference. */
+int rg_matrix_coarse_operator(rg_matrix* m, int levels, rg_matrix** out);
 
 #ifdef __cplusplus
 }

@@ -119,25 +119,39 @@ def test_full_size_other_methods_reach_the_fp64_residual(c2, method, prec, sym):
 def test_full_size_other_solvers_against_oracle(c2, method):
     """PASO's MINRES and BiCGStab and the textbook PCG at 128^3 (symmetric
     storage, the folded and non-temporal paths) against the oracle's
-    restatements.  BiCGStab's residual is not monotone and over ~1200
-    iterations its count moves with the reduction order: the oracle alone
-    takes 1152, 1311 and 1260 iterations on 5, 8 and 16 OpenMP threads, the
-    GPU 1210; so its bar is 10 %, the others match to 2 %."""
+    restatements.  MINRES and PCG match the oracle's count to 2 %.
+    BiCGStab's residual is not monotone and over ~1200 iterations its count
+    moves with the order of the reductions (the oracle's recurrence itself is
+    pinned iteration by iteration in tests/test_oracle_recurrences.py): the
+    bar is the range the oracle spans over 5, 8 and 16 OpenMP threads (its
+    reduction orders), widened by 10 %."""
     qn = c2["q"].cpu().numpy()
     oval = c2["oval"].copy()
     O.nullify(1, c2["ptr"], c2["idx"], oval, qn, qn, 1.0)
     orhs = np.where(qn > 0, 0.0, c2["orhs"])
     xo, dg, _, _ = O.solve(1, c2["ptr"], c2["idx"], oval, orhs, tolerance=1e-8, method=method)
     assert dg["rc"] == 0 and dg["converged"] == 1
+    counts = [dg["num_iter"]]
+    if method == "BICGSTAB":
+        keep = O.num_threads()
+        try:
+            for nt in (5, 8, 16):
+                O.set_num_threads(nt)
+                _, dgt, _, _ = O.solve(1, c2["ptr"], c2["idx"], oval, orhs, tolerance=1e-8, method=method)
+                assert dgt["rc"] == 0
+                counts.append(dgt["num_iter"])
+        finally:
+            O.set_num_threads(keep)
     op, rhs = _constrained(c2, True)
     opts = R.SolverOptions()
     opts.setSymmetryOn()
     opts.setSolverMethod(getattr(R.SolverOptions, method))
     x = op.solve(rhs, opts).cpu().numpy()
-    print(method, "128^3 symmetric iterations gpu", opts.getDiagnostics("num_iter"), "oracle", dg["num_iter"])
+    n_gpu = opts.getDiagnostics("num_iter")
+    print(method, "128^3 symmetric iterations gpu", n_gpu, "oracle", counts, flush=True)
     assert opts.hasConverged()
     bar = 0.10 if method == "BICGSTAB" else 0.02
-    assert abs(opts.getDiagnostics("num_iter") - dg["num_iter"]) <= bar * dg["num_iter"]
+    assert (1 - bar) * min(counts) <= n_gpu <= (1 + bar) * max(counts)
     assert np.linalg.norm(x - xo) <= 1e-6 * np.linalg.norm(xo)
     res = np.linalg.norm(orhs - O.spmv(1, c2["ptr"], c2["idx"], oval, x)) / np.linalg.norm(orhs)
     assert res <= 1e-7

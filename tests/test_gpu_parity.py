@@ -103,6 +103,34 @@ def test_assembly_parity_reduced_and_isotropic(dim, ne):
     assert np.allclose(rhs.cpu().numpy(), orhs, rtol=1e-13, atol=0)
 
 
+@pytest.mark.parametrize("dim,ne,sym", [(3, (5, 4, 6), False), (3, (5, 4, 6), True), (2, (9, 6), False),
+                                        (2, (9, 6), True)])
+def test_assembly_element_constant_isotropic(dim, ne, sym):
+    # element-constant kappa I with element-constant Y (the constant-coefficient
+    # branch, DefaultAssembler3D.cpp:906-993, :2205-2214): the first assembly
+    # writes into the zeroed matrix, the second adds to it; the general
+    # quadrature kernel (A as a full tensor) gives the same entries
+    rng = np.random.default_rng(31 + dim + 2 * sym)
+    length = tuple(0.7 + 0.3 * d for d in range(dim))
+    dom, _ = grid(dim, ne, length)
+    E = dom.getNumElements()
+    kap = np.exp(rng.standard_normal(E))
+    Y = rng.standard_normal(E)
+    op, rhs = gpu_assemble(dom, 1, symmetric=sym, iso=kap, Y=Y)
+    ptr, idx, val = op.exportCSR()
+    _, _, oval, orhs = O.assemble(dim, ne, length, A=kap[:, None, None] * np.eye(dim)[None], Y=Y)
+    ok, worst = entrywise_ok(1, ptr, val, oval)
+    assert ok, worst
+    assert np.max(np.abs(rhs.cpu().numpy() - orhs)) <= 1e-13 * np.abs(orhs).max()
+    gen, _ = gpu_assemble(dom, 1, symmetric=sym, A=kap[:, None, None] * np.eye(dim)[None], Y=Y)
+    ok, worst = entrywise_ok(1, ptr, gen.exportCSR()[2], oval)
+    assert ok, worst
+    E_, Q = dom.getNumElements(), dom.getNumQuadraturePoints()
+    op.assemble({"A": R.Coefficient(to_dev(kap), (dim, dim), E_, Q, dom.device, isotropic=True)}, None)
+    ok, worst = entrywise_ok(1, ptr, op.exportCSR()[2], 2 * oval)
+    assert ok, worst
+
+
 @pytest.mark.parametrize("n,dim,ne", [(3, 3, (4, 3, 3)), (2, 2, (6, 5)), (3, 2, (4, 4))])
 def test_assembly_parity_system(n, dim, ne):
     rng = np.random.default_rng(n * 10 + dim)
