@@ -597,6 +597,28 @@ int rg_matrix_coarse_operator(rg_matrix* m, int levels, rg_matrix** out)
     });
 }
 
+int rg_matrix_vcycle(rg_matrix* m, const double* b, double* x)
+{
+    return guarded([&] {
+        check_matrix(m);
+        RG_CHECK(b && x && b != x, RG_VALUE_ERROR, "vcycle: b and x must be distinct device vectors");
+        use_device(m->ctx);
+        rg::vcycle_unbalanced(m, b, x);
+        return RG_OK;
+    });
+}
+
+int rg_matrix_smooth(rg_matrix* m, const double* b, double* x, int steps)
+{
+    return guarded([&] {
+        check_matrix(m);
+        RG_CHECK(b && x && b != x, RG_VALUE_ERROR, "smooth: b and x must be distinct device vectors");
+        use_device(m->ctx);
+        rg::smooth_unbalanced(m, b, x, steps);
+        return RG_OK;
+    });
+}
+
 int rg_matrix_set_multilevel(rg_matrix* m, int levels, int degree, int coarse_steps)
 {
     return guarded([&] {

@@ -549,6 +549,9 @@ struct Net {
     int nchunk = 64;
     double* scale = nullptr;       // device scalar: input normalisation
     double* red = nullptr;         // reduction scratch (partials + results)
+    rg_matrix* loss_op = nullptr;  // energy-norm loss operator (borrowed; null: cosine loss)
+    double* zbuf = nullptr;        // energy loss: the output z on the nodes and A z
+    double* azbuf = nullptr;
     std::vector<void*> allocs;
 
     void* alloc(size_t bytes)
@@ -614,6 +617,11 @@ void net_init(Net& N, rg_context* ctx, const int32_t* nodes, bool training)
     }
     N.scale = (double*)N.alloc(sizeof(double) * 4);
     N.red = (double*)N.alloc(sizeof(double) * (3 * 1024 + 8));   // partials, then 3 loss dots at 3*1024
+    if (training) {
+        const int64_t nn = (int64_t)nodes[0] * nodes[1] * nodes[2];
+        N.zbuf = (double*)N.alloc(sizeof(double) * nn);
+        N.azbuf = (double*)N.alloc(sizeof(double) * nn);
+    }
 }
 
 __global__ void k_to_bf16(const float* __restrict__ a, bf16* __restrict__ b, int64_t n)
@@ -756,6 +764,47 @@ __global__ void k_loss_grad(Grid3 g, int n0, int n1, int n2, const bf16* __restr
     }
 }
 
+// Energy-norm loss of the output z (physical units) of a level with operator
+// A, for a pair with r = A e:  L = 1 - (r.z)^2 / ((r.e)(z.A z)), the fraction
+// of the error energy e.A e that the energy-optimal step along z leaves.
+// Partials of (r.z, z.Az, r.e).
+__global__ void k_energy_part(int64_t n, const double* __restrict__ r, const double* __restrict__ z,
+                              const double* __restrict__ az, const double* __restrict__ e, double* __restrict__ part)
+{
+    __shared__ double red[3][256];
+    double s[3] = {0.0, 0.0, 0.0};
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        s[0] += r[i] * z[i];
+        s[1] += z[i] * az[i];
+        s[2] += r[i] * e[i];
+    }
+    for (int k = 0; k < 3; ++k) red[k][threadIdx.x] = s[k];
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o)
+            for (int k = 0; k < 3; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 3; ++k) part[k * gridDim.x + blockIdx.x] = red[k][0];
+}
+
+// dL/dy into gy channel 0: with c = r.z, q = z.Az, E = r.e,
+// dL/dz = -(2c / (E q)) (r - (c/q) A z), and z = y / scale.
+__global__ void k_energy_grad(Grid3 g, int n0, int n1, int n2, const double* __restrict__ r,
+                              const double* __restrict__ az, const double* __restrict__ dots,
+                              const double* __restrict__ scale, bf16* __restrict__ gy)
+{
+    const double c = dots[0], q = fmax(dots[1], 1e-300), E = fmax(dots[2], 1e-300);
+    const double f = -2.0 * c / (E * q) / scale[0], t = c / q;
+    const int64_t nn = (int64_t)n0 * n1 * n2;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nn; i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t x = i % n0, yv = (i / n0) % n1, z = i / ((int64_t)n0 * n1);
+        const int64_t p = (((z + 1) * g.p1 + (yv + 1)) * g.p0 + (x + 1)) * 16;
+        gy[p] = (bf16)(float)(f * (r[i] - t * az[i]));
+    }
+}
+
 __global__ void k_adam(float* __restrict__ w, const float* __restrict__ gw, float* __restrict__ m, float* __restrict__ v,
                        int64_t n, float lr, float b1, float b2, float eps, float c1, float c2)
 {
@@ -803,14 +852,24 @@ double net_gradients(Net& N, const double* r, const double* dinv, const float* l
 {
     hipStream_t st = N.ctx->stream;
     const Grid3 &g1 = N.g1, &g2 = N.g2, &g3 = N.g3;
-    net_forward(N, r, dinv, nullptr, logk, nullptr, st);
-    // target in normalised units: e * scale (the output is y / scale)
+    const int64_t nn = (int64_t)N.n[0] * N.n[1] * N.n[2];
+    const bool energy = N.loss_op != nullptr;
+    net_forward(N, r, dinv, nullptr, logk, energy ? N.zbuf : nullptr, st);
     const int np = 256;
-    hipLaunchKernelGGL(k_loss_part, dim3(np), dim3(256), 0, st, g1, N.n[0], N.n[1], N.n[2], N.y.p, e, N.red);
-    hipLaunchKernelGGL(k_loss_final, dim3(1), dim3(64), 0, st, N.red, np, N.red + 3 * 1024);
     RG_HIP(hipMemsetAsync(N.gy.p, 0, sizeof(bf16) * padded_size(g1) * 16, st));
-    hipLaunchKernelGGL(k_loss_grad, dim3(ew_grid((int64_t)N.n[0] * N.n[1] * N.n[2])), dim3(256), 0, st, g1, N.n[0],
-                       N.n[1], N.n[2], N.y.p, e, N.red + 3 * 1024, N.gy.p);
+    if (energy) {
+        launch_spmv_on(N.loss_op, 1.0, N.zbuf, 0.0, N.azbuf, st);
+        hipLaunchKernelGGL(k_energy_part, dim3(np), dim3(256), 0, st, nn, r, N.zbuf, N.azbuf, e, N.red);
+        hipLaunchKernelGGL(k_loss_final, dim3(1), dim3(64), 0, st, N.red, np, N.red + 3 * 1024);
+        hipLaunchKernelGGL(k_energy_grad, dim3(ew_grid(nn)), dim3(256), 0, st, g1, N.n[0], N.n[1], N.n[2], r,
+                           N.azbuf, N.red + 3 * 1024, N.scale, N.gy.p);
+    } else {
+        // cosine loss on the output in normalised units (the output is y / scale)
+        hipLaunchKernelGGL(k_loss_part, dim3(np), dim3(256), 0, st, g1, N.n[0], N.n[1], N.n[2], N.y.p, e, N.red);
+        hipLaunchKernelGGL(k_loss_final, dim3(1), dim3(64), 0, st, N.red, np, N.red + 3 * 1024);
+        hipLaunchKernelGGL(k_loss_grad, dim3(ew_grid(nn)), dim3(256), 0, st, g1, N.n[0], N.n[1], N.n[2], N.y.p, e,
+                           N.red + 3 * 1024, N.gy.p);
+    }
     // output layer (linear)
     layer_wgrad(N, 8, g1, N.a5.p, N.gy.p);
     layer_dgrad(N, 8, g1, N.gy.p, N.ga5.p);
@@ -861,6 +920,7 @@ double net_gradients(Net& N, const double* r, const double* dinv, const float* l
     double dots[3];
     RG_HIP(hipMemcpyAsync(dots, N.red + 3 * 1024, 3 * sizeof(double), hipMemcpyDeviceToHost, st));
     RG_HIP(hipStreamSynchronize(st));
+    if (energy) return 1.0 - dots[0] * dots[0] / (std::max(dots[1], 1e-300) * std::max(dots[2], 1e-300));
     const double ny = std::sqrt(std::max(dots[1], 1e-300)), ne = std::sqrt(std::max(dots[2], 1e-300));
     return 1.0 - dots[0] / (ny * ne);
 }
@@ -1030,6 +1090,21 @@ int rg_lpc_gradients(rg_lpc* h, const double* r, const double* dinv, const float
             }
             RG_HIP(hipStreamSynchronize(N.ctx->stream));
         }
+        return RG_OK;
+    });
+}
+
+int rg_lpc_set_loss_operator(rg_lpc* h, rg_matrix* A)
+{
+    return lpc_guard([&] {
+        RG_CHECK(h && h->training, RG_VALUE_ERROR, "rg_lpc_set_loss_operator: needs a network created for training");
+        if (A) {
+            RG_CHECK(A->g.dim == 3 && A->g.bs == 1 && A->g.nn[0] == h->net.n[0] && A->g.nn[1] == h->net.n[1] &&
+                         A->g.nn[2] == h->net.n[2],
+                     RG_VALUE_ERROR, "rg_lpc_set_loss_operator: the operator's grid must be the network's");
+            RG_CHECK(!A->is_balanced, RG_STATE_ERROR, "rg_lpc_set_loss_operator: balanced matrix is not supported.");
+        }
+        h->net.loss_op = A;
         return RG_OK;
     });
 }

@@ -17,11 +17,14 @@
 //     over [lmax / 10, lmax], lmax the Gershgorin bound
 //     max_i sum_j |a_ij| / a_ii, before (from zero) and after the coarse
 //     correction, values in FP32, vectors in FP64;
-//   * coarsest level: the network y = Net(r_c) (its inputs the Jacobi-scaled
-//     residual and log10 of the level's diagonal relative to that of a unit
-//     coefficient), scaled by alpha = (r_c.y)/(y.A_c y), the step that
-//     minimises the A_c-norm error along y; without a network, `coarse_steps`
-//     Chebyshev steps from zero.
+//   * coarsest level: `coarse_steps` Chebyshev steps from zero, x_c; with a
+//     network, then y = Net(r') for the residual r' = b_c - A_c x_c they
+//     leave (inputs the Jacobi-scaled residual and log10 of the level's
+//     diagonal relative to that of a unit coefficient), low-pass filtered by
+//     the same Chebyshev steps on A_c y = 0 from y, and x_c += alpha y with
+//     alpha = (r'.y)/(y.A_c y), the step that minimises the A_c-norm error
+//     along y -- the network corrects what the smoother leaves, and the step
+//     never increases the level's error energy.
 //
 // The V-cycle runs in the physical (unbalanced) variables below level 0:
 // r_1 = P^T D^-1 r_bal going down, y_bal += D^-1 P x_1 coming up, so that
@@ -227,6 +230,13 @@ __global__ void k_level_logk(const double* __restrict__ main_slot, int64_t N, do
     if (i < N) logk[i] = (float)log10(fmax(main_slot[i], 1e-300) / aref);
 }
 
+// dinv = 1 / a_ii from the main slot
+__global__ void k_inv_main_slot(const double* __restrict__ main_slot, int64_t N, double* __restrict__ dinv)
+{
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i < N) dinv[i] = 1.0 / main_slot[i];
+}
+
 // ------------------------------------------------------------ V-cycle kernels
 // First Chebyshev step from x = 0: d = c D^-1 b, x = d.
 __global__ __launch_bounds__(kBlock) void k_cheb_first(int64_t N, const double* __restrict__ b,
@@ -302,13 +312,12 @@ __global__ __launch_bounds__(kBlock) void k_restrict(MgGeom f, MgGeom c, const d
     bc[I] = s;
 }
 
-// x_f += scale * (P x_c) / bal: one fine node per thread over the (up to 8)
-// coarse nodes that interpolate it; scale (device, may be null) is the
-// coarse step length.
+// x_f += (P x_c) / bal: one fine node per thread over the (up to 8) coarse
+// nodes that interpolate it.
 template <bool BAL>
 __global__ __launch_bounds__(kBlock) void k_prolong(MgGeom f, MgGeom c, const double* __restrict__ xc,
-                                                    const double* __restrict__ bal, const double* __restrict__ scale,
-                                                    double* __restrict__ xf, const KrylovState* guard)
+                                                    const double* __restrict__ bal, double* __restrict__ xf,
+                                                    const KrylovState* guard)
 {
     if (guard_done(guard)) return;
     const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
@@ -325,7 +334,6 @@ __global__ __launch_bounds__(kBlock) void k_prolong(MgGeom f, MgGeom c, const do
         const double w = (ox ? 0.5 : 1.0) * (oy ? 0.5 : 1.0) * (oz ? 0.5 : 1.0);
         s += use ? w * xc[J] : 0.0;
     }
-    if (scale) s *= scale[0];
     if (BAL) s /= bal[i];
     xf[i] += s;
 }
@@ -355,6 +363,17 @@ __global__ __launch_bounds__(kBlock) void k_coarse_alpha(const double* __restric
     double tot[2];
     reduce_partials<2>(partials, G, tot);
     if (threadIdx.x == 0) alpha[0] = tot[1] > 0.0 ? tot[0] / tot[1] : 0.0;
+}
+
+// x += alpha y (alpha in device memory)
+__global__ __launch_bounds__(kBlock) void k_axpy_dev(int64_t N, const double* __restrict__ alpha,
+                                                     const double* __restrict__ y, double* __restrict__ x,
+                                                     const KrylovState* guard)
+{
+    if (guard_done(guard)) return;
+    const double a = alpha[0];
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < N; i += (int64_t)gridDim.x * kBlock)
+        x[i] = fma(a, y[i], x[i]);
 }
 
 // ------------------------------------------------------------ FCG(kDirs)
@@ -519,12 +538,16 @@ struct rg_mg_state {
     double* gpart = nullptr;       // block maxima of the set-up's Gershgorin bounds (one per 256 fine rows)
     double* alpha = nullptr;       // coarse step length
     float* logk = nullptr;         // network input of the coarsest level
-    double* y = nullptr;           // network output
+    double* y = nullptr;           // network output and its filtered copies
+    double* y2 = nullptr;
+    double* zero = nullptr;        // zero right-hand side of the output filter
     double* P = nullptr;           // FCG ring of directions and their products
     double* Q = nullptr;
     double* fcg = nullptr;
     int64_t vld = 0;
     bool with_net = false;
+    int net_level = -1;            // level the network corrects (-1: none)
+    bool balanced = true;          // level 0 is a balanced matrix (D A D, D = m->balance)
 };
 
 namespace {
@@ -543,39 +566,36 @@ double* dalloc(int64_t n)
     return p;
 }
 
-// Number of levels the hierarchy of m gets: down to the network's grid when
-// one is attached, else m->mg_levels (0: until a level has at most 9^3
-// nodes or cannot be halved).
-int plan_levels(const rg_matrix* m, bool* with_net)
+// Levels of the hierarchy of m: m->mg_levels, or (0) until a level has at
+// most 9^3 nodes or cannot be halved.  With a network attached, *net_level
+// is the level whose node grid is the network's (at least one halving; the
+// hierarchy reaches it), else -1.
+int plan_levels(const rg_matrix* m, int* net_level)
 {
     const Geom& g = m->g;
     int ne[3] = {g.ne[0], g.ne[1], g.ne[2]};
     int32_t target[3] = {0, 0, 0};
-    *with_net = m->lpc != nullptr;
-    if (*with_net) lpc_nodes(m->lpc, target);
+    const bool with_net = m->lpc != nullptr;
+    if (with_net) lpc_nodes(m->lpc, target);
+    *net_level = -1;
     int levels = 1;
     for (;;) {
         const int64_t nodes = (int64_t)(ne[0] + 1) * (ne[1] + 1) * (ne[2] + 1);
-        if (*with_net) {
-            if (ne[0] + 1 == target[0] && ne[1] + 1 == target[1] && ne[2] + 1 == target[2]) {
-                RG_CHECK(levels >= 2, RG_VALUE_ERROR,
-                         "FCG_LEARNED: the network's node grid must be a coarse level (at least one halving) of the "
-                         "matrix's grid");
-                return levels;
-            }
-        } else if (m->mg_levels > 0) {
-            if (levels == m->mg_levels) return levels;
-        } else if (nodes <= 9 * 9 * 9) {
-            return levels;
+        if (with_net && ne[0] + 1 == target[0] && ne[1] + 1 == target[1] && ne[2] + 1 == target[2]) {
+            RG_CHECK(levels >= 2, RG_VALUE_ERROR,
+                     "FCG_LEARNED: the network's node grid must be a coarse level (at least one halving) of the "
+                     "matrix's grid");
+            *net_level = levels - 1;
         }
+        const bool deep_enough = m->mg_levels > 0 ? levels >= m->mg_levels : nodes <= 9 * 9 * 9;
+        if (deep_enough && (!with_net || *net_level >= 0)) return levels;
         const bool halvable = ne[0] % 2 == 0 && ne[1] % 2 == 0 && ne[2] % 2 == 0 && ne[0] >= 2 && ne[1] >= 2 &&
                               ne[2] >= 2;
         if (!halvable || levels == kMaxLevels) {
-            RG_CHECK(!*with_net, RG_VALUE_ERROR,
+            RG_CHECK(!with_net || *net_level >= 0, RG_VALUE_ERROR,
                      "FCG_LEARNED: the network's node grid is not a level of this matrix's grid hierarchy "
                      "(every other node per level)");
-            RG_CHECK(m->mg_levels <= 0, RG_VALUE_ERROR,
-                     "FCG_LEARNED: the grid cannot be halved that many times");
+            RG_CHECK(m->mg_levels <= 0, RG_VALUE_ERROR, "FCG_LEARNED: the grid cannot be halved that many times");
             return levels;
         }
         for (int d = 0; d < 3; ++d) ne[d] /= 2;
@@ -598,7 +618,7 @@ void mg_free(rg_matrix* m)
             if (L.A) rg_matrix_destroy(L.A);
         }
     }
-    for (void* p : {(void*)s->coefs, (void*)s->part, (void*)s->gpart, (void*)s->alpha, (void*)s->logk, (void*)s->y, (void*)s->P,
+    for (void* p : {(void*)s->coefs, (void*)s->part, (void*)s->gpart, (void*)s->alpha, (void*)s->logk, (void*)s->y, (void*)s->y2, (void*)s->zero, (void*)s->P,
                     (void*)s->Q, (void*)s->fcg})
         if (p) hipFree(p);
     delete s;
@@ -606,16 +626,18 @@ void mg_free(rg_matrix* m)
 }
 
 // Builds (on first use or when the plan changes) and refreshes the
-// hierarchy of the balanced matrix m: Galerkin operators, their FP32 copies,
-// Jacobi inverses, Chebyshev tables and the network's coefficient input.
-void mg_setup(rg_matrix* m)
+// hierarchy of the matrix m -- balanced (the solver's) or not: Galerkin
+// operators, their FP32 copies, Jacobi inverses, Chebyshev tables and the
+// network's coefficient input.
+void mg_setup(rg_matrix* m, bool balanced)
 {
     const Geom& g = m->g;
     RG_CHECK(g.dim == 3 && g.bs == 1, RG_VALUE_ERROR, "FCG_LEARNED: scalar Brick matrices only");
-    RG_CHECK(m->is_balanced && m->val32_valid, RG_STATE_ERROR, "FCG_LEARNED: matrix not prepared");
-    bool with_net = false;
-    const int nlev = plan_levels(m, &with_net);
-    if (m->mg && (m->mg->nlev != nlev || m->mg->with_net != with_net)) mg_free(m);
+    RG_CHECK(m->is_balanced == balanced && m->val32_valid, RG_STATE_ERROR, "FCG_LEARNED: matrix not prepared");
+    int net_level = -1;
+    const int nlev = plan_levels(m, &net_level);
+    const bool with_net = net_level >= 0;
+    if (m->mg && (m->mg->nlev != nlev || m->mg->net_level != net_level)) mg_free(m);
     hipStream_t st = m->ctx->stream;
     if (!m->mg) {
         if (m->work) m->work->graph_method = -1;   // captured iterations hold the old buffers
@@ -623,6 +645,7 @@ void mg_setup(rg_matrix* m)
         m->mg = s;
         s->nlev = nlev;
         s->with_net = with_net;
+        s->net_level = net_level;
         s->coefs = dalloc((int64_t)kMaxLevels * (2 * kChebSteps + 1));
         s->part = dalloc(2 * kMaxGrid);
         s->gpart = dalloc(grid1(g.N));
@@ -657,15 +680,28 @@ void mg_setup(rg_matrix* m)
             L.r = dalloc(L.g.N);
             L.d = dalloc(L.g.N);
         }
-        const MgGeom& gc = s->lev[nlev - 1].g;
+        const MgGeom& gc = s->lev[std::max(net_level, 0)].g;
         if (with_net) {
             RG_HIP(hipMalloc(&s->logk, sizeof(float) * gc.N));
             s->y = dalloc(gc.N);
+            s->y2 = dalloc(gc.N);
+            s->zero = dalloc(gc.N);
+            RG_HIP(hipMemsetAsync(s->zero, 0, sizeof(double) * gc.N, st));
         }
     }
     rg_mg_state* s = m->mg;
+    if (s->balanced != balanced && m->work) m->work->graph_method = -1;
+    s->balanced = balanced;
     s->lev[0].A = nullptr;
     s->lev[0].dinv = m->dinv;
+    if (!balanced) {
+        // 1 / a_ii of the unbalanced matrix in the Jacobi slot, which a solve
+        // then rebuilds for its balanced matrix
+        const double* main_slot = m->val + (int64_t)(m->symmetric ? 0 : C3) * g.ld;
+        hipLaunchKernelGGL(k_inv_main_slot, dim3(grid1(g.N)), dim3(kBlock), 0, st, main_slot, g.N, m->dinv);
+        RG_HIP(hipGetLastError());
+        m->have_prec = false;
+    }
     // level 0: Gershgorin bound of the balanced operator (the spectrum of
     // D^-1 A does not change under the balancing)
     {
@@ -681,7 +717,7 @@ void mg_setup(rg_matrix* m)
         const rg_matrix* F = (l == 1) ? m : s->lev[l - 1].A;
         const MgGeom f = s->lev[l - 1].g, c = L.g;
         const unsigned G = grid1(c.N);
-        const double* bal = (l == 1) ? m->balance : nullptr;
+        const double* bal = (l == 1 && balanced) ? m->balance : nullptr;
         auto launch = [&](auto SF, auto SC, auto B) {
             hipLaunchKernelGGL((k_galerkin<SF.value, SC.value, B.value>), dim3(G), dim3(kBlock), 0, st, F->val, f, bal,
                                L.A->val, c, L.dinv, s->gpart);
@@ -703,7 +739,7 @@ void mg_setup(rg_matrix* m)
         L.A->val32_valid = true;
     }
     if (s->with_net) {
-        const MgLevel& L = s->lev[s->nlev - 1];
+        const MgLevel& L = s->lev[s->net_level];
         const rg_matrix* A = L.A ? L.A : m;
         // main entry of a unit coefficient: 8 elements x sum_d vol / (9 h_d^2)
         double aref = 0.0;
@@ -758,8 +794,54 @@ double* smooth(rg_mg_state* s, const rg_matrix* Am, int l, const double* b, doub
 
 }  // namespace
 
-// Launches one V-cycle: z = M(r) on the balanced level-0 vectors (r, z
-// distinct, z written in full).
+namespace {
+
+// The network's correction of level l's iterate x for A_l x = b: the residual
+// r' = b - A_l x (FP32 values), y = Net(r'), low-pass filtered by the level's
+// Chebyshev steps on A_l y = 0 from y (they damp the rough components of y,
+// whose energy would otherwise dominate the step; the error the smoothing
+// left is smooth and passes), and x += alpha y with the energy-optimal
+// alpha = (r'.y)/(y.A_l y) (FP64 values), which never raises the level's
+// error energy.
+void net_correct(rg_mg_state* s, rg_matrix* m, int l, const double* b, double* x, hipStream_t st,
+                 const KrylovState* guard)
+{
+    MgLevel& L = s->lev[l];
+    const rg_matrix* A = L.A;
+    const Launch Ls = chunked_launch(A->g.N, spmv_rows_per_thread<float>());
+    by_sym_vt(A->symmetric, [&](auto S) {
+        hipLaunchKernelGGL((k_residual<S.value, float>), dim3(Ls.grid), dim3(kBlock), 0, st, A->val32, A->g.ld,
+                           A->g.N, A->g.s1, A->g.s2, Ls.chunk, b, x, L.r, guard);
+    });
+    lpc_forward_balanced(m->lpc, L.r, L.dinv, nullptr, s->logk, s->y, st);
+    double* yf = s->y;
+    for (int k = 0; k < std::min(m->mg_coarse_steps, kChebSteps); ++k) {
+        double* dst = (yf == s->y) ? s->y2 : s->y;
+        by_sym_vt(A->symmetric, [&](auto S) {
+            hipLaunchKernelGGL((k_cheb_step<S.value, float>), dim3(Ls.grid), dim3(kBlock), 0, st, A->val32, A->g.ld,
+                               A->g.N, A->g.s1, A->g.s2, Ls.chunk, L.dinv, s->zero, yf, dst, L.d, L.coef + 2 * k,
+                               guard);
+        });
+        yf = dst;
+    }
+    const Launch Le = chunked_launch(A->g.N, spmv_rows_per_thread<double>(), 1024);
+    by_sym_vt(A->symmetric, [&](auto S) {
+        hipLaunchKernelGGL((k_coarse_energy<S.value>), dim3(Le.grid), dim3(kBlock), 0, st, A->val, A->g.ld, A->g.N,
+                           A->g.s1, A->g.s2, Le.chunk, yf, L.r, s->part, guard);
+    });
+    hipLaunchKernelGGL(k_coarse_alpha, dim3(1), dim3(kBlock), 0, st, s->part, Le.grid, s->alpha, guard);
+    hipLaunchKernelGGL(k_axpy_dev, dim3(grid_for(A->g.N)), dim3(kBlock), 0, st, A->g.N, s->alpha, yf, x, guard);
+    RG_HIP(hipGetLastError());
+}
+
+}  // namespace
+
+// Launches one V-cycle: z = M(r) on the level-0 vectors (balanced unless the
+// hierarchy was set up for an unbalanced matrix; r, z distinct, z written in
+// full).  Down: `degree` Chebyshev steps from zero, residual, restriction;
+// the coarsest level: coarse_steps Chebyshev steps from zero; up:
+// prolongation, `degree` Chebyshev steps, and at the network's level its
+// correction.
 void mg_vcycle(rg_matrix* m, const double* r, double* z, hipStream_t st, const KrylovState* guard)
 {
     rg_mg_state* s = m->mg;
@@ -780,7 +862,7 @@ void mg_vcycle(rg_matrix* m, const double* r, double* z, hipStream_t st, const K
                                A->g.N, A->g.s1, A->g.s2, Ls.chunk, bl[l], xl[l], L.r, guard);
         });
         MgLevel& C = s->lev[l + 1];
-        if (l == 0)
+        if (l == 0 && s->balanced)
             hipLaunchKernelGGL(k_restrict<true>, dim3(grid1(C.g.N)), dim3(kBlock), 0, st, L.g, C.g, L.r, m->balance,
                                C.b, guard);
         else
@@ -790,55 +872,41 @@ void mg_vcycle(rg_matrix* m, const double* r, double* z, hipStream_t st, const K
         RG_HIP(hipGetLastError());
     }
     // coarsest level
-    MgLevel& Lc = s->lev[nl - 1];
-    const double* coarse_scale = nullptr;
     if (nl == 1) {
-        xl[0] = smooth(s, m, 0, r, nullptr, true, std::max(deg, m->mg_coarse_steps), z, st, guard);
-    } else if (s->with_net) {
-        const rg_matrix* A = Lc.A;
-        lpc_forward_balanced(m->lpc, bl[nl - 1], Lc.dinv, nullptr, s->logk, s->y, st);
-        const Launch Le = chunked_launch(A->g.N, spmv_rows_per_thread<double>(), 1024);
-        by_sym_vt(A->symmetric, [&](auto S) {
-            hipLaunchKernelGGL((k_coarse_energy<S.value>), dim3(Le.grid), dim3(kBlock), 0, st, A->val, A->g.ld,
-                               A->g.N, A->g.s1, A->g.s2, Le.chunk, s->y, bl[nl - 1], s->part, guard);
-        });
-        hipLaunchKernelGGL(k_coarse_alpha, dim3(1), dim3(kBlock), 0, st, s->part, Le.grid, s->alpha, guard);
-        xl[nl - 1] = s->y;
-        coarse_scale = s->alpha;
-    } else {
-        xl[nl - 1] = smooth(s, m, nl - 1, bl[nl - 1], nullptr, true, std::min(m->mg_coarse_steps, kChebSteps),
-                            nullptr, st, guard);
+        smooth(s, m, 0, r, nullptr, true, std::max(deg, m->mg_coarse_steps), z, st, guard);
+        return;
     }
-    RG_HIP(hipGetLastError());
+    xl[nl - 1] = smooth(s, m, nl - 1, bl[nl - 1], nullptr, true, std::min(m->mg_coarse_steps, kChebSteps), nullptr,
+                        st, guard);
+    if (s->net_level == nl - 1) net_correct(s, m, nl - 1, bl[nl - 1], xl[nl - 1], st, guard);
     // up
     for (int l = nl - 2; l >= 0; --l) {
         MgLevel& L = s->lev[l];
         const MgLevel& C = s->lev[l + 1];
-        if (l == 0)
+        if (l == 0 && s->balanced)
             hipLaunchKernelGGL(k_prolong<true>, dim3(grid1(L.g.N)), dim3(kBlock), 0, st, L.g, C.g, xl[l + 1],
-                               m->balance, coarse_scale, xl[l], guard);
+                               m->balance, xl[l], guard);
         else
             hipLaunchKernelGGL(k_prolong<false>, dim3(grid1(L.g.N)), dim3(kBlock), 0, st, L.g, C.g, xl[l + 1],
-                               nullptr, coarse_scale, xl[l], guard);
-        coarse_scale = nullptr;
+                               nullptr, xl[l], guard);
         RG_HIP(hipGetLastError());
         // post-smoothing from the corrected iterate: the Chebyshev recursion
         // restarts (its first step has no previous update)
-        MgLevel& Lm = s->lev[l];
-        const rg_matrix* A = Lm.A ? Lm.A : m;
+        const rg_matrix* A = L.A ? L.A : m;
         const Geom& g = A->g;
         const Launch Ls = chunked_launch(g.N, spmv_rows_per_thread<float>());
         double* cur = xl[l];
         for (int k = 0; k < deg; ++k) {
-            double* dst = (l == 0 && k == deg - 1) ? z : (cur == Lm.xa ? Lm.xb : Lm.xa);
+            double* dst = (l == 0 && k == deg - 1) ? z : (cur == L.xa ? L.xb : L.xa);
             by_sym_vt(A->symmetric, [&](auto S) {
                 hipLaunchKernelGGL((k_cheb_step<S.value, float>), dim3(Ls.grid), dim3(kBlock), 0, st, A->val32, g.ld,
-                                   g.N, g.s1, g.s2, Ls.chunk, Lm.dinv, bl[l], cur, dst, Lm.d, Lm.coef + 2 * k, guard);
+                                   g.N, g.s1, g.s2, Ls.chunk, L.dinv, bl[l], cur, dst, L.d, L.coef + 2 * k, guard);
             });
             cur = dst;
         }
         xl[l] = cur;
         RG_HIP(hipGetLastError());
+        if (l == s->net_level) net_correct(s, m, l, bl[l], xl[l], st, guard);
     }
 }
 
@@ -891,6 +959,74 @@ rg_matrix* coarse_operator(rg_matrix* m, int levels)
         F = C;
     }
     return F;
+}
+
+// x = one V-cycle for A x = b on the unbalanced matrix m (its hierarchy set
+// up in physical variables).
+void vcycle_unbalanced(rg_matrix* m, const double* b, double* x)
+{
+    RG_CHECK(m->g.dim == 3 && m->g.bs == 1, RG_VALUE_ERROR, "vcycle: scalar Brick matrices only");
+    RG_CHECK(!m->is_balanced, RG_STATE_ERROR, "vcycle: balanced matrix is not supported.");
+    if (!m->val32_valid) {
+        if (!m->val32) RG_HIP(hipMalloc(&m->val32, (size_t)m->g.nstore * m->g.ld * sizeof(float)));
+        launch_to_fp32(m);
+        m->val32_valid = true;
+    }
+    mg_setup(m, false);
+    mg_vcycle(m, b, x, m->ctx->stream, nullptr);
+    RG_HIP(hipStreamSynchronize(m->ctx->stream));
+}
+
+// x = `steps` Chebyshev steps from zero on the unbalanced matrix m (FP32
+// values, FP64 vectors, the bound and interval of the multilevel levels).
+void smooth_unbalanced(rg_matrix* m, const double* b, double* x, int steps)
+{
+    RG_CHECK(m->g.dim == 3 && m->g.bs == 1, RG_VALUE_ERROR, "smooth: scalar Brick matrices only");
+    RG_CHECK(!m->is_balanced, RG_STATE_ERROR, "smooth: balanced matrix is not supported.");
+    RG_CHECK(steps >= 1 && steps <= kChebSteps, RG_VALUE_ERROR, "smooth: steps must be 1..16");
+    hipStream_t st = m->ctx->stream;
+    const Geom& g = m->g;
+    if (!m->val32_valid) {
+        if (!m->val32) RG_HIP(hipMalloc(&m->val32, (size_t)g.nstore * g.ld * sizeof(float)));
+        launch_to_fp32(m);
+        m->val32_valid = true;
+    }
+    const MgGeom a = geom_of(m);
+    const unsigned G = grid1(a.N);
+    double* buf = nullptr;
+    const int64_t nc = 2 * kChebSteps + 1;
+    RG_HIP(hipMalloc(&buf, sizeof(double) * (G + nc + 4 * a.N)));
+    double* gp = buf;
+    double* coef = gp + G;
+    double* dinv = coef + nc;
+    double* d = dinv + a.N;
+    double* xa = d + a.N;
+    double* xb = xa + a.N;
+    try {
+        if (m->symmetric) hipLaunchKernelGGL(k_gershgorin<true>, dim3(G), dim3(kBlock), 0, st, m->val, a, gp);
+        else hipLaunchKernelGGL(k_gershgorin<false>, dim3(G), dim3(kBlock), 0, st, m->val, a, gp);
+        hipLaunchKernelGGL(k_cheb_table, dim3(1), dim3(kBlock), 0, st, gp, (int)G, kChebRatio, coef);
+        const double* main_slot = m->val + (int64_t)(m->symmetric ? 0 : C3) * g.ld;
+        hipLaunchKernelGGL(k_inv_main_slot, dim3(G), dim3(kBlock), 0, st, main_slot, a.N, dinv);
+        hipLaunchKernelGGL(k_cheb_first, dim3(grid_for(g.N)), dim3(kBlock), 0, st, g.N, b, dinv, coef,
+                           steps == 1 ? x : xa, d, nullptr);
+        const double* cur = xa;
+        const Launch Ls = chunked_launch(g.N, spmv_rows_per_thread<float>());
+        for (int k = 1; k < steps; ++k) {
+            double* dst = (k == steps - 1) ? x : (cur == xa ? xb : xa);
+            by_sym_vt(m->symmetric, [&](auto S) {
+                hipLaunchKernelGGL((k_cheb_step<S.value, float>), dim3(Ls.grid), dim3(kBlock), 0, st, m->val32, g.ld,
+                                   g.N, g.s1, g.s2, Ls.chunk, dinv, b, cur, dst, d, coef + 2 * k, nullptr);
+            });
+            cur = dst;
+        }
+        RG_HIP(hipGetLastError());
+        RG_HIP(hipStreamSynchronize(st));
+    } catch (...) {
+        hipFree(buf);
+        throw;
+    }
+    hipFree(buf);
 }
 
 // One FCG(kDirs) iteration: z = M(r), the direction, q = A p, the updates.

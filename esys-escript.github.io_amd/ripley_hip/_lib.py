@@ -45,7 +45,8 @@ EXPORTS = [
     "rg_batch_create", "rg_batch_destroy", "rg_matrix_assemble_boundary", "rg_matrix_add_dirac",
     "rg_batch_offsets", "rg_batch_vector", "rg_batch_set_tile", "rg_batch_autotune", "rg_batch_solve",
     "rg_matrix_solve_preconditioner", "rg_matrix_coloring", "rg_read_binary_grid", "rg_write_binary_grid",
-    "rg_matrix_set_multilevel", "rg_matrix_coarse_operator",
+    "rg_matrix_set_multilevel", "rg_matrix_coarse_operator", "rg_lpc_set_loss_operator",
+    "rg_matrix_smooth", "rg_matrix_vcycle",
 ]
 
 
@@ -136,6 +137,9 @@ def load():
         "rg_matrix_set_learned_preconditioner": (I, [P, P, P]),
         "rg_matrix_set_multilevel": (I, [P, I, I, I]),
         "rg_matrix_coarse_operator": (I, [P, I, PP]),
+        "rg_lpc_set_loss_operator": (I, [P, P]),
+        "rg_matrix_smooth": (I, [P, P, P, I]),
+        "rg_matrix_vcycle": (I, [P, P, P]),
         "rg_solve_multi": (I, [P, P, P, I, ctypes.POINTER(RgOptions), P]),
         "rg_matrix_vector_multi": (I, [P, P, P, I]),
         "rg_multi_autotune": (I, [P, I, I, I, P, P]),

@@ -435,6 +435,24 @@ class SystemMatrix:
         self._bind()
         check(load().rg_matrix_set_multilevel(self.handle, int(levels), int(degree), int(coarse_steps)))
 
+    def smooth(self, b, steps):
+        """`steps` Chebyshev steps from zero for A x = b (the multilevel
+        preconditioner's smoother) on this unbalanced matrix."""
+        self._bind()
+        bd = _dev_vec(b, self.size, self.device)
+        x = torch.empty(self.size, dtype=torch.float64, device=self.device)
+        check(load().rg_matrix_smooth(self.handle, _ptr(bd), _ptr(x), int(steps)))
+        return x
+
+    def vcycle(self, b):
+        """One V-cycle of the multilevel preconditioner for A x = b on this
+        unbalanced matrix (its multilevel settings and network)."""
+        self._bind()
+        bd = _dev_vec(b, self.size, self.device)
+        x = torch.empty(self.size, dtype=torch.float64, device=self.device)
+        check(load().rg_matrix_vcycle(self.handle, _ptr(bd), _ptr(x)))
+        return x
+
     def coarseOperator(self, levels=1):
         """The Galerkin operator P^T A P `levels` halvings down (P trilinear
         interpolation), as a SystemMatrix of the coarse Brick."""

@@ -9,15 +9,22 @@ MFMA kernels, FP32 accumulate) is a 3-level 3-D U-Net on a node grid:
     level 3 (1/4, avg pool):  32 -> 64 -> 64
     decoder:  up(64) ++ s2 -> 32,  up(32) ++ s1 -> 16,  16 -> 1 (linear)
 
-It solves the coarsest level of the geometric-multigrid V-cycle that
-preconditions the flexible CG: the level's Galerkin operator A_c = P^T A P
-(trilinear P, the Brick halved per level), input channels u / rms(u) with
-u = r / diag(A_c) and log10(diag(A_c) / diag of a unit coefficient), output
-a correction proportional to the level's error e = A_c^-1 r (the solver
-scales it to minimise the energy-norm error).  It is trained with the
-cosine loss 1 - <y, e>/(|y||e|) on residual/error pairs that the plain
-Jacobi-PCG produces on A_c: r_k = b - A_c x_k and e_k = x* - x_k for
-iterates x_k of the plain solver, with b = A_c x* for smooth random x*.
+It corrects one level of the geometric-multigrid V-cycle that preconditions
+the flexible CG, after the V-cycle's work there (its coarser levels and
+smoothing): on the level's Galerkin operator A_c = P^T A P (trilinear P, the
+Brick halved per level), input channels u / rms(u) with u = r / diag(A_c)
+for the residual r left there and log10(diag(A_c) / diag of a unit
+coefficient), output a correction proportional to the level's error
+e = A_c^-1 r (the solver low-pass filters it and scales it to minimise the
+energy-norm error).  It is trained with the
+energy-norm loss 1 - (r.y)^2 / ((r.e)(y.A_c y)) -- the fraction of the
+level's error energy the energy-optimal step along y leaves, which is what
+the coarse correction removes; the cosine loss 1 - <y, e>/(|y||e|) ignores
+rough output errors that cost much energy -- on residual/error pairs that
+the plain Jacobi-PCG produces on A_c: r_k = b - A_c x_k and e_k = x* - x_k
+for iterates x_k of the plain solver, with b = A_c x* for smooth random x*,
+each after one V-cycle from that level (the solver applies the network to
+the residual the V-cycle leaves there).
 
 The reference has no learned preconditioner (PASO offers Jacobi,
 Gauss-Seidel, ILU0, RILU and AMG, paso/src/Preconditioner.cpp:46-107); the
@@ -129,20 +136,30 @@ class LearnedPreconditioner:
         check(load().rg_lpc_apply(self.handle, _ptr(r), _ptr(dinv), _ptr(lk), _ptr(z)))
         return z
 
-    def gradients(self, r, dinv, logk, e):
-        """(loss, flat FP32 gradient) of the cosine loss of Net(r) against e."""
+    def set_loss_operator(self, op):
+        """The energy-norm loss 1 - (r.z)^2 / ((r.e)(z.A z)) with the level's
+        operator op (an unbalanced SystemMatrix on this grid), or, with None,
+        the cosine loss."""
+        self.ctx.bind_stream()
+        check(load().rg_lpc_set_loss_operator(self.handle, op.handle if op is not None else None))
+        self._loss_op = op
+
+    def gradients(self, r, dinv, logk, e, op=None):
+        """(loss, flat FP32 gradient) of Net(r) against e: the energy-norm
+        loss with op, else the cosine loss."""
         r, dinv, lk, e = self._vec(r), self._vec(dinv), self._vec(logk, torch.float32), self._vec(e)
         g = np.zeros(num_params(), dtype=np.float32)
         loss = ctypes.c_double()
-        self.ctx.bind_stream()
+        self.set_loss_operator(op)
         check(load().rg_lpc_gradients(self.handle, _ptr(r), _ptr(dinv), _ptr(lk), _ptr(e), ctypes.byref(loss),
                                       g.ctypes.data_as(ctypes.c_void_p)))
         return loss.value, g
 
-    def train_step(self, r, dinv, logk, e, lr=1e-3):
+    def train_step(self, r, dinv, logk, e, lr=1e-3, op=None):
+        """One Adam step on the loss of gradients()."""
         r, dinv, lk, e = self._vec(r), self._vec(dinv), self._vec(logk, torch.float32), self._vec(e)
         loss = ctypes.c_double()
-        self.ctx.bind_stream()
+        self.set_loss_operator(op)
         check(load().rg_lpc_train_step(self.handle, _ptr(r), _ptr(dinv), _ptr(lk), _ptr(e), ctypes.c_float(lr),
                                        ctypes.byref(loss)))
         return loss.value
@@ -251,17 +268,19 @@ def smooth_target(nodes, seed, device):
 
 
 def generate_training_set(ne, num_fields, kappa_fn, iterations=DEFAULT_ITERATES, first_seed=1, log=None,
-                          per_field=None, levels=1):
-    """[(r, e, dinv, logk), ...] on the device for the network that solves
-    the level `levels` halvings below Brick(ne): for each seed s in
+                          per_field=None, levels=1, after_vcycle=True):
+    """[(r, e, dinv, logk, A_c), ...] on the device for the network that
+    solves the level `levels` halvings below Brick(ne): for each seed s in
     first_seed .. first_seed+num_fields-1, kappa = kappa_fn(ne, s), the
     Galerkin operator A_c of the constrained fine operator, and the plain
     Jacobi-PCG's residual/error pairs on A_c x = A_c x* (x* smooth, random)
     at the given iteration counts (with per_field = m: m of them per field,
-    chosen by the seed)."""
+    chosen by the seed), each taken after one V-cycle V from that level (the
+    default hierarchy below it): r' = r - A_c V r, e' = e - V r, the residual
+    and error the network sees and corrects in the solver's V-cycle."""
     data = []
     for s in range(first_seed, first_seed + num_fields):
-        prob = BrickPoisson(ne, kappa_fn(ne, s))
+        prob = BrickPoisson(ne, kappa_fn(ne, s), symmetric=True)
         fine = prob.operator()
         make = lambda: fine.coarseOperator(levels)
         C = make()
@@ -272,22 +291,29 @@ def generate_training_set(ne, num_fields, kappa_fn, iterations=DEFAULT_ITERATES,
         if per_field:
             its = [iterations[(7 * s + j) % len(iterations)] for j in range(per_field)]
         for r, e in plain_pcg_pairs(make, xstar, its):
-            data.append((r, e, dinv, logk))
+            # what one V-cycle from this level leaves (its levels below, the
+            # default smoothing): the network corrects that
+            if after_vcycle:
+                xs = C.vcycle(r)
+                C.MatrixVector(-1.0, xs, 1.0, r)
+                e = e - xs
+            data.append((r, e, dinv, logk, C))
         n = s - first_seed + 1
         if log and (n % max(1, num_fields // 20) == 0 or n == num_fields):
             log("training set: field %d/%d (%d pairs)" % (n, num_fields, len(data)))
     return data
 
 
-def train(net, data, steps, lr=1e-3, lr_final=1e-4, seed=0, log=None, log_every=100):
-    """Adam on the cosine loss over random pairs of `data`; the learning rate
+def train(net, data, steps, lr=1e-3, lr_final=1e-4, seed=0, log=None, log_every=100, energy=True):
+    """Adam over random pairs of `data` on the energy-norm loss with each
+    pair's level operator (energy=False: the cosine loss); the learning rate
     decays geometrically from lr to lr_final.  Returns the loss history."""
     rng = np.random.default_rng(seed)
     hist = []
     for k in range(steps):
-        r, e, dinv, logk = data[int(rng.integers(len(data)))]
+        r, e, dinv, logk, op = data[int(rng.integers(len(data)))]
         a = lr * (lr_final / lr) ** (k / max(1, steps - 1))
-        hist.append(net.train_step(r, dinv, logk, e, a))
+        hist.append(net.train_step(r, dinv, logk, e, a, op=op if energy else None))
         if log and (k % log_every == 0 or k == steps - 1):
             log("step %d loss %.4f (mean of last %d: %.4f)" % (k, hist[-1], min(len(hist), log_every),
                                                                float(np.mean(hist[-log_every:]))))
@@ -307,6 +333,7 @@ def _main():
                     help="residual/error pairs per field (0: one per iteration count 1, 2, 4, ..., 256)")
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--cosine", action="store_true", help="train on the cosine loss instead of the energy loss")
     ap.add_argument("--family", default="log10grf", choices=["log10grf", "lognormal"])
     ap.add_argument("--sigma", type=float, default=1.0)
     ap.add_argument("--corr", type=float, default=8.0)
@@ -348,7 +375,7 @@ def _main():
     net = LearnedPreconditioner(tuple(n // 2 ** a.levels + 1 for n in ne), training=True)
     if a.init:
         net.load(a.init)
-    hist = train(net, data, a.steps, lr=a.lr, log=log)
+    hist = train(net, data, a.steps, lr=a.lr, log=log, energy=not a.cosine)
     torch.cuda.synchronize()
     t2 = time.perf_counter()
     net.save(a.out)

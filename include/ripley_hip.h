@@ -390,13 +390,22 @@ int rg_lpc_get_params(rg_lpc* net, float* params);         /* host */
 /* z = Net(r) (device vectors on the node grid; dinv = 1/diag(A), logk
  * float per node, either may be NULL). */
 int rg_lpc_apply(rg_lpc* net, const double* r, const double* dinv, const float* logk, double* z);
-/* Cosine loss 1 - (y.e)/(|y||e|) of Net(r) against the error e (device) and
- * its parameter gradient (host, may be NULL). */
+/* Loss of Net(r) against the error e (device) -- the cosine loss
+ * 1 - (y.e)/(|y||e|), or the energy-norm loss of rg_lpc_set_loss_operator --
+ * and its parameter gradient (host, may be NULL). */
 int rg_lpc_gradients(rg_lpc* net, const double* r, const double* dinv, const float* logk, const double* e,
                      double* loss, float* grads);
 /* One Adam step (lr) on that loss. */
 int rg_lpc_train_step(rg_lpc* net, const double* r, const double* dinv, const float* logk, const double* e,
                       float lr, double* loss);
+/* Loss of rg_lpc_gradients / rg_lpc_train_step: with an operator A (a
+ * scalar Brick matrix on the network's node grid, not balanced, borrowed)
+ * the energy-norm loss 1 - (r.z)^2 / ((r.e)(z.A z)) of the output z for a
+ * pair with r = A e -- the fraction of the error energy that the
+ * energy-optimal step along z leaves, which is what the coarsest level of
+ * the multilevel preconditioner removes; A = NULL restores the cosine loss.
+ * Training networks only. */
+int rg_lpc_set_loss_operator(rg_lpc* net, rg_matrix* A);
 /* Attach a network as the coarsest-level solver of RG_METHOD_FCG_LEARNED's
  * multilevel preconditioner: its node grid must be a coarse level of the
  * matrix's Brick (ne / 2^k elements per axis, k >= 1), which sets the
@@ -406,12 +415,12 @@ int rg_lpc_train_step(rg_lpc* net, const double* r, const double* dinv, const fl
  * detaches. */
 int rg_matrix_set_learned_preconditioner(rg_matrix* m, rg_lpc* net, const float* logk);
 
-/* Hierarchy of RG_METHOD_FCG_LEARNED without a network: `levels` grid levels
- * (0: until a level has at most 9^3 nodes or cannot be halved), `degree`
- * Chebyshev steps before and after each coarse correction (1..16, default
- * 2) and `coarse_steps` Chebyshev steps from zero on the coarsest level
- * (1..16, default 8).  degree applies with a network too.  No counterpart in
- * the reference. */
+/* Hierarchy of RG_METHOD_FCG_LEARNED: `levels` grid levels without a
+ * network (0: until a level has at most 9^3 nodes or cannot be halved; an
+ * attached network's grid sets the levels), `degree` Chebyshev steps before
+ * and after each coarse correction (1..16, default 2) and `coarse_steps`
+ * Chebyshev steps from zero on the coarsest level (1..16, default 8), which
+ * the network's correction follows.  No counterpart in the reference. */
 int rg_matrix_set_multilevel(rg_matrix* m, int levels, int degree, int coarse_steps);
 
 /* The Galerkin operator P^T A P of the matrix `levels` halvings down (P the
@@ -420,6 +429,22 @@ int rg_matrix_set_multilevel(rg_matrix* m, int levels, int degree, int coarse_st
  * (rg_matrix_destroy).  The matrix must not be balanced.  Scalar Brick
  * matrices.  No counterpart in the reference. */
 int rg_matrix_coarse_operator(rg_matrix* m, int levels, rg_matrix** out);
+
+/* x = `steps` (1..16) Chebyshev steps from zero for A x = b on the matrix
+ * (not balanced; scalar Brick), the smoother of the multilevel
+ * preconditioner's levels: Jacobi-scaled, over [g / 10, g] with g the
+ * Gershgorin bound max_i sum_j |a_ij| / a_ii, FP32 values.  x, b: device,
+ * distinct.  No counterpart in the reference (training data of the
+ * coarse-level network). */
+int rg_matrix_smooth(rg_matrix* m, const double* b, double* x, int steps);
+
+/* x = one V-cycle of RG_METHOD_FCG_LEARNED's multilevel preconditioner for
+ * A x = b on the matrix itself (not balanced; scalar Brick; the levels,
+ * smoothing and network of rg_matrix_set_multilevel and
+ * rg_matrix_set_learned_preconditioner).  x, b: device, distinct.  No
+ * counterpart in the reference (training data of the network: the residual
+ * and error a V-cycle leaves). */
+int rg_matrix_vcycle(rg_matrix* m, const double* b, double* x);
 
 #ifdef __cplusplus
 }

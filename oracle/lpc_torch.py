@@ -20,7 +20,8 @@ module restates the network the library defines:
         c6 96->32 relu; nearest up x2, concat [up, s1];
         c7 48->16 relu; c8 16->16 linear; output channel 0
   output z = y0 / s on the nodes.
-  loss: 1 - <y0, e> / (|y0| |e|) over the nodes.
+  loss: 1 - <y0, e> / (|y0| |e|) over the nodes, or the energy-norm loss
+        1 - (r.z)^2 / ((r.e)(z.A z)) of z with the level's operator A.
 
 With bf16=True every weight and every stored activation (and, in the
 library's backward pass, every stored gradient) is rounded to bfloat16 as
@@ -125,6 +126,26 @@ def loss_and_gradients(flat_params, nodes, r, dinv, logk, e, bf16=True, dtype=to
     y = forward(params, x, bf16)[0, 0, :n2, :n1, :n0].reshape(-1).to(torch.float64)
     et = torch.as_tensor(np.asarray(e, dtype=np.float64))
     loss = 1.0 - torch.dot(y, et) / (torch.linalg.norm(y) * torch.linalg.norm(et))
+    loss.backward()
+    return loss.item(), flatten([(w.grad, b.grad) for (w, b) in params]).double().numpy()
+
+
+def loss_and_gradients_energy(flat_params, nodes, r, dinv, logk, e, A, bf16=True, dtype=torch.float32):
+    """(loss, flat gradient) of the energy-norm loss 1 - (r.z)^2/((r.e)(z.A z))
+    of the output z = y / s by autograd (A: scipy sparse operator of the
+    level)."""
+    n0, n1, n2 = nodes
+    params = [(w.clone().requires_grad_(True), b.clone().requires_grad_(True)) for (w, b) in split(flat_params, dtype)]
+    x, s = make_input(nodes, r, dinv, logk, dtype, bf16)
+    y = forward(params, x, bf16)[0, 0, :n2, :n1, :n0].reshape(-1).to(torch.float64)
+    z = y / s
+    At = torch.sparse_csr_tensor(torch.as_tensor(A.indptr, dtype=torch.int64),
+                                 torch.as_tensor(A.indices, dtype=torch.int64),
+                                 torch.as_tensor(A.data, dtype=torch.float64), size=A.shape)
+    rt = torch.as_tensor(np.asarray(r, dtype=np.float64))
+    et = torch.as_tensor(np.asarray(e, dtype=np.float64))
+    az = torch.mv(At, z)
+    loss = 1.0 - torch.dot(rt, z) ** 2 / (torch.dot(rt, et) * torch.dot(z, az))
     loss.backward()
     return loss.item(), flatten([(w.grad, b.grad) for (w, b) in params]).double().numpy()
 

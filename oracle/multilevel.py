@@ -14,9 +14,11 @@ standard geometric multigrid on ripley's vertex-centred grid:
 * smoother: Chebyshev iteration on D^-1 A (D = diag A) over
   [lmax / ratio, lmax], lmax the Gershgorin bound max_i sum_j |a_ij| / a_ii;
 * V-cycle: pre-smoothing from 0, residual, restriction P^T, recursion,
-  prolongation, post-smoothing; the coarsest level solved exactly, by
-  Chebyshev steps, or by a given function (the network) whose output is
-  scaled to minimise the level's energy-norm error along it.
+  prolongation, post-smoothing; the coarsest level solved exactly or by
+  Chebyshev steps; at one level a given function (the network) may correct
+  the iterate: applied to the residual left there, its output low-pass
+  filtered by the level's Chebyshev steps on A y = 0 and scaled to minimise
+  the level's energy-norm error along it.
 
 Pinned in tests/test_oracle_multilevel.py by: P reproducing linear
 functions exactly, the Galerkin product against a dense triple product,
@@ -153,26 +155,44 @@ class Hierarchy:
             self.coarse_inv = np.linalg.inv(self.levels[-1].A.toarray())
         return self.coarse_inv @ b
 
-    def vcycle(self, b, l=0, coarse=None):
+    def vcycle(self, b, l=0, coarse=None, net=None, net_level=None):
         """x ~ A_l^-1 b.  coarse(l, b) solves the coarsest level (default:
-        exactly; "smooth": coarse_steps Chebyshev steps from zero)."""
+        exactly; "smooth": coarse_steps Chebyshev steps from zero).  With
+        net (a function (l, r) -> correction), level net_level ends with
+        net_correct (after its post-smoothing, or after the coarse solve when
+        it is the coarsest level)."""
         if l == len(self.levels) - 1:
             if coarse is None:
-                return self.exact_coarse(l, b)
-            if coarse == "smooth":
-                return self.smooth(l, b, None, self.coarse_steps)
-            return coarse(l, b)
+                x = self.exact_coarse(l, b)
+            elif coarse == "smooth":
+                x = self.smooth(l, b, None, self.coarse_steps)
+            else:
+                x = coarse(l, b)
+            return self.net_correct(l, b, x, net) if (net is not None and l == net_level) else x
         L = self.levels[l]
         x = self.smooth(l, b, None, self.degree)
         r = b - L.S @ x
         if l == 0 and self.bal is not None:
             r = r / self.bal
-        xc = self.vcycle(L.P.T @ r, l + 1, coarse)
+        xc = self.vcycle(L.P.T @ r, l + 1, coarse, net, net_level)
         c = L.P @ xc
         if l == 0 and self.bal is not None:
             c = c / self.bal
-        x = x + c
-        return self.smooth(l, b, x, self.degree)
+        x = self.smooth(l, b, x + c, self.degree)
+        return self.net_correct(l, b, x, net) if (net is not None and l == net_level) else x
+
+    def net_correct(self, l, b, x, net):
+        """The library's network correction of level l's iterate x: the
+        residual r' = b - A x (FP32 values), y = net(l, r') low-pass filtered
+        by coarse_steps Chebyshev steps on A y = 0 from y, then
+        x + alpha y with the energy-optimal alpha (FP64 values)."""
+        L = self.levels[l]
+        r = b - L.S @ x
+        y = net(l, r)
+        y = self.smooth(l, np.zeros_like(y), y, self.coarse_steps)
+        Ay = L.A @ y
+        yAy = y @ Ay
+        return x + y * ((r @ y) / yAy if yAy > 0 else 0.0)
 
     def energy_scaled(self, net):
         """coarse(l, b) for vcycle: y = net(l, b) scaled by (b.y)/(y.A y)."""

@@ -187,9 +187,9 @@ def test_c5_learned_fcg_reaches_fp64_tolerance(c5):
     print("C5 learned multilevel FCG: %d iterations, %d fallbacks, FP64 residual %.3e"
           % (o.getDiagnostics("num_iter"), o.getDiagnostics("fallbacks"), res), flush=True)
     assert o.hasConverged() and o.getDiagnostics("fallbacks") == 0 and res <= 1e-8 * 1.001
-    # the same hierarchy without the network (its coarsest level smoothed)
+    # the same hierarchy without the network
     op, rhs = _c5_system(c5, sym=True)
-    op.setMultilevel(3, 2, 8)
+    op.setMultilevel(0, 2, 8)
     o2 = _opts("FCG_LEARNED")
     o2.setCheckInterval(4)
     x2 = op.solve(rhs, o2)

@@ -103,6 +103,35 @@ def test_gradients_match_oracle(nodes):
         o += ws + cout
 
 
+def test_energy_loss_gradients_match_oracle():
+    # the energy-norm loss with a level operator: a Galerkin operator 9^3 of
+    # a 16^3 Brick, a pair with r = A e
+    print("energy-loss gradients", flush=True)
+    prob = _problem((16, 16, 16), 21)
+    C = prob.operator().coarseOperator(1)
+    nodes = (9, 9, 9)
+    p = _random_params(22)
+    dinv, logk = L.level_inputs(C)
+    e = torch.as_tensor(np.random.default_rng(23).standard_normal(C.size), device="cuda")
+    r = torch.zeros_like(e)
+    C.MatrixVector(1.0, e, 0.0, r)
+    net = L.LearnedPreconditioner(nodes, training=True, params=p)
+    loss, g = net.gradients(r, dinv, logk, e, op=C)
+    ptr, idx, val = C.exportCSR()
+    import scipy.sparse as sp
+    A = sp.csr_matrix((val, idx, ptr), shape=(C.size, C.size))
+    lref, gref = LT.loss_and_gradients_energy(p, nodes, r.cpu().numpy(), dinv.cpu().numpy(), logk.cpu().numpy(),
+                                              e.cpu().numpy(), A, bf16=True)
+    print("energy loss gpu %.5f oracle %.5f, gradient rel. difference %.3e" % (loss, lref, _rel(g, gref)), flush=True)
+    assert abs(loss - lref) < 2e-3 * max(1.0, abs(lref))
+    assert _rel(g, gref) < 5e-2
+    # the cosine loss is back without the operator
+    l2, _ = net.gradients(r, dinv, logk, e)
+    lc, _ = LT.loss_and_gradients(p, nodes, r.cpu().numpy(), dinv.cpu().numpy(), logk.cpu().numpy(),
+                                  e.cpu().numpy(), bf16=True)
+    assert abs(l2 - lc) < 2e-3
+
+
 def test_train_step_is_adam_on_the_gradient():
     nodes = (8, 8, 8)
     p = _random_params(9)
@@ -184,41 +213,47 @@ def test_fcg_needs_a_coarse_level_network():
         op.setLearnedPreconditioner(L.LearnedPreconditioner(prob.nodes))
 
 
-def _cos(a, b):
-    return (torch.dot(a, b) / (torch.linalg.norm(a) * torch.linalg.norm(b))).item()
+def _energy_gain(y, r, e, op):
+    """(r.y)^2 / ((r.e)(y.A y)): the fraction of the error energy e.A e that
+    the energy-optimal step along y removes."""
+    ay = torch.zeros_like(y)
+    op.MatrixVector(1.0, y, 0.0, ay)
+    return (torch.dot(r, y) ** 2 / (torch.dot(r, e) * torch.dot(y, ay))).item()
 
 
 def test_trained_network_gains_over_untrained_and_over_no_network():
-    """Train the coarse-level network on the plain solver's residual/error
-    pairs on Galerkin operators 17^3 of 32^3 Bricks (the library's own
-    forward, backward and Adam kernels), then solve unseen fields with the
-    flexible CG and the same 2-level hierarchy whose coarsest level is
-    (a) the trained network, (b) the untrained network, (c) 8 Chebyshev
-    steps -- the method without the network.  The trained network must need
-    fewer iterations than (b) and (c), and far fewer than plain Jacobi-PCG;
-    the cosine of its correction with the true error is reported beside."""
+    """Train the network on the plain solver's residual/error pairs on
+    Galerkin operators 17^3 of 150 32^3 Bricks, each after one V-cycle from that
+    level (the library's own forward, backward and Adam kernels, energy-norm
+    loss), then solve unseen fields with the flexible CG and the same
+    hierarchy (33^3, 17^3, 9^3) with (a) the trained network correcting the
+    17^3 level, (b) the untrained network, (c) no network -- the method
+    without it.  The trained network must need fewer iterations than (b)
+    and (c), and far fewer than plain Jacobi-PCG; the error energy its step
+    removes on held-out pairs is reported beside that of the untrained
+    network and of the Jacobi direction."""
     ne = (32, 32, 32)
     kfn = lambda n, s: fields.log10_normal_kappa(n, corr_len=4.0, sigma=1.0, seed=s)
-    data = L.generate_training_set(ne, 12, kfn, iterations=(1, 2, 4, 8, 16, 32, 64, 128), first_seed=100)
+    data = L.generate_training_set(ne, 150, kfn, iterations=(1, 2, 4, 8, 16, 32, 64, 128), first_seed=100)
     held = L.generate_training_set(ne, 2, kfn, iterations=(4, 32, 128), first_seed=7)
     nodes = (17, 17, 17)
     net = L.LearnedPreconditioner(nodes, training=True, params=L.init_params(0))
     untrained = L.LearnedPreconditioner(nodes, params=L.init_params(0))
 
-    def mean_cos(pre):
-        return float(np.mean([_cos(pre.apply(r, d, lk), e) for (r, e, d, lk) in held]))
+    def mean_gain(f):
+        return float(np.mean([_energy_gain(f(r, d, lk), r, e, op) for (r, e, d, lk, op) in held]))
 
-    c0 = mean_cos(untrained)
-    cj = float(np.mean([_cos(d * r, e) for (r, e, d, lk) in held]))
-    hist = L.train(net, data, steps=1500, lr=2e-3, lr_final=2e-4)
-    c1 = mean_cos(net)
+    g0 = mean_gain(untrained.apply)
+    gj = mean_gain(lambda r, d, lk: d * r)
+    hist = L.train(net, data, steps=4000, lr=2e-3, lr_final=2e-4)
+    g1 = mean_gain(net.apply)
     trained = L.LearnedPreconditioner(nodes, params=net.get_params())
     its = {}
     for seed in (7, 8):
         test = _problem(ne, seed)
         x1, o1 = _fcg_solve(test, trained)
         _, o0 = _fcg_solve(test, untrained)
-        _, os_ = _fcg_solve(test, None, levels=2, coarse_steps=8)
+        _, os_ = _fcg_solve(test, None, levels=0, coarse_steps=8)
         op = test.operator()
         po = R.SolverOptions()
         po.setSolverMethod(R.SolverOptions.PCG_PLAIN)
@@ -227,10 +262,10 @@ def test_trained_network_gains_over_untrained_and_over_no_network():
         its[seed] = [o.getDiagnostics("num_iter") for o in (o1, o0, os_, po)]
         assert o1.hasConverged() and o1.getDiagnostics("fallbacks") == 0
         assert _true_res(test, x1) <= 1e-8 * 1.001
-    print("cosine(net(r), e): untrained %.3f, trained %.3f, Jacobi %.3f; loss %.3f -> %.3f; iterations "
-          "(trained, untrained, Chebyshev coarse, plain Jacobi-PCG) %s" %
-          (c0, c1, cj, np.mean(hist[:100]), np.mean(hist[-100:]), its), flush=True)
-    assert np.mean(hist[-100:]) < 0.5 * np.mean(hist[:100])
-    assert c1 > c0 + 0.2 and c1 > cj
+    print("error energy removed by the step along y: untrained %.3f, trained %.3f, Jacobi %.3f; loss %.3f -> %.3f; "
+          "iterations (trained, untrained, Chebyshev coarse, plain Jacobi-PCG) %s" %
+          (g0, g1, gj, np.mean(hist[:100]), np.mean(hist[-100:]), its), flush=True)
+    assert np.mean(hist[-100:]) < 0.7 * np.mean(hist[:100])
+    assert g1 > g0 + 0.1 and g1 > gj
     for t, u, c, j in its.values():
         assert t < u and t < c and 5 * t < j

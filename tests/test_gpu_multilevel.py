@@ -102,32 +102,62 @@ def test_vcycle_against_oracle(sym, levels, degree, coarse_steps):
     assert err <= 1e-9
 
 
-def test_vcycle_with_network_against_oracle():
+@pytest.mark.parametrize("steps", [1, 3, 8])
+def test_smooth_against_oracle(steps):
+    # the level smoother on an unbalanced matrix (training data of the network)
+    ne = (8, 6, 10)
+    _, op, _ = _system(ne, True, 9)
+    A = _csr(op)
+    H = ML.Hierarchy(A, tuple(n + 1 for n in ne), levels=1, fp32=True)
+    b = np.random.default_rng(steps).standard_normal(op.size)
+    x = op.smooth(to_dev(b), steps).cpu().numpy()
+    xo = H.smooth(0, b, None, steps)
+    assert np.abs(x - xo).max() <= 1e-9 * np.abs(xo).max()
+
+
+@pytest.mark.parametrize("levels", [2, 3])
+def test_vcycle_with_network_against_oracle(levels):
+    # the network corrects level 1 (9^3): the coarsest level (2 levels) or
+    # an intermediate one (3 levels, 5^3 below)
     ne = (16, 16, 16)
     _, op, _ = _system(ne, True, 6)
     exported = op.exportCSR()
-    H = _oracle_hierarchy(exported, ne, 2)
+    H = _oracle_hierarchy(exported, ne, levels)
     p = (np.random.default_rng(3).standard_normal(L.num_params()) * 0.05).astype(np.float32)
     nodes_c = (9, 9, 9)
     net = L.LearnedPreconditioner(nodes_c, params=p)
+    op.setMultilevel(levels, 2, 8)
     op.setLearnedPreconditioner(net)
     o = R.SolverOptions()
     o.setSolverMethod(R.SolverOptions.FCG_LEARNED)
     r = np.random.default_rng(12).standard_normal(op.size)
     z = op.solvePreconditioner(to_dev(r), o).cpu().numpy()
-    Lc = H.levels[-1]
+    Lc = H.levels[1]
     h = 1.0 / 8
     aref = 8.0 * 3 * h ** 3 / (9.0 * h * h)
 
     def net_cpu(l, b):
         logk = np.log10(Lc.A.diagonal() / aref).astype(np.float32)
         return LT.apply(p, nodes_c, b, Lc.dinv, logk, bf16=True)
-    zo = H.vcycle(r, coarse=H.energy_scaled(net_cpu))
+    zo = H.vcycle(r, coarse="smooth", net=net_cpu, net_level=1)
     zs = H.vcycle(r, coarse="smooth")
     err = np.linalg.norm(z - zo) / np.linalg.norm(zo)
-    print("V-cycle with network: relative difference %.2e (to the smoothed coarse level %.2e)"
-          % (err, np.linalg.norm(z - zs) / np.linalg.norm(zs)), flush=True)
+    print("V-cycle (%d levels) with network: relative difference %.2e (to the V-cycle without it %.2e)"
+          % (levels, err, np.linalg.norm(z - zs) / np.linalg.norm(zs)), flush=True)
     assert err <= 2e-2
+
+
+def test_vcycle_of_unbalanced_matrix_against_oracle():
+    # the V-cycle in physical variables (training data of the network)
+    ne = (16, 8, 16)
+    _, op, _ = _system(ne, False, 13)
+    A = _csr(op)
+    H = ML.Hierarchy(A, tuple(n + 1 for n in ne), levels=3, max_coarse=1, fp32=True)
+    op.setMultilevel(3, 2, 8)
+    b = np.random.default_rng(14).standard_normal(op.size)
+    x = op.vcycle(to_dev(b)).cpu().numpy()
+    xo = H.vcycle(b, coarse="smooth")
+    assert np.abs(x - xo).max() <= 1e-9 * np.abs(xo).max()
 
 
 def _true_res(op_fresh, x, b):

@@ -189,3 +189,11 @@ def test_energy_scaled_coarse_step_minimises_energy_error():
         return e @ (A @ e)
     alpha = x[0] / y[0]
     assert energy(alpha) <= min(energy(alpha * 1.01), energy(alpha * 0.99))
+    # after Chebyshev steps the network's correction never raises the error energy
+    H.coarse_steps = 4
+    x0 = H.smooth(0, b, None, 4)
+
+    def err_energy(v):
+        e = xs - v
+        return e @ (A @ e)
+    assert err_energy(H.net_correct(0, b, x0, lambda l, r: y)) <= err_energy(x0) * (1 + 1e-12)
