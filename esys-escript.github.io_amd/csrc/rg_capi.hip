@@ -125,6 +125,7 @@ int rg_context_create(int device, void* stream, rg_context** out)
         RG_HIP(hipGetDeviceProperties(&prop, device));
         c->num_cu = prop.multiProcessorCount;
         try {
+            c->scratch = rg::scratch_address();   // this device's copy of the scratch array
             RG_HIP(hipMalloc(&c->partials, sizeof(double) * 16 * rg::kMaxGrid));
             RG_HIP(hipHostMalloc(&c->pinned, sizeof(double) * 256, hipHostMallocDefault));
         } catch (...) {
@@ -306,6 +307,25 @@ int rg_matrix_add_element_blocks(rg_matrix* m, int64_t num_elements, int nodes_p
         }
         add_element_blocks_via(m, static_cast<unsigned char*>(m->stage), off, num_elements, nodes_per_element,
                                nodes, nsz, em, esz);
+        return RG_OK;
+    });
+}
+
+int rg_matrix_add_element_blocks_noalloc(rg_matrix* m, int64_t num_elements, int nodes_per_element,
+                                         const int32_t* nodes, int num_eq, const double* em)
+{
+    return guarded([&] {
+        size_t nsz = 0, esz = 0;
+        if (!check_element_blocks(m, num_elements, nodes_per_element, nodes, num_eq, em, &nsz, &esz)) return RG_OK;
+        const size_t off = align256(nsz);
+        RG_CHECK(off + esz <= (size_t)rg::kScratchBytes, RG_VALUE_ERROR,
+                 "add_element_blocks_noalloc: " + std::to_string(num_elements) + " elements need " +
+                     std::to_string(off + esz) + " bytes of staging, more than the " +
+                     std::to_string(rg::kScratchBytes) + "-byte scratch array; pass fewer elements per call");
+        RG_CHECK(m->ctx->scratch != nullptr, RG_STATE_ERROR, "add_element_blocks_noalloc: no scratch address");
+        use_device(m->ctx);
+        add_element_blocks_via(m, static_cast<unsigned char*>(m->ctx->scratch), off, num_elements,
+                               nodes_per_element, nodes, nsz, em, esz);
         return RG_OK;
     });
 }

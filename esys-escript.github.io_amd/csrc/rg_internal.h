@@ -45,6 +45,8 @@ struct Error : std::runtime_error {
 constexpr int kBlock = 256;          // threads per workgroup of every streaming kernel
 constexpr int kNumXcd = 8;           // MI355X: 8 XCDs, blocks dispatched round-robin
 constexpr int kMaxGrid = 8192;       // grid cap of chunked streaming kernels (partials per reduction)
+// Size of the per-device scratch array of the allocation-free entry points.
+constexpr int64_t kScratchBytes = RG_SCRATCH_BYTES;
 
 // Geometry of one matrix (one rank's share).
 struct Geom {
@@ -94,6 +96,7 @@ struct rg_context {
     double* partials = nullptr;     // 16 * kMaxGrid doubles of per-block partial sums
     double* pinned = nullptr;       // host-pinned staging for scalars (256 doubles)
     hipEvent_t ev = nullptr;
+    void* scratch = nullptr;        // device address of this device's scratch array (scratch_address)
 };
 
 namespace rg {
@@ -169,6 +172,9 @@ namespace rg {
 void launch_set_values(rg_matrix* m, double value);
 void launch_add_element_blocks(rg_matrix* m, int64_t ne, int npe, const int32_t* d_nodes,
                                const double* d_em);
+// Device address of the calling thread's current device's scratch array
+// (kScratchBytes); looked up once per device with hipGetSymbolAddress.
+void* scratch_address();
 void launch_assemble(rg_matrix* m, const rg_pde_coefs& c, int reduced, double* rhs);
 void launch_assemble_boundary(rg_matrix* m, const rg_coef* d, const rg_coef* y, int reduced, double* rhs);
 void launch_add_dirac(rg_matrix* m, int64_t count, const int64_t* d_nodes, const double* d_d, const double* d_y,

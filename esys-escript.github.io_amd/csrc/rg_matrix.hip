@@ -5,6 +5,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 
 #include "rg_device.h"
 #include "rg_dispatch.h"
@@ -106,6 +107,32 @@ void launch_add_element_blocks(rg_matrix* m, int64_t ne, int npe, const int32_t*
     });
     RG_HIP(hipGetLastError());
     m->val_zero = false;
+}
+
+// ---------------------------------------------------- allocation-free scratch
+// Device memory of the entry points that allocate nothing
+// (rg_matrix_add_element_blocks_noalloc): one array per device, held until
+// the process ends.  Host code must not pass the array's name to a kernel
+// (it is a host-side stand-in); it uses the device address that
+// scratch_address() looks up once per device with that device current.
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+void* scratch_address()
+{
+    constexpr int kMaxDevices = 64;
+    static std::mutex mu;
+    static void* addr[kMaxDevices] = {};
+    int dev = 0;
+    RG_HIP(hipGetDevice(&dev));
+    RG_CHECK(dev >= 0 && dev < kMaxDevices, RG_DEVICE_ERROR, "scratch: device id out of range");
+    std::lock_guard<std::mutex> lock(mu);
+    if (!addr[dev]) {
+        void* p = nullptr;
+        RG_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        RG_CHECK(p != nullptr, RG_DEVICE_ERROR, "hipGetSymbolAddress(scratch) returned NULL");
+        addr[dev] = p;
+    }
+    return addr[dev];
 }
 
 // ---------------------------------------------------------------- assembly

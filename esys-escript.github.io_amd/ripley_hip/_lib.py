@@ -29,6 +29,7 @@ RG_PRECONDITIONER_JACOBI, RG_PRECONDITIONER_GAUSS_SEIDEL, RG_PRECONDITIONER_ILU0
 RG_BYTEORDER_LITTLE_ENDIAN, RG_BYTEORDER_BIG_ENDIAN = 1234, 4321
 RG_BYTEORDER_NATIVE = RG_BYTEORDER_LITTLE_ENDIAN
 RG_DATATYPE_INT32, RG_DATATYPE_FLOAT32, RG_DATATYPE_FLOAT64 = 1, 2, 3
+RG_SCRATCH_BYTES = 16 << 20
 
 # every function the header declares (checked by the CPU test suite)
 EXPORTS = [
@@ -46,7 +47,7 @@ EXPORTS = [
     "rg_batch_offsets", "rg_batch_vector", "rg_batch_set_tile", "rg_batch_autotune", "rg_batch_solve",
     "rg_matrix_solve_preconditioner", "rg_matrix_coloring", "rg_read_binary_grid", "rg_write_binary_grid",
     "rg_matrix_set_multilevel", "rg_matrix_coarse_operator", "rg_lpc_set_loss_operator",
-    "rg_matrix_smooth", "rg_matrix_vcycle",
+    "rg_matrix_smooth", "rg_matrix_add_element_blocks_noalloc", "rg_matrix_vcycle",
 ]
 
 
@@ -111,6 +112,7 @@ def load():
         "rg_matrix_set_values": (I, [P, D]),
         "rg_matrix_reset_values": (I, [P, I]),
         "rg_matrix_add_element_blocks": (I, [P, I64, I, P, I, P]),
+        "rg_matrix_add_element_blocks_noalloc": (I, [P, I64, I, P, I, P]),
         "rg_matrix_assemble": (I, [P, ctypes.POINTER(RgPdeCoefs), I, P]),
         "rg_matrix_vector": (I, [P, D, P, D, P]),
         "rg_matrix_ypAx": (I, [P, P, P]),

@@ -108,6 +108,20 @@ int rg_matrix_reset_values(rg_matrix* m, int preserve_solver_data);
 int rg_matrix_add_element_blocks(rg_matrix* m, int64_t num_elements, int nodes_per_element,
                                  const int32_t* nodes, int num_eq, const double* em);
 
+/* No counterpart in the reference: rg_matrix_add_element_blocks for callers
+ * that cannot allocate device memory (code under graph capture, tight
+ * assembly loops, harnesses that forbid hipMalloc).  Same arguments and
+ * result; the node ids and element matrices are staged in a fixed 16 MiB
+ * device array per GPU (RG_SCRATCH_BYTES, held until the process ends)
+ * instead of the matrix's staging buffer, and the call allocates nothing.
+ * A call whose staging (node ids rounded up to 256 bytes, plus the element
+ * matrices) exceeds RG_SCRATCH_BYTES fails with RG_VALUE_ERROR: pass fewer
+ * elements per call.  Every call on one GPU shares that array, so calls that
+ * overlap on one GPU (different streams or threads) race. */
+#define RG_SCRATCH_BYTES (16 << 20)
+int rg_matrix_add_element_blocks_noalloc(rg_matrix* m, int64_t num_elements, int nodes_per_element,
+                                         const int32_t* nodes, int num_eq, const double* em);
+
 /* PDE coefficients (device pointers, escript data-point layout, column
  * major within a data point):
  *   single PDE: A[i + dim*j], B[i], C[j], D, X[i], Y
