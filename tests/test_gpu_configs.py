@@ -117,10 +117,14 @@ def test_c4_uneven_rectangles():
         ptr, idx, oval, orhs = O.assemble(2, (n, n), (1.0, 1.0), A=kap[:, None, None] * np.eye(2)[None], Y=1.0)
         O.nullify(1, ptr, idx, oval, qn, qn, 1.0)
         orhs = np.where(qn > 0, 0.0, orhs)
+        # the reference solve to 1e-10: PASO's outer loop may end there on "no
+        # improvement" (code 6) at the edge of FP64, with the solution still
+        # far more accurate than the one compared
         xo, dg, _, _ = O.solve(1, ptr, idx, oval, orhs, tolerance=1e-10)
         err = _rel_err(xs[j].cpu().numpy(), xo)
-        print("C4 uneven system %d (n=%d): relative difference to the oracle %.3e" % (j, n, err), flush=True)
-        assert dg["rc"] == 0 and err <= 1e-6
+        print("C4 uneven system %d (n=%d): relative difference to the oracle %.3e (oracle code %d)"
+              % (j, n, err, dg["rc"]), flush=True)
+        assert dg["rc"] in (0, 6) and err <= 1e-6
 
 
 @pytest.fixture(scope="module")
