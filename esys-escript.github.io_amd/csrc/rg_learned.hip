@@ -65,12 +65,17 @@ inline ConvTiles conv_tiles(const Grid3& g)
     return {(g.d0 + kTX - 1) / kTX, (g.d1 + kTY - 1) / kTY, (g.d2 + kTZ - 1) / kTZ};
 }
 
-template <int CIN, int COUT, bool RELU>
+// On small grids (few output bricks) the COUT/16 channel tiles are split
+// over blockIdx.y, NT tiles per workgroup, so that more workgroups share the
+// work (a 9^3 grid has 9 bricks: with one 16-channel tile each, 4x the
+// workgroups for COUT = 64).
+template <int CIN, int COUT, bool RELU, int NT>
 __global__ __launch_bounds__(256) void k_conv_fwd(Grid3 g, ConvTiles T, const bf16* __restrict__ in,
                                                   const bf16* __restrict__ w, const float* __restrict__ bias,
                                                   bf16* __restrict__ out)
 {
-    constexpr int NT = COUT / 16;
+    static_assert(COUT % (16 * NT) == 0, "channel tiles must divide COUT");
+    const int jt0 = blockIdx.y * NT;        // first 16-channel tile of this workgroup
     constexpr int C8 = CIN / 8;             // 16-byte channel groups per voxel
     constexpr int G8 = 27 * C8;             // groups of 8 along K = 27 * CIN
     constexpr int STEPS = (G8 + 3) / 4;
@@ -105,7 +110,7 @@ __global__ __launch_bounds__(256) void k_conv_fwd(Grid3 g, ConvTiles T, const bf
         bf16x8 b[NT];
 #pragma unroll
         for (int j = 0; j < NT; ++j) {
-            b[j] = *reinterpret_cast<const bf16x8*>(w + ((int64_t)(16 * j + n) * 27 + tap) * CIN + cig * 8);
+            b[j] = *reinterpret_cast<const bf16x8*>(w + ((int64_t)(16 * (jt0 + j) + n) * 27 + tap) * CIN + cig * 8);
             if (!gok) b[j] = bf16x8{};
         }
 #pragma unroll
@@ -131,7 +136,7 @@ __global__ __launch_bounds__(256) void k_conv_fwd(Grid3 g, ConvTiles T, const bf
             const int64_t p = (((int64_t)z + 1) * g.p1 + (y + 1)) * g.p0 + (x + 1);
 #pragma unroll
             for (int j = 0; j < NT; ++j) {
-                const int co = 16 * j + n;
+                const int co = 16 * (jt0 + j) + n;
                 float o = acc[m][j][r] + (bias ? bias[co] : 0.f);
                 if (RELU) o = fmaxf(o, 0.f);
                 out[p * COUT + co] = (bf16)o;
@@ -492,8 +497,19 @@ void conv_fwd_t(hipStream_t st, const Grid3& g, const bf16* in, const bf16* w, c
 {
     const ConvTiles T = conv_tiles(g);
     const int grid = T.nbx * T.nby * T.nbz;
-    if (relu) hipLaunchKernelGGL((k_conv_fwd<CIN, COUT, true>), dim3(grid), dim3(256), 0, st, g, T, in, w, b, out);
-    else hipLaunchKernelGGL((k_conv_fwd<CIN, COUT, false>), dim3(grid), dim3(256), 0, st, g, T, in, w, b, out);
+    constexpr int NT = COUT / 16;
+    // fewer bricks than half the CUs: one channel tile per workgroup
+    if (grid < 128) {
+        if (relu)
+            hipLaunchKernelGGL((k_conv_fwd<CIN, COUT, true, 1>), dim3(grid, NT), dim3(256), 0, st, g, T, in, w, b, out);
+        else
+            hipLaunchKernelGGL((k_conv_fwd<CIN, COUT, false, 1>), dim3(grid, NT), dim3(256), 0, st, g, T, in, w, b, out);
+    } else {
+        if (relu)
+            hipLaunchKernelGGL((k_conv_fwd<CIN, COUT, true, NT>), dim3(grid), dim3(256), 0, st, g, T, in, w, b, out);
+        else
+            hipLaunchKernelGGL((k_conv_fwd<CIN, COUT, false, NT>), dim3(grid), dim3(256), 0, st, g, T, in, w, b, out);
+    }
 }
 
 void conv_fwd(hipStream_t st, const Grid3& g, int cin, int cout, const bf16* in, const bf16* w, const float* b,

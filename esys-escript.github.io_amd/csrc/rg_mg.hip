@@ -101,12 +101,12 @@ __device__ __forceinline__ double entry27(const double* __restrict__ val, int64_
 // and the (at most 8) coarse nodes J whose interpolation reaches j, per axis
 // J = I + e/2 for even e = di + s (weight 1), I + (e-1)/2 and I + (e+1)/2 for
 // odd e (weight 1/2 each).  With BAL the fine matrix is balanced and the
-// entries are divided by bal_i bal_j.  Writes the coarse values (main and
+// entries are multiplied by ibal_i ibal_j (ibal = 1 / balance).  Writes the coarse values (main and
 // upper slots with SYMC), 1 / a_II and the block maximum of the Gershgorin
 // ratio sum_J |a_IJ| / a_II.
 template <bool SYMF, bool SYMC, bool BAL>
 __global__ __launch_bounds__(kBlock) void k_galerkin(const double* __restrict__ fv, MgGeom f,
-                                                     const double* __restrict__ bal, double* __restrict__ cv,
+                                                     const double* __restrict__ ibal, double* __restrict__ cv,
                                                      MgGeom c, double* __restrict__ cdinv, double* __restrict__ gpart)
 {
     const int64_t I = (int64_t)blockIdx.x * kBlock + threadIdx.x;
@@ -123,7 +123,7 @@ __global__ __launch_bounds__(kBlock) void k_galerkin(const double* __restrict__ 
             const bool iok = fx >= 0 && fx < f.n0 && fy >= 0 && fy < f.n1 && fz >= 0 && fz < f.n2;
             const int64_t i = iok ? fx + f.n0 * (fy + f.n1 * fz) : 0;
             const double wi = (dx ? 0.5 : 1.0) * (dy ? 0.5 : 1.0) * (dz ? 0.5 : 1.0);
-            const double ibi = BAL ? 1.0 / bal[i] : 1.0;
+            const double ibi = BAL ? ibal[i] : 1.0;
 #pragma unroll
             for (int s = 0; s < 27; ++s) {
                 const int sx = s % 3 - 1, sy = (s / 3) % 3 - 1, sz = s / 9 - 1;
@@ -131,7 +131,7 @@ __global__ __launch_bounds__(kBlock) void k_galerkin(const double* __restrict__ 
                 const bool ok = iok && jx >= 0 && jx < f.n0 && jy >= 0 && jy < f.n1 && jz >= 0 && jz < f.n2;
                 const int64_t j = ok ? jx + f.n0 * (jy + f.n1 * jz) : 0;
                 double a = entry27<SYMF>(fv, f.ld, i, j, s, ok);
-                if (BAL) a *= ibi / bal[j];
+                if (BAL) a *= ibi * ibal[j];
                 const double t = wi * a;
                 const int ex = dx + sx, ey = dy + sy, ez = dz + sz;
 #pragma unroll
@@ -228,6 +228,13 @@ __global__ void k_level_logk(const double* __restrict__ main_slot, int64_t N, do
 {
     const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (i < N) logk[i] = (float)log10(fmax(main_slot[i], 1e-300) / aref);
+}
+
+// out = 1 / v
+__global__ void k_reciprocal(const double* __restrict__ v, int64_t N, double* __restrict__ out)
+{
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i < N) out[i] = 1.0 / v[i];
 }
 
 // dinv = 1 / a_ii from the main slot
@@ -536,6 +543,7 @@ struct rg_mg_state {
     double* coefs = nullptr;       // kMaxLevels Chebyshev tables
     double* part = nullptr;        // block partials of the coarse-step reduction
     double* gpart = nullptr;       // block maxima of the set-up's Gershgorin bounds (one per 256 fine rows)
+    double* ibal = nullptr;        // 1 / balance of the level-0 matrix
     double* alpha = nullptr;       // coarse step length
     float* logk = nullptr;         // network input of the coarsest level
     double* y = nullptr;           // network output and its filtered copies
@@ -618,7 +626,7 @@ void mg_free(rg_matrix* m)
             if (L.A) rg_matrix_destroy(L.A);
         }
     }
-    for (void* p : {(void*)s->coefs, (void*)s->part, (void*)s->gpart, (void*)s->alpha, (void*)s->logk, (void*)s->y, (void*)s->y2, (void*)s->zero, (void*)s->P,
+    for (void* p : {(void*)s->coefs, (void*)s->part, (void*)s->gpart, (void*)s->ibal, (void*)s->alpha, (void*)s->logk, (void*)s->y, (void*)s->y2, (void*)s->zero, (void*)s->P,
                     (void*)s->Q, (void*)s->fcg})
         if (p) hipFree(p);
     delete s;
@@ -649,6 +657,7 @@ void mg_setup(rg_matrix* m, bool balanced)
         s->coefs = dalloc((int64_t)kMaxLevels * (2 * kChebSteps + 1));
         s->part = dalloc(2 * kMaxGrid);
         s->gpart = dalloc(grid1(g.N));
+        s->ibal = dalloc(g.N);
         s->alpha = dalloc(1);
         s->vld = ((g.N + 63) / 64) * 64;
         s->P = dalloc(kDirs * s->vld);
@@ -717,7 +726,11 @@ void mg_setup(rg_matrix* m, bool balanced)
         const rg_matrix* F = (l == 1) ? m : s->lev[l - 1].A;
         const MgGeom f = s->lev[l - 1].g, c = L.g;
         const unsigned G = grid1(c.N);
-        const double* bal = (l == 1 && balanced) ? m->balance : nullptr;
+        const double* bal = (l == 1 && balanced) ? s->ibal : nullptr;
+        if (bal) {
+            hipLaunchKernelGGL(k_reciprocal, dim3(grid1(g.N)), dim3(kBlock), 0, st, m->balance, g.N, s->ibal);
+            RG_HIP(hipGetLastError());
+        }
         auto launch = [&](auto SF, auto SC, auto B) {
             hipLaunchKernelGGL((k_galerkin<SF.value, SC.value, B.value>), dim3(G), dim3(kBlock), 0, st, F->val, f, bal,
                                L.A->val, c, L.dinv, s->gpart);
