@@ -228,8 +228,10 @@ __global__ __launch_bounds__(kBlock) void k_minit(MArgs a)
     publish_column_sums<KP, 1, 2>(s, a.partials, a.K);
 }
 
-// x += alpha p, r -= alpha q, partial r.D^-1 r and r.r per column
-template <int KP>
+// r -= alpha q (and x += alpha p with WITHX: the last iteration of a chunk,
+// before the residual replacement reads x), partial r.D^-1 r and r.r per
+// column
+template <int KP, bool WITHX>
 __global__ __launch_bounds__(kBlock) void k_mupdate(MArgs a)
 {
     double s[2][1] = {};
@@ -239,7 +241,7 @@ __global__ __launch_bounds__(kBlock) void k_mupdate(MArgs a)
     column_loop<KP>(a, [&](int64_t i, int kk) {
         if (!live) return;
         const int64_t e = i * a.K + kk;
-        a.x[e] = fma(alpha, a.p[e], a.x[e]);
+        if (WITHX) a.x[e] = fma(alpha, a.p[e], a.x[e]);
         const double rv = fma(-alpha, a.q[e], a.r[e]);
         a.r[e] = rv;
         s[0][0] += rv * a.dinv[i] * rv;
@@ -248,17 +250,25 @@ __global__ __launch_bounds__(kBlock) void k_mupdate(MArgs a)
     publish_column_sums<KP, 1, 2>(s, a.partials, a.K);
 }
 
-// p = D^-1 r + beta p
+// x += alpha p where this iteration's x update is pending (the stage after
+// k_mupdate<KP, false> flags it, also when that stage ended the column),
+// then p = D^-1 r + beta p for the running columns: x and p are read once
+// for both updates
 template <int KP>
 __global__ __launch_bounds__(kBlock) void k_mpupdate(MArgs a)
 {
     const int k = threadIdx.x % KP;
-    const bool live = k < a.K && !a.st[k < a.K ? k : 0].done;
-    const double beta = live ? a.st[k].beta : 0.0;
+    const int kc = k < a.K ? k : 0;
+    const bool live = k < a.K && !a.st[kc].done;
+    const bool xpend = k < a.K && a.st[kc].early;
+    const double beta = live ? a.st[kc].beta : 0.0;
+    const double alpha = xpend ? a.st[kc].alpha : 0.0;
     column_loop<KP>(a, [&](int64_t i, int kk) {
-        if (!live) return;
+        if (!live && !xpend) return;
         const int64_t e = i * a.K + kk;
-        a.p[e] = fma(beta, a.p[e], a.dinv[i] * a.r[e]);
+        const double pv = a.p[e];
+        if (xpend) a.x[e] = fma(alpha, pv, a.x[e]);
+        if (live) a.p[e] = fma(beta, pv, a.dinv[i] * a.r[e]);
     });
 }
 
@@ -633,7 +643,10 @@ void dispatch_kp(int K, F&& f)
 
 // ---------------------------------------------------------------- many RHS
 // SpMM tile shapes: (rows per thread R, columns per thread CPT).  A shape
-// applies when CPT divides the number of columns.
+// applies when CPT divides the number of columns.  (Longer row runs per
+// thread, R = 4 and 8, which load R + 2 x windows per plane for 3 R
+// products, spilled registers and ran 3-7x slower on the 128^3 Brick with
+// 64 columns: 4.8-12.5 ms against 1.7 ms per SpMM.)
 constexpr int kNumShapes = 4;
 constexpr int kShapeR[kNumShapes] = {1, 1, 1, 2};
 constexpr int kShapeC[kNumShapes] = {1, 2, 4, 2};
@@ -733,11 +746,17 @@ struct MultiSolver {
                 else hipLaunchKernelGGL((k_mspmv_dot<2, double, P, Rv, Cv>), dim3(L.grid), dim3(kBlock), 0, s, a, m->val);
             }
         });
-        stage<ST_CG_SPMV, 1>(s);
+        // the x update of an iteration is deferred into the direction update
+        // (which reads p anyway) except on the last iteration of a chunk,
+        // whose residual replacement reads x; ST_CGX_SPMV clears the pending
+        // flag of every column, ended ones included
+        stage<ST_CGX_SPMV, 1>(s);
         dispatch_kp(K, [&](auto KPC) {
-            hipLaunchKernelGGL((k_mupdate<KPC.value>), dim3(L.grid), dim3(kBlock), 0, s, a);
+            if (last) hipLaunchKernelGGL((k_mupdate<KPC.value, true>), dim3(L.grid), dim3(kBlock), 0, s, a);
+            else hipLaunchKernelGGL((k_mupdate<KPC.value, false>), dim3(L.grid), dim3(kBlock), 0, s, a);
         });
-        stage<ST_CG_UPDATE, 2>(s);
+        if (last) stage<ST_CG_UPDATE, 2>(s);
+        else stage<ST_CGX_UPDATE, 2>(s);
         if (last) {
             dispatch_shape(K, shape, [&](auto KPC, auto RC, auto CC) {
                 constexpr int P = KPC.value, Rv = RC.value, Cv = CC.value;

@@ -17,6 +17,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <type_traits>
 
 #include "rg_internal.h"
 
@@ -101,28 +102,50 @@ __global__ __launch_bounds__(256) void k_conv_fwd(Grid3 g, ConvTiles T, const bf
     for (int m = 0; m < 4; ++m)
 #pragma unroll
         for (int j = 0; j < NT; ++j) acc[m][j] = f32x4{0.f, 0.f, 0.f, 0.f};
-    for (int s = 0; s < STEPS; ++s) {
-        const int gi = 4 * s + q;
-        const bool gok = gi < G8;
-        const int gg = gok ? gi : 0;
-        const int tap = gg / C8, cig = gg % C8;
-        const int dx = tap % 3, dy = (tap / 3) % 3, dz = tap / 9;   // offsets + 1
-        bf16x8 b[NT];
+    // K steps in groups of U: the U weight loads of a group (global memory,
+    // L2 resident) are issued together before its MFMAs, so one load latency
+    // is paid per group rather than per step -- on the small grids of the
+    // coarse levels, where a workgroup's K loop is the whole kernel, that
+    // latency chain bounds the time
+    constexpr int U = NT <= 2 ? 8 : 4;
+    auto steps = [&](int s0, auto UC) {
+        constexpr int UU = decltype(UC)::value;
+        bf16x8 b[UU][NT];
+        int tapv[UU], cigv[UU];
+        bool gokv[UU];
 #pragma unroll
-        for (int j = 0; j < NT; ++j) {
-            b[j] = *reinterpret_cast<const bf16x8*>(w + ((int64_t)(16 * (jt0 + j) + n) * 27 + tap) * CIN + cig * 8);
-            if (!gok) b[j] = bf16x8{};
+        for (int u = 0; u < UU; ++u) {
+            const int gi = 4 * (s0 + u) + q;
+            gokv[u] = gi < G8;
+            const int gg = gokv[u] ? gi : 0;
+            tapv[u] = gg / C8;
+            cigv[u] = gg % C8;
+#pragma unroll
+            for (int j = 0; j < NT; ++j) {
+                b[u][j] = *reinterpret_cast<const bf16x8*>(w + ((int64_t)(16 * (jt0 + j) + n) * 27 + tapv[u]) * CIN +
+                                                           cigv[u] * 8);
+                if (!gokv[u]) b[u][j] = bf16x8{};
+            }
         }
 #pragma unroll
-        for (int m = 0; m < 4; ++m) {
-            const int row = 4 * wave + m;                 // (ty, tz) row of the brick
-            const int ty = row % kTY, tz = row / kTY;
-            bf16x8 a = tile[(((tz + dz) * kHY + (ty + dy)) * kHX + (n + dx)) * C8 + cig];
-            if (!gok) a = bf16x8{};
+        for (int u = 0; u < UU; ++u) {
+            const int tap = tapv[u];
+            const int dx = tap % 3, dy = (tap / 3) % 3, dz = tap / 9;   // offsets + 1
 #pragma unroll
-            for (int j = 0; j < NT; ++j) acc[m][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[j], acc[m][j], 0, 0, 0);
+            for (int m = 0; m < 4; ++m) {
+                const int row = 4 * wave + m;                 // (ty, tz) row of the brick
+                const int ty = row % kTY, tz = row / kTY;
+                bf16x8 a = tile[(((tz + dz) * kHY + (ty + dy)) * kHX + (n + dx)) * C8 + cigv[u]];
+                if (!gokv[u]) a = bf16x8{};
+#pragma unroll
+                for (int j = 0; j < NT; ++j)
+                    acc[m][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[u][j], acc[m][j], 0, 0, 0);
+            }
         }
-    }
+    };
+    int s = 0;
+    for (; s + U <= STEPS; s += U) steps(s, std::integral_constant<int, U>{});
+    if constexpr (STEPS % U != 0) steps(s, std::integral_constant<int, STEPS % U>{});
     // D rows 4q..4q+3 are the voxels x0 + 4q + r of the row, column n the channel
 #pragma unroll
     for (int m = 0; m < 4; ++m) {

@@ -3,8 +3,8 @@
 // No counterpart in the reference: PASO's preconditioners are Jacobi,
 // Gauss-Seidel, ILU0, RILU and an algebraic multigrid
 // (paso/src/Preconditioner.cpp:46-137).  This is geometric multigrid on
-// ripley's vertex-centred Brick grid, with the learned network as the solver
-// of its coarsest level:
+// ripley's vertex-centred Brick grid, with the learned network correcting
+// one of its levels:
 //
 //   * level l+1 has every other node of level l (ne/2 elements per axis);
 //     prolongation P is trilinear interpolation (per axis P[2I, I] = 1,
@@ -17,14 +17,16 @@
 //     over [lmax / 10, lmax], lmax the Gershgorin bound
 //     max_i sum_j |a_ij| / a_ii, before (from zero) and after the coarse
 //     correction, values in FP32, vectors in FP64;
-//   * coarsest level: `coarse_steps` Chebyshev steps from zero, x_c; with a
-//     network, then y = Net(r') for the residual r' = b_c - A_c x_c they
-//     leave (inputs the Jacobi-scaled residual and log10 of the level's
-//     diagonal relative to that of a unit coefficient), low-pass filtered by
-//     the same Chebyshev steps on A_c y = 0 from y, and x_c += alpha y with
-//     alpha = (r'.y)/(y.A_c y), the step that minimises the A_c-norm error
-//     along y -- the network corrects what the smoother leaves, and the step
-//     never increases the level's error energy.
+//   * coarsest level: `coarse_steps` Chebyshev steps from zero;
+//   * the network's level (any level below the finest, the coarsest
+//     included), after its post-smoothing: y = Net(r') for the residual
+//     r' = b_l - A_l x_l left there (inputs the Jacobi-scaled residual and
+//     log10 of the level's diagonal relative to that of a unit coefficient),
+//     low-pass filtered by `coarse_steps` Chebyshev steps on A_l y = 0 from
+//     y, and x_l += alpha y with alpha = (r'.y)/(y.A_l y), the step that
+//     minimises the A_l-norm error along y -- the network corrects what the
+//     sub-cycle below and the smoother leave, and never increases the
+//     level's error energy.
 //
 // The V-cycle runs in the physical (unbalanced) variables below level 0:
 // r_1 = P^T D^-1 r_bal going down, y_bal += D^-1 P x_1 coming up, so that
@@ -96,78 +98,66 @@ __device__ __forceinline__ double entry27(const double* __restrict__ val, int64_
 }
 
 // ------------------------------------------------------------ set-up kernels
-// Galerkin product A_c = P^T A P for one coarse node per thread: fine node
-// i = 2I + di (weight prod (1 - |di|/2)), its stencil neighbour j = i + s,
-// and the (at most 8) coarse nodes J whose interpolation reaches j, per axis
-// J = I + e/2 for even e = di + s (weight 1), I + (e-1)/2 and I + (e+1)/2 for
-// odd e (weight 1/2 each).  With BAL the fine matrix is balanced and the
-// entries are multiplied by ibal_i ibal_j (ibal = 1 / balance).  Writes the coarse values (main and
-// upper slots with SYMC), 1 / a_II and the block maximum of the Gershgorin
-// ratio sum_J |a_IJ| / a_II.
-template <bool SYMF, bool SYMC, bool BAL>
-__global__ __launch_bounds__(kBlock) void k_galerkin(const double* __restrict__ fv, MgGeom f,
-                                                     const double* __restrict__ ibal, double* __restrict__ cv,
-                                                     MgGeom c, double* __restrict__ cdinv, double* __restrict__ gpart)
+// One axis of the Galerkin product: out = P_AX^T in P_AX with P_AX the
+// linear interpolation along axis AX alone (P = P_z P_y P_x, so three such
+// products, x then y then z, give P^T A P).  Both operators are 27-point
+// stencils (the output on the grid with axis AX halved); one output row per
+// thread: the fine rows i = 2I + r along AX (r = -1, 0, 1, weight 1 - |r|/2)
+// and their stencil neighbours j = i + s; along AX, j sits at offset
+// e = r + s_AX from 2I, which the interpolation maps to the coarse offset e/2
+// (e even, weight 1) or to (e-1)/2 and (e+1)/2 (e odd, weight 1/2 each);
+// across the other axes the offset stays s.  Each output row reads 3 x 27
+// input entries (the direct product P^T A P per coarse node reads 27 x 27).
+// With BAL the input is the balanced level-0 matrix and its entries are
+// multiplied by ibal_i ibal_j (ibal = 1 / balance).  With SYMO only the main
+// and upper output slots are written.
+template <int AX, bool SYMI, bool SYMO, bool BAL>
+__global__ __launch_bounds__(kBlock) void k_galerkin_axis(const double* __restrict__ in, MgGeom fi,
+                                                          const double* __restrict__ ibal, double* __restrict__ out,
+                                                          MgGeom fo)
 {
     const int64_t I = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    double g = 0.0;
-    if (I < c.N) {
-        const int64_t X = I % c.n0, Y = (I / c.n0) % c.n1, Z = I / (c.n0 * c.n1);
-        double acc[27];
+    if (I >= fo.N) return;
+    const int64_t o[3] = {I % fo.n0, (I / fo.n0) % fo.n1, I / (fo.n0 * fo.n1)};
+    const int64_t fn[3] = {fi.n0, fi.n1, fi.n2};
+    double acc[27];
 #pragma unroll
-        for (int t = 0; t < 27; ++t) acc[t] = 0.0;
+    for (int t = 0; t < 27; ++t) acc[t] = 0.0;
 #pragma unroll
-        for (int di = 0; di < 27; ++di) {
-            const int dx = di % 3 - 1, dy = (di / 3) % 3 - 1, dz = di / 9 - 1;
-            const int64_t fx = 2 * X + dx, fy = 2 * Y + dy, fz = 2 * Z + dz;
-            const bool iok = fx >= 0 && fx < f.n0 && fy >= 0 && fy < f.n1 && fz >= 0 && fz < f.n2;
-            const int64_t i = iok ? fx + f.n0 * (fy + f.n1 * fz) : 0;
-            const double wi = (dx ? 0.5 : 1.0) * (dy ? 0.5 : 1.0) * (dz ? 0.5 : 1.0);
-            const double ibi = BAL ? ibal[i] : 1.0;
+    for (int r = -1; r <= 1; ++r) {
+        int64_t f[3] = {o[0], o[1], o[2]};
+        f[AX] = 2 * o[AX] + r;
+        if (f[AX] < 0 || f[AX] >= fn[AX]) continue;
+        const int64_t i = f[0] + fi.n0 * (f[1] + fi.n1 * f[2]);
+        const double wr = r ? 0.5 : 1.0;
+        const double ibi = BAL ? ibal[i] : 1.0;
 #pragma unroll
-            for (int s = 0; s < 27; ++s) {
-                const int sx = s % 3 - 1, sy = (s / 3) % 3 - 1, sz = s / 9 - 1;
-                const int64_t jx = fx + sx, jy = fy + sy, jz = fz + sz;
-                const bool ok = iok && jx >= 0 && jx < f.n0 && jy >= 0 && jy < f.n1 && jz >= 0 && jz < f.n2;
-                const int64_t j = ok ? jx + f.n0 * (jy + f.n1 * jz) : 0;
-                double a = entry27<SYMF>(fv, f.ld, i, j, s, ok);
-                if (BAL) a *= ibi * ibal[j];
-                const double t = wi * a;
-                const int ex = dx + sx, ey = dy + sy, ez = dz + sz;
+        for (int s = 0; s < 27; ++s) {
+            const int sd[3] = {s % 3 - 1, (s / 3) % 3 - 1, s / 9 - 1};
+            const int64_t jx = f[0] + sd[0], jy = f[1] + sd[1], jz = f[2] + sd[2];
+            const bool ok = jx >= 0 && jx < fi.n0 && jy >= 0 && jy < fi.n1 && jz >= 0 && jz < fi.n2;
+            const int64_t j = ok ? jx + fi.n0 * (jy + fi.n1 * jz) : 0;
+            double a = entry27<SYMI>(in, fi.ld, i, j, s, ok);
+            if (BAL) a *= ibi * ibal[j];
+            const int e = r + sd[AX];
+            const int lo = (e & 1) ? (e - 1) / 2 : e / 2;
+            const int hi = (e & 1) ? (e + 1) / 2 : e / 2;
+            const double w = ((e & 1) ? 0.5 : 1.0) * wr;
 #pragma unroll
-                for (int cz = 0; cz < 2; ++cz) {
-                    if (cz == 1 && !(ez & 1)) continue;
-                    const int Dz = (ez & 1) ? ((ez - 1) >> 1) + cz : (ez >> 1);
-                    const double wz = (ez & 1) ? 0.5 : 1.0;
-#pragma unroll
-                    for (int cy = 0; cy < 2; ++cy) {
-                        if (cy == 1 && !(ey & 1)) continue;
-                        const int Dy = (ey & 1) ? ((ey - 1) >> 1) + cy : (ey >> 1);
-                        const double wy = (ey & 1) ? 0.5 : 1.0;
-#pragma unroll
-                        for (int cx = 0; cx < 2; ++cx) {
-                            if (cx == 1 && !(ex & 1)) continue;
-                            const int Dx = (ex & 1) ? ((ex - 1) >> 1) + cx : (ex >> 1);
-                            const double wx = (ex & 1) ? 0.5 : 1.0;
-                            const int D = (Dx + 1) + 3 * (Dy + 1) + 9 * (Dz + 1);
-                            acc[D] = fma(t, wx * wy * wz, acc[D]);
-                        }
-                    }
-                }
+            for (int h = 0; h < 2; ++h) {
+                if (h == 1 && !(e & 1)) continue;
+                int d[3] = {sd[0], sd[1], sd[2]};
+                d[AX] = h ? hi : lo;
+                const int t = (d[0] + 1) + 3 * (d[1] + 1) + 9 * (d[2] + 1);
+                acc[t] = fma(w, a, acc[t]);
             }
         }
-#pragma unroll
-        for (int t = 0; t < 27; ++t) {
-            if (SYMC && t < C3) continue;
-            cv[(int64_t)(SYMC ? t - C3 : t) * c.ld + I] = acc[t];
-        }
-        cdinv[I] = 1.0 / acc[C3];
-        double s = 0.0;
-#pragma unroll
-        for (int t = 0; t < 27; ++t) s += fabs(acc[t]);
-        g = s / acc[C3];
     }
-    publish_block_max(g, gpart);
+#pragma unroll
+    for (int t = 0; t < 27; ++t) {
+        if (SYMO && t < C3) continue;
+        out[(int64_t)(SYMO ? t - C3 : t) * fo.ld + I] = acc[t];
+    }
 }
 
 // Gershgorin ratio max_i sum_j |a_ij| / a_ii of a 27-point operator (block
@@ -526,6 +516,62 @@ int grid_for(int64_t n) { return (int)std::min<int64_t>((n + kBlock - 1) / kBloc
 // the grid of a one-node-per-thread kernel
 unsigned grid1(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
 
+MgGeom make_mg_geom(int64_t n0, int64_t n1, int64_t n2)
+{
+    const int64_t N = n0 * n1 * n2;
+    return {n0, n1, n2, N, ((N + 63) / 64) * 64};
+}
+
+// The two partly coarsened grids of the one-axis products from f to c.
+void galerkin_grids(const MgGeom& f, const MgGeom& c, MgGeom* g1, MgGeom* g2)
+{
+    *g1 = make_mg_geom(c.n0, f.n1, f.n2);
+    *g2 = make_mg_geom(c.n0, c.n1, f.n2);
+}
+
+// Doubles of the intermediate operators of galerkin_product (nslot values per
+// row, the coarse storage's slot count).
+int64_t galerkin_tmp_doubles(const MgGeom& f, const MgGeom& c, int nslot)
+{
+    MgGeom g1, g2;
+    galerkin_grids(f, c, &g1, &g2);
+    return (int64_t)nslot * (g1.ld + g2.ld);
+}
+
+// cv = P^T A_f P (fv on grid f, symmetric storage sf; cv on grid c,
+// storage sc) as the x, y and z one-axis products, the two intermediate
+// operators in tmp (galerkin_tmp_doubles) with the coarse storage; ibal
+// (level 0 of a balanced hierarchy, else null) unbalances the fine entries.
+void galerkin_product(const double* fv, const MgGeom& f, bool sf, const double* ibal, double* cv, const MgGeom& c,
+                      bool sc, double* tmp, hipStream_t st)
+{
+    MgGeom g1, g2;
+    galerkin_grids(f, c, &g1, &g2);
+    double* t1 = tmp;
+    double* t2 = tmp + (int64_t)(sc ? 14 : 27) * g1.ld;
+    auto first = [&](auto SI, auto SO) {
+        if (ibal)
+            hipLaunchKernelGGL((k_galerkin_axis<0, SI.value, SO.value, true>), dim3(grid1(g1.N)), dim3(kBlock), 0, st,
+                               fv, f, ibal, t1, g1);
+        else
+            hipLaunchKernelGGL((k_galerkin_axis<0, SI.value, SO.value, false>), dim3(grid1(g1.N)), dim3(kBlock), 0, st,
+                               fv, f, nullptr, t1, g1);
+    };
+    auto rest = [&](auto S) {
+        hipLaunchKernelGGL((k_galerkin_axis<1, S.value, S.value, false>), dim3(grid1(g2.N)), dim3(kBlock), 0, st, t1,
+                           g1, nullptr, t2, g2);
+        hipLaunchKernelGGL((k_galerkin_axis<2, S.value, S.value, false>), dim3(grid1(c.N)), dim3(kBlock), 0, st, t2,
+                           g2, nullptr, cv, c);
+    };
+    if (sf && sc) first(BC<true>{}, BC<true>{});
+    else if (sf) first(BC<true>{}, BC<false>{});
+    else if (sc) first(BC<false>{}, BC<true>{});
+    else first(BC<false>{}, BC<false>{});
+    if (sc) rest(BC<true>{});
+    else rest(BC<false>{});
+    RG_HIP(hipGetLastError());
+}
+
 }  // namespace
 
 // ================================================================== host side
@@ -544,6 +590,7 @@ struct rg_mg_state {
     double* part = nullptr;        // block partials of the coarse-step reduction
     double* gpart = nullptr;       // block maxima of the set-up's Gershgorin bounds (one per 256 fine rows)
     double* ibal = nullptr;        // 1 / balance of the level-0 matrix
+    double* gtmp = nullptr;        // intermediate operators of the Galerkin products
     double* alpha = nullptr;       // coarse step length
     float* logk = nullptr;         // network input of the coarsest level
     double* y = nullptr;           // network output and its filtered copies
@@ -626,7 +673,8 @@ void mg_free(rg_matrix* m)
             if (L.A) rg_matrix_destroy(L.A);
         }
     }
-    for (void* p : {(void*)s->coefs, (void*)s->part, (void*)s->gpart, (void*)s->ibal, (void*)s->alpha, (void*)s->logk, (void*)s->y, (void*)s->y2, (void*)s->zero, (void*)s->P,
+    for (void* p : {(void*)s->coefs, (void*)s->part, (void*)s->gpart, (void*)s->ibal, (void*)s->gtmp,
+                    (void*)s->alpha, (void*)s->logk, (void*)s->y, (void*)s->y2, (void*)s->zero, (void*)s->P,
                     (void*)s->Q, (void*)s->fcg})
         if (p) hipFree(p);
     delete s;
@@ -689,6 +737,8 @@ void mg_setup(rg_matrix* m, bool balanced)
             L.r = dalloc(L.g.N);
             L.d = dalloc(L.g.N);
         }
+        if (nlev > 1)   // the first product is the largest
+            s->gtmp = dalloc(galerkin_tmp_doubles(s->lev[0].g, s->lev[1].g, m->symmetric ? 14 : 27));
         const MgGeom& gc = s->lev[std::max(net_level, 0)].g;
         if (with_net) {
             RG_HIP(hipMalloc(&s->logk, sizeof(float) * gc.N));
@@ -731,19 +781,14 @@ void mg_setup(rg_matrix* m, bool balanced)
             hipLaunchKernelGGL(k_reciprocal, dim3(grid1(g.N)), dim3(kBlock), 0, st, m->balance, g.N, s->ibal);
             RG_HIP(hipGetLastError());
         }
-        auto launch = [&](auto SF, auto SC, auto B) {
-            hipLaunchKernelGGL((k_galerkin<SF.value, SC.value, B.value>), dim3(G), dim3(kBlock), 0, st, F->val, f, bal,
-                               L.A->val, c, L.dinv, s->gpart);
-        };
-        auto by_b = [&](auto SF, auto SC) {
-            if (bal) launch(SF, SC, BC<true>{});
-            else launch(SF, SC, BC<false>{});
-        };
-        const bool sf = F->symmetric, sc = L.A->symmetric;
-        if (sf && sc) by_b(BC<true>{}, BC<true>{});
-        else if (sf) by_b(BC<true>{}, BC<false>{});
-        else if (sc) by_b(BC<false>{}, BC<true>{});
-        else by_b(BC<false>{}, BC<false>{});
+        galerkin_product(F->val, f, F->symmetric, bal, L.A->val, c, L.A->symmetric, s->gtmp, st);
+        // its Jacobi inverse and Gershgorin bound
+        const double* main_slot = L.A->val + (int64_t)(L.A->symmetric ? 0 : C3) * c.ld;
+        hipLaunchKernelGGL(k_inv_main_slot, dim3(G), dim3(kBlock), 0, st, main_slot, c.N, L.dinv);
+        if (L.A->symmetric)
+            hipLaunchKernelGGL(k_gershgorin<true>, dim3(G), dim3(kBlock), 0, st, L.A->val, c, s->gpart);
+        else
+            hipLaunchKernelGGL(k_gershgorin<false>, dim3(G), dim3(kBlock), 0, st, L.A->val, c, s->gpart);
         hipLaunchKernelGGL(k_cheb_table, dim3(1), dim3(kBlock), 0, st, s->gpart, (int)G, kChebRatio, L.coef);
         RG_HIP(hipGetLastError());
         L.A->val_zero = false;
@@ -946,19 +991,10 @@ rg_matrix* coarse_operator(rg_matrix* m, int levels)
             throw Error(rc, "coarse_operator: cannot create the coarse matrix");
         }
         const MgGeom f = geom_of(F), c = geom_of(C);
-        const unsigned G = grid1(c.N);
         double* tmp = nullptr;
         try {
-            RG_HIP(hipMalloc(&tmp, sizeof(double) * (c.N + G)));
-            auto launch = [&](auto SF, auto SC) {
-                hipLaunchKernelGGL((k_galerkin<SF.value, SC.value, false>), dim3(G), dim3(kBlock), 0, st, F->val, f,
-                                   nullptr, C->val, c, tmp, tmp + c.N);
-            };
-            if (F->symmetric && C->symmetric) launch(BC<true>{}, BC<true>{});
-            else if (F->symmetric) launch(BC<true>{}, BC<false>{});
-            else if (C->symmetric) launch(BC<false>{}, BC<true>{});
-            else launch(BC<false>{}, BC<false>{});
-            RG_HIP(hipGetLastError());
+            RG_HIP(hipMalloc(&tmp, sizeof(double) * galerkin_tmp_doubles(f, c, C->symmetric ? 14 : 27)));
+            galerkin_product(F->val, f, F->symmetric, nullptr, C->val, c, C->symmetric, tmp, st);
             RG_HIP(hipStreamSynchronize(st));
         } catch (...) {
             if (tmp) hipFree(tmp);

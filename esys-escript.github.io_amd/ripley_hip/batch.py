@@ -77,7 +77,7 @@ def autotune_many(op, K, fp32=True, reps=5):
     """(index of the chosen SpMM shape in SPMM_SHAPES, seconds per SpMM of
     each shape; -1 for shapes that do not divide K)."""
     r = ctypes.c_int()
-    t = (ctypes.c_double * 4)()
+    t = (ctypes.c_double * len(SPMM_SHAPES))()
     op._bind()
     check(load().rg_multi_autotune(op.handle, int(K), int(bool(fp32)), int(reps), ctypes.byref(r), t))
     return r.value, list(t)
@@ -145,7 +145,7 @@ class BatchSolver:
     def autotune(self, fp32=True, reps=5):
         """(chosen tile multiplier, seconds per batched SpMV of 1, 2, 4, 8)."""
         m = ctypes.c_int()
-        t = (ctypes.c_double * 4)()
+        t = (ctypes.c_double * len(self.TILES))()
         self.ctx.bind_stream()
         check(load().rg_batch_autotune(self.handle, int(bool(fp32)), int(reps), ctypes.byref(m), t))
         return m.value, list(t)

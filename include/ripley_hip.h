@@ -179,7 +179,12 @@ int rg_matrix_add_dirac(rg_matrix* m, int64_t count, const int64_t* nodes, const
 
 /* paso::SystemMatrix::MatrixVector (paso/src/SystemMatrix_MatrixVector.cpp:
  * 62-91): y = alpha*A*x + beta*y (device x, y).  Fails with RG_STATE_ERROR
- * on a balanced matrix ("MatrixVector: balanced matrix is not supported."). */
+ * on a balanced matrix ("MatrixVector: balanced matrix is not supported.").
+ * x is assumed finite: at the x0/x1 grid edges a stencil slot that leaves
+ * the grid holds 0 and is multiplied by the neighbouring line's entry of x
+ * (no per-slot branch in the kernel), so an Inf or NaN there reaches rows
+ * that PASO's CSR product does not couple to it.  The same holds for
+ * rg_matrix_ypAx and the solvers' products. */
 int rg_matrix_vector(rg_matrix* m, double alpha, const double* x, double beta, double* y);
 
 /* AbstractSystemMatrix::ypAx (paso/src/SystemMatrix.h:646-671): y += A*x. */
@@ -362,12 +367,14 @@ int rg_solve_multi(rg_matrix* m, double* X, const double* B, int nrhs, const rg_
 /* Y = A X for a row-major num_rows x nrhs block (device). */
 int rg_matrix_vector_multi(rg_matrix* m, const double* X, double* Y, int nrhs);
 /* Times reps SpMMs for each tile shape -- (rows, columns) per thread of
- * (1,1), (1,2), (1,4), (2,2); shapes whose column count does not divide nrhs
- * are skipped (time -1) -- keeps the fastest for this matrix and returns its
- * index in *shape; times (host, may be NULL) receives 4 seconds/SpMM. */
+ * (1,1), (1,2), (1,4), (2,2); shapes whose column count does not divide
+ * nrhs are skipped (time -1) -- keeps the fastest for this matrix and
+ * returns its index in *shape; times (host, may be NULL) receives
+ * RG_MULTI_SHAPES seconds/SpMM. */
+#define RG_MULTI_SHAPES 4
 int rg_multi_autotune(rg_matrix* m, int nrhs, int fp32, int reps, int* shape, double* times);
-/* Fixes the SpMM shape (0..3 as above) or, with -1, autotunes on the next
- * solve. */
+/* Fixes the SpMM shape (0..RG_MULTI_SHAPES-1 as above) or, with -1,
+ * autotunes on the next solve. */
 int rg_multi_set_shape(rg_matrix* m, int shape);
 
 /* A batch of independent systems (count matrices of one dimension, sizes

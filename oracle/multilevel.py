@@ -20,11 +20,16 @@ standard geometric multigrid on ripley's vertex-centred grid:
   filtered by the level's Chebyshev steps on A y = 0 and scaled to minimise
   the level's energy-norm error along it.
 
-Pinned in tests/test_oracle_multilevel.py by: P reproducing linear
-functions exactly, the Galerkin product against a dense triple product,
-the Chebyshev smoother against the closed-form Chebyshev polynomial of
-D^-1 A applied to b, the V-cycle's symmetry (a symmetric preconditioner for
-symmetric smoothing) and its convergence on a Poisson problem.
+Pinned in tests/test_oracle_multilevel.py by: P reproducing trilinear
+functions exactly; the Galerkin product equal to the Q1 assembly on the
+coarse grid for constant and 2x2x2-blockwise constant kappa (nested
+conforming spaces); the Chebyshev smoother against the closed-form
+Chebyshev error polynomial from an eigendecomposition; the V-cycle's
+symmetry and convergence on a Poisson problem; the balanced form against
+the physical one; the flexible CG against scipy's CG with a fixed
+preconditioner; the energy-scaled step minimising the energy-norm error.
+Planted mistakes (an interpolation weight, a Chebyshev coefficient's sign,
+a missing balance factor) fail these checks.
 """
 import numpy as np
 import scipy.sparse as sp

@@ -46,16 +46,20 @@ def test_matrix_vector_many(dim, ne, K):
     op, _, (ptr, idx, val, _), _ = _system(dim, ne, 1)
     rng = np.random.default_rng(K)
     X = rng.standard_normal((op.size, K))
-    Y = RB.matrix_vector_many(op, to_dev(X)).cpu().numpy()
-    for k in range(K):
-        ref = O.spmv(1, ptr, idx, val, X[:, k])
-        scale = O.spmv(1, ptr, idx, np.abs(val), np.abs(X[:, k]))
-        assert np.all(np.abs(Y[:, k] - ref) <= 1e-13 * scale + 1e-300)
+    ref = np.stack([O.spmv(1, ptr, idx, val, X[:, k]) for k in range(K)], axis=1)
+    scale = np.stack([O.spmv(1, ptr, idx, np.abs(val), np.abs(X[:, k])) for k in range(K)], axis=1)
+    # every SpMM tile shape whose column count divides K
+    for shape, (_, cpt) in enumerate(RB.SPMM_SHAPES):
+        if K % cpt:
+            continue
+        RB.set_shape_many(op, shape)
+        Y = RB.matrix_vector_many(op, to_dev(X)).cpu().numpy()
+        assert np.all(np.abs(Y - ref) <= 1e-13 * scale + 1e-300), shape
 
 
 @pytest.mark.parametrize("method", ["PCG_PLAIN", "PCG_MIXED"])
 @pytest.mark.parametrize("K", [1, 7, 64])
-@pytest.mark.parametrize("shape", [0, 1, 2, 3])
+@pytest.mark.parametrize("shape", list(range(len(RB.SPMM_SHAPES))))
 def test_solve_many_matches_single_solves(method, K, shape):
     print("solve_many", method, K, shape, flush=True)
     op, rhs, (ptr, idx, val, orhs), q = _system(3, (10, 9, 8), 2, sigma=1.5)

@@ -10,7 +10,7 @@ their full sizes, checked against the oracle's PASO solves on the host
   batch: every system's FP64 residual, the three largest against the oracle;
 * Brick 192^3, kappa = exp(GRF) sigma_log 2, correlation 12 cells: plain
   Jacobi-PCG against the oracle's solution, and the multilevel flexible CG
-  with the learned coarse-level solver (49^3) to the FP64 residual 1e-8
+  with the learned level correction (25^3) to the FP64 residual 1e-8
   without fallback, in fewer iterations than the same hierarchy without
   the network.
 
@@ -182,7 +182,8 @@ def test_c5_learned_fcg_reaches_fp64_tolerance(c5):
                            "lpc_coarse_lognormal_s2_l12.npy")
     op, rhs = _c5_system(c5, sym=True)
     ne = c5["ne"]
-    net = learned.LearnedPreconditioner(tuple(n // 4 + 1 for n in ne), params=np.load(weights, allow_pickle=False))
+    # the network on the level three halvings down (25^3 nodes)
+    net = learned.LearnedPreconditioner(tuple(n // 8 + 1 for n in ne), params=np.load(weights, allow_pickle=False))
     op.setLearnedPreconditioner(net)
     o = _opts("FCG_LEARNED")
     o.setCheckInterval(4)
