@@ -122,6 +122,10 @@ __device__ __forceinline__ void spmm_rows(const VT* __restrict__ val, const MArg
         for (int r = 0; r < R; ++r)
 #pragma unroll
             for (int c = 0; c < CPT; ++c) acc[r][c] = 0.0;
+        // all nine (dy, dz) planes unrolled: their x window loads are in
+        // flight together (one wave per SIMD for 4 columns per thread, yet
+        // faster than one plane at a time: 1.9 against 2.7 ms on the 128^3
+        // Brick with 64 columns)
 #pragma unroll
         for (int pl = 0; pl < NPL; ++pl) {
             const int dy = pl % 3 - 1, dz = (DIM == 3) ? pl / 3 - 1 : 0;
@@ -152,6 +156,164 @@ __device__ __forceinline__ void spmm_rows(const VT* __restrict__ val, const MArg
         for (int r = 0; r < R; ++r)
             if (active && i0 + r < rr.r1) epi(i0 + r, k0, acc[r]);
     }
+}
+
+// ------------------------------------------- z-marching SpMM (64 columns)
+// Y = A X for exactly 64 columns on a Brick: a workgroup owns a 16 x 4
+// (x, y) tile of nodes, all 64 columns (lane = column, wave = y line of the
+// tile) and a segment of kZSeg z-planes, and marches up z.  X enters LDS
+// one z-plane of the tile at a time (18 x 6 nodes with the halo, 54 KB),
+// prefetched into registers while the previous plane is in use; each X
+// plane contributes to the three output planes it touches (its dz = +1, 0,
+// -1 slots), whose sums stay in registers, and the lowest of them is then
+// complete and handed to the epilogue.  Each X value is read from L2 about
+// 1.7 times (the halo) instead of once per stencil slot, and the matrix
+// values -- the same for the 64 lanes of a wave -- come through the scalar
+// cache.
+constexpr int kZY = 4, kZK = 64, kZSeg = 16;
+constexpr int kZHY = kZY + 2;
+static_assert(kZY * 64 == kBlock, "z-march tile: one wave per y line");
+
+struct ZArgs {
+    int64_t n0, n1, n2, ld;
+    int nbx, nby, nseg;
+};
+
+// TX: x nodes per thread (the tile is TX x 4 nodes)
+inline ZArgs zmarch_args(const Geom& g, int TX)
+{
+    ZArgs z;
+    z.n0 = g.nn[0]; z.n1 = g.nn[1]; z.n2 = g.nn[2]; z.ld = g.ld;
+    z.nbx = (int)((z.n0 + TX - 1) / TX);
+    z.nby = (int)((z.n1 + kZY - 1) / kZY);
+    z.nseg = (int)((z.n2 + kZSeg - 1) / kZSeg);
+    return z;
+}
+
+// TX consecutive values of one stencil slot (a uniform address: one wide
+// scalar load)
+template <int TX, typename VT>
+struct alignas(sizeof(VT)) SlotRun {
+    VT v[TX];
+};
+
+// epi(row, sum, x at the row) for every output row of the workgroup
+template <int kZX, typename VT, class Epi>
+__device__ __forceinline__ void spmm_zmarch(const VT* __restrict__ val, const ZArgs z, const double* __restrict__ X,
+                                            Epi&& epi)
+{
+    constexpr int kZHX = kZX + 2;
+    constexpr int kZRows = kZHX * kZHY;                             // halo rows per plane
+    constexpr int kZPer = (kZRows * (kZK / 2) + kBlock - 1) / kBlock;   // 16-byte pieces staged per thread
+    __shared__ double xs[kZRows * kZK];
+    const int c = threadIdx.x & 63;
+    const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int b = blockIdx.x;
+    const int bx = b % z.nbx, by = (b / z.nbx) % z.nby, bs = b / (z.nbx * z.nby);
+    const int64_t x0 = (int64_t)bx * kZX, y0 = (int64_t)by * kZY;
+    const int64_t y = y0 + wv;
+    const bool yok = y < z.n1;
+    const int64_t za = (int64_t)bs * kZSeg, zb = min(za + (int64_t)kZSeg, z.n2);
+    const int nx = (int)min((int64_t)kZX, z.n0 - x0);
+    // staging: 32 threads per halo row, 16 bytes (2 columns) each; the row
+    // of a piece depends on the lane, so its address arithmetic stays in
+    // vector registers
+    double2 pre[kZPer];
+    const int ch = threadIdx.x & 31;
+    auto fetch = [&](int64_t zp) {
+#pragma unroll
+        for (int k = 0; k < kZPer; ++k) {
+            const int r = (int)(threadIdx.x >> 5) + (kBlock / 32) * k;   // halo row
+            const int hy = r / kZHX, hx = r % kZHX;
+            const int64_t gx = x0 - 1 + hx, gy = y0 - 1 + hy;
+            const bool ok = r < kZRows && gx >= 0 && gx < z.n0 && gy >= 0 && gy < z.n1;
+            pre[k] = ok ? *reinterpret_cast<const double2*>(X + ((zp * z.n1 + gy) * z.n0 + gx) * kZK + 2 * ch)
+                        : double2{0.0, 0.0};
+        }
+    };
+    double am[kZX], ac[kZX], ap[kZX], pc[kZX];   // output planes zp-1, zp, zp+1; x at plane zp-1
+#pragma unroll
+    for (int x = 0; x < kZX; ++x) am[x] = ac[x] = ap[x] = pc[x] = 0.0;
+    // the staged X plane's contributions to output plane zo (dz = zp - zo):
+    // the slot's values for the tile's TX nodes in one scalar load (a
+    // partial tile reads on past its line, into the next rows or the value
+    // arrays' slack: those sums are never written)
+    auto add_slot = [&](int64_t zo, int dz, int dy, int dx, const double (&w)[kZHX], double (&A)[kZX]) {
+        const int64_t rowb = (zo * z.n1 + y) * z.n0 + x0;
+        const SlotRun<kZX, VT> t = *reinterpret_cast<const SlotRun<kZX, VT>*>(
+            val + (int64_t)((dx + 1) + 3 * (dy + 1) + 9 * (dz + 1)) * z.ld + rowb);
+#pragma unroll
+        for (int x = 0; x < kZX; ++x) A[x] = fma((double)t.v[x], w[x + 1 + dx], A[x]);
+    };
+    auto finish = [&](int64_t zo, const double (&A)[kZX]) {
+        const int64_t rowb = (zo * z.n1 + y) * z.n0 + x0;
+#pragma unroll
+        for (int x = 0; x < kZX; ++x)
+            if (x < nx) epi(rowb + x, A[x], pc[x]);
+    };
+    const int64_t zfirst = za > 0 ? za - 1 : 0;
+    const int64_t zlast = zb < z.n2 ? zb : z.n2 - 1;
+    fetch(zfirst);
+    for (int64_t zp = zfirst; zp <= zlast; ++zp) {
+        __syncthreads();   // every wave is done with the previous plane
+#pragma unroll
+        for (int k = 0; k < kZPer; ++k) {
+            const int r = (int)(threadIdx.x >> 5) + (kBlock / 32) * k;
+            if (r < kZRows) *reinterpret_cast<double2*>(xs + r * kZK + 2 * ch) = pre[k];
+        }
+        __syncthreads();
+        if (zp < zlast) fetch(zp + 1);   // in flight while this plane is used
+        if (yok) {
+            const bool cm = zp - 1 >= za, cc = zp >= za && zp < zb, cp = zp + 1 < zb;
+            // one y offset at a time: its 9 slot runs are the scalar loads
+            // in flight (all 27 at once would spill scalar registers)
+#pragma unroll 1
+            for (int dy = -1; dy <= 1; ++dy) {
+                double w[kZHX];
+#pragma unroll
+                for (int hx = 0; hx < kZHX; ++hx) w[hx] = xs[((wv + 1 + dy) * kZHX + hx) * kZK + c];
+#pragma unroll
+                for (int dx = -1; dx <= 1; ++dx) {
+                    if (cm) add_slot(zp - 1, 1, dy, dx, w, am);
+                    if (cc) add_slot(zp, 0, dy, dx, w, ac);
+                    if (cp) add_slot(zp + 1, -1, dy, dx, w, ap);
+                }
+            }
+            if (cm) finish(zp - 1, am);   // output plane zp-1 has all its X planes
+#pragma unroll
+            for (int x = 0; x < kZX; ++x) pc[x] = xs[((wv + 1) * kZHX + x + 1) * kZK + c];
+        }
+#pragma unroll
+        for (int x = 0; x < kZX; ++x) {
+            am[x] = ac[x];
+            ac[x] = ap[x];
+            ap[x] = 0.0;
+        }
+    }
+    // the top plane of the Brick has no X plane above it
+    if (yok && zlast == z.n2 - 1 && zlast >= za && zlast < zb) finish(zlast, am);
+}
+
+// q = A p and the partial p.q per column (64 columns, z-marching)
+template <int TX, typename VT>
+__global__ __launch_bounds__(kBlock) void k_mspmv_dot_z(MArgs a, ZArgs z, const VT* __restrict__ val)
+{
+    double s[1][1] = {{0.0}};
+    const int c = threadIdx.x & 63;
+    spmm_zmarch<TX, VT>(val, z, a.p, [&](int64_t i, double q, double p) {
+        a.q[i * kZK + c] = q;
+        s[0][0] = fma(q, p, s[0][0]);
+    });
+    publish_column_sums<kZK, 1, 1>(s, a.partials, a.K);
+}
+
+// Y = A X (64 columns, z-marching)
+template <int TX, typename VT>
+__global__ __launch_bounds__(kBlock) void k_mspmv_z(ZArgs z, const VT* __restrict__ val, const double* __restrict__ X,
+                                                    double* __restrict__ Y)
+{
+    const int c = threadIdx.x & 63;
+    spmm_zmarch<TX, VT>(val, z, X, [&](int64_t i, double q, double) { Y[i * kZK + c] = q; });
 }
 
 // q = A p, partial p.q per column
@@ -618,8 +780,7 @@ void prepare_matrix(rg_matrix* m, bool fp32, double* dinv)
     RG_CHECK(!failed, RG_VALUE_ERROR, "batched solve: zero on the main diagonal");
     if (fp32 && !m->val32_valid) {
         if (!m->val32) {
-            const size_t cnt = (size_t)m->g.nstore * m->g.ld;
-            RG_HIP(hipMalloc(&m->val32, cnt * sizeof(float)));
+            RG_HIP(hipMalloc(&m->val32, value_count(m->g) * sizeof(float)));
         }
         launch_to_fp32(m);
         m->val32_valid = true;
@@ -643,13 +804,23 @@ void dispatch_kp(int K, F&& f)
 
 // ---------------------------------------------------------------- many RHS
 // SpMM tile shapes: (rows per thread R, columns per thread CPT).  A shape
-// applies when CPT divides the number of columns.  (Longer row runs per
-// thread, R = 4 and 8, which load R + 2 x windows per plane for 3 R
-// products, spilled registers and ran 3-7x slower on the 128^3 Brick with
-// 64 columns: 4.8-12.5 ms against 1.7 ms per SpMM.)
-constexpr int kNumShapes = 4;
-constexpr int kShapeR[kNumShapes] = {1, 1, 1, 2};
-constexpr int kShapeC[kNumShapes] = {1, 2, 4, 2};
+// applies when CPT divides the number of columns.  Shapes 4 and 5 are the
+// z-marching LDS tiles (exactly 64 columns, Brick only; their R is the x
+// run of 16 or 8 nodes per thread).  Longer row runs per thread (R = 4, 8:
+// R + 2 x windows per plane for 3 R products) spilled with the planes
+// unrolled and ran 2.7-3.1 ms with them rolled, against 1.9 ms.
+constexpr int kNumShapes = 6;
+constexpr int kZShape = 4, kZShapeEnd = 6;
+constexpr int kShapeR[kNumShapes] = {1, 1, 1, 2, 16, 8};
+constexpr int kShapeC[kNumShapes] = {1, 2, 4, 2, kZK, kZK};
+
+inline bool is_zshape(int shape) { return shape >= kZShape && shape < kZShapeEnd; }
+
+inline bool shape_applies(int shape, int K, int dim)
+{
+    if (is_zshape(shape)) return K == kZK && dim == 3;
+    return K % kShapeC[shape] == 0;
+}
 
 inline int pow2_at_least(int v)
 {
@@ -679,32 +850,64 @@ void dispatch_shape(int K, int shape, F&& f)
     if (R == 1 && C == 1) go(IC<1>{}, IC<1>{});
     else if (R == 1 && C == 2) go(IC<1>{}, IC<2>{});
     else if (R == 1 && C == 4) go(IC<1>{}, IC<4>{});
-    else go(IC<2>{}, IC<2>{});
+    else if (R == 2 && C == 2) go(IC<2>{}, IC<2>{});
+    else RG_CHECK(false, RG_VALUE_ERROR, "SpMM shape has no kernel");
 }
 
 struct MultiSolver {
     rg_matrix* m;
     int K, KP, shape;
+    int rshape;     // shape of the row-chunked SpMM kernels (the FP64 residual replacement)
     bool fp32;
+    bool zm;        // z-marching SpMM
     hipStream_t st;
     MArgs a;
-    Launch L;       // SpMM / vector kernels
+    Launch L;       // row-chunked SpMM / vector kernels
+    ZArgs z{};
+    int Gs;         // workgroups (partials per column) of the iteration's SpMM
 
     MultiSolver(rg_matrix* mm, int k, int shp, bool f32) : m(mm), K(k), shape(shp), fp32(f32), st(mm->ctx->stream)
     {
         KP = pow2_at_least(K);
-        const int kpc = pow2_at_least(K / kShapeC[shape]);
-        const int64_t rpp = (int64_t)(kBlock / kpc) * kShapeR[shape];
+        zm = is_zshape(shape);
+        RG_CHECK(shape_applies(shape, K, m->g.dim), RG_VALUE_ERROR,
+                 "SpMM shape does not apply to this column count or dimension");
+        rshape = zm ? 2 : shape;
+        const int kpc = pow2_at_least(K / kShapeC[rshape]);
+        const int64_t rpp = (int64_t)(kBlock / kpc) * kShapeR[rshape];
         L = chunked_launch((m->g.N + rpp - 1) / rpp * kBlock, 1, 2048);
         // chunked_launch counted threads; convert to rows per block
         L.chunk = L.chunk / kBlock * rpp;
         std::memset(&a, 0, sizeof(a));
         a.N = m->g.N; a.ld = m->g.ld; a.s1 = m->g.s1; a.s2 = m->g.s2; a.chunk = L.chunk; a.K = K;
+        Gs = L.grid;
+        if (zm) {
+            z = zmarch_args(m->g, kShapeR[shape]);
+            Gs = z.nbx * z.nby * z.nseg;
+        }
+    }
+
+    // partials per column of the reducing kernels (SpMM and vector kernels)
+    int max_grid() const { return std::max(L.grid, Gs); }
+
+    template <class F>
+    void by_tx(F&& f)
+    {
+        if (kShapeR[shape] == 16) f(IC<16>{});
+        else f(IC<8>{});
     }
 
     void spmm(const double* X, double* Y)
     {
         const Geom& g = m->g;
+        if (zm) {
+            by_tx([&](auto TX) {
+                if (fp32) hipLaunchKernelGGL((k_mspmv_z<TX.value, float>), dim3(Gs), dim3(kBlock), 0, st, z, m->val32, X, Y);
+                else hipLaunchKernelGGL((k_mspmv_z<TX.value, double>), dim3(Gs), dim3(kBlock), 0, st, z, m->val, X, Y);
+            });
+            RG_HIP(hipGetLastError());
+            return;
+        }
         dispatch_shape(K, shape, [&](auto KPC, auto RC, auto CC) {
             constexpr int P = KPC.value, Rv = RC.value, Cv = CC.value;
             if (g.dim == 3) {
@@ -719,9 +922,10 @@ struct MultiSolver {
     }
 
     template <int STAGE, int NV>
-    void stage(hipStream_t s)
+    void stage(hipStream_t s, int G = -1)
     {
-        hipLaunchKernelGGL((k_mstage<STAGE, NV>), dim3(K), dim3(kBlock), 0, s, a.st, a.partials, L.grid, K);
+        hipLaunchKernelGGL((k_mstage<STAGE, NV>), dim3(K), dim3(kBlock), 0, s, a.st, a.partials, G < 0 ? L.grid : G,
+                           K);
     }
 
     void init(hipStream_t s)
@@ -735,22 +939,42 @@ struct MultiSolver {
 
     void iteration(hipStream_t s, bool last)
     {
-        const Geom& g = m->g;
-        dispatch_shape(K, shape, [&](auto KPC, auto RC, auto CC) {
-            constexpr int P = KPC.value, Rv = RC.value, Cv = CC.value;
-            if (g.dim == 3) {
-                if (fp32) hipLaunchKernelGGL((k_mspmv_dot<3, float, P, Rv, Cv>), dim3(L.grid), dim3(kBlock), 0, s, a, m->val32);
-                else hipLaunchKernelGGL((k_mspmv_dot<3, double, P, Rv, Cv>), dim3(L.grid), dim3(kBlock), 0, s, a, m->val);
-            } else {
-                if (fp32) hipLaunchKernelGGL((k_mspmv_dot<2, float, P, Rv, Cv>), dim3(L.grid), dim3(kBlock), 0, s, a, m->val32);
-                else hipLaunchKernelGGL((k_mspmv_dot<2, double, P, Rv, Cv>), dim3(L.grid), dim3(kBlock), 0, s, a, m->val);
-            }
-        });
+        spmm_dot(s);
         // the x update of an iteration is deferred into the direction update
         // (which reads p anyway) except on the last iteration of a chunk,
         // whose residual replacement reads x; ST_CGX_SPMV clears the pending
         // flag of every column, ended ones included
-        stage<ST_CGX_SPMV, 1>(s);
+        stage<ST_CGX_SPMV, 1>(s, Gs);
+        rest_of_iteration(s, last);
+    }
+
+    // q = A p with the partials of p.q (a.p, a.q, a.partials)
+    void spmm_dot(hipStream_t s)
+    {
+        const Geom& g = m->g;
+        if (zm) {
+            by_tx([&](auto TX) {
+                if (fp32) hipLaunchKernelGGL((k_mspmv_dot_z<TX.value, float>), dim3(Gs), dim3(kBlock), 0, s, a, z, m->val32);
+                else hipLaunchKernelGGL((k_mspmv_dot_z<TX.value, double>), dim3(Gs), dim3(kBlock), 0, s, a, z, m->val);
+            });
+        } else {
+            dispatch_shape(K, shape, [&](auto KPC, auto RC, auto CC) {
+                constexpr int P = KPC.value, Rv = RC.value, Cv = CC.value;
+                if (g.dim == 3) {
+                    if (fp32) hipLaunchKernelGGL((k_mspmv_dot<3, float, P, Rv, Cv>), dim3(L.grid), dim3(kBlock), 0, s, a, m->val32);
+                    else hipLaunchKernelGGL((k_mspmv_dot<3, double, P, Rv, Cv>), dim3(L.grid), dim3(kBlock), 0, s, a, m->val);
+                } else {
+                    if (fp32) hipLaunchKernelGGL((k_mspmv_dot<2, float, P, Rv, Cv>), dim3(L.grid), dim3(kBlock), 0, s, a, m->val32);
+                    else hipLaunchKernelGGL((k_mspmv_dot<2, double, P, Rv, Cv>), dim3(L.grid), dim3(kBlock), 0, s, a, m->val);
+                }
+            });
+        }
+        RG_HIP(hipGetLastError());
+    }
+
+    void rest_of_iteration(hipStream_t s, bool last)
+    {
+        const Geom& g = m->g;
         dispatch_kp(K, [&](auto KPC) {
             if (last) hipLaunchKernelGGL((k_mupdate<KPC.value, true>), dim3(L.grid), dim3(kBlock), 0, s, a);
             else hipLaunchKernelGGL((k_mupdate<KPC.value, false>), dim3(L.grid), dim3(kBlock), 0, s, a);
@@ -758,7 +982,7 @@ struct MultiSolver {
         if (last) stage<ST_CG_UPDATE, 2>(s);
         else stage<ST_CGX_UPDATE, 2>(s);
         if (last) {
-            dispatch_shape(K, shape, [&](auto KPC, auto RC, auto CC) {
+            dispatch_shape(K, rshape, [&](auto KPC, auto RC, auto CC) {
                 constexpr int P = KPC.value, Rv = RC.value, Cv = CC.value;
                 if (g.dim == 3) hipLaunchKernelGGL((k_mreplace<3, P, Rv, Cv>), dim3(L.grid), dim3(kBlock), 0, s, a, m->val);
                 else hipLaunchKernelGGL((k_mreplace<2, P, Rv, Cv>), dim3(L.grid), dim3(kBlock), 0, s, a, m->val);
@@ -772,7 +996,8 @@ struct MultiSolver {
     }
 };
 
-// Times `reps` SpMMs for each applicable shape; returns the best.
+// Times `reps` SpMMs of each applicable shape as the solver runs them (with
+// the partial dot products p.q); returns the best.
 int autotune_multi(rg_matrix* m, int K, bool fp32, const double* X, double* Y, int reps, double* times)
 {
     hipStream_t st = m->ctx->stream;
@@ -781,13 +1006,18 @@ int autotune_multi(rg_matrix* m, int K, bool fp32, const double* X, double* Y, i
     RG_HIP(hipEventCreate(&e1));
     int best = 0;
     float bt = 1e30f;
+    DevArray<double> part;
     for (int c = 0; c < kNumShapes; ++c) {
         if (times) times[c] = -1.0;
-        if (K % kShapeC[c]) continue;
+        if (!shape_applies(c, K, m->g.dim)) continue;
         MultiSolver S(m, K, c, fp32);
-        S.spmm(X, Y);   // warm-up
+        part.alloc((size_t)K * S.max_grid());
+        S.a.p = const_cast<double*>(X);
+        S.a.q = Y;
+        S.a.partials = part.p;
+        S.spmm_dot(st);   // warm-up
         RG_HIP(hipEventRecord(e0, st));
-        for (int k = 0; k < reps; ++k) S.spmm(X, Y);
+        for (int k = 0; k < reps; ++k) S.spmm_dot(st);
         RG_HIP(hipEventRecord(e1, st));
         RG_HIP(hipEventSynchronize(e1));
         float ms = 0.f;
@@ -826,12 +1056,12 @@ int solve_multi(rg_matrix* m, double* X, const double* B, int K, const rg_option
     sts.alloc(K);
     prepare_matrix(m, fp32, dinv.p);
     int shape = m->multi_shape - 1;
-    if (shape < 0 || K % kShapeC[shape]) {
+    if (shape < 0 || !shape_applies(shape, K, g.dim)) {
         shape = autotune_multi(m, K, fp32, B, q.p, 5, nullptr);
         m->multi_shape = shape + 1;
     }
     MultiSolver S(m, K, shape, fp32);
-    part.alloc((size_t)2 * K * S.L.grid);
+    part.alloc((size_t)2 * K * S.max_grid());
     S.a.dinv = dinv.p; S.a.b = B; S.a.x = X; S.a.r = r.p; S.a.p = p.p; S.a.q = q.p;
     S.a.st = sts.p; S.a.partials = part.p;
     init_states(st, sts.p, K, opt.tolerance, opt.iter_max);
@@ -853,7 +1083,7 @@ void multi_vector(rg_matrix* m, const double* X, double* Y, int K)
     RG_CHECK(K >= 1 && K <= 64, RG_VALUE_ERROR, "matrix_vector_multi: 1 to 64 columns");
     RG_CHECK(!m->is_balanced, RG_STATE_ERROR, "matrix_vector_multi: the matrix is balanced (set or reset values first)");
     int shape = m->multi_shape - 1;
-    if (shape < 0 || K % kShapeC[shape]) shape = 0;
+    if (shape < 0 || !shape_applies(shape, K, m->g.dim)) shape = 0;
     MultiSolver S(m, K, shape, false);
     S.spmm(X, Y);
 }
@@ -1146,7 +1376,7 @@ int rg_multi_set_shape(rg_matrix* m, int shape)
     return batch_guard([&] {
         RG_CHECK(m, RG_VALUE_ERROR, "multi_set_shape: NULL matrix");
         RG_CHECK(shape >= -1 && shape < rg::kNumShapes, RG_VALUE_ERROR,
-                 "multi_set_shape: shape must be -1 (autotune) or 0..3");
+                 "multi_set_shape: shape must be -1 (autotune) or 0..5");
         m->multi_shape = shape + 1;
         return RG_OK;
     });

@@ -208,7 +208,7 @@ int rg_matrix_create(rg_context* ctx, int dim, const int32_t* ne, const double* 
         m->flags = flags;
         m->symmetric = sym;
         try {
-            const size_t nval = (size_t)g.nstore * g.bs * g.bs * g.ld;
+            const size_t nval = value_count(g);
             RG_HIP(hipMalloc(&m->val, nval * sizeof(double)));
             RG_HIP(hipMemsetAsync(m->val, 0, nval * sizeof(double), ctx->stream));
             RG_HIP(hipMalloc(&m->dinv, (size_t)g.bs * g.bs * g.ld * sizeof(double)));

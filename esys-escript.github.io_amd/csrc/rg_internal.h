@@ -66,6 +66,13 @@ struct Geom {
     int64_t nnz = 0;     // PASO block nnz
 };
 
+// Values held by a matrix's value arrays (val, val32): the stored slots and
+// kValSlack more, so that wide loads of a run of consecutive rows that ends
+// past the last row (the partial tiles of the 64-column SpMM) stay inside
+// the allocation; the slack is never read into a result.
+constexpr int64_t kValSlack = 64;
+inline size_t value_count(const Geom& g) { return (size_t)g.nstore * g.bs * g.bs * g.ld + kValSlack; }
+
 // Scalars of one in-flight Krylov solve, resident in device memory and
 // updated by the last workgroup of each reducing kernel.
 struct KrylovState {

@@ -726,8 +726,7 @@ void mg_setup(rg_matrix* m, bool balanced)
                 }
                 const int rc = rg_matrix_create(m->ctx, 3, ne, len, 1, m->symmetric ? RG_MATRIX_SYMMETRIC : 0, &L.A);
                 RG_CHECK(rc == RG_OK, rc, "FCG_LEARNED: cannot create a coarse level");
-                const size_t cnt = (size_t)L.A->g.nstore * L.A->g.ld;
-                RG_HIP(hipMalloc(&L.A->val32, cnt * sizeof(float)));
+                RG_HIP(hipMalloc(&L.A->val32, value_count(L.A->g) * sizeof(float)));
                 L.dinv = dalloc(L.A->g.N);
                 L.g = geom_of(L.A);
                 L.b = dalloc(L.g.N);
@@ -1017,7 +1016,7 @@ void vcycle_unbalanced(rg_matrix* m, const double* b, double* x)
     RG_CHECK(m->g.dim == 3 && m->g.bs == 1, RG_VALUE_ERROR, "vcycle: scalar Brick matrices only");
     RG_CHECK(!m->is_balanced, RG_STATE_ERROR, "vcycle: balanced matrix is not supported.");
     if (!m->val32_valid) {
-        if (!m->val32) RG_HIP(hipMalloc(&m->val32, (size_t)m->g.nstore * m->g.ld * sizeof(float)));
+        if (!m->val32) RG_HIP(hipMalloc(&m->val32, value_count(m->g) * sizeof(float)));
         launch_to_fp32(m);
         m->val32_valid = true;
     }
@@ -1036,7 +1035,7 @@ void smooth_unbalanced(rg_matrix* m, const double* b, double* x, int steps)
     hipStream_t st = m->ctx->stream;
     const Geom& g = m->g;
     if (!m->val32_valid) {
-        if (!m->val32) RG_HIP(hipMalloc(&m->val32, (size_t)g.nstore * g.ld * sizeof(float)));
+        if (!m->val32) RG_HIP(hipMalloc(&m->val32, value_count(g) * sizeof(float)));
         launch_to_fp32(m);
         m->val32_valid = true;
     }

@@ -1583,8 +1583,7 @@ int solve(rg_matrix* m, double* x, const double* b, const rg_options& opt, rg_di
     int method = opt.method;
     if ((method == RG_METHOD_PCG_MIXED || method == RG_METHOD_FCG_LEARNED) && !m->val32_valid) {
         if (!m->val32) {
-            const size_t cnt = (size_t)m->g.nstore * m->g.bs * m->g.bs * m->g.ld;
-            RG_HIP(hipMalloc(&m->val32, cnt * sizeof(float)));
+            RG_HIP(hipMalloc(&m->val32, value_count(m->g) * sizeof(float)));
         }
         launch_to_fp32(m);
         m->val32_valid = true;
@@ -1700,7 +1699,7 @@ void solve_preconditioner(rg_matrix* m, const rg_options& opt, double* x, const 
         RG_CHECK(prec == RG_PRECONDITIONER_JACOBI && m->g.bs == 1 && m->g.dim == 3, RG_VALUE_ERROR,
                  "solvePreconditioner: the multilevel preconditioner needs a scalar Brick matrix and Jacobi");
         if (!m->val32_valid) {
-            if (!m->val32) RG_HIP(hipMalloc(&m->val32, (size_t)m->g.nstore * m->g.ld * sizeof(float)));
+            if (!m->val32) RG_HIP(hipMalloc(&m->val32, value_count(m->g) * sizeof(float)));
             launch_to_fp32(m);
             m->val32_valid = true;
         }

@@ -70,7 +70,10 @@ def matrix_vector_many(op, X):
     return Y
 
 
-SPMM_SHAPES = ((1, 1), (1, 2), (1, 4), (2, 2))   # (rows, columns) per thread
+# (rows, columns) per thread; (16, 64) and (8, 64) are the z-marching LDS
+# tiles (16 or 8 x-nodes per thread, exactly 64 columns, Brick matrices only)
+SPMM_SHAPES = ((1, 1), (1, 2), (1, 4), (2, 2), (16, 64), (8, 64))
+ZMARCH_SHAPES = (4, 5)
 
 
 def autotune_many(op, K, fp32=True, reps=5):

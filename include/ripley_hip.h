@@ -366,12 +366,14 @@ int rg_solve_multi(rg_matrix* m, double* X, const double* B, int nrhs, const rg_
                    rg_diagnostics* diag);
 /* Y = A X for a row-major num_rows x nrhs block (device). */
 int rg_matrix_vector_multi(rg_matrix* m, const double* X, double* Y, int nrhs);
-/* Times reps SpMMs for each tile shape -- (rows, columns) per thread of
- * (1,1), (1,2), (1,4), (2,2); shapes whose column count does not divide
- * nrhs are skipped (time -1) -- keeps the fastest for this matrix and
- * returns its index in *shape; times (host, may be NULL) receives
- * RG_MULTI_SHAPES seconds/SpMM. */
-#define RG_MULTI_SHAPES 4
+/* Times reps SpMMs (with the partial products p.q, as the solver runs
+ * them) for each tile shape -- (rows, columns) per thread of
+ * 0: (1,1), 1: (1,2), 2: (1,4), 3: (2,2), and 4, 5: z-marching tiles of
+ * 16 x 4 and 8 x 4 nodes with their X planes staged in LDS (exactly 64
+ * columns, Brick matrices only); shapes that do not apply are skipped
+ * (time -1) -- keeps the fastest for this matrix and returns its index in
+ * *shape; times (host, may be NULL) receives RG_MULTI_SHAPES seconds/SpMM. */
+#define RG_MULTI_SHAPES 6
 int rg_multi_autotune(rg_matrix* m, int nrhs, int fp32, int reps, int* shape, double* times);
 /* Fixes the SpMM shape (0..RG_MULTI_SHAPES-1 as above) or, with -1,
  * autotunes on the next solve. */

@@ -40,7 +40,9 @@ def _opts(method, tol=1e-9):
     return o
 
 
-@pytest.mark.parametrize("dim,ne", [(3, (9, 8, 7)), (2, (30, 21))])
+# (33, 9, 40): full and partial 16-node x tiles, partial 4-line y tiles and
+# three z segments of the z-marching SpMM
+@pytest.mark.parametrize("dim,ne", [(3, (9, 8, 7)), (3, (33, 9, 40)), (2, (30, 21))])
 @pytest.mark.parametrize("K", [1, 5, 16, 64])
 def test_matrix_vector_many(dim, ne, K):
     op, _, (ptr, idx, val, _), _ = _system(dim, ne, 1)
@@ -48,7 +50,8 @@ def test_matrix_vector_many(dim, ne, K):
     X = rng.standard_normal((op.size, K))
     ref = np.stack([O.spmv(1, ptr, idx, val, X[:, k]) for k in range(K)], axis=1)
     scale = np.stack([O.spmv(1, ptr, idx, np.abs(val), np.abs(X[:, k])) for k in range(K)], axis=1)
-    # every SpMM tile shape whose column count divides K
+    # every SpMM tile shape whose column count divides K (the z-marching
+    # shape falls back to the first one on a Rectangle)
     for shape, (_, cpt) in enumerate(RB.SPMM_SHAPES):
         if K % cpt:
             continue
@@ -100,8 +103,12 @@ def test_solve_many_zero_column_and_errors():
 def test_multi_autotune():
     op, *_ = _system(3, (16, 16, 16), 5)
     best, times = RB.autotune_many(op, 32, fp32=True, reps=3)
-    assert 0 <= best < len(RB.SPMM_SHAPES) and all(t > 0 for t in times)
-    assert times[best] == min(times)
+    # every shape but the 64-column z-marching ones applies to 32 columns
+    ok = [j for j in range(len(times)) if j not in RB.ZMARCH_SHAPES]
+    assert best in ok and all(times[j] > 0 for j in ok) and all(times[j] == -1 for j in RB.ZMARCH_SHAPES)
+    assert times[best] == min(times[j] for j in ok)
+    best, times = RB.autotune_many(op, 64, fp32=True, reps=3)
+    assert all(t > 0 for t in times) and times[best] == min(times)
     # a column count that only the one-column shape divides
     best, times = RB.autotune_many(op, 7, fp32=False, reps=2)
     assert best == 0 and times[0] > 0 and all(t == -1 for t in times[1:])
