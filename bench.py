@@ -723,7 +723,7 @@ def main():
     if "learned_sym" in methods:
         P.attach_network(weights)
     results = {}
-    log(args, "reference solution (FP64 PASO PCG, rtol 1e-12)")
+    log(args, "reference solution (FP64 PASO PCG, rtol 1e-11)")
     xref = reference_solution(P, torch)
     for m in methods:
         log(args, "method", m)
