@@ -85,9 +85,9 @@ class Problem:
         self.op.nullifyRowsAndCols(self.q, self.q, 1.0)
         self.rhs.masked_fill_(self.q > 0, 0.0)
 
-    def attach_network(self, weights, halvings=2):
-        """The learned coarse-level solver for this grid: the network on the
-        level `halvings` times coarser (33^3 nodes for 128^3), weights trained
+    def attach_network(self, weights, halvings=3):
+        """The learned level correction for this grid: the network on the
+        level `halvings` times coarser (17^3 nodes for 128^3), weights trained
         by ripley_hip.learned on Galerkin operators of the same field family."""
         from ripley_hip import learned
         nodes = tuple(n // 2 ** halvings + 1 for n in self.ne)
@@ -116,8 +116,8 @@ class Problem:
         elif method == "paso_ilu0":
             o.setPreconditioner(R.SolverOptions.ILU0)
         if method == "learned_sym":
-            # the multilevel flexible CG with the trained network on its
-            # coarsest level
+            # the multilevel flexible CG with the trained network correcting
+            # its 17^3 level
             self.op.setLearnedPreconditioner(self.net)
             o.setCheckInterval(4)
         elif method == "mg_sym":
