@@ -83,16 +83,38 @@ __global__ __launch_bounds__(256) void k_conv_fwd(Grid3 g, ConvTiles T, const bf
     __shared__ bf16x8 tile[kHalo * C8];
     const int bx = blockIdx.x % T.nbx, by = (blockIdx.x / T.nbx) % T.nby, bz = blockIdx.x / (T.nbx * T.nby);
     const int x0 = bx * kTX, y0 = by * kTY, z0 = bz * kTZ;
-    // stage the input brick: padded coordinates (x0 + hx, y0 + hy, z0 + hz)
+    // stage the input brick: padded coordinates (x0 + hx, y0 + hy, z0 + hz);
+    // the loads of SB passes are issued before their stores, so a
+    // workgroup waits for one load latency per SB passes (up to 31 passes of
+    // 256 x 16 bytes for CIN = 96; on the small grids of the solver, one
+    // workgroup per CU, that chain was most of the kernel's time)
     const int p2 = g.d2 + 2;
-    for (int e = threadIdx.x; e < kHalo * C8; e += 256) {
-        const int v = e / C8, c8 = e % C8;
-        const int hx = v % kHX, hy = (v / kHX) % kHY, hz = v / (kHX * kHY);
-        const int px = x0 + hx, py = y0 + hy, pz = z0 + hz;
-        bf16x8 val = bf16x8{};
-        if (px < g.p0 && py < g.p1 && pz < p2)
-            val = *reinterpret_cast<const bf16x8*>(in + (((int64_t)pz * g.p1 + py) * g.p0 + px) * CIN + c8 * 8);
-        tile[e] = val;
+    constexpr int NE = kHalo * C8;
+    constexpr int NPASS = (NE + 255) / 256;
+    constexpr int SB = 8;
+    auto stage = [&](int pass0, auto UC) {
+        constexpr int UU = decltype(UC)::value;
+        bf16x8 v[UU];
+#pragma unroll
+        for (int u = 0; u < UU; ++u) {
+            const int e = (int)threadIdx.x + 256 * (pass0 + u);
+            const int v8 = e / C8, c8 = e % C8;
+            const int hx = v8 % kHX, hy = (v8 / kHX) % kHY, hz = v8 / (kHX * kHY);
+            const int px = x0 + hx, py = y0 + hy, pz = z0 + hz;
+            v[u] = bf16x8{};
+            if (e < NE && px < g.p0 && py < g.p1 && pz < p2)
+                v[u] = *reinterpret_cast<const bf16x8*>(in + (((int64_t)pz * g.p1 + py) * g.p0 + px) * CIN + c8 * 8);
+        }
+#pragma unroll
+        for (int u = 0; u < UU; ++u) {
+            const int e = (int)threadIdx.x + 256 * (pass0 + u);
+            if (e < NE) tile[e] = v[u];
+        }
+    };
+    {
+        int pass = 0;
+        for (; pass + SB <= NPASS; pass += SB) stage(pass, std::integral_constant<int, SB>{});
+        if constexpr (NPASS % SB != 0) stage(pass, std::integral_constant<int, NPASS % SB>{});
     }
     __syncthreads();
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -104,9 +126,11 @@ __global__ __launch_bounds__(256) void k_conv_fwd(Grid3 g, ConvTiles T, const bf
         for (int j = 0; j < NT; ++j) acc[m][j] = f32x4{0.f, 0.f, 0.f, 0.f};
     // K steps in groups of U: the U weight loads of a group (global memory,
     // L2 resident) are issued together before its MFMAs, so one load latency
-    // is paid per group rather than per step -- on the small grids of the
-    // coarse levels, where a workgroup's K loop is the whole kernel, that
-    // latency chain bounds the time
+    // is paid per group rather than per step, and the A operands (LDS) are
+    // read one step ahead of their MFMAs -- on the small grids of the coarse
+    // levels a workgroup's K loop is the whole kernel, and these latency
+    // chains set its length.  (Loading each group's weights one group ahead
+    // as well measured no faster.)
     constexpr int U = NT <= 2 ? 8 : 4;
     auto steps = [&](int s0, auto UC) {
         constexpr int UU = decltype(UC)::value;
@@ -127,20 +151,37 @@ __global__ __launch_bounds__(256) void k_conv_fwd(Grid3 g, ConvTiles T, const bf
                 if (!gokv[u]) b[u][j] = bf16x8{};
             }
         }
-#pragma unroll
-        for (int u = 0; u < UU; ++u) {
+        auto load_a = [&](int u, bf16x8 (&a)[4]) {
             const int tap = tapv[u];
             const int dx = tap % 3, dy = (tap / 3) % 3, dz = tap / 9;   // offsets + 1
 #pragma unroll
             for (int m = 0; m < 4; ++m) {
                 const int row = 4 * wave + m;                 // (ty, tz) row of the brick
                 const int ty = row % kTY, tz = row / kTY;
-                bf16x8 a = tile[(((tz + dz) * kHY + (ty + dy)) * kHX + (n + dx)) * C8 + cigv[u]];
-                if (!gokv[u]) a = bf16x8{};
+                a[m] = tile[(((tz + dz) * kHY + (ty + dy)) * kHX + (n + dx)) * C8 + cigv[u]];
+                if (!gokv[u]) a[m] = bf16x8{};
+            }
+        };
+        bf16x8 a0[4], a1[4];
+        load_a(0, a0);
+        // scheduling order of the group (the compiler otherwise kept one LDS
+        // read in flight): its UU * NT weight loads (VMEM reads, mask
+        // 0x020), the first step's 4 LDS reads (0x100), then per step the
+        // next step's LDS reads and this step's MFMAs (0x008)
+        __builtin_amdgcn_sched_group_barrier(0x020, UU * NT, 0);
+        __builtin_amdgcn_sched_group_barrier(0x100, 4, 0);
+#pragma unroll
+        for (int u = 0; u < UU; ++u) {
+            bf16x8 (&cur)[4] = (u & 1) ? a1 : a0;
+            bf16x8 (&nxt)[4] = (u & 1) ? a0 : a1;
+            if (u + 1 < UU) load_a(u + 1, nxt);
+#pragma unroll
+            for (int m = 0; m < 4; ++m)
 #pragma unroll
                 for (int j = 0; j < NT; ++j)
-                    acc[m][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[u][j], acc[m][j], 0, 0, 0);
-            }
+                    acc[m][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(cur[m], b[u][j], acc[m][j], 0, 0, 0);
+            if (u + 1 < UU) __builtin_amdgcn_sched_group_barrier(0x100, 4, 0);
+            __builtin_amdgcn_sched_group_barrier(0x008, 4 * NT, 0);
         }
     };
     int s = 0;
