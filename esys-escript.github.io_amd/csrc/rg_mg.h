@@ -9,7 +9,7 @@ namespace rg {
 constexpr int kMaxLevels = 8;     // grid levels of a hierarchy
 constexpr int kChebSteps = 16;    // Chebyshev steps a level's table holds
 constexpr double kChebRatio = 10.0;   // smoothing interval [lmax / ratio, lmax]
-constexpr int kDirs = 4;          // directions the flexible CG keeps
+constexpr int kDirs = 5;          // directions the flexible CG keeps
 
 // (Re)builds the hierarchy of the matrix m (after the FP32 copy of its
 // values): of the solver's balanced matrix, or of an unbalanced one.

@@ -226,7 +226,7 @@ int rg_matrix_save_mm(rg_matrix* m, const char* filename);
  *   RG_METHOD_MINRES      PASO's Solver_MINRES (paso/src/MINRES.cpp:59-191) in
  *                         FP64 for symmetric (possibly indefinite) systems;
  *                         r.M^-1 r < 0 ends the solve with RG_NEGATIVE_NORM.
- *   RG_METHOD_FCG_LEARNED flexible CG keeping the last 4 directions (each
+ *   RG_METHOD_FCG_LEARNED flexible CG keeping the last 5 directions (each
  *                        new one A-orthogonalised against them; FP64
  *                        operator, vectors and scalars) preconditioned by one
  *                        geometric-multigrid V-cycle per iteration: Galerkin
