@@ -197,7 +197,7 @@ struct alignas(sizeof(VT)) SlotRun {
     VT v[TX];
 };
 
-// epi(row, sum, x at the row) for every output row of the workgroup
+// epi(row, sum) for every output row of the workgroup
 template <int kZX, typename VT, class Epi>
 __device__ __forceinline__ void spmm_zmarch(const VT* __restrict__ val, const ZArgs z, const double* __restrict__ X,
                                             Epi&& epi)
@@ -231,9 +231,9 @@ __device__ __forceinline__ void spmm_zmarch(const VT* __restrict__ val, const ZA
                         : double2{0.0, 0.0};
         }
     };
-    double am[kZX], ac[kZX], ap[kZX], pc[kZX];   // output planes zp-1, zp, zp+1; x at plane zp-1
+    double am[kZX], ac[kZX], ap[kZX];   // output planes zp-1, zp, zp+1
 #pragma unroll
-    for (int x = 0; x < kZX; ++x) am[x] = ac[x] = ap[x] = pc[x] = 0.0;
+    for (int x = 0; x < kZX; ++x) am[x] = ac[x] = ap[x] = 0.0;
     // the staged X plane's contributions to output plane zo (dz = zp - zo):
     // the slot's values for the tile's TX nodes in one scalar load (a
     // partial tile reads on past its line, into the next rows or the value
@@ -249,7 +249,7 @@ __device__ __forceinline__ void spmm_zmarch(const VT* __restrict__ val, const ZA
         const int64_t rowb = (zo * z.n1 + y) * z.n0 + x0;
 #pragma unroll
         for (int x = 0; x < kZX; ++x)
-            if (x < nx) epi(rowb + x, A[x], pc[x]);
+            if (x < nx) epi(rowb + x, A[x]);
     };
     const int64_t zfirst = za > 0 ? za - 1 : 0;
     const int64_t zlast = zb < z.n2 ? zb : z.n2 - 1;
@@ -280,8 +280,6 @@ __device__ __forceinline__ void spmm_zmarch(const VT* __restrict__ val, const ZA
                 }
             }
             if (cm) finish(zp - 1, am);   // output plane zp-1 has all its X planes
-#pragma unroll
-            for (int x = 0; x < kZX; ++x) pc[x] = xs[((wv + 1) * kZHX + x + 1) * kZK + c];
         }
 #pragma unroll
         for (int x = 0; x < kZX; ++x) {
@@ -300,9 +298,12 @@ __global__ __launch_bounds__(kBlock) void k_mspmv_dot_z(MArgs a, ZArgs z, const 
 {
     double s[1][1] = {{0.0}};
     const int c = threadIdx.x & 63;
-    spmm_zmarch<TX, VT>(val, z, a.p, [&](int64_t i, double q, double p) {
+    // p at the row from memory again (its plane was staged two planes
+    // earlier, usually still in L2): holding it in registers until the
+    // row's sum is complete cost occupancy
+    spmm_zmarch<TX, VT>(val, z, a.p, [&](int64_t i, double q) {
         a.q[i * kZK + c] = q;
-        s[0][0] = fma(q, p, s[0][0]);
+        s[0][0] = fma(q, a.p[i * kZK + c], s[0][0]);
     });
     publish_column_sums<kZK, 1, 1>(s, a.partials, a.K);
 }
@@ -313,7 +314,7 @@ __global__ __launch_bounds__(kBlock) void k_mspmv_z(ZArgs z, const VT* __restric
                                                     double* __restrict__ Y)
 {
     const int c = threadIdx.x & 63;
-    spmm_zmarch<TX, VT>(val, z, X, [&](int64_t i, double q, double) { Y[i * kZK + c] = q; });
+    spmm_zmarch<TX, VT>(val, z, X, [&](int64_t i, double q) { Y[i * kZK + c] = q; });
 }
 
 // q = A p, partial p.q per column
