@@ -159,17 +159,17 @@ __device__ __forceinline__ void spmm_rows(const VT* __restrict__ val, const MArg
 }
 
 // ------------------------------------------- z-marching SpMM (64 columns)
-// Y = A X for exactly 64 columns on a Brick: a workgroup owns a 16 x 4
-// (x, y) tile of nodes, all 64 columns (lane = column, wave = y line of the
+// Y = A X for exactly 64 columns on a Brick: a workgroup owns a TX x 4
+// (x, y) tile of nodes (TX = 16 or 8), all 64 columns (lane = column, wave = y line of the
 // tile) and a segment of kZSeg z-planes, and marches up z.  X enters LDS
-// one z-plane of the tile at a time (18 x 6 nodes with the halo, 54 KB),
+// one z-plane of the tile at a time (TX + 2 by 6 nodes with the halo),
 // prefetched into registers while the previous plane is in use; each X
 // plane contributes to the three output planes it touches (its dz = +1, 0,
 // -1 slots), whose sums stay in registers, and the lowest of them is then
 // complete and handed to the epilogue.  Each X value is read from L2 about
-// 1.7 times (the halo) instead of once per stencil slot, and the matrix
-// values -- the same for the 64 lanes of a wave -- come through the scalar
-// cache.
+// 1.9 times (the halo) instead of once per stencil slot, and the matrix
+// values -- the same for the 64 lanes of a wave -- are staged in LDS one
+// plane ahead and read back as broadcasts.
 constexpr int kZY = 4, kZK = 64, kZSeg = 16;
 constexpr int kZHY = kZY + 2;
 static_assert(kZY * 64 == kBlock, "z-march tile: one wave per y line");
@@ -190,13 +190,6 @@ inline ZArgs zmarch_args(const Geom& g, int TX)
     return z;
 }
 
-// TX consecutive values of one stencil slot (a uniform address: one wide
-// scalar load)
-template <int TX, typename VT>
-struct alignas(sizeof(VT)) SlotRun {
-    VT v[TX];
-};
-
 // epi(row, sum) for every output row of the workgroup
 template <int kZX, typename VT, class Epi>
 __device__ __forceinline__ void spmm_zmarch(const VT* __restrict__ val, const ZArgs z, const double* __restrict__ X,
@@ -205,7 +198,7 @@ __device__ __forceinline__ void spmm_zmarch(const VT* __restrict__ val, const ZA
     constexpr int kZHX = kZX + 2;
     constexpr int kZRows = kZHX * kZHY;                             // halo rows per plane
     constexpr int kZPer = (kZRows * (kZK / 2) + kBlock - 1) / kBlock;   // 16-byte pieces staged per thread
-    __shared__ double xs[kZRows * kZK];
+    __shared__ __align__(16) double xs[kZRows * kZK];
     const int c = threadIdx.x & 63;
     const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const int b = blockIdx.x;
@@ -231,19 +224,58 @@ __device__ __forceinline__ void spmm_zmarch(const VT* __restrict__ val, const ZA
                         : double2{0.0, 0.0};
         }
     };
+    // the matrix values of the tile's rows, one output plane at a time, in a
+    // ring of three planes in LDS (plane zo in slot zo % 3): staged one X
+    // plane ahead like X (16-byte pieces, the row of a piece from the lane),
+    // read back as wave-uniform (broadcast) LDS reads.  A partial tile's
+    // pieces run on past its line, into the next rows or the value arrays'
+    // slack; those products are never written.
+    constexpr int VPC = 16 / (int)sizeof(VT);              // values per piece
+    constexpr int CPL = kZX / VPC;                         // pieces per line
+    constexpr int NCH = 27 * kZY * CPL;                    // pieces per output plane
+    constexpr int VPER = (NCH + kBlock - 1) / kBlock;
+    static_assert(kZX % VPC == 0, "z-march tile width");
+    __shared__ __align__(16) VT vs[3 * 27 * kZY * kZX];
+    uint4 vpre[VPER];
+    auto fetch_vals = [&](int64_t zo) {
+#pragma unroll
+        for (int k = 0; k < VPER; ++k) {
+            const int e = (int)threadIdx.x + kBlock * k;
+            const int slot = e / (kZY * CPL), rem = e % (kZY * CPL);
+            const int ty = rem / CPL, pc = rem % CPL;
+            const bool ok = e < NCH && zo >= za && zo < zb && y0 + ty < z.n1;
+            uint4 v = uint4{0u, 0u, 0u, 0u};
+            if (ok)
+                __builtin_memcpy(&v, val + (int64_t)slot * z.ld + ((zo * z.n1 + y0 + ty) * z.n0 + x0) + pc * VPC,
+                                 sizeof(v));
+            vpre[k] = v;
+        }
+    };
+    auto store_vals = [&](int64_t zo) {
+        VT* dst = vs + (int)(zo % 3) * (27 * kZY * kZX);
+#pragma unroll
+        for (int k = 0; k < VPER; ++k) {
+            const int e = (int)threadIdx.x + kBlock * k;
+            if (e < NCH) {
+                const int slot = e / (kZY * CPL), rem = e % (kZY * CPL);
+                const int ty = rem / CPL, pc = rem % CPL;
+                *reinterpret_cast<uint4*>(dst + (slot * kZY + ty) * kZX + pc * VPC) = vpre[k];
+            }
+        }
+    };
     double am[kZX], ac[kZX], ap[kZX];   // output planes zp-1, zp, zp+1
 #pragma unroll
     for (int x = 0; x < kZX; ++x) am[x] = ac[x] = ap[x] = 0.0;
-    // the staged X plane's contributions to output plane zo (dz = zp - zo):
-    // the slot's values for the tile's TX nodes in one scalar load (a
-    // partial tile reads on past its line, into the next rows or the value
-    // arrays' slack: those sums are never written)
+    // the staged X plane's contributions to output plane zo (dz = zp - zo)
     auto add_slot = [&](int64_t zo, int dz, int dy, int dx, const double (&w)[kZHX], double (&A)[kZX]) {
-        const int64_t rowb = (zo * z.n1 + y) * z.n0 + x0;
-        const SlotRun<kZX, VT> t = *reinterpret_cast<const SlotRun<kZX, VT>*>(
-            val + (int64_t)((dx + 1) + 3 * (dy + 1) + 9 * (dz + 1)) * z.ld + rowb);
+        const VT* vp = vs + (int)(zo % 3) * (27 * kZY * kZX) +
+                       (((dx + 1) + 3 * (dy + 1) + 9 * (dz + 1)) * kZY + wv) * kZX;
+        VT t[kZX];
 #pragma unroll
-        for (int x = 0; x < kZX; ++x) A[x] = fma((double)t.v[x], w[x + 1 + dx], A[x]);
+        for (int k = 0; k < kZX / VPC; ++k)
+            *reinterpret_cast<uint4*>(t + k * VPC) = *reinterpret_cast<const uint4*>(vp + k * VPC);
+#pragma unroll
+        for (int x = 0; x < kZX; ++x) A[x] = fma((double)t[x], w[x + 1 + dx], A[x]);
     };
     auto finish = [&](int64_t zo, const double (&A)[kZX]) {
         const int64_t rowb = (zo * z.n1 + y) * z.n0 + x0;
@@ -253,6 +285,11 @@ __device__ __forceinline__ void spmm_zmarch(const VT* __restrict__ val, const ZA
     };
     const int64_t zfirst = za > 0 ? za - 1 : 0;
     const int64_t zlast = zb < z.n2 ? zb : z.n2 - 1;
+    // the first two output planes' values, then X plane zfirst
+    fetch_vals(zfirst);
+    store_vals(zfirst);
+    fetch_vals(zfirst + 1);
+    store_vals(zfirst + 1);
     fetch(zfirst);
     for (int64_t zp = zfirst; zp <= zlast; ++zp) {
         __syncthreads();   // every wave is done with the previous plane
@@ -261,13 +298,17 @@ __device__ __forceinline__ void spmm_zmarch(const VT* __restrict__ val, const ZA
             const int r = (int)(threadIdx.x >> 5) + (kBlock / 32) * k;
             if (r < kZRows) *reinterpret_cast<double2*>(xs + r * kZK + 2 * ch) = pre[k];
         }
+        // plane zp+1's values (fetched during the previous step) replace
+        // plane zp-2's, finished then
+        if (zp > zfirst) store_vals(zp + 1);
         __syncthreads();
-        if (zp < zlast) fetch(zp + 1);   // in flight while this plane is used
+        if (zp < zlast) {   // in flight while this plane is used
+            fetch(zp + 1);
+            fetch_vals(zp + 2);
+        }
         if (yok) {
             const bool cm = zp - 1 >= za, cc = zp >= za && zp < zb, cp = zp + 1 < zb;
-            // one y offset at a time: its 9 slot runs are the scalar loads
-            // in flight (all 27 at once would spill scalar registers)
-#pragma unroll 1
+#pragma unroll
             for (int dy = -1; dy <= 1; ++dy) {
                 double w[kZHX];
 #pragma unroll
