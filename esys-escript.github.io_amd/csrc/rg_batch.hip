@@ -349,6 +349,25 @@ __global__ __launch_bounds__(kBlock) void k_mspmv_dot_z(MArgs a, ZArgs z, const 
     publish_column_sums<kZK, 1, 1>(s, a.partials, a.K);
 }
 
+// r = b - A x (FP64 values) with the partials of r.D^-1 r and r.r per
+// column (64 columns, z-marching)
+template <int TX>
+__global__ __launch_bounds__(kBlock) void k_mreplace_z(MArgs a, ZArgs z, const double* __restrict__ val)
+{
+    double s[2][1] = {{0.0}, {0.0}};
+    const int c = threadIdx.x & 63;
+    const bool live = !a.st[c].done;
+    spmm_zmarch<TX, double>(val, z, a.x, [&](int64_t i, double ax) {
+        if (!live) return;
+        const int64_t e = i * kZK + c;
+        const double rv = a.b[e] - ax;
+        a.r[e] = rv;
+        s[0][0] += rv * a.dinv[i] * rv;
+        s[1][0] += rv * rv;
+    });
+    publish_column_sums<kZK, 1, 2>(s, a.partials, a.K);
+}
+
 // Y = A X (64 columns, z-marching)
 template <int TX, typename VT>
 __global__ __launch_bounds__(kBlock) void k_mspmv_z(ZArgs z, const VT* __restrict__ val, const double* __restrict__ X,
@@ -1024,12 +1043,19 @@ struct MultiSolver {
         if (last) stage<ST_CG_UPDATE, 2>(s);
         else stage<ST_CGX_UPDATE, 2>(s);
         if (last) {
-            dispatch_shape(K, rshape, [&](auto KPC, auto RC, auto CC) {
-                constexpr int P = KPC.value, Rv = RC.value, Cv = CC.value;
-                if (g.dim == 3) hipLaunchKernelGGL((k_mreplace<3, P, Rv, Cv>), dim3(L.grid), dim3(kBlock), 0, s, a, m->val);
-                else hipLaunchKernelGGL((k_mreplace<2, P, Rv, Cv>), dim3(L.grid), dim3(kBlock), 0, s, a, m->val);
-            });
-            stage<ST_CG_REPLACE, 2>(s);
+            if (zm) {
+                by_tx([&](auto TX) {
+                    hipLaunchKernelGGL((k_mreplace_z<TX.value>), dim3(Gs), dim3(kBlock), 0, s, a, z, m->val);
+                });
+                stage<ST_CG_REPLACE, 2>(s, Gs);
+            } else {
+                dispatch_shape(K, rshape, [&](auto KPC, auto RC, auto CC) {
+                    constexpr int P = KPC.value, Rv = RC.value, Cv = CC.value;
+                    if (g.dim == 3) hipLaunchKernelGGL((k_mreplace<3, P, Rv, Cv>), dim3(L.grid), dim3(kBlock), 0, s, a, m->val);
+                    else hipLaunchKernelGGL((k_mreplace<2, P, Rv, Cv>), dim3(L.grid), dim3(kBlock), 0, s, a, m->val);
+                });
+                stage<ST_CG_REPLACE, 2>(s);
+            }
         }
         dispatch_kp(K, [&](auto KPC) {
             hipLaunchKernelGGL((k_mpupdate<KPC.value>), dim3(L.grid), dim3(kBlock), 0, s, a);
