@@ -263,13 +263,38 @@ __global__ __launch_bounds__(kBlock) void k_cheb_step(const VT* __restrict__ val
     if (guard_done(guard)) return;
     const double c1 = coef[0], c2 = coef[1];
     const RowRange rr = block_rows(N, chunk);
-    stencil_block<3, 1, SYM, VT>(val, ld, N, s1, s2, rr.r0, rr.r1, xin, [&](int64_t i, const double (&acc)[1]) {
+    auto row = [&](int64_t i, const double (&acc)[1]) {
         const double r = b[i] - acc[0];
         // c1 = 0 (a first step): the previous update is not read
         const double dn = c2 * dinv[i] * r + (c1 != 0.0 ? c1 * d[i] : 0.0);
         d[i] = dn;
         xout[i] = xin[i] + dn;
-    });
+    };
+    constexpr int R = spmv_rows_per_thread<VT>();
+    // the vector streams R rows per access where every one is 16-byte
+    // aligned at the run starts (multiples of R rows)
+    const bool runs = ((reinterpret_cast<uintptr_t>(b) | reinterpret_cast<uintptr_t>(dinv) |
+                        reinterpret_cast<uintptr_t>(xin) | reinterpret_cast<uintptr_t>(xout) |
+                        reinterpret_cast<uintptr_t>(d)) & 15) == 0;
+    if (runs) {
+        stencil_block_runs<3, SYM, VT>(val, ld, N, s1, s2, rr.r0, rr.r1, xin, [&](int64_t i0, const double (&acc)[R]) {
+            double bv[R], dv[R], xv[R], dn[R], xo[R];
+            load_run<R>(b + i0, bv);
+            load_run<R>(dinv + i0, dv);
+            load_run<R>(xin + i0, xv);
+            double dp[R];
+            if (c1 != 0.0) load_run<R>(d + i0, dp);
+#pragma unroll
+            for (int k = 0; k < R; ++k) {
+                dn[k] = c2 * dv[k] * (bv[k] - acc[k]) + (c1 != 0.0 ? c1 * dp[k] : 0.0);
+                xo[k] = xv[k] + dn[k];
+            }
+            store_run<R>(d + i0, dn);
+            store_run<R>(xout + i0, xo);
+        }, row);
+    } else {
+        stencil_block<3, 1, SYM, VT>(val, ld, N, s1, s2, rr.r0, rr.r1, xin, row);
+    }
 }
 
 // r = b - A x
@@ -281,8 +306,19 @@ __global__ __launch_bounds__(kBlock) void k_residual(const VT* __restrict__ val,
 {
     if (guard_done(guard)) return;
     const RowRange rr = block_rows(N, chunk);
-    stencil_block<3, 1, SYM, VT>(val, ld, N, s1, s2, rr.r0, rr.r1, x,
-                                 [&](int64_t i, const double (&acc)[1]) { r[i] = b[i] - acc[0]; });
+    auto row = [&](int64_t i, const double (&acc)[1]) { r[i] = b[i] - acc[0]; };
+    constexpr int R = spmv_rows_per_thread<VT>();
+    if (((reinterpret_cast<uintptr_t>(b) | reinterpret_cast<uintptr_t>(r)) & 15) == 0) {
+        stencil_block_runs<3, SYM, VT>(val, ld, N, s1, s2, rr.r0, rr.r1, x, [&](int64_t i0, const double (&acc)[R]) {
+            double bv[R], rv[R];
+            load_run<R>(b + i0, bv);
+#pragma unroll
+            for (int k = 0; k < R; ++k) rv[k] = bv[k] - acc[k];
+            store_run<R>(r + i0, rv);
+        }, row);
+    } else {
+        stencil_block<3, 1, SYM, VT>(val, ld, N, s1, s2, rr.r0, rr.r1, x, row);
+    }
 }
 
 // b_c = P^T (r / bal): one coarse node per thread over its 27 fine nodes.

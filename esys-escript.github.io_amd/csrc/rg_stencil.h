@@ -324,4 +324,54 @@ __device__ __forceinline__ void stencil_block(const VT* __restrict__ val, int64_
     }
 }
 
+// As stencil_block for scalar rows (bs = 1), handing the vectorised path's
+// R consecutive rows to epi_run(i0, acc[R]) at once (the epilogue can then
+// move its vector streams R rows per access) and the other rows one by one
+// to epi_row(i, acc[1]).
+template <int DIM, bool SYM, typename VT, class EpiRun, class EpiRow>
+__device__ __forceinline__ void stencil_block_runs(const VT* __restrict__ val, int64_t ld, int64_t N, int64_t s1,
+                                                   int64_t s2, int64_t r0, int64_t r1, const double* __restrict__ x,
+                                                   EpiRun&& epi_run, EpiRow&& epi_row)
+{
+    constexpr int R = spmv_rows_per_thread<VT>();
+    const int64_t reach = s2 * (DIM == 3) + s1 + R + 2;
+    const bool fast = ((reinterpret_cast<uintptr_t>(x) & 7) == 0) && (r0 % R == 0) && ((r1 - r0) % R == 0) &&
+                      (r0 >= reach) && (r1 + reach <= N);
+    if (fast) {
+        for (int64_t i0 = r0 + (int64_t)threadIdx.x * R; i0 < r1; i0 += (int64_t)kBlock * R) {
+            double acc[R];
+            if (SYM) stencil_rows_s1_sym<DIM, VT, R>(val, ld, s1, s2, i0, x, acc);
+            else stencil_rows_s1<DIM, VT, R>(val, ld, s1, s2, i0, x, acc);
+            epi_run(i0, acc);
+        }
+    } else {
+        const bool interior = (r0 >= reach) && (r1 + reach <= N);
+        for (int64_t i = r0 + threadIdx.x; i < r1; i += kBlock) {
+            double acc[1] = {0.0};
+            if (interior) stencil_row<DIM, 1, SYM, false>(val, ld, N, s1, s2, i, x, acc);
+            else stencil_row<DIM, 1, SYM, true>(val, ld, N, s1, s2, i, x, acc);
+            epi_row(i, acc);
+        }
+    }
+}
+
+// R consecutive doubles at p (p 16-byte aligned, R even) in 16-byte accesses
+template <int R>
+__device__ __forceinline__ void load_run(const double* __restrict__ p, double (&v)[R])
+{
+    static_assert(R % 2 == 0, "runs of an even number of rows");
+#pragma unroll
+    for (int k = 0; k < R / 2; ++k) {
+        const double2 t = *reinterpret_cast<const double2*>(p + 2 * k);
+        v[2 * k] = t.x;
+        v[2 * k + 1] = t.y;
+    }
+}
+template <int R>
+__device__ __forceinline__ void store_run(double* __restrict__ p, const double (&v)[R])
+{
+#pragma unroll
+    for (int k = 0; k < R / 2; ++k) *reinterpret_cast<double2*>(p + 2 * k) = double2{v[2 * k], v[2 * k + 1]};
+}
+
 }  // namespace rg
