@@ -297,6 +297,80 @@ __global__ __launch_bounds__(kBlock) void k_cheb_step(const VT* __restrict__ val
     }
 }
 
+// All `steps` Chebyshev steps of a small level (N <= kSmallNodes, one node
+// per thread of one workgroup): the thread keeps its row's 27 values and
+// neighbour indices in registers, x (double-buffered) and the previous
+// update d live in LDS, and a barrier separates the steps (each reads its
+// neighbours' x of the previous step).  The same recurrence as
+// k_cheb_first + k_cheb_step (FROM_ZERO: the first step from x = 0 with
+// coef[1]; otherwise from xin with step 0's c1 = 0), one launch instead of
+// `steps` on the coarsest levels (9^3, 7^3 nodes), where each launch costs
+// more than its work.  (With several nodes per thread and the values
+// re-read from L2 each step, a 17^3 level ran slower this way than as
+// separate launches: each step then waits on those loads.)
+constexpr int kSmallNodes = 1024;
+
+template <bool SYM, bool FROM_ZERO>
+__global__ __launch_bounds__(kSmallNodes) void k_cheb_small(const float* __restrict__ val, MgGeom a,
+                                                            const double* __restrict__ dinv,
+                                                            const double* __restrict__ b,
+                                                            const double* __restrict__ xin, double* __restrict__ xout,
+                                                            double* __restrict__ dout, const double* __restrict__ coef,
+                                                            int steps, const KrylovState* guard)
+{
+    if (guard_done(guard)) return;
+    __shared__ double xs[2][kSmallNodes];
+    const int N = (int)a.N;
+    const int i = threadIdx.x;
+    const bool own = i < N;
+    // this row's values and neighbours (value 0 and neighbour i outside the grid)
+    float v[27];
+    int nb[27];
+    {
+        const int n0 = (int)a.n0, n1 = (int)a.n1, n2 = (int)a.n2, ld = (int)a.ld;
+        const int x = i % n0, y = (i / n0) % n1, z = i / (n0 * n1);
+#pragma unroll
+        for (int t = 0; t < 27; ++t) {
+            const int jx = x + t % 3 - 1, jy = y + (t / 3) % 3 - 1, jz = z + t / 9 - 1;
+            const bool ok = own && jx >= 0 && jx < n0 && jy >= 0 && jy < n1 && jz >= 0 && jz < n2;
+            const int j = ok ? jx + n0 * (jy + n1 * jz) : 0;
+            int k;
+            if (!SYM) k = t * ld + i;
+            else if (t >= C3) k = (t - C3) * ld + i;
+            else k = (C3 - t) * ld + j;
+            v[t] = ok ? val[k] : 0.f;
+            nb[t] = j;
+        }
+    }
+    const double bi = own ? b[i] : 0.0, di = own ? dinv[i] : 0.0;
+    double d = 0.0;
+    int k0 = 0;
+    if (FROM_ZERO) {
+        d = coef[1] * di * bi;
+        k0 = 1;
+    }
+    if (own) xs[0][i] = FROM_ZERO ? d : xin[i];
+    __syncthreads();
+    int cur = 0;
+    for (int k = k0; k < steps; ++k) {
+        const double c1 = coef[2 * k], c2 = coef[2 * k + 1];
+        if (own) {
+            double acc = 0.0;
+#pragma unroll
+            for (int t = 0; t < 27; ++t) acc = fma((double)v[t], xs[cur][nb[t]], acc);
+            // c1 = 0 (a first step): the previous update is not read
+            d = c2 * di * (bi - acc) + (c1 != 0.0 ? c1 * d : 0.0);
+            xs[cur ^ 1][i] = xs[cur][i] + d;
+        }
+        cur ^= 1;
+        __syncthreads();   // the step's x is complete before the next reads it
+    }
+    if (own) {
+        xout[i] = xs[cur][i];
+        dout[i] = d;
+    }
+}
+
 // r = b - A x
 template <bool SYM, typename VT>
 __global__ __launch_bounds__(kBlock) void k_residual(const VT* __restrict__ val, int64_t ld, int64_t N, int64_t s1,
@@ -856,12 +930,34 @@ void by_sym_vt(bool sym, F&& f)
 // Chebyshev smoothing of level l: `steps` steps from x (x_from_zero: from 0,
 // b only).  Returns the buffer holding the result (xa or xb of the level, or
 // `out` for the last step when out is not null).
+// steps Chebyshev steps of a level with at most kSmallNodes nodes in one
+// launch (k_cheb_small), from zero (x null) or from x, into dst.
+void cheb_small(rg_mg_state* s, const rg_matrix* A, int l, const double* b, const double* x, double* dst, int steps,
+                hipStream_t st, const KrylovState* guard)
+{
+    MgLevel& L = s->lev[l];
+    by_sym_vt(A->symmetric, [&](auto S) {
+        if (x)
+            hipLaunchKernelGGL((k_cheb_small<S.value, false>), dim3(1), dim3(kSmallNodes), 0, st, A->val32, L.g,
+                               L.dinv, b, x, dst, L.d, L.coef, steps, guard);
+        else
+            hipLaunchKernelGGL((k_cheb_small<S.value, true>), dim3(1), dim3(kSmallNodes), 0, st, A->val32, L.g,
+                               L.dinv, b, nullptr, dst, L.d, L.coef, steps, guard);
+    });
+    RG_HIP(hipGetLastError());
+}
+
 double* smooth(rg_mg_state* s, const rg_matrix* Am, int l, const double* b, double* x, bool from_zero, int steps,
                double* out, hipStream_t st, const KrylovState* guard)
 {
     MgLevel& L = s->lev[l];
     const rg_matrix* A = L.A ? L.A : Am;
     const Geom& g = A->g;
+    if (g.N <= kSmallNodes && steps >= 2) {
+        double* dst = out ? out : (x == L.xa ? L.xb : L.xa);
+        cheb_small(s, A, l, b, from_zero ? nullptr : x, dst, steps, st, guard);
+        return dst;
+    }
     const Launch Ls = chunked_launch(g.N, spmv_rows_per_thread<float>());
     double* cur = x;
     int k = 0;
@@ -908,7 +1004,11 @@ void net_correct(rg_mg_state* s, rg_matrix* m, int l, const double* b, double* x
     });
     lpc_forward_balanced(m->lpc, L.r, L.dinv, nullptr, s->logk, s->y, st);
     double* yf = s->y;
-    for (int k = 0; k < std::min(m->mg_coarse_steps, kChebSteps); ++k) {
+    const int fsteps = std::min(m->mg_coarse_steps, kChebSteps);
+    if (A->g.N <= kSmallNodes && fsteps >= 2) {
+        cheb_small(s, A, l, s->zero, s->y, s->y2, fsteps, st, guard);
+        yf = s->y2;
+    } else for (int k = 0; k < fsteps; ++k) {
         double* dst = (yf == s->y) ? s->y2 : s->y;
         by_sym_vt(A->symmetric, [&](auto S) {
             hipLaunchKernelGGL((k_cheb_step<S.value, float>), dim3(Ls.grid), dim3(kBlock), 0, st, A->val32, A->g.ld,
@@ -989,7 +1089,11 @@ void mg_vcycle(rg_matrix* m, const double* r, double* z, hipStream_t st, const K
         const Geom& g = A->g;
         const Launch Ls = chunked_launch(g.N, spmv_rows_per_thread<float>());
         double* cur = xl[l];
-        for (int k = 0; k < deg; ++k) {
+        if (g.N <= kSmallNodes && deg >= 2) {
+            double* dst = (l == 0) ? z : (cur == L.xa ? L.xb : L.xa);
+            cheb_small(s, A, l, bl[l], cur, dst, deg, st, guard);
+            cur = dst;
+        } else for (int k = 0; k < deg; ++k) {
             double* dst = (l == 0 && k == deg - 1) ? z : (cur == L.xa ? L.xb : L.xa);
             by_sym_vt(A->symmetric, [&](auto S) {
                 hipLaunchKernelGGL((k_cheb_step<S.value, float>), dim3(Ls.grid), dim3(kBlock), 0, st, A->val32, g.ld,
