@@ -10,10 +10,11 @@
 //
 // Both solve the unbalanced system from x = 0 with the textbook Jacobi-PCG
 // (PCG.cpp:62-368 without PASO's residual smoothing), the SpMV in FP64 or
-// FP32 values (RG_METHOD_PCG_PLAIN / RG_METHOD_PCG_MIXED), all vectors,
-// dots and updates in FP64, and an FP64 residual replacement r = b - A x at
-// the end of every chunk; a column converges only on that FP64 residual:
-// ||b - A x||_2 <= tol ||b||_2 per column.  Tile shapes (rows per thread of
+// FP32 values (RG_METHOD_PCG_PLAIN / RG_METHOD_PCG_MIXED), the vectors,
+// dots and updates in FP64 (with FP32 values on the z-marching SpMM the
+// directions p and q = A p in FP32), and an FP64 residual replacement
+// r = b - A x at the end of every chunk; a column converges only on that
+// FP64 residual: ||b - A x||_2 <= tol ||b||_2 per column.  Tile shapes (rows per thread of
 // the SpMM, rows per tile of the batch) are picked by timing candidates on
 // the actual operator (autotune).
 #include <hip/hip_runtime.h>
@@ -22,6 +23,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 #include "rg_device.h"
@@ -49,11 +51,24 @@ struct MArgs {
     double* r;
     double* p;
     double* q;
+    float* pf;               // p and q in FP32 (the FP32-value z-marching path)
+    float* qf;
+    double* p2;              // the other direction block: the previous direction, and the
+    float* pf2;              // target of the next one
     KrylovState* st;         // K states
     double* partials;        // [quantity][K][G]
     int64_t N, ld, s1, s2, chunk;
     int K;
 };
+
+// the direction vector p or q = A p of element type PT (FP64, or FP32 on the
+// FP32-value z-marching path)
+template <class PT>
+__device__ __forceinline__ PT* dir_vec(double* d, float* f)
+{
+    if constexpr (std::is_same_v<PT, float>) return f;
+    else return d;
+}
 
 // Per-column block sums of NV values, thread t holding CPT consecutive
 // columns k = (t % KPC) * CPT + c in v[q][c]; partials[(q*K + k)*gridDim.x +
@@ -190,15 +205,18 @@ inline ZArgs zmarch_args(const Geom& g, int TX)
     return z;
 }
 
-// epi(row, sum) for every output row of the workgroup
-template <int kZX, typename VT, class Epi>
-__device__ __forceinline__ void spmm_zmarch(const VT* __restrict__ val, const ZArgs z, const double* __restrict__ X,
+// epi(row, sum) for every output row of the workgroup; X holds FP64 or
+// FP32 entries (XT), the sums are FP64
+template <int kZX, typename VT, typename XT, class Epi>
+__device__ __forceinline__ void spmm_zmarch(const VT* __restrict__ val, const ZArgs z, const XT* __restrict__ X,
                                             Epi&& epi)
 {
     constexpr int kZHX = kZX + 2;
-    constexpr int kZRows = kZHX * kZHY;                             // halo rows per plane
-    constexpr int kZPer = (kZRows * (kZK / 2) + kBlock - 1) / kBlock;   // 16-byte pieces staged per thread
-    __shared__ __align__(16) double xs[kZRows * kZK];
+    constexpr int kZRows = kZHX * kZHY;                  // halo rows per plane
+    constexpr int XPC = 16 / (int)sizeof(XT);            // X entries per 16-byte piece
+    constexpr int TPR = kZK / XPC;                       // staging threads per halo row
+    constexpr int kZPer = (kZRows * TPR + kBlock - 1) / kBlock;   // pieces staged per thread
+    __shared__ __align__(16) XT xs[kZRows * kZK];
     const int c = threadIdx.x & 63;
     const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const int b = blockIdx.x;
@@ -208,20 +226,20 @@ __device__ __forceinline__ void spmm_zmarch(const VT* __restrict__ val, const ZA
     const bool yok = y < z.n1;
     const int64_t za = (int64_t)bs * kZSeg, zb = min(za + (int64_t)kZSeg, z.n2);
     const int nx = (int)min((int64_t)kZX, z.n0 - x0);
-    // staging: 32 threads per halo row, 16 bytes (2 columns) each; the row
-    // of a piece depends on the lane, so its address arithmetic stays in
+    // staging: TPR threads per halo row, 16 bytes (XPC columns) each; the
+    // row of a piece depends on the lane, so its address arithmetic stays in
     // vector registers
-    double2 pre[kZPer];
-    const int ch = threadIdx.x & 31;
+    uint4 pre[kZPer];
+    const int ch = threadIdx.x % TPR;
     auto fetch = [&](int64_t zp) {
 #pragma unroll
         for (int k = 0; k < kZPer; ++k) {
-            const int r = (int)(threadIdx.x >> 5) + (kBlock / 32) * k;   // halo row
+            const int r = (int)(threadIdx.x / TPR) + (kBlock / TPR) * k;   // halo row
             const int hy = r / kZHX, hx = r % kZHX;
             const int64_t gx = x0 - 1 + hx, gy = y0 - 1 + hy;
             const bool ok = r < kZRows && gx >= 0 && gx < z.n0 && gy >= 0 && gy < z.n1;
-            pre[k] = ok ? *reinterpret_cast<const double2*>(X + ((zp * z.n1 + gy) * z.n0 + gx) * kZK + 2 * ch)
-                        : double2{0.0, 0.0};
+            pre[k] = ok ? *reinterpret_cast<const uint4*>(X + ((zp * z.n1 + gy) * z.n0 + gx) * kZK + XPC * ch)
+                        : uint4{0u, 0u, 0u, 0u};
         }
     };
     // the matrix values of the tile's rows, one output plane at a time, in a
@@ -295,8 +313,8 @@ __device__ __forceinline__ void spmm_zmarch(const VT* __restrict__ val, const ZA
         __syncthreads();   // every wave is done with the previous plane
 #pragma unroll
         for (int k = 0; k < kZPer; ++k) {
-            const int r = (int)(threadIdx.x >> 5) + (kBlock / 32) * k;
-            if (r < kZRows) *reinterpret_cast<double2*>(xs + r * kZK + 2 * ch) = pre[k];
+            const int r = (int)(threadIdx.x / TPR) + (kBlock / TPR) * k;
+            if (r < kZRows) *reinterpret_cast<uint4*>(xs + r * kZK + XPC * ch) = pre[k];
         }
         // plane zp+1's values (fetched during the previous step) replace
         // plane zp-2's, finished then
@@ -312,7 +330,7 @@ __device__ __forceinline__ void spmm_zmarch(const VT* __restrict__ val, const ZA
             for (int dy = -1; dy <= 1; ++dy) {
                 double w[kZHX];
 #pragma unroll
-                for (int hx = 0; hx < kZHX; ++hx) w[hx] = xs[((wv + 1 + dy) * kZHX + hx) * kZK + c];
+                for (int hx = 0; hx < kZHX; ++hx) w[hx] = (double)xs[((wv + 1 + dy) * kZHX + hx) * kZK + c];
 #pragma unroll
                 for (int dx = -1; dx <= 1; ++dx) {
                     if (cm) add_slot(zp - 1, 1, dy, dx, w, am);
@@ -333,18 +351,277 @@ __device__ __forceinline__ void spmm_zmarch(const VT* __restrict__ val, const ZA
     if (yok && zlast == z.n2 - 1 && zlast >= za && zlast < zb) finish(zlast, am);
 }
 
-// q = A p and the partial p.q per column (64 columns, z-marching)
-template <int TX, typename VT>
+// ------------------------------- z-marching SpMM, one x node per lane
+// The tile and the march of spmm_zmarch with an 8-node x run, but lane =
+// g + 8 xl: column group g (columns 8g .. 8g+7) of the node at x offset xl.
+// A lane reads its node's matrix value once per slot (eight distinct LDS
+// addresses per wave, the values staged in LDS as FP64 so that none is
+// converted per use) and the X entries of its 8 columns at each (dx, dy)
+// as 16-byte LDS reads; it keeps 3 x 8 sums (output planes zp-1, zp, zp+1).
+constexpr int kZN = 8;   // x nodes of the tile
+constexpr int kZG = 8;   // columns per lane
+
+// epi(row, g, sums[8]) for every output row of the workgroup
+template <typename VT, typename XT, class Epi>
+__device__ __forceinline__ void spmm_zmarch_n(const VT* __restrict__ val, const ZArgs z, const XT* __restrict__ X,
+                                              Epi&& epi)
+{
+    constexpr int kZX = kZN;
+    constexpr int kZHX = kZX + 2;
+    constexpr int kZRows = kZHX * kZHY;
+    constexpr int XPC = 16 / (int)sizeof(XT);
+    constexpr int TPR = kZK / XPC;
+    constexpr int kZPer = (kZRows * TPR + kBlock - 1) / kBlock;
+    __shared__ __align__(16) XT xs[kZRows * kZK];
+    const int lane = threadIdx.x & 63;
+    const int g = lane % kZG, xl = lane / kZG;
+    const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int b = blockIdx.x;
+    const int bx = b % z.nbx, by = (b / z.nbx) % z.nby, bs = b / (z.nbx * z.nby);
+    const int64_t x0 = (int64_t)bx * kZX, y0 = (int64_t)by * kZY;
+    const int64_t y = y0 + wv;
+    const bool yok = y < z.n1;
+    const int64_t za = (int64_t)bs * kZSeg, zb = min(za + (int64_t)kZSeg, z.n2);
+    const int nx = (int)min((int64_t)kZX, z.n0 - x0);
+    uint4 pre[kZPer];
+    const int ch = threadIdx.x % TPR;
+    auto fetch = [&](int64_t zp) {
+#pragma unroll
+        for (int k = 0; k < kZPer; ++k) {
+            const int r = (int)(threadIdx.x / TPR) + (kBlock / TPR) * k;   // halo row
+            const int hy = r / kZHX, hx = r % kZHX;
+            const int64_t gx = x0 - 1 + hx, gy = y0 - 1 + hy;
+            const bool ok = r < kZRows && gx >= 0 && gx < z.n0 && gy >= 0 && gy < z.n1;
+            pre[k] = ok ? *reinterpret_cast<const uint4*>(X + ((zp * z.n1 + gy) * z.n0 + gx) * kZK + XPC * ch)
+                        : uint4{0u, 0u, 0u, 0u};
+        }
+    };
+    // matrix values: a ring of three output planes in LDS as FP64, staged
+    // one X plane ahead (16-byte pieces of the value arrays; a partial
+    // tile's pieces run on past its line into the next rows or the arrays'
+    // slack, and those products are never written)
+    constexpr int VPC = 16 / (int)sizeof(VT);
+    constexpr int CPL = kZX / VPC;
+    constexpr int NCH = 27 * kZY * CPL;
+    constexpr int VPER = (NCH + kBlock - 1) / kBlock;
+    constexpr int kPlaneV = 27 * kZY * kZX;
+    __shared__ __align__(16) double vs[3 * kPlaneV];
+    uint4 vpre[VPER];
+    auto fetch_vals = [&](int64_t zo) {
+#pragma unroll
+        for (int k = 0; k < VPER; ++k) {
+            const int e = (int)threadIdx.x + kBlock * k;
+            const int slot = e / (kZY * CPL), rem = e % (kZY * CPL);
+            const int ty = rem / CPL, pc = rem % CPL;
+            const bool ok = e < NCH && zo >= za && zo < zb && y0 + ty < z.n1;
+            uint4 v = uint4{0u, 0u, 0u, 0u};
+            if (ok)
+                __builtin_memcpy(&v, val + (int64_t)slot * z.ld + ((zo * z.n1 + y0 + ty) * z.n0 + x0) + pc * VPC,
+                                 sizeof(v));
+            vpre[k] = v;
+        }
+    };
+    auto store_vals = [&](int64_t zo) {
+        double* dst = vs + (int)(zo % 3) * kPlaneV;
+#pragma unroll
+        for (int k = 0; k < VPER; ++k) {
+            const int e = (int)threadIdx.x + kBlock * k;
+            if (e < NCH) {
+                const int slot = e / (kZY * CPL), rem = e % (kZY * CPL);
+                const int ty = rem / CPL, pc = rem % CPL;
+                VT t[VPC];
+                __builtin_memcpy(t, &vpre[k], sizeof(t));
+                double* d = dst + (slot * kZY + ty) * kZX + pc * VPC;
+#pragma unroll
+                for (int j = 0; j < VPC; j += 2) *reinterpret_cast<double2*>(d + j) = double2{(double)t[j], (double)t[j + 1]};
+            }
+        }
+    };
+    double am[kZG], ac[kZG], ap[kZG];   // output planes zp-1, zp, zp+1
+#pragma unroll
+    for (int c = 0; c < kZG; ++c) am[c] = ac[c] = ap[c] = 0.0;
+    auto finish = [&](int64_t zo, const double (&A)[kZG]) {
+        if (xl < nx) epi((zo * z.n1 + y) * z.n0 + x0 + xl, g, A);
+    };
+    const int64_t zfirst = za > 0 ? za - 1 : 0;
+    const int64_t zlast = zb < z.n2 ? zb : z.n2 - 1;
+    fetch_vals(zfirst);
+    store_vals(zfirst);
+    fetch_vals(zfirst + 1);
+    store_vals(zfirst + 1);
+    fetch(zfirst);
+    for (int64_t zp = zfirst; zp <= zlast; ++zp) {
+        __syncthreads();   // every wave is done with the previous plane
+#pragma unroll
+        for (int k = 0; k < kZPer; ++k) {
+            const int r = (int)(threadIdx.x / TPR) + (kBlock / TPR) * k;
+            if (r < kZRows) *reinterpret_cast<uint4*>(xs + r * kZK + XPC * ch) = pre[k];
+        }
+        if (zp > zfirst) store_vals(zp + 1);
+        __syncthreads();
+        if (zp < zlast) {   // in flight while this plane is used
+            fetch(zp + 1);
+            fetch_vals(zp + 2);
+        }
+        if (yok) {
+            const bool cm = zp - 1 >= za, cc = zp >= za && zp < zb, cp = zp + 1 < zb;
+            const double* vm = vs + (int)((zp + 2) % 3) * kPlaneV + wv * kZX + xl;   // plane zp-1
+            const double* vc = vs + (int)(zp % 3) * kPlaneV + wv * kZX + xl;
+            const double* vp = vs + (int)((zp + 1) % 3) * kPlaneV + wv * kZX + xl;
+#pragma unroll
+            for (int dy = -1; dy <= 1; ++dy) {
+#pragma unroll
+                for (int dx = -1; dx <= 1; ++dx) {
+                    XT wt[kZG];
+                    const XT* src = xs + ((wv + 1 + dy) * kZHX + (xl + 1 + dx)) * kZK + kZG * g;
+#pragma unroll
+                    for (int k = 0; k < kZG / XPC; ++k)
+                        *reinterpret_cast<uint4*>(wt + k * XPC) = *reinterpret_cast<const uint4*>(src + k * XPC);
+                    double w[kZG];
+#pragma unroll
+                    for (int c = 0; c < kZG; ++c) w[c] = (double)wt[c];
+                    const int s0 = (dx + 1) + 3 * (dy + 1);   // slot of dz = -1
+                    if (cm) {   // X plane zp is output plane zp-1's dz = +1 neighbour
+                        const double v = vm[(s0 + 18) * kZY * kZX];
+#pragma unroll
+                        for (int c = 0; c < kZG; ++c) am[c] = fma(v, w[c], am[c]);
+                    }
+                    if (cc) {
+                        const double v = vc[(s0 + 9) * kZY * kZX];
+#pragma unroll
+                        for (int c = 0; c < kZG; ++c) ac[c] = fma(v, w[c], ac[c]);
+                    }
+                    if (cp) {
+                        const double v = vp[s0 * kZY * kZX];
+#pragma unroll
+                        for (int c = 0; c < kZG; ++c) ap[c] = fma(v, w[c], ap[c]);
+                    }
+                }
+            }
+            if (cm) finish(zp - 1, am);
+        }
+#pragma unroll
+        for (int c = 0; c < kZG; ++c) {
+            am[c] = ac[c];
+            ac[c] = ap[c];
+            ap[c] = 0.0;
+        }
+    }
+    if (yok && zlast == z.n2 - 1 && zlast >= za && zlast < zb) finish(zlast, am);
+}
+
+// Per-column block sums of NV values held as v[q][c] for column kZG g + c
+// by lane g + kZG xl of every wave; partials[(q*K + k)*gridDim.x +
+// blockIdx.x] in a fixed order (64 columns)
+template <int NV>
+__device__ __forceinline__ void publish_group_sums(double (&v)[NV][kZG], double* partials, int K)
+{
+    __shared__ double red[NV][kBlock / 64][kZK];
+#pragma unroll
+    for (int q = 0; q < NV; ++q)
+#pragma unroll
+        for (int c = 0; c < kZG; ++c)
+#pragma unroll
+            for (int o = kZG; o < 64; o <<= 1) v[q][c] += __shfl_xor(v[q][c], o, 64);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (lane < kZG)
+#pragma unroll
+        for (int q = 0; q < NV; ++q)
+#pragma unroll
+            for (int c = 0; c < kZG; ++c) red[q][wv][lane * kZG + c] = v[q][c];
+    __syncthreads();
+    if ((int)threadIdx.x < K) {
+#pragma unroll
+        for (int q = 0; q < NV; ++q) {
+            double s = 0.0;
+#pragma unroll
+            for (int w = 0; w < kBlock / 64; ++w) s += red[q][w][threadIdx.x];
+            partials[((int64_t)q * K + threadIdx.x) * gridDim.x + blockIdx.x] = s;
+        }
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ void load8(const T* p, T (&t)[kZG])
+{
+#pragma unroll
+    for (int k = 0; k < kZG * (int)sizeof(T) / 16; ++k)
+        *reinterpret_cast<uint4*>(t + k * 16 / sizeof(T)) = *reinterpret_cast<const uint4*>(p + k * 16 / sizeof(T));
+}
+template <typename T>
+__device__ __forceinline__ void store8(T* p, const T (&t)[kZG])
+{
+#pragma unroll
+    for (int k = 0; k < kZG * (int)sizeof(T) / 16; ++k)
+        *reinterpret_cast<uint4*>(p + k * 16 / sizeof(T)) = *reinterpret_cast<const uint4*>(t + k * 16 / sizeof(T));
+}
+
+// q = A p and the partial p.q per column, one x node per lane
+template <typename VT, typename PT>
+__global__ __launch_bounds__(kBlock) void k_mspmv_dot_zn(MArgs a, ZArgs z, const VT* __restrict__ val)
+{
+    double s[1][kZG] = {};
+    const PT* P = dir_vec<PT>(a.p, a.pf);
+    PT* Q = dir_vec<PT>(a.q, a.qf);
+    spmm_zmarch_n<VT>(val, z, P, [&](int64_t i, int g, const double (&A)[kZG]) {
+        PT qs[kZG], pv[kZG];
+#pragma unroll
+        for (int c = 0; c < kZG; ++c) qs[c] = (PT)A[c];
+        store8(Q + i * kZK + kZG * g, qs);
+        load8(P + i * kZK + kZG * g, pv);
+#pragma unroll
+        for (int c = 0; c < kZG; ++c) s[0][c] = fma((double)qs[c], (double)pv[c], s[0][c]);
+    });
+    publish_group_sums<1>(s, a.partials, a.K);
+}
+
+// r = b - A x (FP64 values), partials of r.D^-1 r and r.r, one x node per lane
+__global__ __launch_bounds__(kBlock) void k_mreplace_zn(MArgs a, ZArgs z, const double* __restrict__ val)
+{
+    double s[2][kZG] = {};
+    const int g = (threadIdx.x & 63) % kZG;
+    bool live[kZG];
+#pragma unroll
+    for (int c = 0; c < kZG; ++c) live[c] = !a.st[kZG * g + c].done;
+    spmm_zmarch_n<double>(val, z, a.x, [&](int64_t i, int gg, const double (&A)[kZG]) {
+        const double di = a.dinv[i];
+#pragma unroll
+        for (int c = 0; c < kZG; ++c) {
+            if (!live[c]) continue;
+            const int64_t e = i * kZK + kZG * gg + c;
+            const double rv = a.b[e] - A[c];
+            a.r[e] = rv;
+            s[0][c] += rv * di * rv;
+            s[1][c] += rv * rv;
+        }
+    });
+    publish_group_sums<2>(s, a.partials, a.K);
+}
+
+// Y = A X, one x node per lane
+template <typename VT>
+__global__ __launch_bounds__(kBlock) void k_mspmv_zn(ZArgs z, const VT* __restrict__ val, const double* __restrict__ X,
+                                                     double* __restrict__ Y)
+{
+    spmm_zmarch_n<VT>(val, z, X, [&](int64_t i, int g, const double (&A)[kZG]) { store8(Y + i * kZK + kZG * g, A); });
+}
+
+// q = A p and the partial p.q per column (64 columns, z-marching); with
+// PT = float q is rounded to FP32 and the dot taken of the stored values
+template <int TX, typename VT, typename PT>
 __global__ __launch_bounds__(kBlock) void k_mspmv_dot_z(MArgs a, ZArgs z, const VT* __restrict__ val)
 {
     double s[1][1] = {{0.0}};
     const int c = threadIdx.x & 63;
+    const PT* P = dir_vec<PT>(a.p, a.pf);
+    PT* Q = dir_vec<PT>(a.q, a.qf);
     // p at the row from memory again (its plane was staged two planes
     // earlier, usually still in L2): holding it in registers until the
     // row's sum is complete cost occupancy
-    spmm_zmarch<TX, VT>(val, z, a.p, [&](int64_t i, double q) {
-        a.q[i * kZK + c] = q;
-        s[0][0] = fma(q, a.p[i * kZK + c], s[0][0]);
+    spmm_zmarch<TX, VT>(val, z, P, [&](int64_t i, double q) {
+        const PT qs = (PT)q;
+        Q[i * kZK + c] = qs;
+        s[0][0] = fma((double)qs, (double)P[i * kZK + c], s[0][0]);
     });
     publish_column_sums<kZK, 1, 1>(s, a.partials, a.K);
 }
@@ -434,38 +711,47 @@ __device__ __forceinline__ void column_loop(const MArgs& a, F&& f)
 }
 
 // r = b, p = D^-1 r, partial r.D^-1 r and r.r (x = 0)
-template <int KP>
+template <int KP, typename PT>
 __global__ __launch_bounds__(kBlock) void k_minit(MArgs a)
 {
     double s[2][1] = {};
+    PT* P = dir_vec<PT>(a.p, a.pf);
     column_loop<KP>(a, [&](int64_t i, int k) {
         const int64_t e = i * a.K + k;
         const double rv = a.b[e];
         const double z = a.dinv[i] * rv;
         a.r[e] = rv;
         a.x[e] = 0.0;
-        a.p[e] = z;
+        P[e] = (PT)z;
         s[0][0] += rv * z;
         s[1][0] += rv * rv;
     });
     publish_column_sums<KP, 1, 2>(s, a.partials, a.K);
 }
 
-// r -= alpha q (and x += alpha p with WITHX: the last iteration of a chunk,
-// before the residual replacement reads x), partial r.D^-1 r and r.r per
-// column
-template <int KP, bool WITHX>
+// r -= alpha q, partial r.D^-1 r and r.r per column.  With WITHX (the
+// last iteration of a chunk, before the residual replacement reads x) also
+// x += alpha_prev p_prev (pending from the previous iteration, also for a
+// column that ended there) + alpha p.
+template <int KP, bool WITHX, typename PT>
 __global__ __launch_bounds__(kBlock) void k_mupdate(MArgs a)
 {
     double s[2][1] = {};
     const int k = threadIdx.x % KP;
-    const bool live = k < a.K && !a.st[k < a.K ? k : 0].done;
-    const double alpha = live ? a.st[k].alpha : 0.0;
+    const int kc = k < a.K ? k : 0;
+    const bool live = k < a.K && !a.st[kc].done;
+    const bool xprev = WITHX && k < a.K && a.st[kc].early_prev;
+    const double alpha = live ? a.st[kc].alpha : 0.0;
+    const double alpha_prev = xprev ? a.st[kc].alpha_prev : 0.0;
+    const PT* P = dir_vec<PT>(a.p, a.pf);
+    const PT* P2 = dir_vec<PT>(a.p2, a.pf2);
+    const PT* Q = dir_vec<PT>(a.q, a.qf);
     column_loop<KP>(a, [&](int64_t i, int kk) {
-        if (!live) return;
+        if (!live && !xprev) return;
         const int64_t e = i * a.K + kk;
-        if (WITHX) a.x[e] = fma(alpha, a.p[e], a.x[e]);
-        const double rv = fma(-alpha, a.q[e], a.r[e]);
+        if (WITHX) a.x[e] = fma(alpha_prev, (double)P2[e], fma(alpha, (double)P[e], a.x[e]));
+        if (!live) return;
+        const double rv = fma(-alpha, (double)Q[e], a.r[e]);
         a.r[e] = rv;
         s[0][0] += rv * a.dinv[i] * rv;
         s[1][0] += rv * rv;
@@ -473,25 +759,29 @@ __global__ __launch_bounds__(kBlock) void k_mupdate(MArgs a)
     publish_column_sums<KP, 1, 2>(s, a.partials, a.K);
 }
 
-// x += alpha p where this iteration's x update is pending (the stage after
-// k_mupdate<KP, false> flags it, also when that stage ended the column),
-// then p = D^-1 r + beta p for the running columns: x and p are read once
-// for both updates
-template <int KP>
+// p_next = D^-1 r + beta p into the other direction block (a.p2) for the
+// running columns.  x is updated every second iteration: with ODD also x +=
+// alpha_prev p_prev + alpha p where pending (p_prev is read from the block
+// that p_next then overwrites), so x and p are read once for two updates.
+template <int KP, typename PT, bool ODD>
 __global__ __launch_bounds__(kBlock) void k_mpupdate(MArgs a)
 {
     const int k = threadIdx.x % KP;
     const int kc = k < a.K ? k : 0;
     const bool live = k < a.K && !a.st[kc].done;
-    const bool xpend = k < a.K && a.st[kc].early;
+    const bool xcur = ODD && k < a.K && a.st[kc].early;
+    const bool xprev = ODD && k < a.K && a.st[kc].early_prev;
     const double beta = live ? a.st[kc].beta : 0.0;
-    const double alpha = xpend ? a.st[kc].alpha : 0.0;
+    const double alpha = xcur ? a.st[kc].alpha : 0.0;
+    const double alpha_prev = xprev ? a.st[kc].alpha_prev : 0.0;
+    const PT* P = dir_vec<PT>(a.p, a.pf);
+    PT* P2 = dir_vec<PT>(a.p2, a.pf2);
     column_loop<KP>(a, [&](int64_t i, int kk) {
-        if (!live && !xpend) return;
+        if (!live && !xcur && !xprev) return;
         const int64_t e = i * a.K + kk;
-        const double pv = a.p[e];
-        if (xpend) a.x[e] = fma(alpha, pv, a.x[e]);
-        if (live) a.p[e] = fma(beta, pv, a.dinv[i] * a.r[e]);
+        const double pv = (double)P[e];
+        if (ODD && (xcur || xprev)) a.x[e] = fma(alpha_prev, (double)P2[e], fma(alpha, pv, a.x[e]));
+        if (live) P2[e] = (PT)fma(beta, pv, a.dinv[i] * a.r[e]);
     });
 }
 
@@ -870,10 +1160,11 @@ void dispatch_kp(int K, F&& f)
 // run of 16 or 8 nodes per thread).  Longer row runs per thread (R = 4, 8:
 // R + 2 x windows per plane for 3 R products) spilled with the planes
 // unrolled and ran 2.7-3.1 ms with them rolled, against 1.9 ms.
-constexpr int kNumShapes = 6;
-constexpr int kZShape = 4, kZShapeEnd = 6;
-constexpr int kShapeR[kNumShapes] = {1, 1, 1, 2, 16, 8};
-constexpr int kShapeC[kNumShapes] = {1, 2, 4, 2, kZK, kZK};
+constexpr int kNumShapes = 7;
+constexpr int kZShape = 4, kZShapeEnd = 7;
+constexpr int kZNodeShape = 6;   // z-marching, one x node per lane
+constexpr int kShapeR[kNumShapes] = {1, 1, 1, 2, 16, 8, kZN};
+constexpr int kShapeC[kNumShapes] = {1, 2, 4, 2, kZK, kZK, kZK};
 
 inline bool is_zshape(int shape) { return shape >= kZShape && shape < kZShapeEnd; }
 
@@ -921,6 +1212,11 @@ struct MultiSolver {
     int rshape;     // shape of the row-chunked SpMM kernels (the FP64 residual replacement)
     bool fp32;
     bool zm;        // z-marching SpMM
+    bool pf32;      // p and q = A p stored in FP32 (FP32 values on the z-marching SpMM)
+    double* pd[2] = {nullptr, nullptr};   // the two direction blocks (FP64 or, with pf32, FP32)
+    float* pfl[2] = {nullptr, nullptr};
+    int cur = 0;    // direction block of the current iteration
+    bool odd = false;   // parity of the current iteration (x is updated on odd ones)
     hipStream_t st;
     MArgs a;
     Launch L;       // row-chunked SpMM / vector kernels
@@ -931,6 +1227,7 @@ struct MultiSolver {
     {
         KP = pow2_at_least(K);
         zm = is_zshape(shape);
+        pf32 = zm && fp32;
         RG_CHECK(shape_applies(shape, K, m->g.dim), RG_VALUE_ERROR,
                  "SpMM shape does not apply to this column count or dimension");
         rshape = zm ? 2 : shape;
@@ -958,14 +1255,47 @@ struct MultiSolver {
         else f(IC<8>{});
     }
 
+    // the z-marching kernels of this shape
+    void z_spmm(hipStream_t s, const double* X, double* Y)
+    {
+        if (shape == kZNodeShape) {
+            if (fp32) hipLaunchKernelGGL((k_mspmv_zn<float>), dim3(Gs), dim3(kBlock), 0, s, z, m->val32, X, Y);
+            else hipLaunchKernelGGL((k_mspmv_zn<double>), dim3(Gs), dim3(kBlock), 0, s, z, m->val, X, Y);
+            return;
+        }
+        by_tx([&](auto TX) {
+            if (fp32) hipLaunchKernelGGL((k_mspmv_z<TX.value, float>), dim3(Gs), dim3(kBlock), 0, s, z, m->val32, X, Y);
+            else hipLaunchKernelGGL((k_mspmv_z<TX.value, double>), dim3(Gs), dim3(kBlock), 0, s, z, m->val, X, Y);
+        });
+    }
+    void z_spmm_dot(hipStream_t s)
+    {
+        if (shape == kZNodeShape) {
+            if (fp32) hipLaunchKernelGGL((k_mspmv_dot_zn<float, float>), dim3(Gs), dim3(kBlock), 0, s, a, z, m->val32);
+            else hipLaunchKernelGGL((k_mspmv_dot_zn<double, double>), dim3(Gs), dim3(kBlock), 0, s, a, z, m->val);
+            return;
+        }
+        by_tx([&](auto TX) {
+            if (fp32) hipLaunchKernelGGL((k_mspmv_dot_z<TX.value, float, float>), dim3(Gs), dim3(kBlock), 0, s, a, z, m->val32);
+            else hipLaunchKernelGGL((k_mspmv_dot_z<TX.value, double, double>), dim3(Gs), dim3(kBlock), 0, s, a, z, m->val);
+        });
+    }
+    void z_replace(hipStream_t s)
+    {
+        if (shape == kZNodeShape) {
+            hipLaunchKernelGGL(k_mreplace_zn, dim3(Gs), dim3(kBlock), 0, s, a, z, m->val);
+            return;
+        }
+        by_tx([&](auto TX) {
+            hipLaunchKernelGGL((k_mreplace_z<TX.value>), dim3(Gs), dim3(kBlock), 0, s, a, z, m->val);
+        });
+    }
+
     void spmm(const double* X, double* Y)
     {
         const Geom& g = m->g;
         if (zm) {
-            by_tx([&](auto TX) {
-                if (fp32) hipLaunchKernelGGL((k_mspmv_z<TX.value, float>), dim3(Gs), dim3(kBlock), 0, st, z, m->val32, X, Y);
-                else hipLaunchKernelGGL((k_mspmv_z<TX.value, double>), dim3(Gs), dim3(kBlock), 0, st, z, m->val, X, Y);
-            });
+            z_spmm(st, X, Y);
             RG_HIP(hipGetLastError());
             return;
         }
@@ -982,6 +1312,15 @@ struct MultiSolver {
         RG_HIP(hipGetLastError());
     }
 
+    // f(IC<KP>, PT{}) for the vector kernels: KP lanes per row, direction
+    // vectors of type PT
+    template <class F>
+    void by_kp(F&& f)
+    {
+        if (pf32) f(IC<kZK>{}, float{});
+        else dispatch_kp(K, [&](auto KPC) { f(KPC, double{}); });
+    }
+
     template <int STAGE, int NV>
     void stage(hipStream_t s, int G = -1)
     {
@@ -991,22 +1330,35 @@ struct MultiSolver {
 
     void init(hipStream_t s)
     {
-        dispatch_kp(K, [&](auto KPC) {
-            hipLaunchKernelGGL((k_minit<KPC.value>), dim3(L.grid), dim3(kBlock), 0, s, a);
+        by_kp([&](auto KPC, auto pt) {
+            hipLaunchKernelGGL((k_minit<KPC.value, decltype(pt)>), dim3(L.grid), dim3(kBlock), 0, s, a);
         });
         stage<ST_BATCH_INIT, 2>(s);
         RG_HIP(hipGetLastError());
     }
 
+    // One iteration.  The x update is deferred into the direction update of
+    // every second (odd) iteration, which adds alpha_prev p_prev + alpha p
+    // and so reads and writes x once per two iterations; the direction
+    // blocks alternate (p in block cur, the next direction into cur ^ 1,
+    // over the previous one).  A chunk starts on an even iteration and
+    // ends on an odd one, whose x update goes into the r update (the
+    // residual replacement reads x).  ST_CGX_SPMV clears the pending flag
+    // of every column, ended ones included; ST_CGX2_SPMV moves it to the
+    // previous one.
     void iteration(hipStream_t s, bool last)
     {
+        RG_CHECK(!last || odd, RG_VALUE_ERROR, "solve_multi: a chunk must hold an even number of iterations");
+        a.p = pd[cur];
+        a.pf = pfl[cur];
+        a.p2 = pd[cur ^ 1];
+        a.pf2 = pfl[cur ^ 1];
         spmm_dot(s);
-        // the x update of an iteration is deferred into the direction update
-        // (which reads p anyway) except on the last iteration of a chunk,
-        // whose residual replacement reads x; ST_CGX_SPMV clears the pending
-        // flag of every column, ended ones included
-        stage<ST_CGX_SPMV, 1>(s, Gs);
+        if (odd) stage<ST_CGX2_SPMV, 1>(s, Gs);
+        else stage<ST_CGX_SPMV, 1>(s, Gs);
         rest_of_iteration(s, last);
+        cur ^= 1;
+        odd = !odd;
     }
 
     // q = A p with the partials of p.q (a.p, a.q, a.partials)
@@ -1014,10 +1366,7 @@ struct MultiSolver {
     {
         const Geom& g = m->g;
         if (zm) {
-            by_tx([&](auto TX) {
-                if (fp32) hipLaunchKernelGGL((k_mspmv_dot_z<TX.value, float>), dim3(Gs), dim3(kBlock), 0, s, a, z, m->val32);
-                else hipLaunchKernelGGL((k_mspmv_dot_z<TX.value, double>), dim3(Gs), dim3(kBlock), 0, s, a, z, m->val);
-            });
+            z_spmm_dot(s);
         } else {
             dispatch_shape(K, shape, [&](auto KPC, auto RC, auto CC) {
                 constexpr int P = KPC.value, Rv = RC.value, Cv = CC.value;
@@ -1036,17 +1385,16 @@ struct MultiSolver {
     void rest_of_iteration(hipStream_t s, bool last)
     {
         const Geom& g = m->g;
-        dispatch_kp(K, [&](auto KPC) {
-            if (last) hipLaunchKernelGGL((k_mupdate<KPC.value, true>), dim3(L.grid), dim3(kBlock), 0, s, a);
-            else hipLaunchKernelGGL((k_mupdate<KPC.value, false>), dim3(L.grid), dim3(kBlock), 0, s, a);
+        by_kp([&](auto KPC, auto pt) {
+            using PT = decltype(pt);
+            if (last) hipLaunchKernelGGL((k_mupdate<KPC.value, true, PT>), dim3(L.grid), dim3(kBlock), 0, s, a);
+            else hipLaunchKernelGGL((k_mupdate<KPC.value, false, PT>), dim3(L.grid), dim3(kBlock), 0, s, a);
         });
         if (last) stage<ST_CG_UPDATE, 2>(s);
         else stage<ST_CGX_UPDATE, 2>(s);
         if (last) {
             if (zm) {
-                by_tx([&](auto TX) {
-                    hipLaunchKernelGGL((k_mreplace_z<TX.value>), dim3(Gs), dim3(kBlock), 0, s, a, z, m->val);
-                });
+                z_replace(s);
                 stage<ST_CG_REPLACE, 2>(s, Gs);
             } else {
                 dispatch_shape(K, rshape, [&](auto KPC, auto RC, auto CC) {
@@ -1057,8 +1405,10 @@ struct MultiSolver {
                 stage<ST_CG_REPLACE, 2>(s);
             }
         }
-        dispatch_kp(K, [&](auto KPC) {
-            hipLaunchKernelGGL((k_mpupdate<KPC.value>), dim3(L.grid), dim3(kBlock), 0, s, a);
+        by_kp([&](auto KPC, auto pt) {
+            using PT = decltype(pt);
+            if (odd && !last) hipLaunchKernelGGL((k_mpupdate<KPC.value, PT, true>), dim3(L.grid), dim3(kBlock), 0, s, a);
+            else hipLaunchKernelGGL((k_mpupdate<KPC.value, PT, false>), dim3(L.grid), dim3(kBlock), 0, s, a);
         });
         RG_HIP(hipGetLastError());
     }
@@ -1082,6 +1432,10 @@ int autotune_multi(rg_matrix* m, int K, bool fp32, const double* X, double* Y, i
         part.alloc((size_t)K * S.max_grid());
         S.a.p = const_cast<double*>(X);
         S.a.q = Y;
+        // timing only: with FP32 directions the first half of X's and Y's
+        // bytes serve as the FP32 blocks
+        S.a.pf = reinterpret_cast<float*>(const_cast<double*>(X));
+        S.a.qf = reinterpret_cast<float*>(Y);
         S.a.partials = part.p;
         S.spmm_dot(st);   // warm-up
         RG_HIP(hipEventRecord(e0, st));
@@ -1116,25 +1470,37 @@ int solve_multi(rg_matrix* m, double* X, const double* B, int K, const rg_option
     hipStream_t st = m->ctx->stream;
     const size_t nv = (size_t)g.N * K;
     DevArray<double> dinv, r, p, q, part;
+    DevArray<float> pf, qf;
     DevArray<KrylovState> sts;
     dinv.alloc(g.ld);
     r.alloc(nv);
-    p.alloc(nv);
-    q.alloc(nv);
     sts.alloc(K);
     prepare_matrix(m, fp32, dinv.p);
     int shape = m->multi_shape - 1;
     if (shape < 0 || !shape_applies(shape, K, g.dim)) {
-        shape = autotune_multi(m, K, fp32, B, q.p, 5, nullptr);
+        shape = autotune_multi(m, K, fp32, B, r.p, 5, nullptr);
         m->multi_shape = shape + 1;
     }
     MultiSolver S(m, K, shape, fp32);
+    DevArray<double> p2;
+    DevArray<float> pf2;
+    if (S.pf32) {
+        pf.alloc(nv);
+        pf2.alloc(nv);
+        qf.alloc(nv);
+    } else {
+        p.alloc(nv);
+        p2.alloc(nv);
+        q.alloc(nv);
+    }
+    S.pd[0] = p.p; S.pd[1] = p2.p; S.pfl[0] = pf.p; S.pfl[1] = pf2.p;
     part.alloc((size_t)2 * K * S.max_grid());
-    S.a.dinv = dinv.p; S.a.b = B; S.a.x = X; S.a.r = r.p; S.a.p = p.p; S.a.q = q.p;
+    S.a.dinv = dinv.p; S.a.b = B; S.a.x = X; S.a.r = r.p; S.a.p = p.p; S.a.q = q.p; S.a.pf = pf.p; S.a.qf = qf.p;
     S.a.st = sts.p; S.a.partials = part.p;
     init_states(st, sts.p, K, opt.tolerance, opt.iter_max);
-    S.init(st);
-    const int check = opt.check_interval > 0 ? opt.check_interval : 32;
+    S.init(st);   // p into block 0 (S.cur)
+    int check = opt.check_interval > 0 ? opt.check_interval : 32;
+    check += check & 1;   // chunks of whole iteration pairs (the x update spans two)
     ChunkRunner run;
     run.stream = st;
     std::vector<KrylovState> fin;
@@ -1444,7 +1810,7 @@ int rg_multi_set_shape(rg_matrix* m, int shape)
     return batch_guard([&] {
         RG_CHECK(m, RG_VALUE_ERROR, "multi_set_shape: NULL matrix");
         RG_CHECK(shape >= -1 && shape < rg::kNumShapes, RG_VALUE_ERROR,
-                 "multi_set_shape: shape must be -1 (autotune) or 0..5");
+                 "multi_set_shape: shape must be -1 (autotune) or 0..6");
         m->multi_shape = shape + 1;
         return RG_OK;
     });

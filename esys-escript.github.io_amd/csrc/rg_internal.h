@@ -77,7 +77,8 @@ inline size_t value_count(const Geom& g) { return (size_t)g.nstore * g.bs * g.bs
 // updated by the last workgroup of each reducing kernel.
 struct KrylovState {
     double tau, tau_new, delta, alpha, beta, gamma1, norm, tol;
-    double rho, rnorm0, rho_prev, pad1;
+    double rho, rnorm0, rho_prev;
+    double alpha_prev;      // many-RHS PCG: alpha of the previous iteration (its x update pending)
     int32_t num_iter, maxit, done, status;
     int32_t replace_mode;   // 1: convergence is only declared on a replaced (true) residual
     int32_t early;          // BiCGStab: converged on ||s|| this iteration (x += alpha phat pending);
@@ -85,7 +86,8 @@ struct KrylovState {
                             // PCG with the deferred x update: x += alpha p pending
     int32_t pend;           // textbook PCG with the folded update stage: the update kernel's
                             // totals (pend_rz, pend_rr) await their commit
-    int32_t pad;
+    int32_t early_prev;     // many-RHS PCG: x += alpha_prev p_prev pending (x is updated every
+                            // second iteration)
     // MINRES (paso/src/MINRES.cpp:67-69): the last two Givens rotations, the
     // coefficients of the w recurrence, c * eta for the x update and the
     // scaling of the preconditioned residual norm

@@ -32,7 +32,7 @@ enum Stage {
     ST_BATCH_INIT,
     ST_BICG_RHO, ST_BICG_ALPHA, ST_BICG_S, ST_BICG_XA, ST_BICG_OMEGA, ST_BICG_END,
     ST_MR_INIT, ST_MR_DELTA, ST_MR_GAMMA,
-    ST_CGX_SPMV, ST_CGX_UPDATE, ST_CG_COMMIT
+    ST_CGX_SPMV, ST_CGX_UPDATE, ST_CG_COMMIT, ST_CGX2_SPMV
 };
 
 // Stages that start a solve run even when the state says done.
@@ -41,7 +41,7 @@ __device__ __forceinline__ bool stage_starts()
 {
     return STAGE == ST_PASO_INIT || STAGE == ST_CG_INIT || STAGE == ST_FCG_INIT || STAGE == ST_BATCH_INIT ||
            STAGE == ST_BICG_XA || STAGE == ST_MR_INIT || STAGE == ST_MR_DELTA || STAGE == ST_CGX_SPMV ||
-           STAGE == ST_CG_COMMIT;
+           STAGE == ST_CG_COMMIT || STAGE == ST_CGX2_SPMV;
 }
 
 // The update stage of the textbook PCG when k_p_update<.., FOLD> has
@@ -113,12 +113,19 @@ __device__ __forceinline__ void stage_apply(KrylovState* st, const double (&tot)
         if (st->norm <= st->tol) { st->done = 1; st->status = KS_CONVERGED; }
         else if (st->maxit <= 0) { st->done = 1; st->status = KS_MAXITER; }
         else if (fabs(tot[0]) <= 0.0) { st->done = 1; st->status = KS_BREAKDOWN; }
-    } else if constexpr (STAGE == ST_CG_SPMV || STAGE == ST_CGX_SPMV) {
+    } else if constexpr (STAGE == ST_CG_SPMV || STAGE == ST_CGX_SPMV || STAGE == ST_CGX2_SPMV) {
         // the deferred-x variant also retires the previous iteration's
-        // pending x update, and so runs after the solve has ended too
+        // pending x update, and so runs after the solve has ended too; the
+        // paired variant (x updated every second iteration) keeps it pending
+        // as the previous one instead
         if (STAGE == ST_CGX_SPMV) {
             st->early = 0;
             cg_commit(st);
+        }
+        if (STAGE == ST_CGX2_SPMV) {
+            st->early_prev = st->early;
+            st->alpha_prev = st->alpha;
+            st->early = 0;
         }
         if (!st->done) {
             st->delta = tot[0];

@@ -70,10 +70,12 @@ def matrix_vector_many(op, X):
     return Y
 
 
-# (rows, columns) per thread; (16, 64) and (8, 64) are the z-marching LDS
-# tiles (16 or 8 x-nodes per thread, exactly 64 columns, Brick matrices only)
-SPMM_SHAPES = ((1, 1), (1, 2), (1, 4), (2, 2), (16, 64), (8, 64))
-ZMARCH_SHAPES = (4, 5)
+# (rows, columns) per thread of the row-chunked SpMM tiles 0-3, the column
+# entry dividing the column count; 4-6 are the z-marching LDS tiles, which
+# need exactly 64 columns (Brick matrices only): (16, 64) and (8, 64) give a
+# thread 16 or 8 x-nodes of one column, (1, 64) one x-node of 8 columns
+SPMM_SHAPES = ((1, 1), (1, 2), (1, 4), (2, 2), (16, 64), (8, 64), (1, 64))
+ZMARCH_SHAPES = (4, 5, 6)
 
 
 def autotune_many(op, K, fp32=True, reps=5):

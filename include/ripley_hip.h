@@ -361,7 +361,11 @@ int rg_matrix_coloring(rg_matrix* m, int32_t* colours, int* num_colors);
 /* Many right-hand sides on one matrix: X, B device, row-major
  * num_rows x nrhs (column index fastest), 1 <= nrhs <= 64.  The SpMM reads
  * the matrix once per iteration for all columns; its tile shape is
- * autotuned on the first call (rg_multi_autotune sets it explicitly). */
+ * autotuned on the first call (rg_multi_autotune sets it explicitly).
+ * With RG_METHOD_PCG_MIXED on a z-marching shape (4-6 below) the search
+ * direction p and q = A p are stored in FP32 (r, x, the dot products and
+ * the updates stay FP64).  x is updated every second iteration (two
+ * directions at once), so check_interval is rounded up to an even count. */
 int rg_solve_multi(rg_matrix* m, double* X, const double* B, int nrhs, const rg_options* opt,
                    rg_diagnostics* diag);
 /* Y = A X for a row-major num_rows x nrhs block (device). */
@@ -369,11 +373,13 @@ int rg_matrix_vector_multi(rg_matrix* m, const double* X, double* Y, int nrhs);
 /* Times reps SpMMs (with the partial products p.q, as the solver runs
  * them) for each tile shape -- (rows, columns) per thread of
  * 0: (1,1), 1: (1,2), 2: (1,4), 3: (2,2), and 4, 5: z-marching tiles of
- * 16 x 4 and 8 x 4 nodes with their X planes staged in LDS (exactly 64
- * columns, Brick matrices only); shapes that do not apply are skipped
+ * 16 x 4 and 8 x 4 nodes with their X planes staged in LDS, one column per
+ * lane; 6: the 8 x 4 z-marching tile with one node and 8 columns per lane
+ * (4-6: exactly 64 columns, Brick matrices only); shapes that do not apply
+ * are skipped
  * (time -1) -- keeps the fastest for this matrix and returns its index in
  * *shape; times (host, may be NULL) receives RG_MULTI_SHAPES seconds/SpMM. */
-#define RG_MULTI_SHAPES 6
+#define RG_MULTI_SHAPES 7
 int rg_multi_autotune(rg_matrix* m, int nrhs, int fp32, int reps, int* shape, double* times);
 /* Fixes the SpMM shape (0..RG_MULTI_SHAPES-1 as above) or, with -1,
  * autotunes on the next solve. */

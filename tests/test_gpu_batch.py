@@ -83,6 +83,51 @@ def test_solve_many_matches_single_solves(method, K, shape):
     assert np.linalg.norm(X[:, K - 1] - xo) <= 1e-6 * np.linalg.norm(xo)
 
 
+def _pcg_iterate(ptr, idx, val, b, m):
+    """The m-th iterate of textbook Jacobi-PCG from x = 0 (CSR, numpy)."""
+    import scipy.sparse as sp
+    A = sp.csr_matrix((val, idx, ptr), shape=(len(ptr) - 1, len(ptr) - 1))
+    dinv = 1.0 / A.diagonal()
+    x = np.zeros_like(b)
+    r = b.copy()
+    z = dinv * r
+    p = z.copy()
+    rho = r @ z
+    for _ in range(m):
+        q = A @ p
+        alpha = rho / (p @ q)
+        x += alpha * p
+        r -= alpha * q
+        z = dinv * r
+        rho, rho_old = r @ z, rho
+        p = z + (rho / rho_old) * p
+    return x
+
+
+# x is updated every second iteration (two directions at once) and a chunk
+# ends on the second of a pair: stopping after an odd or even number of
+# iterations, inside a chunk or at its end, must leave x at that iterate
+@pytest.mark.parametrize("shape", [0, 6])
+@pytest.mark.parametrize("m", [7, 10, 16, 17])
+def test_solve_many_iteration_limit_leaves_that_iterate(shape, m):
+    print("solve_many iteration limit", shape, m, flush=True)
+    op, rhs, (ptr, idx, val, orhs), q = _system(3, (10, 9, 8), 2, sigma=1.5)
+    K = 64
+    RB.set_shape_many(op, shape)
+    rng = np.random.default_rng(5)
+    B = rng.standard_normal((op.size, K))
+    B[q > 0, :] = 0.0
+    o = _opts("PCG_PLAIN", tol=1e-14)
+    o.setIterMax(m)
+    o.setAcceptanceConvergenceFailureOn()
+    X, diags = RB.solve_many(op, to_dev(B), o)
+    X = X.cpu().numpy()
+    assert all(d["num_iter"] == m and not d["converged"] for d in diags)
+    for k in (0, 1, K - 1):
+        ref = _pcg_iterate(ptr, idx, val, B[:, k], m)
+        assert np.linalg.norm(X[:, k] - ref) <= 1e-10 * np.linalg.norm(ref), k
+
+
 def test_solve_many_zero_column_and_errors():
     op, rhs, (ptr, idx, val, orhs), q = _system(2, (16, 12), 4)
     B = np.stack([orhs, np.zeros_like(orhs), 2.0 * orhs], axis=1)
