@@ -66,15 +66,30 @@ inline ConvTiles conv_tiles(const Grid3& g)
     return {(g.d0 + kTX - 1) / kTX, (g.d1 + kTY - 1) / kTY, (g.d2 + kTZ - 1) / kTZ};
 }
 
+// Where a convolution's input comes from (SRC): the input tensor itself
+// (kSrcPlain), or formed while the input brick is staged -- the decoder's
+// concatenation of the nearest-neighbour x2 upsampled coarse tensor `in`
+// (channels [0, CUP), grid gs) with the skip tensor `in2` (channels [CUP,
+// CIN), the output grid) (kSrcCatUp), or the encoder's 2x2x2 average
+// pooling of the fine tensor `in` (grid gs) (kSrcPool).  The staged values
+// are the ones k_up_fwd / k_copy_channels / k_pool_fwd would have written.
+enum { kSrcPlain = 0, kSrcCatUp = 1, kSrcPool = 2 };
+struct ConvSrc {
+    const bf16* in;
+    const bf16* in2;
+    Grid3 gs;
+};
+
 // On small grids (few output bricks) the COUT/16 channel tiles are split
 // over blockIdx.y, NT tiles per workgroup, so that more workgroups share the
 // work (a 9^3 grid has 9 bricks: with one 16-channel tile each, 4x the
 // workgroups for COUT = 64).
-template <int CIN, int COUT, bool RELU, int NT>
-__global__ __launch_bounds__(256) void k_conv_fwd(Grid3 g, ConvTiles T, const bf16* __restrict__ in,
+template <int CIN, int COUT, bool RELU, int NT, int SRC = kSrcPlain, int CUP = 0>
+__global__ __launch_bounds__(256) void k_conv_fwd(Grid3 g, ConvTiles T, const ConvSrc src,
                                                   const bf16* __restrict__ w, const float* __restrict__ bias,
                                                   bf16* __restrict__ out)
 {
+    const bf16* __restrict__ in = src.in;
     static_assert(COUT % (16 * NT) == 0, "channel tiles must divide COUT");
     const int jt0 = blockIdx.y * NT;        // first 16-channel tile of this workgroup
     constexpr int C8 = CIN / 8;             // 16-byte channel groups per voxel
@@ -91,7 +106,8 @@ __global__ __launch_bounds__(256) void k_conv_fwd(Grid3 g, ConvTiles T, const bf
     const int p2 = g.d2 + 2;
     constexpr int NE = kHalo * C8;
     constexpr int NPASS = (NE + 255) / 256;
-    constexpr int SB = 8;
+    constexpr int SB = SRC == kSrcPool ? 2 : 8;   // a pooled entry is 8 loads
+    const Grid3& gs = src.gs;
     auto stage = [&](int pass0, auto UC) {
         constexpr int UU = decltype(UC)::value;
         bf16x8 v[UU];
@@ -102,8 +118,37 @@ __global__ __launch_bounds__(256) void k_conv_fwd(Grid3 g, ConvTiles T, const bf
             const int hx = v8 % kHX, hy = (v8 / kHX) % kHY, hz = v8 / (kHX * kHY);
             const int px = x0 + hx, py = y0 + hy, pz = z0 + hz;
             v[u] = bf16x8{};
-            if (e < NE && px < g.p0 && py < g.p1 && pz < p2)
-                v[u] = *reinterpret_cast<const bf16x8*>(in + (((int64_t)pz * g.p1 + py) * g.p0 + px) * CIN + c8 * 8);
+            if constexpr (SRC == kSrcPlain) {
+                if (e < NE && px < g.p0 && py < g.p1 && pz < p2)
+                    v[u] = *reinterpret_cast<const bf16x8*>(in + (((int64_t)pz * g.p1 + py) * g.p0 + px) * CIN + c8 * 8);
+            } else {
+                // interior voxels only (the padding is zero)
+                if (e < NE && px >= 1 && px <= g.d0 && py >= 1 && py <= g.d1 && pz >= 1 && pz <= g.d2) {
+                    const int x = px - 1, y = py - 1, z = pz - 1;
+                    if constexpr (SRC == kSrcCatUp) {
+                        if (c8 * 8 < CUP) {
+                            const int64_t pc = (((int64_t)(z / 2) + 1) * gs.p1 + (y / 2) + 1) * gs.p0 + (x / 2) + 1;
+                            v[u] = *reinterpret_cast<const bf16x8*>(in + pc * CUP + c8 * 8);
+                        } else {
+                            const int64_t pf = ((int64_t)pz * g.p1 + py) * g.p0 + px;
+                            v[u] = *reinterpret_cast<const bf16x8*>(src.in2 + pf * (CIN - CUP) + c8 * 8 - CUP);
+                        }
+                    } else {
+                        float sum[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                        for (int k = 0; k < 8; ++k) {
+                            const int64_t pf = (((int64_t)(2 * z + (k >> 2)) + 1) * gs.p1 + (2 * y + ((k >> 1) & 1)) + 1) *
+                                                   gs.p0 +
+                                               (2 * x + (k & 1)) + 1;
+                            const bf16x8 t = *reinterpret_cast<const bf16x8*>(in + pf * CIN + c8 * 8);
+#pragma unroll
+                            for (int j = 0; j < 8; ++j) sum[j] += (float)t[j];
+                        }
+#pragma unroll
+                        for (int j = 0; j < 8; ++j) v[u][j] = (bf16)(0.125f * sum[j]);
+                    }
+                }
+            }
         }
 #pragma unroll
         for (int u = 0; u < UU; ++u) {
@@ -556,8 +601,8 @@ inline Grid3 make_grid(int d0, int d1, int d2)
 }
 inline int64_t padded_size(const Grid3& g) { return (int64_t)(g.d0 + 2) * (g.d1 + 2) * (g.d2 + 2); }
 
-template <int CIN, int COUT>
-void conv_fwd_t(hipStream_t st, const Grid3& g, const bf16* in, const bf16* w, const float* b, bf16* out, bool relu)
+template <int CIN, int COUT, int SRC = kSrcPlain, int CUP = 0>
+void conv_fwd_t(hipStream_t st, const Grid3& g, const ConvSrc& in, const bf16* w, const float* b, bf16* out, bool relu)
 {
     const ConvTiles T = conv_tiles(g);
     const int grid = T.nbx * T.nby * T.nbz;
@@ -565,14 +610,14 @@ void conv_fwd_t(hipStream_t st, const Grid3& g, const bf16* in, const bf16* w, c
     // fewer bricks than half the CUs: one channel tile per workgroup
     if (grid < 128) {
         if (relu)
-            hipLaunchKernelGGL((k_conv_fwd<CIN, COUT, true, 1>), dim3(grid, NT), dim3(256), 0, st, g, T, in, w, b, out);
+            hipLaunchKernelGGL((k_conv_fwd<CIN, COUT, true, 1, SRC, CUP>), dim3(grid, NT), dim3(256), 0, st, g, T, in, w, b, out);
         else
-            hipLaunchKernelGGL((k_conv_fwd<CIN, COUT, false, 1>), dim3(grid, NT), dim3(256), 0, st, g, T, in, w, b, out);
+            hipLaunchKernelGGL((k_conv_fwd<CIN, COUT, false, 1, SRC, CUP>), dim3(grid, NT), dim3(256), 0, st, g, T, in, w, b, out);
     } else {
         if (relu)
-            hipLaunchKernelGGL((k_conv_fwd<CIN, COUT, true, NT>), dim3(grid), dim3(256), 0, st, g, T, in, w, b, out);
+            hipLaunchKernelGGL((k_conv_fwd<CIN, COUT, true, NT, SRC, CUP>), dim3(grid), dim3(256), 0, st, g, T, in, w, b, out);
         else
-            hipLaunchKernelGGL((k_conv_fwd<CIN, COUT, false, NT>), dim3(grid), dim3(256), 0, st, g, T, in, w, b, out);
+            hipLaunchKernelGGL((k_conv_fwd<CIN, COUT, false, NT, SRC, CUP>), dim3(grid), dim3(256), 0, st, g, T, in, w, b, out);
     }
 }
 
@@ -580,7 +625,7 @@ void conv_fwd(hipStream_t st, const Grid3& g, int cin, int cout, const bf16* in,
               bf16* out, bool relu)
 {
 #define RG_CONV(CI, CO) \
-    if (cin == CI && cout == CO) return conv_fwd_t<CI, CO>(st, g, in, w, b, out, relu);
+    if (cin == CI && cout == CO) return conv_fwd_t<CI, CO>(st, g, ConvSrc{in, nullptr, g}, w, b, out, relu);
     RG_CONV(16, 16) RG_CONV(16, 32) RG_CONV(32, 32) RG_CONV(32, 64) RG_CONV(64, 64) RG_CONV(96, 32)
     RG_CONV(48, 16) RG_CONV(32, 16) RG_CONV(64, 32) RG_CONV(32, 96) RG_CONV(16, 48)
 #undef RG_CONV
@@ -757,9 +802,11 @@ __global__ void k_rms_final(const double* __restrict__ part, int np, int64_t n, 
 // Forward pass: y = Net(u / rms(u), logk) with u = bal * dinv * r; z = y[...,0] / (scale bal).
 // With bal (the balance vector of a balanced system, A_b = D A D) the network
 // sees the unbalanced residual's Jacobi-scaled form and returns the balanced
-// correction D^-1 e.
+// correction D^-1 e.  With materialize (training: the backward pass reads
+// them) the pooled and concatenated layer inputs are written out; otherwise
+// the convolutions that read them form them while staging their input.
 void net_forward(Net& N, const double* r, const double* dinv, const double* bal, const float* logk, double* z,
-                 hipStream_t st)
+                 hipStream_t st, bool materialize = false)
 {
     const int64_t nn = (int64_t)N.n[0] * N.n[1] * N.n[2];
     const Grid3 &g1 = N.g1, &g2 = N.g2, &g3 = N.g3;
@@ -769,6 +816,20 @@ void net_forward(Net& N, const double* r, const double* dinv, const double* bal,
                        bal, logk, N.scale, 16, N.x0.p);
     conv_fwd(st, g1, 16, 16, N.x0.p, N.W16[0], N.B32[0], N.a1.p, true);
     conv_fwd(st, g1, 16, 16, N.a1.p, N.W16[1], N.B32[1], N.s1.p, true);
+    if (!materialize) {
+        conv_fwd_t<16, 32, kSrcPool>(st, g2, ConvSrc{N.s1.p, nullptr, g1}, N.W16[2], N.B32[2], N.a2.p, true);
+        conv_fwd(st, g2, 32, 32, N.a2.p, N.W16[3], N.B32[3], N.s2.p, true);
+        conv_fwd_t<32, 64, kSrcPool>(st, g3, ConvSrc{N.s2.p, nullptr, g2}, N.W16[4], N.B32[4], N.a3.p, true);
+        conv_fwd(st, g3, 64, 64, N.a3.p, N.W16[5], N.B32[5], N.a3b.p, true);
+        conv_fwd_t<96, 32, kSrcCatUp, 64>(st, g2, ConvSrc{N.a3b.p, N.s2.p, g3}, N.W16[6], N.B32[6], N.a4.p, true);
+        conv_fwd_t<48, 16, kSrcCatUp, 32>(st, g1, ConvSrc{N.a4.p, N.s1.p, g2}, N.W16[7], N.B32[7], N.a5.p, true);
+        conv_fwd(st, g1, 16, 16, N.a5.p, N.W16[8], N.B32[8], N.y.p, false);
+        if (z)
+            hipLaunchKernelGGL(k_take_output, dim3(ew_grid(nn)), dim3(256), 0, st, g1, N.n[0], N.n[1], N.n[2], N.y.p,
+                               16, N.scale, bal, z);
+        RG_HIP(hipGetLastError());
+        return;
+    }
     hipLaunchKernelGGL(k_pool_fwd, dim3(ew_grid(g2.nvox * 2)), dim3(256), 0, st, g1, g2, 16, N.s1.p, N.p1.p);
     conv_fwd(st, g2, 16, 32, N.p1.p, N.W16[2], N.B32[2], N.a2.p, true);
     conv_fwd(st, g2, 32, 32, N.a2.p, N.W16[3], N.B32[3], N.s2.p, true);
@@ -934,7 +995,7 @@ double net_gradients(Net& N, const double* r, const double* dinv, const float* l
     const Grid3 &g1 = N.g1, &g2 = N.g2, &g3 = N.g3;
     const int64_t nn = (int64_t)N.n[0] * N.n[1] * N.n[2];
     const bool energy = N.loss_op != nullptr;
-    net_forward(N, r, dinv, nullptr, logk, energy ? N.zbuf : nullptr, st);
+    net_forward(N, r, dinv, nullptr, logk, energy ? N.zbuf : nullptr, st, true);
     const int np = 256;
     RG_HIP(hipMemsetAsync(N.gy.p, 0, sizeof(bf16) * padded_size(g1) * 16, st));
     if (energy) {
