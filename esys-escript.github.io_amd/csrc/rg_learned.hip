@@ -70,24 +70,43 @@ inline ConvTiles conv_tiles(const Grid3& g)
 // (kSrcPlain), or formed while the input brick is staged -- the decoder's
 // concatenation of the nearest-neighbour x2 upsampled coarse tensor `in`
 // (channels [0, CUP), grid gs) with the skip tensor `in2` (channels [CUP,
-// CIN), the output grid) (kSrcCatUp), or the encoder's 2x2x2 average
-// pooling of the fine tensor `in` (grid gs) (kSrcPool).  The staged values
-// are the ones k_up_fwd / k_copy_channels / k_pool_fwd would have written.
-enum { kSrcPlain = 0, kSrcCatUp = 1, kSrcPool = 2 };
+// CIN), the output grid) (kSrcCatUp), the encoder's 2x2x2 average pooling
+// of the fine tensor `in` (grid gs) (kSrcPool), or the network input from
+// the solver's vectors on the n0 x n1 x n2 node grid: channel 0 = scale *
+// bal * dinv * r, channel 1 = logk, the others 0 (kSrcInput).  The staged
+// values are the ones k_up_fwd / k_copy_channels / k_pool_fwd /
+// k_make_input would have written.
+enum { kSrcPlain = 0, kSrcCatUp = 1, kSrcPool = 2, kSrcInput = 3 };
 struct ConvSrc {
     const bf16* in;
     const bf16* in2;
     Grid3 gs;
+    const double* r = nullptr;      // kSrcInput
+    const double* dinv = nullptr;
+    const double* bal = nullptr;
+    const float* logk = nullptr;
+    const double* scale = nullptr;
+    int n0 = 0, n1 = 0, n2 = 0;
+};
+
+// With ZOUT the convolution also hands its output channel 0 back to the
+// solver, z[i] = y0 / (scale bal[i]) on the node grid (what k_take_output
+// computes from the stored output).
+struct ConvZOut {
+    double* z = nullptr;
+    const double* scale = nullptr;
+    const double* bal = nullptr;
+    int n0 = 0, n1 = 0, n2 = 0;
 };
 
 // On small grids (few output bricks) the COUT/16 channel tiles are split
 // over blockIdx.y, NT tiles per workgroup, so that more workgroups share the
 // work (a 9^3 grid has 9 bricks: with one 16-channel tile each, 4x the
 // workgroups for COUT = 64).
-template <int CIN, int COUT, bool RELU, int NT, int SRC = kSrcPlain, int CUP = 0>
+template <int CIN, int COUT, bool RELU, int NT, int SRC = kSrcPlain, int CUP = 0, bool ZOUT = false>
 __global__ __launch_bounds__(256) void k_conv_fwd(Grid3 g, ConvTiles T, const ConvSrc src,
                                                   const bf16* __restrict__ w, const float* __restrict__ bias,
-                                                  bf16* __restrict__ out)
+                                                  bf16* __restrict__ out, const ConvZOut zo)
 {
     const bf16* __restrict__ in = src.in;
     static_assert(COUT % (16 * NT) == 0, "channel tiles must divide COUT");
@@ -125,7 +144,15 @@ __global__ __launch_bounds__(256) void k_conv_fwd(Grid3 g, ConvTiles T, const Co
                 // interior voxels only (the padding is zero)
                 if (e < NE && px >= 1 && px <= g.d0 && py >= 1 && py <= g.d1 && pz >= 1 && pz <= g.d2) {
                     const int x = px - 1, y = py - 1, z = pz - 1;
-                    if constexpr (SRC == kSrcCatUp) {
+                    if constexpr (SRC == kSrcInput) {
+                        if (c8 == 0 && x < src.n0 && y < src.n1 && z < src.n2) {
+                            const int64_t i = x + (int64_t)src.n0 * (y + (int64_t)src.n1 * z);
+                            const double sc = src.scale ? src.scale[0] : 1.0;
+                            v[u][0] = (bf16)(float)(sc * (src.bal ? src.bal[i] : 1.0) *
+                                                    (src.dinv ? src.dinv[i] : 1.0) * src.r[i]);
+                            v[u][1] = (bf16)(src.logk ? src.logk[i] : 0.f);
+                        }
+                    } else if constexpr (SRC == kSrcCatUp) {
                         if (c8 * 8 < CUP) {
                             const int64_t pc = (((int64_t)(z / 2) + 1) * gs.p1 + (y / 2) + 1) * gs.p0 + (x / 2) + 1;
                             v[u] = *reinterpret_cast<const bf16x8*>(in + pc * CUP + c8 * 8);
@@ -249,6 +276,13 @@ __global__ __launch_bounds__(256) void k_conv_fwd(Grid3 g, ConvTiles T, const Co
                 float o = acc[m][j][r] + (bias ? bias[co] : 0.f);
                 if (RELU) o = fmaxf(o, 0.f);
                 out[p * COUT + co] = (bf16)o;
+                if constexpr (ZOUT) {
+                    if (co == 0 && x < zo.n0 && y < zo.n1 && z < zo.n2) {
+                        const int64_t i = x + (int64_t)zo.n0 * (y + (int64_t)zo.n1 * z);
+                        const double inv = zo.scale ? 1.0 / zo.scale[0] : 1.0;
+                        zo.z[i] = (double)(float)(bf16)o * inv / (zo.bal ? zo.bal[i] : 1.0);
+                    }
+                }
             }
         }
     }
@@ -601,8 +635,9 @@ inline Grid3 make_grid(int d0, int d1, int d2)
 }
 inline int64_t padded_size(const Grid3& g) { return (int64_t)(g.d0 + 2) * (g.d1 + 2) * (g.d2 + 2); }
 
-template <int CIN, int COUT, int SRC = kSrcPlain, int CUP = 0>
-void conv_fwd_t(hipStream_t st, const Grid3& g, const ConvSrc& in, const bf16* w, const float* b, bf16* out, bool relu)
+template <int CIN, int COUT, int SRC = kSrcPlain, int CUP = 0, bool ZOUT = false>
+void conv_fwd_t(hipStream_t st, const Grid3& g, const ConvSrc& in, const bf16* w, const float* b, bf16* out, bool relu,
+                const ConvZOut& zo = {})
 {
     const ConvTiles T = conv_tiles(g);
     const int grid = T.nbx * T.nby * T.nbz;
@@ -610,14 +645,14 @@ void conv_fwd_t(hipStream_t st, const Grid3& g, const ConvSrc& in, const bf16* w
     // fewer bricks than half the CUs: one channel tile per workgroup
     if (grid < 128) {
         if (relu)
-            hipLaunchKernelGGL((k_conv_fwd<CIN, COUT, true, 1, SRC, CUP>), dim3(grid, NT), dim3(256), 0, st, g, T, in, w, b, out);
+            hipLaunchKernelGGL((k_conv_fwd<CIN, COUT, true, 1, SRC, CUP, ZOUT>), dim3(grid, NT), dim3(256), 0, st, g, T, in, w, b, out, zo);
         else
-            hipLaunchKernelGGL((k_conv_fwd<CIN, COUT, false, 1, SRC, CUP>), dim3(grid, NT), dim3(256), 0, st, g, T, in, w, b, out);
+            hipLaunchKernelGGL((k_conv_fwd<CIN, COUT, false, 1, SRC, CUP, ZOUT>), dim3(grid, NT), dim3(256), 0, st, g, T, in, w, b, out, zo);
     } else {
         if (relu)
-            hipLaunchKernelGGL((k_conv_fwd<CIN, COUT, true, NT, SRC, CUP>), dim3(grid), dim3(256), 0, st, g, T, in, w, b, out);
+            hipLaunchKernelGGL((k_conv_fwd<CIN, COUT, true, NT, SRC, CUP, ZOUT>), dim3(grid), dim3(256), 0, st, g, T, in, w, b, out, zo);
         else
-            hipLaunchKernelGGL((k_conv_fwd<CIN, COUT, false, NT, SRC, CUP>), dim3(grid), dim3(256), 0, st, g, T, in, w, b, out);
+            hipLaunchKernelGGL((k_conv_fwd<CIN, COUT, false, NT, SRC, CUP, ZOUT>), dim3(grid), dim3(256), 0, st, g, T, in, w, b, out, zo);
     }
 }
 
@@ -782,6 +817,29 @@ __global__ void k_rms_part(const double* __restrict__ r, const double* __restric
     if (threadIdx.x == 0) part[blockIdx.x] = red[0];
 }
 
+// The same scale from one workgroup, for the small levels the solver applies
+// the network on (one launch instead of two)
+__global__ __launch_bounds__(1024) void k_rms_small(const double* __restrict__ r, const double* __restrict__ dinv,
+                                                    const double* __restrict__ bal, int64_t n, double* __restrict__ scale)
+{
+    __shared__ double red[1024];
+    double s = 0.0;
+    for (int64_t i = threadIdx.x; i < n; i += 1024) {
+        const double u = (bal ? bal[i] : 1.0) * (dinv ? dinv[i] : 1.0) * r[i];
+        s += u * u;
+    }
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 512; o > 0; o >>= 1) {
+        if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const double rms = sqrt(red[0] / (double)n);
+        scale[0] = rms > 0.0 ? 1.0 / rms : 1.0;
+    }
+}
+
 __global__ void k_rms_final(const double* __restrict__ part, int np, int64_t n, double* __restrict__ scale)
 {
     __shared__ double red[256];
@@ -810,26 +868,43 @@ void net_forward(Net& N, const double* r, const double* dinv, const double* bal,
 {
     const int64_t nn = (int64_t)N.n[0] * N.n[1] * N.n[2];
     const Grid3 &g1 = N.g1, &g2 = N.g2, &g3 = N.g3;
-    hipLaunchKernelGGL(k_rms_part, dim3(1024), dim3(256), 0, st, r, dinv, bal, nn, N.red);
-    hipLaunchKernelGGL(k_rms_final, dim3(1), dim3(256), 0, st, N.red, 1024, nn, N.scale);
-    hipLaunchKernelGGL(k_make_input, dim3(ew_grid(g1.nvox)), dim3(256), 0, st, g1, N.n[0], N.n[1], N.n[2], r, dinv,
-                       bal, logk, N.scale, 16, N.x0.p);
-    conv_fwd(st, g1, 16, 16, N.x0.p, N.W16[0], N.B32[0], N.a1.p, true);
-    conv_fwd(st, g1, 16, 16, N.a1.p, N.W16[1], N.B32[1], N.s1.p, true);
     if (!materialize) {
+        // the input scale in one launch on small grids, the input formed by
+        // the first convolution, z handed back by the last
+        if (nn <= (int64_t)1 << 17) {
+            hipLaunchKernelGGL(k_rms_small, dim3(1), dim3(1024), 0, st, r, dinv, bal, nn, N.scale);
+        } else {
+            hipLaunchKernelGGL(k_rms_part, dim3(1024), dim3(256), 0, st, r, dinv, bal, nn, N.red);
+            hipLaunchKernelGGL(k_rms_final, dim3(1), dim3(256), 0, st, N.red, 1024, nn, N.scale);
+        }
+        ConvSrc in0{nullptr, nullptr, g1};
+        in0.r = r; in0.dinv = dinv; in0.bal = bal; in0.logk = logk; in0.scale = N.scale;
+        in0.n0 = N.n[0]; in0.n1 = N.n[1]; in0.n2 = N.n[2];
+        conv_fwd_t<16, 16, kSrcInput>(st, g1, in0, N.W16[0], N.B32[0], N.a1.p, true);
+        conv_fwd(st, g1, 16, 16, N.a1.p, N.W16[1], N.B32[1], N.s1.p, true);
         conv_fwd_t<16, 32, kSrcPool>(st, g2, ConvSrc{N.s1.p, nullptr, g1}, N.W16[2], N.B32[2], N.a2.p, true);
         conv_fwd(st, g2, 32, 32, N.a2.p, N.W16[3], N.B32[3], N.s2.p, true);
         conv_fwd_t<32, 64, kSrcPool>(st, g3, ConvSrc{N.s2.p, nullptr, g2}, N.W16[4], N.B32[4], N.a3.p, true);
         conv_fwd(st, g3, 64, 64, N.a3.p, N.W16[5], N.B32[5], N.a3b.p, true);
         conv_fwd_t<96, 32, kSrcCatUp, 64>(st, g2, ConvSrc{N.a3b.p, N.s2.p, g3}, N.W16[6], N.B32[6], N.a4.p, true);
         conv_fwd_t<48, 16, kSrcCatUp, 32>(st, g1, ConvSrc{N.a4.p, N.s1.p, g2}, N.W16[7], N.B32[7], N.a5.p, true);
-        conv_fwd(st, g1, 16, 16, N.a5.p, N.W16[8], N.B32[8], N.y.p, false);
-        if (z)
-            hipLaunchKernelGGL(k_take_output, dim3(ew_grid(nn)), dim3(256), 0, st, g1, N.n[0], N.n[1], N.n[2], N.y.p,
-                               16, N.scale, bal, z);
+        if (z) {
+            ConvZOut zo;
+            zo.z = z; zo.scale = N.scale; zo.bal = bal; zo.n0 = N.n[0]; zo.n1 = N.n[1]; zo.n2 = N.n[2];
+            conv_fwd_t<16, 16, kSrcPlain, 0, true>(st, g1, ConvSrc{N.a5.p, nullptr, g1}, N.W16[8], N.B32[8], N.y.p,
+                                                   false, zo);
+        } else {
+            conv_fwd(st, g1, 16, 16, N.a5.p, N.W16[8], N.B32[8], N.y.p, false);
+        }
         RG_HIP(hipGetLastError());
         return;
     }
+    hipLaunchKernelGGL(k_rms_part, dim3(1024), dim3(256), 0, st, r, dinv, bal, nn, N.red);
+    hipLaunchKernelGGL(k_rms_final, dim3(1), dim3(256), 0, st, N.red, 1024, nn, N.scale);
+    hipLaunchKernelGGL(k_make_input, dim3(ew_grid(g1.nvox)), dim3(256), 0, st, g1, N.n[0], N.n[1], N.n[2], r, dinv,
+                       bal, logk, N.scale, 16, N.x0.p);
+    conv_fwd(st, g1, 16, 16, N.x0.p, N.W16[0], N.B32[0], N.a1.p, true);
+    conv_fwd(st, g1, 16, 16, N.a1.p, N.W16[1], N.B32[1], N.s1.p, true);
     hipLaunchKernelGGL(k_pool_fwd, dim3(ew_grid(g2.nvox * 2)), dim3(256), 0, st, g1, g2, 16, N.s1.p, N.p1.p);
     conv_fwd(st, g2, 16, 32, N.p1.p, N.W16[2], N.B32[2], N.a2.p, true);
     conv_fwd(st, g2, 32, 32, N.a2.p, N.W16[3], N.B32[3], N.s2.p, true);
