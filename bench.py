@@ -119,13 +119,11 @@ class Problem:
             # the multilevel flexible CG with the trained network correcting
             # its 17^3 level
             self.op.setLearnedPreconditioner(self.net)
-            o.setCheckInterval(4)
         elif method == "mg_sym":
             # the same flexible CG without the network: the hierarchy goes
             # down to 9^3 nodes, its coarsest level smoothed
             self.op.setLearnedPreconditioner(None)
             self.op.setMultilevel(0, 2, 8)
-            o.setCheckInterval(4)
         self.op.setToSolution(self.x, self.rhs, o)
         return o
 
@@ -664,10 +662,8 @@ def bench_c5(R, torch, np, args):
             net = learned.LearnedPreconditioner(tuple(n // 8 + 1 for n in ne),
                                                 params=np.load(weights, allow_pickle=False))
             op.setLearnedPreconditioner(net)
-            o.setCheckInterval(4)
         elif method == "MULTILEVEL":
             op.setMultilevel(0, 2, 8)
-            o.setCheckInterval(4)
         op.solve(rhs, o)   # warm-up: set-up and graph capture
         op, rhs = system(sym)
         if method == "FCG_LEARNED":

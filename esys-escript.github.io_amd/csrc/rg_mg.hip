@@ -496,6 +496,7 @@ struct FcgArgs {
     KrylovState* st;
     double* partials;
     int64_t n, vld, chunk;
+    const double* wn = nullptr;   // k_fcgm_update: the residual norm is taken of r / wn
 };
 
 __global__ __launch_bounds__(kBlock) void k_fcgm_dots(FcgArgs a)
@@ -608,7 +609,8 @@ __global__ __launch_bounds__(kBlock) void k_fcgm_update(FcgArgs a)
         a.x[i] = fma(alpha, p[i], a.x[i]);
         const double rn = fma(-alpha, q[i], a.r[i]);
         a.r[i] = rn;
-        s[0] = fma(rn, rn, s[0]);
+        const double rw = a.wn ? rn / a.wn[i] : rn;
+        s[0] = fma(rw, rw, s[0]);
     }
     publish_block_sums<1>(s, a.partials);
 }
@@ -1218,7 +1220,8 @@ void smooth_unbalanced(rg_matrix* m, const double* b, double* x, int steps)
 }
 
 // One FCG(kDirs) iteration: z = M(r), the direction, q = A p, the updates.
-void mg_fcg_iteration(rg_matrix* m, double* x, double* r, double* z, KrylovState* st_dev, hipStream_t st)
+void mg_fcg_iteration(rg_matrix* m, double* x, double* r, double* z, KrylovState* st_dev, hipStream_t st,
+                      const double* norm_w)
 {
     rg_mg_state* s = m->mg;
     mg_vcycle(m, r, z, st, st_dev);
@@ -1234,6 +1237,7 @@ void mg_fcg_iteration(rg_matrix* m, double* x, double* r, double* z, KrylovState
     a.partials = m->ctx->partials;
     a.n = g.N;
     a.vld = s->vld;
+    a.wn = norm_w;
     const int vg = 1024;
     a.chunk = 0;
     hipLaunchKernelGGL(k_fcgm_dots, dim3(vg), dim3(kBlock), 0, st, a);

@@ -288,6 +288,7 @@ struct PcgArgs {
     int64_t N, ld, s1, s2, chunk;
     const double* fold_partials;   // k_p_update<.., FOLD>: the update kernel's partials (2 x fold_G)
     int fold_G;
+    const double* wn;       // k_cg_init: the residual norm is taken of r / wn (nullptr: of r)
 };
 
 // Loop over the node range of this block, two nodes per thread for BS == 1.
@@ -585,7 +586,8 @@ __global__ __launch_bounds__(kBlock) void k_cg_init(PcgArgs a, const DT* __restr
             const double rk = r[k];
             p[k] = z[c];
             s[0] += rk * z[c];
-            s[1] += rk * rk;
+            const double rw = a.wn ? rk / a.wn[k] : rk;
+            s[1] += rw * rw;
         }
     }
     publish_block_sums<2>(s, a.partials);
@@ -989,7 +991,7 @@ struct SolverImpl {
     Launch L;     // node kernels without pairing (init, Jacobi sweeps)
     int prec = RG_PRECONDITIONER_JACOBI;
     bool nt = false;   // non-temporal vector streams in the update kernels (stream_policy)
-
+    const double* norm_w = nullptr;   // the flexible CG's residual norm is taken of r / norm_w
 
     explicit SolverImpl(rg_matrix* mm) : m(mm), w(mm->work), st(mm->ctx->stream), L(chunked_launch(mm->g.N)) {}
 
@@ -1004,6 +1006,7 @@ struct SolverImpl {
         a.partials = m->ctx->partials;
         a.fold_partials = nullptr;
         a.fold_G = 0;
+        a.wn = norm_w;
         a.N = m->g.N; a.ld = m->g.ld; a.s1 = m->g.s1; a.s2 = m->g.s2; a.chunk = l.chunk;
         return a;
     }
@@ -1265,7 +1268,7 @@ struct SolverImpl {
     {
         if (method == RG_METHOD_BICGSTAB) bicg_iteration(sweeps);
         else if (method == RG_METHOD_MINRES) mr_iteration(sweeps);
-        else if (method == RG_METHOD_FCG_LEARNED) mg_fcg_iteration(m, w->xw, w->r, w->z, w->st, st);
+        else if (method == RG_METHOD_FCG_LEARNED) mg_fcg_iteration(m, w->xw, w->r, w->z, w->st, st, norm_w);
         else if (method == RG_METHOD_PCG) paso_iteration(sweeps);
         else if (method == RG_METHOD_PCG_PLAIN)
             cg_iteration<double, double>(m->val, m->dinv, sweeps, false, folds(method, sweeps));
@@ -1315,7 +1318,7 @@ struct SolverImpl {
             if (folds(method, sweeps)) stage<ST_CG_COMMIT, 1>(1);
             return;
         }
-        const int key = method + 8 * prec;
+        const int key = method + 8 * prec + (norm_w ? 64 : 0);
         if (!w->graph || w->graph_method != key || w->graph_sweeps != sweeps || w->graph_iters != iters) {
             if (w->graph) {
                 RG_HIP(hipGraphExecDestroy(w->graph));
@@ -1599,7 +1602,6 @@ int solve(rg_matrix* m, double* x, const double* b, const rg_options& opt, rg_di
     double true2_of_b = 0.0, truemax_of_b = 0.0;
     if (true_res) S.norms(b, &true2_of_b, &truemax_of_b);
     int status = RG_OK;
-    const int check = opt.check_interval > 0 ? opt.check_interval : 32;
     const bool use_graph = true;
     if (norm2_of_b <= 0.) {
         hipLaunchKernelGGL(k_fill, dim3(vgrid), dim3(kBlock), 0, st, x, 0.0, n);
@@ -1632,8 +1634,21 @@ int solve(rg_matrix* m, double* x, const double* b, const rg_options& opt, rg_di
             const bool more = true_res ? (true2 > tolerance * true2_of_b)
                                        : (norm2 > tolerance * norm2_of_b || normmax > tolerance * norm_max_of_b);
             if (more) {
+                // the flexible CG tracks the unbalanced residual norm itself
+                // with the true-residual criterion (its r / balance), so its
+                // tolerance is the target with a margin for the recursive
+                // residual's drift; the other methods scale the target by
+                // the observed ratio of balanced to unbalanced norms
+                const bool weighted = true_res && method == RG_METHOD_FCG_LEARNED;
+                S.norm_w = weighted ? m->balance : nullptr;
                 double tol = tolerance * std::min(norm2_of_b, 0.1 * norm2 / normmax * norm_max_of_b);
-                if (true_res) tol = 0.5 * tolerance * true2_of_b * (norm2 / std::max(true2, 1e-300));
+                if (weighted) tol = 0.8 * tolerance * true2_of_b;
+                else if (true_res) tol = 0.5 * tolerance * true2_of_b * (norm2 / std::max(true2, 1e-300));
+                // iterations per replayed graph and host poll: the flexible
+                // CG's iteration is some 50 launches, and the chunk launched
+                // behind the converged one runs as no-ops
+                const int check = opt.check_interval > 0 ? opt.check_interval
+                                                         : (method == RG_METHOD_FCG_LEARNED ? 1 : 32);
                 if (opt.verbose) printf("Solver: new tolerance = %g\n", tol);
                 last2 = norm2;
                 lastmax = normmax;

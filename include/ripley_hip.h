@@ -245,6 +245,10 @@ int rg_matrix_save_mm(rg_matrix* m, const char* filename);
  * criterion:
  *   RG_CRITERION_PASO           balanced ||r||_2 and ||r||_inf tests (Solver.cpp:241)
  *   RG_CRITERION_TRUE_RESIDUAL  unbalanced FP64 ||b - A x||_2 <= tolerance ||b||_2
+ *                               (RG_METHOD_FCG_LEARNED stops its inner solve on
+ *                               the unbalanced norm of its own residual at 0.8 x
+ *                               the target; the other methods on the balanced
+ *                               norm, scaled by the observed ratio)
  * preconditioner (Preconditioner_alloc / Preconditioner_solve,
  * paso/src/Preconditioner.cpp:46-137) for RG_METHOD_PCG, RG_METHOD_PCG_PLAIN,
  * RG_METHOD_BICGSTAB and RG_METHOD_MINRES; the FP32 and learned methods use
@@ -268,7 +272,8 @@ typedef struct {
     int32_t method;
     int32_t verbose;
     int32_t accept_failed_convergence;
-    int32_t check_interval;      /* iterations per replayed graph / host check (0: 32) */
+    int32_t check_interval;      /* iterations per replayed graph / host check (0: 32; 1 for
+                                    RG_METHOD_FCG_LEARNED) */
     int32_t criterion;
     int32_t preconditioner;      /* RG_PRECONDITIONER_*, default Jacobi */
 } rg_options;
