@@ -191,6 +191,27 @@ def test_fcg_converges_fp64_verified():
     assert np.linalg.norm(x.cpu().numpy() - xo) <= 1e-6 * np.linalg.norm(xo)
 
 
+@pytest.mark.parametrize("with_net", [True, False])
+def test_fcg_true_residual_stop_is_one_inner_solve(with_net):
+    # under the true-residual criterion the flexible CG stops on the
+    # unbalanced norm of its own residual: the first outer step's inner
+    # solve already meets the target and the second step only confirms it
+    # (the chunk length left at its default, one iteration)
+    ne = (16, 14, 12)
+    prob = _problem(ne, 17, sigma=1.5)
+    net = L.LearnedPreconditioner((9, 8, 7), params=_random_params(18)) if with_net else None
+    x, o = _fcg_solve(prob, net, tol=1e-8, check=0, levels=2)
+    assert o.hasConverged()
+    assert _true_res(prob, x) <= 1e-8 * 1.001
+    print("true-residual stop: network %s, %d iterations, %d outer steps, %d fallbacks"
+          % (with_net, o.getDiagnostics("num_iter"), o.getDiagnostics("outer_steps"), o.getDiagnostics("fallbacks")),
+          flush=True)
+    if not with_net:
+        assert o.getDiagnostics("fallbacks") == 0
+    if o.getDiagnostics("fallbacks") == 0:   # a fallback continues with the Jacobi-PCG's own stop
+        assert o.getDiagnostics("outer_steps") == 2
+
+
 def test_fcg_falls_back_to_jacobi_pcg():
     # a network whose output is not finite makes the first direction break
     # down: the solver must switch to the Jacobi-PCG and still converge
