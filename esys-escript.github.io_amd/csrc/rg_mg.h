@@ -18,7 +18,9 @@ void mg_setup(rg_matrix* m, bool balanced = true);
 // work once guard->done is set (guard may be null).
 void mg_vcycle(rg_matrix* m, const double* r, double* z, hipStream_t st, const KrylovState* guard);
 // One iteration of the flexible CG on x, r with the V-cycle (z: work vector).
-// norm_w (nullptr: none): the residual norm the stopping test reads is taken of r / norm_w
+// norm_w (nullptr: none): the matrix's balance vector; the residual norm the
+// stopping test reads is then taken of r / balance (through 1 / balance kept
+// by mg_setup)
 void mg_fcg_iteration(rg_matrix* m, double* x, double* r, double* z, KrylovState* st_dev, hipStream_t st,
                       const double* norm_w = nullptr);
 void mg_free(rg_matrix* m);

@@ -496,7 +496,7 @@ struct FcgArgs {
     KrylovState* st;
     double* partials;
     int64_t n, vld, chunk;
-    const double* wn = nullptr;   // k_fcgm_update: the residual norm is taken of r / wn
+    const double* wn = nullptr;   // k_fcgm_update: the residual norm is taken of r * wn (wn = 1 / balance)
 };
 
 __global__ __launch_bounds__(kBlock) void k_fcgm_dots(FcgArgs a)
@@ -609,7 +609,7 @@ __global__ __launch_bounds__(kBlock) void k_fcgm_update(FcgArgs a)
         a.x[i] = fma(alpha, p[i], a.x[i]);
         const double rn = fma(-alpha, q[i], a.r[i]);
         a.r[i] = rn;
-        const double rw = a.wn ? rn / a.wn[i] : rn;
+        const double rw = a.wn ? rn * a.wn[i] : rn;
         s[0] = fma(rw, rw, s[0]);
     }
     publish_block_sums<1>(s, a.partials);
@@ -882,16 +882,18 @@ void mg_setup(rg_matrix* m, bool balanced)
         hipLaunchKernelGGL(k_cheb_table, dim3(1), dim3(kBlock), 0, st, s->gpart, (int)G, kChebRatio, s->lev[0].coef);
         RG_HIP(hipGetLastError());
     }
+    // 1 / balance: the level-0 Galerkin product's scaling and the flexible
+    // CG's unbalanced residual norm
+    if (balanced) {
+        hipLaunchKernelGGL(k_reciprocal, dim3(grid1(g.N)), dim3(kBlock), 0, st, m->balance, g.N, s->ibal);
+        RG_HIP(hipGetLastError());
+    }
     for (int l = 1; l < s->nlev; ++l) {
         MgLevel& L = s->lev[l];
         const rg_matrix* F = (l == 1) ? m : s->lev[l - 1].A;
         const MgGeom f = s->lev[l - 1].g, c = L.g;
         const unsigned G = grid1(c.N);
         const double* bal = (l == 1 && balanced) ? s->ibal : nullptr;
-        if (bal) {
-            hipLaunchKernelGGL(k_reciprocal, dim3(grid1(g.N)), dim3(kBlock), 0, st, m->balance, g.N, s->ibal);
-            RG_HIP(hipGetLastError());
-        }
         galerkin_product(F->val, f, F->symmetric, bal, L.A->val, c, L.A->symmetric, s->gtmp, st);
         // its Jacobi inverse and Gershgorin bound
         const double* main_slot = L.A->val + (int64_t)(L.A->symmetric ? 0 : C3) * c.ld;
@@ -1237,7 +1239,9 @@ void mg_fcg_iteration(rg_matrix* m, double* x, double* r, double* z, KrylovState
     a.partials = m->ctx->partials;
     a.n = g.N;
     a.vld = s->vld;
-    a.wn = norm_w;
+    RG_CHECK(!norm_w || (m->is_balanced && norm_w == m->balance), RG_STATE_ERROR,
+             "FCG_LEARNED: the unbalanced residual norm needs the balanced matrix's balance vector");
+    a.wn = norm_w ? s->ibal : nullptr;   // norm_w is the balance; ibal its reciprocal
     const int vg = 1024;
     a.chunk = 0;
     hipLaunchKernelGGL(k_fcgm_dots, dim3(vg), dim3(kBlock), 0, st, a);
