@@ -128,6 +128,30 @@ def test_solve_many_iteration_limit_leaves_that_iterate(shape, m):
         assert np.linalg.norm(X[:, k] - ref) <= 1e-10 * np.linalg.norm(ref), k
 
 
+# columns that start converged (zero right-hand sides) next to running ones
+# on the 64-column z-marching shapes, FP32 directions with PCG_MIXED: the
+# ended columns stay exactly zero through the alternating direction blocks
+@pytest.mark.parametrize("shape", list(RB.ZMARCH_SHAPES))
+def test_solve_many_zero_columns_zmarch_fp32_directions(shape):
+    op, rhs, (ptr, idx, val, orhs), q = _system(3, (12, 10, 9), 4, sigma=1.5)
+    K = 64
+    RB.set_shape_many(op, shape)
+    rng = np.random.default_rng(7)
+    B = rng.standard_normal((op.size, K))
+    B[q > 0, :] = 0.0
+    zero = [3, 17, 40]
+    B[:, zero] = 0.0
+    X, diags = RB.solve_many(op, to_dev(B), _opts("PCG_MIXED"))
+    X = X.cpu().numpy()
+    assert all(d["converged"] for d in diags)
+    for k in range(K):
+        if k in zero:
+            assert np.all(X[:, k] == 0.0) and diags[k]["num_iter"] == 0
+        else:
+            res = np.linalg.norm(B[:, k] - O.spmv(1, ptr, idx, val, X[:, k])) / np.linalg.norm(B[:, k])
+            assert res <= 1e-9 * 1.001, (k, res)
+
+
 def test_solve_many_zero_column_and_errors():
     op, rhs, (ptr, idx, val, orhs), q = _system(2, (16, 12), 4)
     B = np.stack([orhs, np.zeros_like(orhs), 2.0 * orhs], axis=1)
