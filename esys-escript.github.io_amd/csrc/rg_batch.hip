@@ -1155,11 +1155,13 @@ void dispatch_kp(int K, F&& f)
 
 // ---------------------------------------------------------------- many RHS
 // SpMM tile shapes: (rows per thread R, columns per thread CPT).  A shape
-// applies when CPT divides the number of columns.  Shapes 4 and 5 are the
-// z-marching LDS tiles (exactly 64 columns, Brick only; their R is the x
-// run of 16 or 8 nodes per thread).  Longer row runs per thread (R = 4, 8:
-// R + 2 x windows per plane for 3 R products) spilled with the planes
-// unrolled and ran 2.7-3.1 ms with them rolled, against 1.9 ms.
+// applies when CPT divides the number of columns.  Shapes 4-6 are the
+// z-marching LDS tiles (exactly 64 columns, Brick only): 4 and 5 one column
+// per lane over an x run of 16 or 8 nodes, 6 one x node and 8 columns per
+// lane of an 8-node run (spmm_zmarch_n).  Longer row runs per thread on the
+// row-chunked tiles (R = 4, 8: R + 2 x windows per plane for 3 R products)
+// spilled with the planes unrolled and ran 2.7-3.1 ms with them rolled,
+// against 1.9 ms.
 constexpr int kNumShapes = 7;
 constexpr int kZShape = 4, kZShapeEnd = 7;
 constexpr int kZNodeShape = 6;   // z-marching, one x node per lane
