@@ -749,7 +749,12 @@ __global__ __launch_bounds__(kBlock) void k_mupdate(MArgs a)
     column_loop<KP>(a, [&](int64_t i, int kk) {
         if (!live && !xprev) return;
         const int64_t e = i * a.K + kk;
-        if (WITHX) a.x[e] = fma(alpha_prev, (double)P2[e], fma(alpha, (double)P[e], a.x[e]));
+        if (WITHX) {   // only the pending directions are read (an ended column's may never have been written)
+            double xv = a.x[e];
+            if (live) xv = fma(alpha, (double)P[e], xv);
+            if (xprev) xv = fma(alpha_prev, (double)P2[e], xv);
+            a.x[e] = xv;
+        }
         if (!live) return;
         const double rv = fma(-alpha, (double)Q[e], a.r[e]);
         a.r[e] = rv;
@@ -779,9 +784,15 @@ __global__ __launch_bounds__(kBlock) void k_mpupdate(MArgs a)
     column_loop<KP>(a, [&](int64_t i, int kk) {
         if (!live && !xcur && !xprev) return;
         const int64_t e = i * a.K + kk;
-        const double pv = (double)P[e];
-        if (ODD && (xcur || xprev)) a.x[e] = fma(alpha_prev, (double)P2[e], fma(alpha, pv, a.x[e]));
-        if (live) P2[e] = (PT)fma(beta, pv, a.dinv[i] * a.r[e]);
+        // only the pending directions are read: the current one of a column
+        // that ended before writing it may never have been written
+        if (ODD && (xcur || xprev)) {
+            double xv = a.x[e];
+            if (xcur) xv = fma(alpha, (double)P[e], xv);
+            if (xprev) xv = fma(alpha_prev, (double)P2[e], xv);
+            a.x[e] = xv;
+        }
+        if (live) P2[e] = (PT)fma(beta, (double)P[e], a.dinv[i] * a.r[e]);
     });
 }
 
@@ -1496,6 +1507,10 @@ int solve_multi(rg_matrix* m, double* X, const double* B, int K, const rg_option
         q.alloc(nv);
     }
     S.pd[0] = p.p; S.pd[1] = p2.p; S.pfl[0] = pf.p; S.pfl[1] = pf2.p;
+    // the second direction block starts zero (an ended column's entries are
+    // never read, but stay finite)
+    if (S.pf32) RG_HIP(hipMemsetAsync(pf2.p, 0, sizeof(float) * nv, st));
+    else RG_HIP(hipMemsetAsync(p2.p, 0, sizeof(double) * nv, st));
     part.alloc((size_t)2 * K * S.max_grid());
     S.a.dinv = dinv.p; S.a.b = B; S.a.x = X; S.a.r = r.p; S.a.p = p.p; S.a.q = q.p; S.a.pf = pf.p; S.a.qf = qf.p;
     S.a.st = sts.p; S.a.partials = part.p;

@@ -106,9 +106,11 @@ def _pcg_iterate(ptr, idx, val, b, m):
 
 # x is updated every second iteration (two directions at once) and a chunk
 # ends on the second of a pair: stopping after an odd or even number of
-# iterations, inside a chunk or at its end, must leave x at that iterate
+# iterations, inside a chunk or at its end -- also after the first, before
+# the second direction block was ever written -- must leave x at that
+# iterate
 @pytest.mark.parametrize("shape", [0, 6])
-@pytest.mark.parametrize("m", [7, 10, 16, 17])
+@pytest.mark.parametrize("m", [1, 2, 7, 10, 16, 17])
 def test_solve_many_iteration_limit_leaves_that_iterate(shape, m):
     print("solve_many iteration limit", shape, m, flush=True)
     op, rhs, (ptr, idx, val, orhs), q = _system(3, (10, 9, 8), 2, sigma=1.5)
