@@ -55,3 +55,14 @@ def test_cpp_header_compiles_against_library(tmp_path):
     subprocess.check_call([gxx, "-std=c++17", "-I", os.path.join(ROOT, "include"), str(src), "-L", libdir,
                            "-lripley_hip", "-Wl,-rpath," + libdir, "-o", str(exe)])
     assert subprocess.call([str(exe)]) == 0
+
+
+def test_header_constants_match_the_python_mirror():
+    # the SpMM shape table of the many-RHS solver: one entry per shape the
+    # header counts, the z-marching ones needing all 64 columns
+    from ripley_hip import batch as RB
+    text = open(HEADER).read()
+    shapes = int(re.search(r"#define\s+RG_MULTI_SHAPES\s+(\d+)", text).group(1))
+    assert len(RB.SPMM_SHAPES) == shapes
+    assert all(RB.SPMM_SHAPES[j][1] == 64 for j in RB.ZMARCH_SHAPES)
+    assert all(RB.SPMM_SHAPES[j][1] < 64 for j in range(shapes) if j not in RB.ZMARCH_SHAPES)
