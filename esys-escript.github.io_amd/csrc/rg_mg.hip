@@ -463,22 +463,18 @@ __global__ __launch_bounds__(kBlock) void k_coarse_energy(const double* __restri
 }
 
 // alpha = (r.y) / (y.A y), 0 when y.A y is not positive.
-__global__ __launch_bounds__(kBlock) void k_coarse_alpha(const double* __restrict__ partials, int G,
-                                                         double* __restrict__ alpha, const KrylovState* guard)
+// x += alpha y with the energy-optimal alpha = (r'.y)/(y.A y) (0 when
+// y.A y is not positive), reduced from k_coarse_energy's partials by every
+// workgroup itself, in the same fixed order everywhere (one launch instead
+// of a one-workgroup stage and the update)
+__global__ __launch_bounds__(kBlock) void k_coarse_step(int64_t N, const double* __restrict__ partials, int G,
+                                                        const double* __restrict__ y, double* __restrict__ x,
+                                                        const KrylovState* guard)
 {
     if (guard_done(guard)) return;
     double tot[2];
     reduce_partials<2>(partials, G, tot);
-    if (threadIdx.x == 0) alpha[0] = tot[1] > 0.0 ? tot[0] / tot[1] : 0.0;
-}
-
-// x += alpha y (alpha in device memory)
-__global__ __launch_bounds__(kBlock) void k_axpy_dev(int64_t N, const double* __restrict__ alpha,
-                                                     const double* __restrict__ y, double* __restrict__ x,
-                                                     const KrylovState* guard)
-{
-    if (guard_done(guard)) return;
-    const double a = alpha[0];
+    const double a = tot[1] > 0.0 ? tot[0] / tot[1] : 0.0;
     for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < N; i += (int64_t)gridDim.x * kBlock)
         x[i] = fma(a, y[i], x[i]);
 }
@@ -703,7 +699,6 @@ struct rg_mg_state {
     double* gpart = nullptr;       // block maxima of the set-up's Gershgorin bounds (one per 256 fine rows)
     double* ibal = nullptr;        // 1 / balance of the level-0 matrix
     double* gtmp = nullptr;        // intermediate operators of the Galerkin products
-    double* alpha = nullptr;       // coarse step length
     float* logk = nullptr;         // network input of the coarsest level
     double* y = nullptr;           // network output and its filtered copies
     double* y2 = nullptr;
@@ -786,7 +781,7 @@ void mg_free(rg_matrix* m)
         }
     }
     for (void* p : {(void*)s->coefs, (void*)s->part, (void*)s->gpart, (void*)s->ibal, (void*)s->gtmp,
-                    (void*)s->alpha, (void*)s->logk, (void*)s->y, (void*)s->y2, (void*)s->zero, (void*)s->P,
+                    (void*)s->logk, (void*)s->y, (void*)s->y2, (void*)s->zero, (void*)s->P,
                     (void*)s->Q, (void*)s->fcg})
         if (p) hipFree(p);
     delete s;
@@ -818,7 +813,6 @@ void mg_setup(rg_matrix* m, bool balanced)
         s->part = dalloc(2 * kMaxGrid);
         s->gpart = dalloc(grid1(g.N));
         s->ibal = dalloc(g.N);
-        s->alpha = dalloc(1);
         s->vld = ((g.N + 63) / 64) * 64;
         s->P = dalloc(kDirs * s->vld);
         s->Q = dalloc(kDirs * s->vld);
@@ -1026,8 +1020,8 @@ void net_correct(rg_mg_state* s, rg_matrix* m, int l, const double* b, double* x
         hipLaunchKernelGGL((k_coarse_energy<S.value>), dim3(Le.grid), dim3(kBlock), 0, st, A->val, A->g.ld, A->g.N,
                            A->g.s1, A->g.s2, Le.chunk, yf, L.r, s->part, guard);
     });
-    hipLaunchKernelGGL(k_coarse_alpha, dim3(1), dim3(kBlock), 0, st, s->part, Le.grid, s->alpha, guard);
-    hipLaunchKernelGGL(k_axpy_dev, dim3(grid_for(A->g.N)), dim3(kBlock), 0, st, A->g.N, s->alpha, yf, x, guard);
+    hipLaunchKernelGGL(k_coarse_step, dim3(grid_for(A->g.N)), dim3(kBlock), 0, st, A->g.N, s->part, Le.grid, yf, x,
+                       guard);
     RG_HIP(hipGetLastError());
 }
 
